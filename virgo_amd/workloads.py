@@ -1,0 +1,108 @@
+"""Synthetic circuits in the reference's layered-circuit file layout.
+
+A circuit is a dict {"n": int64[d], "ty": int32[sum n], "u": int64[...],
+"v": int64[...]} holding the d file layers of the reference format
+(zk_verifier.cpp:15-217): file layer 1 is the input layer (type 3 gates whose
+u is the value), each later layer lists its gates in gate order.  The same
+arrays feed the HIP prover's C ABI and the text writer below.
+
+Workload randomness is splitmix64 from a stated seed (counter form:
+x_i = mix(seed + (i+1) * 0x9E3779B97F4A7C15)).
+"""
+import numpy as np
+
+GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(seed, n, stream=0):
+    """n outputs of splitmix64 started at seed ^ (stream * golden^2)."""
+    with np.errstate(over="ignore"):
+        base = np.uint64(seed) ^ (np.uint64(stream) * np.uint64(0xD1B54A32D192ED03))
+        z = base + (np.arange(1, n + 1, dtype=np.uint64) * GAMMA)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def _pack(layers):
+    n = np.array([len(t) for (t, _, _) in layers], dtype=np.int64)
+    ty = np.concatenate([t for (t, _, _) in layers]).astype(np.int32)
+    u = np.concatenate([a for (_, a, _) in layers]).astype(np.int64)
+    v = np.concatenate([b for (_, _, b) in layers]).astype(np.int64)
+    return {"n": n, "ty": ty, "u": u, "v": v}
+
+
+def matmul_circuit(mat, seed=1):
+    """C = A*B over F_p with true matmul wiring: mult gate i*n^2+j*n+k takes
+    (A[i][k], B[k][j]); sum gate i*n+j adds [i*n^2+j*n, +n).  Inputs uniform
+    in [0, 2^31) (the reference truncates input values through int,
+    zk_prover.cpp:60)."""
+    n = int(mat)
+    assert n & (n - 1) == 0
+    vals = (splitmix64(seed, 2 * n * n) >> np.uint64(33)).astype(np.int64)
+    inp = (np.full(2 * n * n, 3, np.int32), vals, np.zeros(2 * n * n, np.int64))
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    mult = (np.ones(n ** 3, np.int32), (i * n + k).astype(np.int64), (n * n + k * n + j).astype(np.int64))
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+    start = (ii * n * n + jj * n).astype(np.int64)
+    summ = (np.full(n * n, 5, np.int32), start, start + n)
+    return _pack([inp, mult, summ])
+
+
+def random_circuit(log_width, depth, seed=1, log_inputs=None):
+    """Random layered circuit: `depth` compute layers of 2^log_width gates,
+    each gate add or mult with probability 1/2, u and v uniform over the
+    previous layer; 2^log_inputs inputs uniform in [0, 2^31)."""
+    if log_inputs is None:
+        log_inputs = log_width
+    W, NI = 1 << log_width, 1 << log_inputs
+    vals = (splitmix64(seed, NI, 1) >> np.uint64(33)).astype(np.int64)
+    layers = [(np.full(NI, 3, np.int32), vals, np.zeros(NI, np.int64))]
+    prev = NI
+    for layer in range(depth):
+        r = splitmix64(seed, 3 * W, 2 + layer)
+        ty = (r[:W] & np.uint64(1)).astype(np.int32)
+        u = (r[W:2 * W] % np.uint64(prev)).astype(np.int64)
+        v = (r[2 * W:] % np.uint64(prev)).astype(np.int64)
+        layers.append((ty, u, v))
+        prev = W
+    return _pack(layers)
+
+
+def write_reference_files(circ, circuit_path, meta_path):
+    """Writes the circuit in the reference's text format (circuit + meta)."""
+    n = circ["n"]
+    off = 0
+    with open(circuit_path, "w") as f:
+        f.write("%d\n" % len(n))
+        for cnt in n:
+            f.write("%d " % cnt)
+            parts = []
+            for g in range(int(cnt)):
+                parts.append("%d %d %d %d" % (circ["ty"][off + g], g, circ["u"][off + g], circ["v"][off + g]))
+            f.write(" ".join(parts) + "\n")
+            off += int(cnt)
+    with open(meta_path, "w") as f:
+        for _ in n:
+            f.write("0 0 1 0 0\n")
+
+
+def read_reference_files(circuit_path):
+    """Parses the reference's circuit text format (meta is not needed: the
+    parallel-layer flags only change how the verifier evaluates predicates)."""
+    toks = open(circuit_path).read().split()
+    pos = 0
+    d = int(toks[pos]); pos += 1
+    ns, ty, u, v = [], [], [], []
+    for _ in range(d):
+        cnt = int(toks[pos]); pos += 1
+        ns.append(cnt)
+        arr = np.array(toks[pos:pos + 4 * cnt], dtype=np.int64).reshape(cnt, 4)
+        pos += 4 * cnt
+        if not np.array_equal(arr[:, 1], np.arange(cnt)):
+            raise ValueError("gates must be in sorted order, and full [0, 2^n - 1]")
+        ty.append(arr[:, 0]); u.append(arr[:, 2]); v.append(arr[:, 3])
+    return {"n": np.array(ns, np.int64), "ty": np.concatenate(ty).astype(np.int32),
+            "u": np.concatenate(u), "v": np.concatenate(v)}

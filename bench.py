@@ -1,0 +1,114 @@
+"""Benchmark: single-proof Virgo prover latency on one MI355X.
+
+Workload (the headline configuration): a random layered circuit with 16
+compute layers of 2^20 gates (add/mult with probability 1/2, wiring uniform
+over the previous layer, splitmix64 seed 1) and 2^20 inputs uniform in
+[0, 2^31), proved with the reference's ZK-GKR sumcheck and the Virgo
+polynomial commitment of the input layer (rate 1/32, 64+1 slices, FRI down to
+32, 33 queries).  A step is one complete proof: inputs and gates are resident
+in HBM when the timed region starts; the step ends with the full transcript
+assembled in host memory.
+
+Prints one JSON line.  `value` is milliseconds per proof (lower is better).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+# int32 VALU lane-ops/s of MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz
+PEAK_VALU_GOPS = 256 * 64 * 2.4
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log-width", type=int, default=20)
+    ap.add_argument("--depth", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-log-width", type=int, default=16)
+    return ap.parse_args()
+
+
+def cpu_baseline(args):
+    """Times the CPU oracle (single-threaded C restatement of the reference
+    prover) on a bounded sample: the same circuit family at 2^sample gates per
+    layer, scaled linearly by width to the 2^20-gate workload."""
+    from oracle import oracle as O
+    from virgo_amd import workloads as W
+    lw = args.cpu_sample_log_width
+    circ = W.random_circuit(lw, args.depth, seed=1)
+    t0 = time.perf_counter()
+    O.prove(circ)
+    dt = time.perf_counter() - t0
+    scale = 2 ** (args.log_width - lw)
+    return {"value": round(dt * 1e3 * scale, 1), "unit": "ms per proof (scaled)", "cores": 1, "kind": "port",
+            "sample": "oracle prove of random circuit 2^%d x %d layers: %.2f s, x%d width scaling"
+                      % (lw, args.depth, dt, scale)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+    import virgo_amd as V
+    from virgo_amd import workloads as W
+    circ = W.random_circuit(args.log_width, args.depth, seed=1)
+    p = V.Prover(circ)
+    for _ in range(args.warmup):
+        p.prove()
+    stage = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tr = p.prove()
+        stage.append(p.stage_ms.copy())
+    dt = time.perf_counter() - t0
+    ms = dt * 1e3 / args.steps
+    import numpy as np
+    st = np.mean(np.array(stage), axis=0)
+    names = ["evaluate", "gkr", "commit_l", "commit_h", "fri", "queries"]
+    n = 20
+    # Dominant work: SHA3 leaf chains of the two initial commitments.
+    # Algorithmic ops: 2 trees x 2^(n-2) leaves x 65 Keccak-f x 24 rounds x
+    # 192 32-bit VALU ops per round (theta 40, rho/pi 100, chi 50, iota 2).
+    keccak_ops = 2 * (2 ** (n - 2)) * 65 * 24 * 192
+    commit_ms = st[2] + st[3]
+    achieved = keccak_ops / (commit_ms * 1e-3) / 1e9
+    line = {
+        "metric": "prover ms @2^20-gate circuit; batched proofs/s vs plain per-proof GPU loop",
+        "value": round(ms, 3),
+        "unit": "ms per proof",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 F_{p^2} (p=2^61-1), exact",
+        "data": "synthetic",
+        "config": {"workload": "random layered circuit 2^%d gates x %d layers, 2^%d inputs, seed 1"
+                               % (args.log_width, args.depth, args.log_width),
+                   "transcript_bytes": len(tr),
+                   "stage_ms": {k: round(float(v), 3) for k, v in zip(names, st)}},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 1), "peak": PEAK_VALU_GOPS, "unit": "Gop/s",
+                     "frac": round(achieved / PEAK_VALU_GOPS, 4), "traffic": None,
+                     "kernel": "commit_l+commit_h stages (NTT + SHA3 leaf chains + Merkle)"},
+    }
+    p.close()
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,90 @@
+/*
+ * virgo.h -- C ABI of the MI355X Virgo prover (libvirgo_hip.so).
+ *
+ * Plain pointers and sizes only.  Field elements are 16 bytes
+ * {uint64 img, uint64 real}, the memory layout of the reference's
+ * prime_field::field_element (include/linear_gkr/prime_field.h:93).
+ * Every entry point returns 0 on success and a negative code on error;
+ * vg_last_error() then describes the failure.  Nothing here aborts.
+ *
+ * The reference's prover is a C++ class driven in-process by its verifier
+ * (src/linear_gkr/zk_verifier.cpp:910-1253).  Its challenge stream is glibc
+ * rand() with the default seed 1, and the order of draws depends only on the
+ * circuit shape, so the whole proof is run here as one device-resident
+ * schedule; vg_prove returns the prover's messages as a transcript in the
+ * order the reference verifier consumes them.
+ */
+#ifndef VIRGO_H
+#define VIRGO_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { uint64_t img, real; } vg_fe;
+
+/* A layered circuit in the reference's file layout (zk_verifier.cpp:15-217):
+ * d file layers; layer 1 holds the inputs (type 3: value u, type 2: zero);
+ * gates of each layer are in gate order, concatenated over layers. */
+typedef struct {
+    int d;
+    const int64_t *n;
+    const int32_t *ty;
+    const int64_t *u;
+    const int64_t *v;
+} vg_circuit_desc;
+
+typedef struct vg_ctx vg_ctx;
+
+/* Selects the HIP device for this thread's later calls. */
+int vg_init(int device);
+const char *vg_last_error(void);
+void vg_free(void *p);
+
+/* Builds the layered circuit exactly as zk_verifier::read_circuit does
+ * (input padding to 2^15 or 2^17, relay layer 1, bit lengths), uploads gates
+ * and inputs, and allocates every device buffer of one proof.
+ * stream: a hipStream_t, or NULL for a stream owned by the context. */
+int vg_ctx_create(vg_ctx **ctx, const vg_circuit_desc *c, void *stream);
+int vg_ctx_destroy(vg_ctx *ctx);
+/* Bytes of device memory held by the context. */
+int64_t vg_ctx_device_bytes(const vg_ctx *ctx);
+
+/* Replaces zk_verifier::verify's prover calls (zk_verifier.cpp:922-1226) and
+ * poly_commit_prover::commit_private_array / commit_public_array /
+ * commit_phase (poly_commit.h:42-351, vpd_verifier.cpp:46-78) plus the
+ * query answers of fri::request_init_value_with_merkle /
+ * request_step_commit (fri.cpp:148-289).
+ * Writes a malloc'd transcript (free with vg_free).  stage_ms (may be NULL)
+ * receives 6 timings in ms: evaluate, gkr, commit_l, commit_h, fri, queries. */
+int vg_prove(vg_ctx *ctx, uint8_t **transcript, size_t *len, double *stage_ms);
+/* Same, but only enqueues the device work of the proof on the context's
+ * stream and returns; vg_finish waits and assembles the transcript.  Lets
+ * several contexts on different streams overlap on one device. */
+int vg_prove_async(vg_ctx *ctx);
+int vg_finish(vg_ctx *ctx, uint8_t **transcript, size_t *len, double *stage_ms);
+
+/* Output layer of the last proof (2^bits elements). */
+int vg_output(vg_ctx *ctx, vg_fe *out, size_t cap, int *bits);
+
+/* ---- single-operation entry points on host arrays ---- */
+
+/* fast_fourier_transform (RS_polynomial.cpp:27-167): result[k] =
+ * sum_{j<coef_len} c_j w^{jk}, k < order, w = root of unity of order `order`. */
+int vg_fft(const vg_fe *coef, int coef_len, int order, const vg_fe *rou, vg_fe *result);
+/* inverse_fast_fourier_transform (RS_polynomial.cpp:169-240) */
+int vg_ifft(const vg_fe *evals, int coef_len, int order, const vg_fe *rou, vg_fe *result);
+/* my_hhash (my_hhash.h:39-46): n independent SHA3-256 digests of 64 bytes */
+int vg_sha3_64(const uint8_t *in, uint8_t *out, size_t n);
+/* merkle_tree_prover::create_tree (merkle_tree.cpp:4-57): heap tree of
+ * 2*pad digests, pad = next power of two >= n_leaf. */
+int vg_merkle_tree(const uint8_t *leaves, int n_leaf, uint8_t *tree);
+/* element-wise F_{p^2} product (prime_field.h:125-151) */
+int vg_fe_mul(const vg_fe *a, const vg_fe *b, vg_fe *out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

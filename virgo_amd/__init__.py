@@ -1,0 +1,193 @@
+"""MI355X-native Virgo prover.
+
+Python front end over libvirgo_hip.so (the C ABI declared in
+include/virgo.h).  The proving work runs in hand-written HIP kernels for
+gfx950; this module only marshals arrays.  There is no CPU fallback: if the
+HIP library or the GPU is missing, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import workloads  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvirgo_hip.so")
+FE = np.dtype([("img", "<u8"), ("real", "<u8")])     # prime_field.h:93 layout
+
+EXPORTS = ["vg_init", "vg_last_error", "vg_free", "vg_ctx_create", "vg_ctx_destroy", "vg_ctx_device_bytes",
+           "vg_prove", "vg_prove_async", "vg_finish", "vg_output", "vg_fft", "vg_ifft", "vg_sha3_64",
+           "vg_merkle_tree", "vg_fe_mul"]
+
+
+class VirgoError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE, "-j4"], check=True)
+
+
+def lib():
+    """Loads libvirgo_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VirgoError("libvirgo_hip.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.vg_init.argtypes = [ctypes.c_int]
+        L.vg_last_error.restype = ctypes.c_char_p
+        L.vg_free.argtypes = [vp]
+        L.vg_ctx_create.argtypes = [ctypes.POINTER(vp), vp, vp]
+        L.vg_ctx_destroy.argtypes = [vp]
+        L.vg_ctx_device_bytes.argtypes = [vp]
+        L.vg_ctx_device_bytes.restype = ctypes.c_int64
+        L.vg_prove.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t), vp]
+        L.vg_prove_async.argtypes = [vp]
+        L.vg_finish.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t), vp]
+        L.vg_output.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]
+        L.vg_fft.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.vg_ifft.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.vg_sha3_64.argtypes = [vp, vp, ctypes.c_size_t]
+        L.vg_merkle_tree.argtypes = [vp, ctypes.c_int, vp]
+        L.vg_fe_mul.argtypes = [vp, vp, vp, ctypes.c_size_t]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise VirgoError("virgo error %d: %s" % (rc, lib().vg_last_error().decode()))
+
+
+_inited = False
+
+
+def init(device=0):
+    global _inited
+    if not _inited:
+        _check(lib().vg_init(device))
+        _inited = True
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("d", ctypes.c_int), ("n", ctypes.c_void_p), ("ty", ctypes.c_void_p),
+                ("u", ctypes.c_void_p), ("v", ctypes.c_void_p)]
+
+
+class Prover:
+    """One proof context: the circuit resident in HBM plus every buffer of a
+    proof.  prove() runs the whole Virgo prover on the device and returns the
+    transcript bytes (the prover messages in the reference verifier's order)."""
+
+    def __init__(self, circ, stream=None):
+        init()
+        self._keep = tuple(np.ascontiguousarray(circ[k], dtype=t) for k, t in
+                           (("n", np.int64), ("ty", np.int32), ("u", np.int64), ("v", np.int64)))
+        n, ty, u, v = self._keep
+        d = _Desc(len(n), _p(n), _p(ty), _p(u), _p(v))
+        self._ctx = ctypes.c_void_p()
+        _check(lib().vg_ctx_create(ctypes.byref(self._ctx), ctypes.byref(d), stream))
+        self.stage_ms = np.zeros(6, dtype=np.float64)
+
+    def device_bytes(self):
+        return int(lib().vg_ctx_device_bytes(self._ctx))
+
+    def prove(self):
+        out = ctypes.c_void_p()
+        ln = ctypes.c_size_t()
+        _check(lib().vg_prove(self._ctx, ctypes.byref(out), ctypes.byref(ln), _p(self.stage_ms)))
+        data = ctypes.string_at(out, ln.value)
+        lib().vg_free(out)
+        return data
+
+    def prove_async(self):
+        _check(lib().vg_prove_async(self._ctx))
+
+    def finish(self):
+        out = ctypes.c_void_p()
+        ln = ctypes.c_size_t()
+        _check(lib().vg_finish(self._ctx, ctypes.byref(out), ctypes.byref(ln), _p(self.stage_ms)))
+        data = ctypes.string_at(out, ln.value)
+        lib().vg_free(out)
+        return data
+
+    def output(self):
+        cap = 1 << 26
+        buf = np.zeros(cap, dtype=FE)
+        bits = ctypes.c_int()
+        _check(lib().vg_output(self._ctx, _p(buf), cap, ctypes.byref(bits)))
+        return buf[: 1 << bits.value].copy()
+
+    def close(self):
+        if self._ctx:
+            lib().vg_ctx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def prove(circ):
+    p = Prover(circ)
+    try:
+        return p.prove()
+    finally:
+        p.close()
+
+
+def fe_mul(a, b):
+    init()
+    a = np.ascontiguousarray(a, dtype=FE)
+    b = np.ascontiguousarray(b, dtype=FE)
+    o = np.zeros_like(a)
+    _check(lib().vg_fe_mul(_p(a), _p(b), _p(o), len(a)))
+    return o
+
+
+def fft(coef, order, rou):
+    init()
+    coef = np.ascontiguousarray(coef, dtype=FE)
+    o = np.zeros(order, dtype=FE)
+    _check(lib().vg_fft(_p(coef), len(coef), order, _p(np.ascontiguousarray(rou, dtype=FE)), _p(o)))
+    return o
+
+
+def ifft(evals, coef_len, rou):
+    init()
+    evals = np.ascontiguousarray(evals, dtype=FE)
+    o = np.zeros(min(coef_len, len(evals)), dtype=FE)
+    _check(lib().vg_ifft(_p(evals), coef_len, len(evals), _p(np.ascontiguousarray(rou, dtype=FE)), _p(o)))
+    return o
+
+
+def sha3_64(blocks):
+    init()
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 64)
+    o = np.zeros((len(blocks), 32), dtype=np.uint8)
+    _check(lib().vg_sha3_64(_p(blocks), _p(o), len(blocks)))
+    return o
+
+
+def merkle_tree(leaves):
+    init()
+    leaves = np.ascontiguousarray(leaves, dtype=np.uint8).reshape(-1, 32)
+    pad = 1
+    while pad < len(leaves):
+        pad *= 2
+    tree = np.zeros((2 * pad, 32), dtype=np.uint8)
+    _check(lib().vg_merkle_tree(_p(leaves), len(leaves), _p(tree)))
+    return tree

@@ -1,0 +1,818 @@
+// CDNA4 (gfx950) kernels of the Virgo prover hot path.  All arithmetic is
+// exact F_{p^2} integer work on the VALU (no MFMA: there is no dense float
+// contraction here).  Reductions and atomics may run in any order because
+// every field operation returns the canonical representative.
+#include "vg_kernels.hpp"
+#include "vg_keccak.hpp"
+#include <cstring>
+
+namespace vg {
+namespace {
+
+constexpr int TPB = 256;
+constexpr int NTT_LM = 11;          // LDS-resident NTT block: 2^11 elements = 32 KB
+constexpr int FOLD_TAIL = 4096;     // fold rounds with <= this many outputs run in one block
+constexpr int MAX_PARTS = 1024;     // max blocks of a fold round (partials per round)
+
+__host__ __device__ inline unsigned nblk(int64_t n, int t = TPB) { return (unsigned)((n + t - 1) / t); }
+
+__device__ __forceinline__ uint64_t red64(uint64_t x)       // x mod p for any 64-bit x
+{
+    uint64_t r = (x & P) + (x >> 61);
+    return r >= P ? r - P : r;
+}
+
+// ---- exact modular atomics (CAS on each 64-bit component) ----
+__device__ __forceinline__ void atomic_add_p(uint64_t *a, uint64_t x)
+{
+    if (x == 0) return;
+    unsigned long long *p = (unsigned long long *)a;
+    unsigned long long old = *p, assumed;
+    do {
+        assumed = old;
+        old = atomicCAS(p, assumed, (unsigned long long)addp(assumed, x));
+    } while (old != assumed);
+}
+__device__ __forceinline__ void atomic_add_fe(fe *a, const fe &x)
+{
+    atomic_add_p(&a->real, x.real);
+    atomic_add_p(&a->img, x.img);
+}
+
+// ---- reductions ----
+__device__ __forceinline__ fe wave_sum(fe x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        fe y;
+        y.real = __shfl_xor((unsigned long long)x.real, off);
+        y.img = __shfl_xor((unsigned long long)x.img, off);
+        x = add(x, y);
+    }
+    return x;
+}
+// block-wide sum of K field elements; result valid in thread 0
+template <int K>
+__device__ __forceinline__ void block_sum(fe *v)
+{
+    __shared__ fe red[K][16];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+        for (int k = 0; k < K; ++k) red[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < K; ++k) {
+            fe s = red[k][0];
+            for (int i = 1; i < nw; ++i) s = add(s, red[k][i]);
+            v[k] = s;
+        }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------
+// circuit evaluation (zk_prover.cpp:54-157)
+// ---------------------------------------------------------------------
+__global__ void k_zero(uint32_t *p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+__global__ void k_eval_input(const int32_t *u, fe *out, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = mk(red64((uint64_t)(int64_t)u[i]), 0);   // field_element((int)u), :60-65
+}
+__global__ void k_eval_layer(const uint8_t *ty, const int32_t *gu, const int32_t *gv, const fe *p, fe *o, int n)
+{
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int u = gu[g], v = gv[g];
+    fe r;
+    switch (ty[g]) {
+    case 0: r = add(p[u], p[v]); break;
+    case 1: r = mul(p[u], p[v]); break;
+    case 3: r = mk(red64((uint64_t)(int64_t)u), 0); break;
+    case 4: case 10: r = p[u]; break;
+    case 5: { r = fzero(); for (int k = u; k < v; ++k) r = add(r, p[k]); break; }
+    case 6: r = sub(fone(), p[u]); break;
+    case 7: r = sub(p[u], p[v]); break;
+    case 8: { fe x = p[u], y = p[v]; r = sub(add(x, y), mul(mul(mk(2, 0), x), y)); break; }
+    case 9: { fe x = p[u], y = p[v]; r = sub(y, mul(x, y)); break; }
+    case 12: { r = fzero(); for (int k = u; k <= v; ++k) r = add(r, mul_base(p[k], 1ull << (k - u))); break; }
+    case 13: r = mul(p[u], sub(fone(), p[v])); break;
+    default: r = fzero(); break;
+    }
+    o[g] = r;
+}
+
+// ---------------------------------------------------------------------
+// multilinear evaluation by halving (V_res, zk_prover.cpp:30-52)
+// ---------------------------------------------------------------------
+__global__ void k_mle_fold(const fe *in, fe *out, int64_t n_out, const fe *rp)
+{
+    fe r = *rp, om = sub(fone(), r);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_out; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = add(mul(in[2 * i], om), mul(in[2 * i + 1], r));
+}
+__global__ __launch_bounds__(1024) void k_mle_tail(fe *a, fe *b, int64_t n, const fe *r, int nr, fe *dst)
+{
+    for (int k = 0; k < nr; ++k) {
+        fe rr = r[k], om = sub(fone(), rr);
+        int64_t no = n >> 1;
+        for (int64_t i = threadIdx.x; i < no; i += blockDim.x) b[i] = add(mul(a[2 * i], om), mul(a[2 * i + 1], rr));
+        __syncthreads();
+        fe *t = a; a = b; b = t;
+        n = no;
+    }
+    if (threadIdx.x == 0) *dst = a[0];
+}
+
+// ---------------------------------------------------------------------
+// eq / beta half tables (zk_prover.cpp:391-418, 802-822)
+// ---------------------------------------------------------------------
+__global__ void k_half_tables(fe *f, fe *s, const fe *f0, const fe *r, int fh, int sh)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    fe one = fone();
+    if (j < (1 << fh)) {
+        fe acc = f0 ? *f0 : one;
+        for (int i = 0; i < fh; ++i) acc = mul(acc, ((j >> i) & 1) ? r[i] : sub(one, r[i]));
+        f[j] = acc;
+    } else {
+        j -= 1 << fh;
+        if (j < (1 << sh)) {
+            fe acc = one;
+            for (int i = 0; i < sh; ++i) acc = mul(acc, ((j >> i) & 1) ? r[fh + i] : sub(one, r[fh + i]));
+            s[j] = acc;
+        }
+    }
+}
+__global__ void k_eq_expand(fe *out, const fe *lo, const fe *hi, int fh, int64_t n)
+{
+    int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (x < n) out[x] = mul(lo[x & ((1 << fh) - 1)], hi[x >> fh]);
+}
+
+struct bg_t { const fe *g0f, *g0s, *g1f, *g1s; int fh; };
+struct bu_t { const fe *f, *s; int fh; };
+__device__ __forceinline__ fe beta_g(const bg_t &b, int i)
+{
+    int lo = i & ((1 << b.fh) - 1), hi = i >> b.fh;
+    return add(mul(b.g0f[lo], b.g0s[hi]), mul(b.g1f[lo], b.g1s[hi]));
+}
+__device__ __forceinline__ fe beta_u(const bu_t &b, int u)
+{
+    return mul(b.f[u & ((1 << b.fh) - 1)], b.s[u >> b.fh]);
+}
+
+// ---------------------------------------------------------------------
+// sumcheck phase-1 table build: scatter by u (zk_prover.cpp:420-587)
+// ---------------------------------------------------------------------
+__global__ void k_p1_scatter(const uint8_t *gty, const int32_t *gu, const int32_t *gv, int n, const fe *pv, bg_t bg,
+                             fe *amb, fe *aVb)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ty = gty[i], u = gu[i], v = gv[i];
+    if (ty == 2 || ty == 3) return;
+    fe tmp = beta_g(bg, i);
+    switch (ty) {
+    case 0: atomic_add_fe(&aVb[u], mul(pv[v], tmp)); atomic_add_fe(&amb[u], tmp); break;
+    case 1: atomic_add_fe(&amb[u], mul(pv[v], tmp)); break;
+    case 5: for (int j = u; j < v; ++j) atomic_add_fe(&amb[j], tmp); break;
+    case 12: for (int j = u; j <= v; ++j) { atomic_add_fe(&amb[j], tmp); tmp = dbl(tmp); } break;
+    case 4: atomic_add_fe(&amb[u], beta_g(bg, u)); break;
+    case 6: atomic_add_fe(&amb[u], neg(tmp)); atomic_add_fe(&aVb[u], tmp); break;
+    case 7: atomic_add_fe(&aVb[u], neg(mul(pv[v], tmp))); atomic_add_fe(&amb[u], tmp); break;
+    case 8: { fe tv = mul(tmp, pv[v]); atomic_add_fe(&aVb[u], tv); atomic_add_fe(&amb[u], sub(tmp, dbl(tv))); break; }
+    case 13: { fe tv = mul(tmp, pv[v]); atomic_add_fe(&amb[u], sub(tmp, tv)); break; }
+    case 9: { fe tv = mul(tmp, pv[v]); atomic_add_fe(&aVb[u], tv); atomic_add_fe(&amb[u], neg(tv)); break; }
+    case 10: atomic_add_fe(&amb[u], tmp); break;
+    default: break;
+    }
+}
+
+// ---------------------------------------------------------------------
+// sumcheck phase-2 table build: scatter by v (zk_prover.cpp:845-972).
+// Gates whose target is index 0 (sum, exp-sum, NOT, relay) are reduced in
+// the block first.
+// ---------------------------------------------------------------------
+__global__ void k_p2_scatter(const uint8_t *gty, const int32_t *gu, const int32_t *gv, int n, bg_t bg, bu_t bu,
+                             const fe *vup, fe *amb, fe *aVb)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    fe acc0 = fzero();
+    if (i < n) {
+        int ty = gty[i], u = gu[i], v = gv[i];
+        if (ty != 2 && ty != 3 && ty != 4) {
+            fe vu = *vup;
+            fe tg = beta_g(bg, i);
+            switch (ty) {
+            case 1: atomic_add_fe(&amb[v], mul(mul(tg, beta_u(bu, u)), vu)); break;
+            case 0: { fe t = mul(tg, beta_u(bu, u)); atomic_add_fe(&amb[v], t); atomic_add_fe(&aVb[v], mul(t, vu)); break; }
+            case 5: { fe t = mul(tg, vu); for (int j = u; j < v; ++j) acc0 = add(acc0, mul(t, beta_u(bu, j))); break; }
+            case 12: { fe t = mul(tg, vu); for (int j = u; j <= v; ++j) { acc0 = add(acc0, mul(t, beta_u(bu, j))); t = dbl(t); } break; }
+            case 6: { fe t = mul(tg, beta_u(bu, u)); acc0 = add(acc0, sub(t, mul(t, vu))); break; }
+            case 7: { fe t = mul(tg, beta_u(bu, u)); atomic_add_fe(&amb[v], neg(t)); atomic_add_fe(&aVb[v], mul(t, vu)); break; }
+            case 8: { fe t = mul(tg, beta_u(bu, u)), tv = mul(t, vu); atomic_add_fe(&amb[v], sub(t, dbl(tv))); atomic_add_fe(&aVb[v], tv); break; }
+            case 13: { fe t = mul(tg, beta_u(bu, u)), tv = mul(t, vu); atomic_add_fe(&amb[v], neg(tv)); atomic_add_fe(&aVb[v], tv); break; }
+            case 9: { fe t = mul(tg, beta_u(bu, u)); atomic_add_fe(&amb[v], sub(t, mul(vu, t))); break; }
+            case 10: { fe t = mul(tg, beta_u(bu, u)); acc0 = add(acc0, mul(t, vu)); break; }
+            default: break;
+            }
+        }
+    }
+    int any = __syncthreads_or(!is_zero(acc0));
+    if (!any) return;
+    block_sum<1>(&acc0);
+    if (threadIdx.x == 0) atomic_add_fe(&aVb[0], acc0);
+}
+
+// ---------------------------------------------------------------------
+// bookkeeping fold (zk_prover.cpp:600-633, 992-1025)
+// ---------------------------------------------------------------------
+struct tabs { lpoly *V, *aV, *am; };
+struct ctabs { const lpoly *V, *aV, *am; };
+
+__device__ __forceinline__ void fold_one(bool first, const fe *Vb, const fe *aVb, const fe *amb, ctabs in, const fe &r,
+                                         int64_t i, lpoly &V, lpoly &aV, lpoly &am)
+{
+    if (first) {
+        fe x0 = Vb[2 * i], x1 = Vb[2 * i + 1];
+        V.b = x0; V.a = sub(x1, x0);
+        x0 = aVb[2 * i]; x1 = aVb[2 * i + 1];
+        aV.b = x0; aV.a = sub(x1, x0);
+        x0 = amb[2 * i]; x1 = amb[2 * i + 1];
+        am.b = x0; am.a = sub(x1, x0);
+    } else {
+        lpoly y0 = in.V[2 * i], y1 = in.V[2 * i + 1];
+        V.b = add(mul(y0.a, r), y0.b); V.a = sub(add(mul(y1.a, r), y1.b), V.b);
+        y0 = in.aV[2 * i]; y1 = in.aV[2 * i + 1];
+        aV.b = add(mul(y0.a, r), y0.b); aV.a = sub(add(mul(y1.a, r), y1.b), aV.b);
+        y0 = in.am[2 * i]; y1 = in.am[2 * i + 1];
+        am.b = add(mul(y0.a, r), y0.b); am.a = sub(add(mul(y1.a, r), y1.b), am.b);
+    }
+}
+__device__ __forceinline__ void fold_acc(fe *acc, const lpoly &V, const lpoly &aV, const lpoly &am)
+{
+    acc[0] = add(acc[0], mul(am.a, V.a));
+    acc[1] = add(add(acc[1], add(mul(am.a, V.b), mul(am.b, V.a))), aV.a);
+    acc[2] = add(add(acc[2], mul(am.b, V.b)), aV.b);
+}
+// block 0 first reduces the previous round's per-block partials
+__device__ void reduce_prev(const fe *prev, int nprev, fe *dst)
+{
+    if (blockIdx.x != 0 || nprev == 0) return;
+    fe acc[3] = {fzero(), fzero(), fzero()};
+    for (int k = threadIdx.x; k < nprev; k += blockDim.x)
+        for (int c = 0; c < 3; ++c) acc[c] = add(acc[c], prev[3 * k + c]);
+    block_sum<3>(acc);
+    if (threadIdx.x == 0) { dst[0] = acc[0]; dst[1] = acc[1]; dst[2] = acc[2]; }
+}
+__global__ __launch_bounds__(TPB) void k_fold(int first, const fe *Vb, const fe *aVb, const fe *amb, ctabs in, const fe *rp,
+                                             tabs out, int64_t n_out, fe *part, const fe *prev_part, int nprev, fe *prev_dst)
+{
+    reduce_prev(prev_part, nprev, prev_dst);
+    fe r = first ? fzero() : *rp;
+    fe acc[3] = {fzero(), fzero(), fzero()};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_out; i += (int64_t)gridDim.x * blockDim.x) {
+        lpoly V, aV, am;
+        fold_one(first, Vb, aVb, amb, in, r, i, V, aV, am);
+        out.V[i] = V; out.aV[i] = aV; out.am[i] = am;
+        fold_acc(acc, V, aV, am);
+    }
+    block_sum<3>(acc);
+    if (threadIdx.x == 0) { part[3 * blockIdx.x] = acc[0]; part[3 * blockIdx.x + 1] = acc[1]; part[3 * blockIdx.x + 2] = acc[2]; }
+}
+// remaining rounds j0..L-1 in one block; X holds round j0's input (unless j0 == 0)
+__global__ __launch_bounds__(1024) void k_fold_tail(int j0, int L, const fe *Vb, const fe *aVb, const fe *amb, tabs X, tabs Y,
+                                                    const fe *r, fe vconst, const fe *prev_part, int nprev, fe *sums, fe *end)
+{
+    if (nprev) {
+        fe acc[3] = {fzero(), fzero(), fzero()};
+        for (int k = threadIdx.x; k < nprev; k += blockDim.x)
+            for (int c = 0; c < 3; ++c) acc[c] = add(acc[c], prev_part[3 * k + c]);
+        block_sum<3>(acc);
+        if (threadIdx.x == 0) { sums[3 * (j0 - 1)] = acc[0]; sums[3 * (j0 - 1) + 1] = acc[1]; sums[3 * (j0 - 1) + 2] = acc[2]; }
+    }
+    for (int j = j0; j < L; ++j) {
+        int64_t n_out = (int64_t)1 << (L - 1 - j);
+        fe rr = j == 0 ? fzero() : r[j - 1];
+        ctabs in = {X.V, X.aV, X.am};
+        fe acc[3] = {fzero(), fzero(), fzero()};
+        for (int64_t i = threadIdx.x; i < n_out; i += blockDim.x) {
+            lpoly V, aV, am;
+            fold_one(j == 0, Vb, aVb, amb, in, rr, i, V, aV, am);
+            Y.V[i] = V; Y.aV[i] = aV; Y.am[i] = am;
+            fold_acc(acc, V, aV, am);
+        }
+        block_sum<3>(acc);      // contains __syncthreads: Y complete before the swap
+        if (threadIdx.x == 0) { sums[3 * j] = acc[0]; sums[3 * j + 1] = acc[1]; sums[3 * j + 2] = acc[2]; }
+        tabs t = X; X = Y; Y = t;
+    }
+    if (threadIdx.x == 0) {
+        lpoly V0 = X.V[0], am0 = X.am[0];
+        end[0] = V0.a; end[1] = V0.b; end[2] = am0.a; end[3] = am0.b;
+        end[4] = add(add(mul(V0.a, r[L - 1]), V0.b), vconst);
+    }
+}
+
+// ---------------------------------------------------------------------
+// wiring predicates combined with v_u, v_v (zk_verifier.cpp:219-606, :1149)
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_general(const uint8_t *gty, const int32_t *gu, const int32_t *gv, int n, bg_t bg,
+                                                bu_t bu, bu_t bv, const fe *vup, const fe *vvp, fe *part)
+{
+    fe vu = *vup, vv = *vvp, one = fone();
+    fe uv = mul(vu, vv);
+    fe acc = fzero();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int ty = gty[i], u = gu[i], v = gv[i];
+        if (ty == 2 || ty == 3 || ty == 4) continue;
+        fe tg = beta_g(bg, i);
+        fe f;
+        switch (ty) {
+        case 0: f = add(vu, vv); break;
+        case 1: f = uv; break;
+        case 6: f = sub(one, vu); break;
+        case 7: f = sub(vu, vv); break;
+        case 8: f = sub(add(vu, vv), dbl(uv)); break;
+        case 9: f = sub(vv, uv); break;
+        case 13: f = mul(vu, sub(one, vv)); break;
+        default: f = vu; break;                          /* 5, 10, 12 */
+        }
+        if (ty == 5 || ty == 12) {
+            fe b0 = mul(tg, beta_u(bv, 0));
+            fe s = fzero();
+            for (int j = u; (ty == 5) ? (j < v) : (j <= v); ++j) {
+                s = add(s, mul(b0, beta_u(bu, j)));
+                if (ty == 12) b0 = dbl(b0);
+            }
+            acc = add(acc, mul(s, f));
+        } else {
+            int vv_idx = (ty == 10) ? 0 : v;
+            acc = add(acc, mul(mul(mul(tg, beta_u(bu, u)), beta_u(bv, vv_idx)), f));
+        }
+    }
+    block_sum<1>(&acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+__global__ void k_reduce(const fe *part, int n, fe *dst)
+{
+    fe acc = fzero();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) acc = add(acc, part[k]);
+    block_sum<1>(&acc);
+    if (threadIdx.x == 0) *dst = acc;
+}
+
+// ---------------------------------------------------------------------
+// NTT.  DIT radix-2 with bit-reversed loads and natural-order output:
+// stages [0, log_m) in LDS on blocks of 2^log_m, later stages in registers
+// (radix 2^g passes).  Twiddles w_N^e = tw[e * SS/N] from one table of
+// w_SS powers; inverse uses tw[SS - e].
+// ---------------------------------------------------------------------
+struct ntt_in {
+    int kind;                 // 0 plain, 1 l*q product at codeword positions, 2 coset twist
+    const fe *src; int64_t stride;
+    const fe *l, *q; int64_t cw_S, cw_stride, step; int log_R;
+    const int *lens; int log_Rl; int hmode;
+};
+__device__ __forceinline__ fe ntt_fetch(const ntt_in &in, int64_t b, int64_t j, const fe *tw, int64_t SS)
+{
+    if (in.kind == 0) return in.src[b * in.stride + j];
+    if (in.kind == 1) {
+        int64_t k = cw_idx(j * in.step, in.cw_S, in.log_R);
+        return mul(in.l[b * in.cw_stride + k], in.q[b * in.cw_stride + k]);
+    }
+    int64_t slice = b >> in.log_Rl, s = b & ((1 << in.log_Rl) - 1);
+    int len = in.lens[slice];
+    if (j >= len) return fzero();
+    fe c = in.src[slice * in.stride + (in.hmode ? len : 0) + j];
+    return mul(c, tw[(s * j) & (SS - 1)]);
+}
+__device__ __forceinline__ fe twid(const fe *tw, int64_t e, int64_t SS, int inverse)
+{
+    return inverse ? tw[(SS - e) & (SS - 1)] : tw[e];
+}
+__global__ __launch_bounds__(TPB) void k_ntt_lds(ntt_in in, fe *out, int64_t out_stride, int log_n, int log_m, int inverse,
+                                                fe scale, int do_scale, const fe *tw, int64_t SS, int log_SS)
+{
+    __shared__ fe sh[1 << NTT_LM];
+    int64_t b = blockIdx.y;
+    int64_t q = blockIdx.x;
+    int M = 1 << log_m;
+    for (int i = threadIdx.x; i < M; i += blockDim.x) {
+        int64_t pos = (q << log_m) + i;
+        int64_t j = log_n ? (int64_t)(__brevll((unsigned long long)pos) >> (64 - log_n)) : 0;
+        sh[i] = ntt_fetch(in, b, j, tw, SS);
+    }
+    __syncthreads();
+    for (int s = 0; s < log_m; ++s) {
+        int half = 1 << s;
+        for (int t = threadIdx.x; t < M / 2; t += blockDim.x) {
+            int k = t & (half - 1);
+            int i0 = ((t >> s) << (s + 1)) | k, i1 = i0 + half;
+            fe w = twid(tw, (int64_t)k << (log_SS - s - 1), SS, inverse);
+            fe a = sh[i0], c = mul(w, sh[i1]);
+            sh[i0] = add(a, c);
+            sh[i1] = sub(a, c);
+        }
+        __syncthreads();
+    }
+    fe *o = out + b * out_stride + (q << log_m);
+    for (int i = threadIdx.x; i < M; i += blockDim.x) o[i] = do_scale ? mul(sh[i], scale) : sh[i];
+}
+// stages [s0, s0+g) in registers; thread = (batch, base with bits [s0,s0+g) clear)
+__global__ __launch_bounds__(TPB) void k_ntt_regs(fe *data, int64_t stride, int64_t nb, int log_n, int s0, int g, int inverse,
+                                                 fe scale, int do_scale, const fe *tw, int64_t SS, int log_SS)
+{
+    int64_t per = (int64_t)1 << (log_n - g);
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= nb * per) return;
+    int64_t b = tid / per, r = tid % per;
+    int64_t lo = r & (((int64_t)1 << s0) - 1), hi = r >> s0;
+    int64_t base = (hi << (s0 + g)) | lo;
+    fe *d = data + b * stride;
+    fe x[16];
+    int G = 1 << g;
+    for (int m = 0; m < G; ++m) x[m] = d[base + ((int64_t)m << s0)];
+    for (int t = 0; t < g; ++t) {
+        int s = s0 + t;
+        for (int m = 0; m < G; ++m) {
+            if (m & (1 << t)) continue;
+            int m1 = m | (1 << t);
+            int64_t e = lo + ((int64_t)(m & ((1 << t) - 1)) << s0);
+            fe w = twid(tw, e << (log_SS - s - 1), SS, inverse);
+            fe a = x[m], c = mul(w, x[m1]);
+            x[m] = add(a, c);
+            x[m1] = sub(a, c);
+        }
+    }
+    for (int m = 0; m < G; ++m) d[base + ((int64_t)m << s0)] = do_scale ? mul(x[m], scale) : x[m];
+}
+
+// tw[e] = T1[e & 1023] * T2[e >> 10]
+__global__ void k_twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2)
+{
+    int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e < n) tw[e] = mul(T1[e & 1023], T2[e >> 10]);
+}
+
+// ---------------------------------------------------------------------
+// commitment helpers (poly_commit.h:203-330)
+// ---------------------------------------------------------------------
+__global__ void k_slice_sums(const fe *lq, int64_t stride, int nslice, int64_t real, int64_t m, fe *c0, fe *all_sum)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslice) return;
+    int64_t len = (i == nslice - 1) ? m : real;
+    fe c = add(lq[i * stride], lq[i * stride + len]);
+    c0[i] = c;
+    all_sum[i] = mul_base(c, (uint64_t)len);
+}
+__global__ void k_vo(const fe *l, const fe *q, const fe *h, const fe *c0, int nslice, int64_t SS, int log_R, int64_t real,
+                     int64_t m, const fe *tw, fe *vo)
+{
+    int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (id >= (int64_t)nslice * SS) return;
+    int slice = (int)(id / SS);
+    int64_t st = id - slice * SS;
+    int64_t cols = SS >> log_R;
+    int64_t s = st / cols, t = st - s * cols;
+    int64_t p = (t << log_R) + s;
+    int64_t e = (slice == nslice - 1) ? m : real;
+    fe xe = tw[(p * e) & (SS - 1)], xinv = tw[(SS - p) & (SS - 1)];
+    fe g = sub(mul(l[id], q[id]), mul(sub(xe, fone()), h[id]));
+    vo[id] = mul_base(mul(sub(g, c0[slice]), xinv), (uint64_t)e);
+}
+
+// ---------------------------------------------------------------------
+// hashing: leaf chains and Merkle levels
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void chain_step(uint64_t *h, const fe &a, const fe &b)
+{
+    uint64_t m[8] = {a.img, a.real, b.img, b.real, h[0], h[1], h[2], h[3]};
+    sha3_64(m, h);
+}
+__device__ __forceinline__ void store_digest(uint8_t *tree, int64_t node, const uint64_t *h)
+{
+    ulonglong2 *d = (ulonglong2 *)(tree + node * 32);
+    d[0] = make_ulonglong2(h[0], h[1]);
+    d[1] = make_ulonglong2(h[2], h[3]);
+}
+// leaf (s-major thread order) -> natural leaf index i and the row/column of its pair
+__device__ __forceinline__ int64_t leaf_of(int64_t tid, int64_t nleaf, int log_R)
+{
+    int64_t cl = nleaf >> log_R;
+    if (cl == 0) return tid;
+    int64_t s = tid / cl, t = tid - s * cl;
+    return (t << log_R) + s;
+}
+__global__ __launch_bounds__(TPB) void k_leaf_init(const fe *cw, int64_t S, int log_R, uint8_t *tree)
+{
+    int64_t half = S / 2;
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= half) return;
+    int64_t i = leaf_of(tid, half, log_R);
+    int64_t k0 = cw_idx(i, S, log_R), k1 = cw_idx(i + half, S, log_R);
+    uint64_t h[4] = {0, 0, 0, 0};
+    for (int j = 0; j <= 64; ++j) chain_step(h, cw[j * S + k0], cw[j * S + k1]);
+    store_digest(tree, half + i, h);
+}
+__global__ __launch_bounds__(TPB) void k_merkle_level(uint8_t *tree, int64_t start, int64_t n)
+{
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    k += start;
+    const uint64_t *c = (const uint64_t *)(tree + 2 * k * 32);
+    uint64_t m[8], h[4];
+    for (int i = 0; i < 8; ++i) m[i] = c[i];
+    sha3_64(m, h);
+    store_digest(tree, k, h);
+}
+__global__ __launch_bounds__(1024) void k_merkle_tail(uint8_t *tree, int64_t n)
+{
+    for (; n >= 1; n >>= 1) {
+        for (int64_t k = n + threadIdx.x; k < 2 * n; k += blockDim.x) {
+            const uint64_t *c = (const uint64_t *)(tree + 2 * k * 32);
+            uint64_t m[8], h[4];
+            for (int i = 0; i < 8; ++i) m[i] = c[i];
+            sha3_64(m, h);
+            store_digest(tree, k, h);
+        }
+        __syncthreads();
+    }
+}
+// FRI step: fold with r (fri.cpp:321-340, 367-379), write the folded codeword,
+// hash each new leaf (fri.cpp:397-420).
+__global__ __launch_bounds__(TPB) void k_fri_step(const fe *prev, int64_t S, int log_R, const fe *rp, const fe *tw,
+                                                 int64_t SS, fe *next, uint8_t *tree)
+{
+    int64_t Sn = S / 2, nleaf = S / 4;
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= nleaf) return;
+    int64_t i = leaf_of(tid, nleaf, log_R);
+    fe r = *rp;
+    const fe inv2 = mk(1ull << 60, 0);
+    int64_t p[2] = {i, i + nleaf};
+    int64_t ka[2], kb[2], kn[2];
+    fe imr[2];
+    int64_t sh = SS / S;
+    for (int z = 0; z < 2; ++z) {
+        ka[z] = cw_idx(p[z], S, log_R);
+        kb[z] = cw_idx(p[z] + Sn, S, log_R);
+        kn[z] = cw_idx(p[z], Sn, log_R);
+        imr[z] = mul(tw[(SS - p[z] * sh) & (SS - 1)], r);
+    }
+    uint64_t h[4] = {0, 0, 0, 0};
+    for (int j = 0; j <= 64; ++j) {
+        fe f[2];
+        for (int z = 0; z < 2; ++z) {
+            fe a = prev[j * S + ka[z]], b = prev[j * S + kb[z]];
+            f[z] = mul(inv2, add(add(a, b), mul(imr[z], sub(a, b))));
+            next[j * Sn + kn[z]] = f[z];
+        }
+        chain_step(h, f[0], f[1]);
+    }
+    store_digest(tree, nleaf + i, h);
+}
+__global__ void k_gather16(const uint64_t *src, int64_t n, fe *out)
+{
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k < n) out[k] = *(const fe *)(uintptr_t)src[k];
+}
+
+__global__ void k_sha3_batch(const uint64_t *in, uint64_t *out, int64_t n)
+{
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint64_t m[8], h[4];
+    for (int i = 0; i < 8; ++i) m[i] = in[8 * k + i];
+    sha3_64(m, h);
+    for (int i = 0; i < 4; ++i) out[4 * k + i] = h[i];
+}
+// pad leaves of a Merkle tree: H(0^64) (merkle_tree.cpp:105-117)
+__global__ void k_merkle_pad(uint8_t *tree, int64_t pad, int64_t n_leaf)
+{
+    int64_t k = n_leaf + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= pad) return;
+    uint64_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0}, h[4];
+    sha3_64(m, h);
+    store_digest(tree, pad + k, h);
+}
+__global__ void k_fe_mul(const fe *a, const fe *b, fe *o, int64_t n)
+{
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k < n) o[k] = mul(a[k], b[k]);
+}
+
+int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
+
+void ntt_run(const ntt_in &in, fe *out, int64_t out_stride, int64_t nb, int log_n, int inverse, fe scale, int do_scale,
+             const fe *tw, int64_t SS, hipStream_t st)
+{
+    int log_SS = ilog2(SS);
+    int log_m = log_n < NTT_LM ? log_n : NTT_LM;
+    int rest = log_n - log_m;
+    dim3 grid((unsigned)((int64_t)1 << (log_n - log_m)), (unsigned)nb);
+    hipLaunchKernelGGL(k_ntt_lds, grid, dim3(TPB), 0, st, in, out, out_stride, log_n, log_m, inverse, scale,
+                       rest == 0 ? do_scale : 0, tw, SS, log_SS);
+    int s0 = log_m;
+    while (rest > 0) {
+        int g = rest > 4 ? 4 : rest;
+        rest -= g;
+        int64_t threads = nb * ((int64_t)1 << (log_n - g));
+        hipLaunchKernelGGL(k_ntt_regs, dim3(nblk(threads)), dim3(TPB), 0, st, out, out_stride, nb, log_n, s0, g, inverse,
+                           scale, rest == 0 ? do_scale : 0, tw, SS, log_SS);
+        s0 += g;
+    }
+}
+
+}  // namespace
+
+namespace launch {
+
+void zero(void *p, size_t bytes, hipStream_t s)
+{
+    int64_t n = (int64_t)(bytes / 4);
+    if (!n) return;
+    unsigned g = nblk(n);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_zero, dim3(g), dim3(TPB), 0, s, (uint32_t *)p, n);
+}
+void eval_input(const int32_t *u, fe *out, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_eval_input, dim3(nblk(n)), dim3(TPB), 0, s, u, out, n);
+}
+void eval_layer(gates_dev g, const fe *prev, fe *out, hipStream_t s)
+{
+    int n = 1 << g.bl;
+    hipLaunchKernelGGL(k_eval_layer, dim3(nblk(n)), dim3(TPB), 0, s, g.ty, g.u, g.v, prev, out, n);
+}
+void mle_eval(const fe *in, int log_n, const fe *r, fe *t0, fe *t1, fe *dst, hipStream_t s)
+{
+    int64_t n = (int64_t)1 << log_n;
+    int k = 0;
+    const fe *src = in;
+    fe *bufs[2] = {t0, t1};
+    int cur = 0;
+    while (n > FOLD_TAIL) {
+        int64_t no = n / 2;
+        unsigned g = nblk(no);
+        if (g > 4096) g = 4096;
+        hipLaunchKernelGGL(k_mle_fold, dim3(g), dim3(TPB), 0, s, src, bufs[cur], no, r + k);
+        src = bufs[cur];
+        cur ^= 1;
+        n = no;
+        ++k;
+    }
+    // tail: copy into the ping-pong buffers if still reading the input
+    if (src == in) {
+        (void)hipMemcpyAsync(t0, in, sizeof(fe) * n, hipMemcpyDeviceToDevice, s);
+        src = t0; cur = 1;
+    }
+    hipLaunchKernelGGL(k_mle_tail, dim3(1), dim3(1024), 0, s, (fe *)src, bufs[cur], n, r + k, log_n - k, dst);
+}
+void half_tables(fe *f, fe *s, const fe *f0, const fe *r, int len, hipStream_t st)
+{
+    int fh = len >> 1, sh = len - fh;
+    int n = (1 << fh) + (1 << sh);
+    hipLaunchKernelGGL(k_half_tables, dim3(nblk(n)), dim3(TPB), 0, st, f, s, f0, r, fh, sh);
+}
+void eq_table(fe *out, const fe *r, int len, fe *scratch, hipStream_t st)
+{
+    int fh = len >> 1;
+    fe *lo = scratch, *hi = scratch + (1 << fh);
+    half_tables(lo, hi, nullptr, r, len, st);
+    int64_t n = (int64_t)1 << len;
+    hipLaunchKernelGGL(k_eq_expand, dim3(nblk(n)), dim3(TPB), 0, st, out, lo, hi, fh, n);
+}
+void p1_scatter(gates_dev g, const fe *prevV, beta_g bg, fe *amb, fe *aVb, hipStream_t s)
+{
+    int n = 1 << g.bl;
+    bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
+    hipLaunchKernelGGL(k_p1_scatter, dim3(nblk(n)), dim3(TPB), 0, s, g.ty, g.u, g.v, n, prevV, b, amb, aVb);
+}
+void p2_scatter(gates_dev g, beta_g bg, beta_u bu, const fe *vu, fe *amb, fe *aVb, hipStream_t s)
+{
+    int n = 1 << g.bl;
+    bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
+    bu_t u = {bu.f, bu.s, bu.fh};
+    hipLaunchKernelGGL(k_p2_scatter, dim3(nblk(n)), dim3(TPB), 0, s, g.ty, g.u, g.v, n, b, u, vu, amb, aVb);
+}
+void sumcheck_phase(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *sums,
+                    fe *end, hipStream_t s)
+{
+    int j = 0;
+    int cur = 0;
+    const fe *prev_part = nullptr;
+    int nprev = 0;
+    fe *part[2] = {ws.part, ws.part + 3 * MAX_PARTS};
+    while (j < L && ((int64_t)1 << (L - 1 - j)) > FOLD_TAIL) {
+        int64_t n_out = (int64_t)1 << (L - 1 - j);
+        unsigned g = nblk(n_out);
+        if (g > MAX_PARTS) g = MAX_PARTS;
+        ctabs in = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
+        tabs out = {ws.V[cur], ws.aV[cur], ws.am[cur]};
+        hipLaunchKernelGGL(k_fold, dim3(g), dim3(TPB), 0, s, j == 0 ? 1 : 0, Vb, aVb, amb, in, j == 0 ? r : r + j - 1, out,
+                           n_out, part[j & 1], prev_part, nprev, j > 0 ? sums + 3 * (j - 1) : sums);
+        prev_part = part[j & 1];
+        nprev = (int)g;
+        cur ^= 1;
+        ++j;
+    }
+    tabs X = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
+    tabs Y = {ws.V[cur], ws.aV[cur], ws.am[cur]};
+    hipLaunchKernelGGL(k_fold_tail, dim3(1), dim3(1024), 0, s, j, L, Vb, aVb, amb, X, Y, r, vconst, prev_part, nprev, sums, end);
+}
+void general_value(gates_dev g, beta_g bg, beta_u bu, beta_u bv, const fe *vu, const fe *vv, fe *part, unsigned *counter,
+                   fe *dst, hipStream_t s)
+{
+    (void)counter;
+    int n = 1 << g.bl;
+    unsigned blocks = nblk(n);
+    if (blocks > MAX_PARTS) blocks = MAX_PARTS;
+    bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
+    bu_t u = {bu.f, bu.s, bu.fh}, v = {bv.f, bv.s, bv.fh};
+    hipLaunchKernelGGL(k_general, dim3(blocks), dim3(TPB), 0, s, g.ty, g.u, g.v, n, b, u, v, vu, vv, part);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, part, (int)blocks, dst);
+}
+void twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_twiddles, dim3(nblk(n)), dim3(TPB), 0, s, tw, n, T1, T2);
+}
+void intt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, fe inv_n, const fe *tw, int64_t SS, hipStream_t s)
+{
+    ntt_in ni;
+    ni.kind = in.kind; ni.src = in.src; ni.stride = in.src_stride;
+    ni.l = in.l; ni.q = in.q; ni.cw_S = in.cw_S; ni.cw_stride = in.cw_stride; ni.step = in.step; ni.log_R = in.log_R;
+    ni.lens = nullptr; ni.log_Rl = 0; ni.hmode = 0;
+    ntt_run(ni, out, out_stride, nb, log_n, 1, inv_n, 1, tw, SS, s);
+}
+void lde(const fe *coef, int64_t coef_stride, const int *lens, int nb, fe *out, int log_C, int log_R, const fe *tw,
+         int64_t SS, hipStream_t s, int hmode)
+{
+    ntt_in ni;
+    memset(&ni, 0, sizeof ni);
+    ni.kind = 2; ni.src = coef; ni.stride = coef_stride; ni.lens = lens; ni.log_Rl = log_R; ni.hmode = hmode;
+    ntt_run(ni, out, (int64_t)1 << log_C, (int64_t)nb << log_R, log_C, 0, fone(), 0, tw, SS, s);
+}
+void virtual_oracle(const fe *l, const fe *q, const fe *h, const fe *c0, int nslice, int64_t SS, int log_R, int64_t real,
+                    int64_t m, const fe *tw, fe *vo, hipStream_t s)
+{
+    int64_t n = (int64_t)nslice * SS;
+    hipLaunchKernelGGL(k_vo, dim3(nblk(n)), dim3(TPB), 0, s, l, q, h, c0, nslice, SS, log_R, real, m, tw, vo);
+}
+void slice_sums(const fe *lq_coef, int64_t stride, int nslice, int64_t real, int64_t m, fe *c0, fe *all_sum, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_slice_sums, dim3(1), dim3(128), 0, s, lq_coef, stride, nslice, real, m, c0, all_sum);
+}
+void leaf_hash_init(const fe *cw, int64_t S, int log_R, uint8_t *tree, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_leaf_init, dim3(nblk(S / 2)), dim3(TPB), 0, s, cw, S, log_R, tree);
+}
+void merkle(uint8_t *tree, int64_t n_leaf, hipStream_t s)
+{
+    int64_t n = n_leaf / 2;
+    while (n > 1024) {
+        hipLaunchKernelGGL(k_merkle_level, dim3(nblk(n)), dim3(TPB), 0, s, tree, n, n);
+        n /= 2;
+    }
+    if (n >= 1) hipLaunchKernelGGL(k_merkle_tail, dim3(1), dim3(1024), 0, s, tree, n);
+}
+void fri_step(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, int64_t SS, fe *next, uint8_t *tree,
+              hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fri_step, dim3(nblk(S / 4)), dim3(TPB), 0, s, prev, S, log_R, r, tw, SS, next, tree);
+}
+void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int inverse, fe scale, int do_scale, const fe *tw,
+         int64_t SS, hipStream_t s)
+{
+    ntt_in ni;
+    memset(&ni, 0, sizeof ni);
+    ni.kind = in.kind; ni.src = in.src; ni.stride = in.src_stride;
+    ni.l = in.l; ni.q = in.q; ni.cw_S = in.cw_S; ni.cw_stride = in.cw_stride; ni.step = in.step; ni.log_R = in.log_R;
+    ntt_run(ni, out, out_stride, nb, log_n, inverse, scale, do_scale, tw, SS, s);
+}
+void sha3_batch(const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s)
+{
+    if (n) hipLaunchKernelGGL(k_sha3_batch, dim3(nblk(n)), dim3(TPB), 0, s, (const uint64_t *)in, (uint64_t *)out, n);
+}
+void merkle_pad(uint8_t *tree, int64_t pad, int64_t n_leaf, hipStream_t s)
+{
+    if (pad > n_leaf) hipLaunchKernelGGL(k_merkle_pad, dim3(nblk(pad - n_leaf)), dim3(TPB), 0, s, tree, pad, n_leaf);
+}
+void fe_mul(const fe *a, const fe *b, fe *o, int64_t n, hipStream_t s)
+{
+    if (n) hipLaunchKernelGGL(k_fe_mul, dim3(nblk(n)), dim3(TPB), 0, s, a, b, o, n);
+}
+
+void gather16(const uint64_t *src, int64_t n, fe *out, hipStream_t s)
+{
+    if (n) hipLaunchKernelGGL(k_gather16, dim3(nblk(n)), dim3(TPB), 0, s, src, n, out);
+}
+
+}  // namespace launch
+}  // namespace vg

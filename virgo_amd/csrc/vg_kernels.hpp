@@ -1,0 +1,112 @@
+// Host-side launchers of the CDNA4 kernels of the Virgo prover.  Every
+// launcher enqueues on the given stream and returns immediately; all pointers
+// are device pointers.
+#pragma once
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+#include "vg_field.hpp"
+
+namespace vg {
+
+// Codeword storage used by the commitment: each slice of S natural positions
+// is kept as R rows ("cosets" s) by S/R columns t, natural position
+// p = t*R + s stored at s*(S/R) + t.  Every FRI pair (p, p+S/2) and every
+// Merkle leaf pair then lies in one row, so all passes read contiguously.
+struct cw_layout {
+    int R;          // rows (cosets), R <= 32
+    int log_R;
+};
+__host__ __device__ inline int64_t cw_idx(int64_t p, int64_t S, int log_R)
+{
+    int64_t R = (int64_t)1 << log_R;
+    return (p & (R - 1)) * (S >> log_R) + (p >> log_R);
+}
+
+struct gates_dev {            // one layer's gates, SoA
+    const uint8_t *ty;
+    const int32_t *u, *v;
+    int bl;
+};
+
+namespace launch {
+
+void zero(void *p, size_t bytes, hipStream_t s);
+void eval_input(const int32_t *u, fe *out, int n, hipStream_t s);
+void eval_layer(gates_dev g, const fe *prev, fe *out, hipStream_t s);
+
+// out[j] = in[2j](1-r[k]) + in[2j+1] r[k] for all rounds k < nr, result -> dst
+void mle_eval(const fe *in, int log_n, const fe *r, fe *tmp0, fe *tmp1, fe *dst, hipStream_t s);
+
+// f[j] = f0 * prod_{i<fh} eq(bit_i(j), r[i]), s[j] = prod_{i<sh} eq(bit_i(j), r[fh+i]),
+// fh = len/2, sh = len - fh (zk_prover.cpp:391-418).  f0 may be null (= 1).
+void half_tables(fe *f, fe *s, const fe *f0, const fe *r, int len, hipStream_t st);
+// full eq table out[x] = prod_i eq(bit_i(x), r[i]) (zk_verifier.cpp:838-847)
+void eq_table(fe *out, const fe *r, int len, fe *scratch, hipStream_t s);
+
+struct beta_g { const fe *g0f, *g0s, *g1f, *g1s; int fh; };
+struct beta_u { const fe *f, *s; int fh; };
+
+void p1_scatter(gates_dev g, const fe *prevV, beta_g bg, fe *amb, fe *aVb, hipStream_t s);
+void p2_scatter(gates_dev g, beta_g bg, beta_u bu, const fe *vu, fe *amb, fe *aVb, hipStream_t s);
+
+struct fold_ws {              // workspace of a sumcheck phase
+    lpoly *V[2], *aV[2], *am[2];
+    fe *part;                 // per-block partial sums (3 per block)
+    unsigned *counter;
+};
+// Runs every round of one sumcheck phase on tables of 2^L entries.
+// Round j folds with r[j-1] (r[-1] unused); round sums (a,b,c) -> sums[3j..].
+// After the last round: end[0..3] = V[0].a, V[0].b, am[0].a, am[0].b and
+// end[4] = V[0].a * r[L-1] + V[0].b + vconst (= v_u or v_v).
+void sumcheck_phase(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst,
+                    fold_ws ws, fe *sums, fe *end, hipStream_t s);
+
+// sum over gates of beta_g(g) beta_u(u) beta_v(v) f_type(v_u, v_v)
+// (zk_verifier.cpp:219-606 and the combination at :1149)
+void general_value(gates_dev g, beta_g bg, beta_u bu, beta_u bv, const fe *vu, const fe *vv,
+                   fe *part, unsigned *counter, fe *dst, hipStream_t s);
+
+// tw[e] = w^e, e < n, from T1[j] = w^j (j < 1024) and T2[j] = w^(1024 j)
+void twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2, hipStream_t s);
+
+// Batched inverse NTT: out[b][k] = N^-1 sum_j in(b, j) w_N^{-jk}, natural order.
+//   kind 0: in(b, j) = src[b*src_stride + j]
+//   kind 1: in(b, j) = l[b][idx(j*step)] * q[b][idx(j*step)]  (codeword layout)
+struct intt_src {
+    int kind;
+    const fe *src; int64_t src_stride;
+    const fe *l, *q; int64_t cw_S, cw_stride; int64_t step; int log_R;
+};
+void intt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, fe inv_n, const fe *tw, int64_t SS, hipStream_t s);
+// Coset LDE: for each slice b and row s < R, out[b][s][t] = sum_{j<len_b} c_b[j] w_SS^{(tR+s) j}.
+// coef[b] has lens[b] (<= C) entries at coef + b*coef_stride.
+// hmode: coefficients of slice b start at coef + b*coef_stride + lens[b] (upper half of lq)
+void lde(const fe *coef, int64_t coef_stride, const int *lens, int nb, fe *out, int log_C, int log_R,
+         const fe *tw, int64_t SS, hipStream_t s, int hmode = 0);
+// virtual oracle of commit_public_array (poly_commit.h:246-263, 305-322)
+void virtual_oracle(const fe *l, const fe *q, const fe *h, const fe *c0, int nslice, int64_t SS, int log_R,
+                    int64_t real, int64_t m, const fe *tw, fe *vo, hipStream_t s);
+// all_sum[i] = (lq_coef[0] + lq_coef[len_i]) * len_i; c0[i] = lq_coef[0] + lq_coef[len_i]
+void slice_sums(const fe *lq_coef, int64_t stride, int nslice, int64_t real, int64_t m, fe *c0, fe *all_sum, hipStream_t s);
+
+// Merkle leaves of an initial commitment (fri.cpp:97-126): leaf i hashes the
+// 64 slice pairs (p_i, p_i + S/2) then the mask pair; written to tree[S/2 + i].
+void leaf_hash_init(const fe *cw, int64_t S, int log_R, uint8_t *tree, hipStream_t s);
+// heap Merkle tree over n_leaf leaves already at tree[n_leaf..2n_leaf) (merkle_tree.cpp:46-56)
+void merkle(uint8_t *tree, int64_t n_leaf, hipStream_t s);
+// one FRI step (fri.cpp:291-430): fold 65 slices of size S with r, write the
+// S/2 codeword and the leaves of its Merkle tree (tree[S/4 + i]).
+void fri_step(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, int64_t SS, fe *next,
+              uint8_t *tree, hipStream_t s);
+// generic batched NTT (inverse: w^-1), natural order in and out
+void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int inverse, fe scale, int do_scale, const fe *tw,
+         int64_t SS, hipStream_t s);
+void sha3_batch(const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s);
+// fills leaves [n_leaf, pad) of a tree with pad leaves with H(0^64)
+void merkle_pad(uint8_t *tree, int64_t pad, int64_t n_leaf, hipStream_t s);
+void fe_mul(const fe *a, const fe *b, fe *o, int64_t n, hipStream_t s);
+// out[k] = 16 bytes at address src[k]
+void gather16(const uint64_t *src, int64_t n, fe *out, hipStream_t s);
+
+}  // namespace launch
+}  // namespace vg

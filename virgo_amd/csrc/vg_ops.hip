@@ -1,0 +1,145 @@
+// Single-operation entry points of the C ABI on host arrays: the drop-in
+// replacements of fast_fourier_transform / inverse_fast_fourier_transform
+// (RS_polynomial.cpp:27-240), my_hhash (my_hhash.h:39-46),
+// merkle_tree_prover::create_tree (merkle_tree.cpp:4-57) and the field
+// product (prime_field.h:125-151).  Each call copies in, runs the HIP kernels
+// on the null stream and copies out.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vg_kernels.hpp"
+#include "../../include/virgo.h"
+
+using namespace vg;
+
+void vg_set_error(const std::string &msg);   // vg_prover.hip
+
+namespace {
+struct err_proxy {
+    void operator=(const std::string &m) { vg_set_error(m); }
+    void operator=(const char *m) { vg_set_error(m); }
+} op_err;
+int fail(const char *what, hipError_t e)
+{
+    vg_set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return -20;
+}
+#define OPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return fail(#x, e_); } while (0)
+
+struct dbuf {
+    void *p = nullptr;
+    ~dbuf() { if (p) hipFree(p); }
+};
+int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
+bool pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
+
+// tw[e] = w^e for e < n on the device
+int make_twiddles(const fe &w, int64_t n, dbuf &tw)
+{
+    std::vector<fe> t(1024 + (n >> 10) + 1);
+    fe x = fone();
+    for (int j = 0; j < 1024; ++j) { t[j] = x; x = mul(x, w); }
+    fe y = fone();
+    for (int64_t j = 0; j < (n >> 10) + 1; ++j) { t[1024 + j] = y; y = mul(y, x); }
+    dbuf T;
+    OPCHK(hipMalloc(&T.p, sizeof(fe) * t.size()));
+    OPCHK(hipMemcpy(T.p, t.data(), sizeof(fe) * t.size(), hipMemcpyHostToDevice));
+    OPCHK(hipMalloc(&tw.p, sizeof(fe) * n));
+    launch::twiddles((fe *)tw.p, n, (fe *)T.p, (fe *)T.p + 1024, 0);
+    OPCHK(hipDeviceSynchronize());
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int vg_fft(const vg_fe *coef, int coef_len, int order, const vg_fe *rou, vg_fe *result)
+{
+    if (!pow2(coef_len) || !pow2(order) || coef_len > order) { op_err = "vg_fft: sizes must be powers of two, coef_len <= order"; return -1; }
+    fe w; memcpy(&w, rou, sizeof w);
+    dbuf tw, buf;
+    int rc = make_twiddles(w, order, tw);
+    if (rc) return rc;
+    OPCHK(hipMalloc(&buf.p, sizeof(fe) * order));
+    OPCHK(hipMemset(buf.p, 0, sizeof(fe) * order));
+    OPCHK(hipMemcpy(buf.p, coef, sizeof(fe) * coef_len, hipMemcpyHostToDevice));
+    dbuf out;
+    OPCHK(hipMalloc(&out.p, sizeof(fe) * order));
+    launch::ntt({0, (const fe *)buf.p, order, nullptr, nullptr, 0, 0, 0, 0}, (fe *)out.p, order, 1, ilog2(order), 0, fone(), 0,
+                (const fe *)tw.p, order, 0);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(result, out.p, sizeof(fe) * order, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vg_ifft(const vg_fe *evals, int coef_len, int order, const vg_fe *rou, vg_fe *result)
+{
+    if (!pow2(coef_len) || !pow2(order)) { op_err = "vg_ifft: sizes must be powers of two"; return -1; }
+    if (coef_len > order) coef_len = order;            // RS_polynomial.cpp:171-177
+    int64_t step = order / coef_len;
+    std::vector<fe> sub(coef_len);
+    for (int i = 0; i < coef_len; ++i) memcpy(&sub[i], &evals[(int64_t)i * step], sizeof(fe));
+    fe w; memcpy(&w, rou, sizeof w);
+    fe nw = fone();
+    for (int64_t i = 0; i < step; ++i) nw = mul(nw, w);
+    dbuf tw, in, out;
+    int rc = make_twiddles(nw, coef_len, tw);
+    if (rc) return rc;
+    OPCHK(hipMalloc(&in.p, sizeof(fe) * coef_len));
+    OPCHK(hipMalloc(&out.p, sizeof(fe) * coef_len));
+    OPCHK(hipMemcpy(in.p, sub.data(), sizeof(fe) * coef_len, hipMemcpyHostToDevice));
+    fe inv_n = finv(mk((uint64_t)coef_len, 0));
+    launch::ntt({0, (const fe *)in.p, coef_len, nullptr, nullptr, 0, 0, 0, 0}, (fe *)out.p, coef_len, 1, ilog2(coef_len), 1,
+                inv_n, 1, (const fe *)tw.p, coef_len, 0);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(result, out.p, sizeof(fe) * coef_len, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vg_sha3_64(const uint8_t *in, uint8_t *out, size_t n)
+{
+    if (!n) return 0;
+    dbuf a, b;
+    OPCHK(hipMalloc(&a.p, 64 * n));
+    OPCHK(hipMalloc(&b.p, 32 * n));
+    OPCHK(hipMemcpy(a.p, in, 64 * n, hipMemcpyHostToDevice));
+    launch::sha3_batch((const uint8_t *)a.p, (uint8_t *)b.p, (int64_t)n, 0);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(out, b.p, 32 * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vg_merkle_tree(const uint8_t *leaves, int n_leaf, uint8_t *tree)
+{
+    if (n_leaf < 1) { op_err = "vg_merkle_tree: no leaves"; return -1; }
+    int64_t pad = 1;
+    while (pad < n_leaf) pad *= 2;
+    dbuf t;
+    OPCHK(hipMalloc(&t.p, 64 * pad));
+    OPCHK(hipMemset(t.p, 0, 64 * pad));
+    OPCHK(hipMemcpy((uint8_t *)t.p + 32 * pad, leaves, 32 * (size_t)n_leaf, hipMemcpyHostToDevice));
+    launch::merkle_pad((uint8_t *)t.p, pad, n_leaf, 0);
+    launch::merkle((uint8_t *)t.p, pad, 0);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(tree, t.p, 64 * pad, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vg_fe_mul(const vg_fe *a, const vg_fe *b, vg_fe *out, size_t n)
+{
+    if (!n) return 0;
+    dbuf x, y, z;
+    OPCHK(hipMalloc(&x.p, 16 * n));
+    OPCHK(hipMalloc(&y.p, 16 * n));
+    OPCHK(hipMalloc(&z.p, 16 * n));
+    OPCHK(hipMemcpy(x.p, a, 16 * n, hipMemcpyHostToDevice));
+    OPCHK(hipMemcpy(y.p, b, 16 * n, hipMemcpyHostToDevice));
+    launch::fe_mul((const fe *)x.p, (const fe *)y.p, (fe *)z.p, (int64_t)n, 0);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(out, z.p, 16 * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,1040 @@
+// Host driver of the MI355X Virgo prover and the C ABI of libvirgo_hip.so.
+//
+// One proof is one device-resident schedule on one stream.  The reference's
+// challenge stream (glibc rand(), default seed 1) is drawn in the order of
+// zk_verifier::verify (src/linear_gkr/zk_verifier.cpp:910-1253), which depends
+// only on the circuit shape, so every challenge is uploaded before the first
+// kernel and no round waits for the host.  The prover's mask bookkeeping
+// (zk_prover.cpp:162-333 and the mask terms of every sumcheck round) is host
+// scalar work on challenges only; it is computed beside the device work and
+// added to the device round sums when the transcript is assembled.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vg_kernels.hpp"
+#include "../../include/virgo.h"
+
+using namespace vg;
+
+static thread_local std::string g_err;
+void vg_set_error(const std::string &msg) { g_err = msg; }
+#define FAILV(code, ...) do { char b_[512]; snprintf(b_, sizeof b_, __VA_ARGS__); g_err = b_; return (code); } while (0)
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) FAILV(-20, "%s: %s", #x, hipGetErrorString(e_)); } while (0)
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// glibc rand() TYPE_3 (additive feedback, r[i] = r[i-3] + r[i-31]), seed 1;
+// prime_field::random (prime_field.cpp:35-41) draws real, then img.
+// ---------------------------------------------------------------------------
+struct glibc_rand {
+    uint32_t s[34];
+    uint64_t k;
+    explicit glibc_rand(uint32_t seed)
+    {
+        int32_t r[34];
+        r[0] = (int32_t)(seed ? seed : 1);
+        for (int i = 1; i < 31; ++i) {
+            int64_t t = (16807LL * r[i - 1]) % 2147483647LL;
+            if (t < 0) t += 2147483647LL;
+            r[i] = (int32_t)t;
+        }
+        for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
+        for (int i = 0; i < 34; ++i) s[i] = (uint32_t)r[i];
+        k = 34;
+        for (int i = 0; i < 310; ++i) next();
+    }
+    uint32_t next()
+    {
+        uint32_t v = s[(k - 31) % 34] + s[(k - 3) % 34];
+        s[k % 34] = v;
+        ++k;
+        return v >> 1;
+    }
+    fe field()
+    {
+        fe r;
+        r.real = next() % P;
+        r.img = next() % P;
+        return r;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// circuit, as zk_verifier::read_circuit builds it (zk_verifier.cpp:15-217)
+// ---------------------------------------------------------------------------
+struct host_circuit {
+    int depth = 0;                         // total_depth (layer 0 = inputs)
+    std::vector<int> bl;
+    std::vector<std::vector<uint8_t>> ty;
+    std::vector<std::vector<int32_t>> u, v;
+};
+
+int bitlen(int64_t x) { int c = 0; while (x) { ++c; x >>= 1; } return c; }
+
+int build_circuit(const vg_circuit_desc *d, host_circuit &C)
+{
+    int D = d->d;
+    if (D < 2) FAILV(-1, "circuit needs an input layer and at least one gate layer");
+    C.depth = D + 1;
+    C.bl.assign(D + 1, 0);
+    C.ty.assign(D + 1, {});
+    C.u.assign(D + 1, {});
+    C.v.assign(D + 1, {});
+    size_t off = 0;
+    for (int i = 1; i <= D; ++i) {
+        int64_t n = d->n[i - 1];
+        int pad_req = D > 3 ? 17 : 15;
+        int64_t n_pad = (i == 1 && n < (1LL << pad_req)) ? (1LL << pad_req) : n;
+        if (n < 1 || (n != 1 && (n & (n - 1)))) FAILV(-2, "layer %d: gate count %lld is not a power of two", i, (long long)n);
+        if (i == 1 && n == 1) FAILV(-2, "single-input circuits are not supported");
+        if (n_pad > (1LL << 30)) FAILV(-2, "layer %d too wide", i);
+        int64_t alloc = n == 1 ? 2 : n_pad;
+        int li = i == 1 ? 0 : i;
+        C.ty[li].assign(alloc, 2); C.u[li].assign(alloc, 0); C.v[li].assign(alloc, 0);
+        if (i == 1) { C.ty[1].assign(alloc, 4); C.u[1].assign(alloc, 0); C.v[1].assign(alloc, 0); }
+        int prev_bl = C.bl[i - 1];
+        for (int64_t j = 0; j < n; ++j, ++off) {
+            int t = d->ty[off];
+            int64_t uu = d->u[off], vv = d->v[off];
+            if (t != 3) {
+                if (t == 5) {
+                    if (!(uu >= 0 && uu < (1LL << prev_bl)) || !(vv > uu && vv <= (1LL << prev_bl)))
+                        FAILV(-3, "layer %d gate %lld: sum range out of bounds", i, (long long)j);
+                } else if (!(uu >= 0 && uu < (1LL << prev_bl)) || !(vv >= 0 && vv < (1LL << prev_bl)))
+                    FAILV(-3, "layer %d gate %lld: input index out of bounds", i, (long long)j);
+            }
+            if (t == 6 || t == 10) vv = 0;
+            if (t == 13 && uu != vv) FAILV(-3, "bit-test gate needs u == v");
+            if (t < 0 || t > 13 || t == 11) FAILV(-3, "unknown gate type %d", t);
+            if (i != 1) {
+                C.ty[i][j] = (uint8_t)t; C.u[i][j] = (int32_t)uu; C.v[i][j] = (int32_t)vv;
+            } else {
+                if (t != 2 && t != 3) FAILV(-3, "input layer gates must be type 2 or 3");
+                C.ty[1][j] = 4; C.u[1][j] = (int32_t)j; C.v[1][j] = 0;
+                C.ty[0][j] = (uint8_t)t; C.u[0][j] = (int32_t)uu; C.v[0][j] = (int32_t)vv;   // (int) truncation, zk_prover.cpp:60
+            }
+        }
+        if (i == 1) {
+            for (int64_t g = n; g < n_pad; ++g) { C.ty[1][g] = 4; C.u[1][g] = (int32_t)g; C.ty[0][g] = 3; C.u[0][g] = 0; }
+            n = n_pad;
+        }
+        if (n == 1) {
+            C.ty[i][1] = 2; C.u[i][1] = 0; C.v[i][1] = 0;
+            C.bl[i] = 1;
+        } else {
+            C.bl[i] = bitlen(n - 1);
+            if (i == 1) C.bl[0] = C.bl[1];
+        }
+        // a direct-relay gate indexes the layer's own beta table with u
+        // (zk_prover.cpp:496-497); the reference reads past it otherwise
+        for (size_t j = 0; j < C.ty[i].size(); ++j)
+            if (C.ty[i][j] == 4 && C.u[i][j] >= (1LL << C.bl[i]))
+                FAILV(-3, "layer %d: relay gate %zu reads u=%d beyond its own layer size", i, j, C.u[i][j]);
+    }
+    if (C.bl[0] < 6 + 1) FAILV(-2, "input layer too small for the commitment");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// challenge / mask schedule (draw order of zk_verifier::verify,
+// zk_prover::generate_maskR / generate_maskpoly_pre_rho, commit_phase and
+// verify_poly_commitment)
+// ---------------------------------------------------------------------------
+struct layer_sched {
+    int bl = 0;
+    fe rho, maskR[6], maskr[2], r_c, random_combine, linear_coeff, linear_combine, alpha, beta;
+    std::vector<fe> maskpoly, r_u, r_v;
+};
+struct schedule {
+    std::vector<fe> r0, r1, fri_r;
+    std::vector<layer_sched> L;
+    int64_t pow[33];
+};
+
+void build_schedule(const host_circuit &C, schedule &S)
+{
+    glibc_rand g(1);
+    int bl_out = C.bl[C.depth - 1];
+    S.r0.resize(bl_out); S.r1.resize(bl_out);
+    for (auto &x : S.r0) x = g.field();
+    for (auto &x : S.r1) x = g.field();
+    S.L.assign(C.depth, layer_sched());
+    for (int i = C.depth - 1; i >= 1; --i) {
+        layer_sched &l = S.L[i];
+        int bl = l.bl = C.bl[i - 1];
+        l.rho = g.field();
+        if (i > 1) for (auto &x : l.maskR) x = g.field();
+        l.maskr[0] = g.field(); l.maskr[1] = g.field();
+        l.maskpoly.resize((2 * bl + 1) * 2 + 1 + 6);
+        for (auto &x : l.maskpoly) x = g.field();
+        l.r_u.resize(bl); l.r_v.resize(bl);
+        for (auto &x : l.r_u) x = g.field();
+        for (auto &x : l.r_v) x = g.field();
+        l.r_c = g.field();
+        l.random_combine = g.field();
+        l.linear_coeff = g.field();
+        l.linear_combine = g.field();
+        l.alpha = g.field();
+        l.beta = g.field();
+    }
+    int n = C.bl[0];
+    int cw = 1 << (n - 1);
+    S.fri_r.clear();
+    while (cw > 32) { S.fri_r.push_back(g.field()); cw /= 2; }
+    for (int rep = 0; rep < 33; ++rep) {
+        int64_t p;
+        do { p = (int64_t)(g.next() % (uint32_t)(1u << (n - 1))); } while (p < (1LL << (n - 6)) || p % 2 == 1);
+        S.pow[rep] = p;
+    }
+}
+
+struct qpoly { fe a, b, c; };
+fe q_eval(const qpoly &p, const fe &x) { return add(mul(add(mul(p.a, x), p.b), x), p.c); }
+fe dbl_n(fe x, int n) { for (int i = 0; i < n; ++i) x = add(x, x); return x; }
+
+// Host part of every prover message of one layer.  Device round sums are
+// added to these (field addition, so the split is exact).
+struct layer_host {
+    int bl = 0;
+    fe maskpoly_sumc;
+    std::vector<fe> p1;           // 3 per round; the last round has 6 (a..f before reordering)
+    std::vector<fe> p2;
+    fe lb1[4], lb2[4];            // sumRc.a, .b, .c, Z for the last-bit terms
+    fe vu_const, vv_const;
+    fe fin[3];                    // final round a, b (without general value), c
+    fe query, rg1, rg2, Iuv, preZu, preZv;
+};
+
+// The scalar state of zk_prover (zk_prover.h:283-348) that does not touch
+// the bookkeeping tables.
+struct mask_machine {
+    fe alpha, beta, rho, maskpoly_sumc, maskpoly_sumr;
+    std::vector<fe> maskpoly;
+    fe maskR[6], maskR_sumcu, maskR_sumcv, preZu, preZv, Zu, Zv, preu1, prev1, Iuv, prepreu1, preprev1;
+    qpoly Rg1, Rg2, sumRc;
+    fe maskr[2];
+    int lu = 0, lv = 0;
+    fe inv2;
+    std::vector<fe> pri;
+    mask_machine()
+    {
+        alpha = beta = rho = maskpoly_sumc = maskpoly_sumr = fzero();
+        for (auto &x : maskR) x = fzero();
+        maskR_sumcu = maskR_sumcv = preZu = preZv = Zu = Zv = preu1 = prev1 = Iuv = prepreu1 = preprev1 = fzero();
+        Rg1 = Rg2 = sumRc = {fzero(), fzero(), fzero()};
+        maskr[0] = maskr[1] = fzero();
+        inv2 = finv(mk(2, 0));
+    }
+    void gen_maskR(const layer_sched &l, int layer_id)       // zk_prover.cpp:192-255
+    {
+        for (auto &x : maskR) pri.push_back(x);
+        prepreu1 = preu1; preprev1 = prev1;
+        fe *m = maskR;
+        Rg1.a = m[4]; Rg1.b = add(m[3], mul(m[5], preu1));
+        Rg1.c = add(add(m[0], mul(m[1], preu1)), mul(mul(m[2], preu1), preu1));
+        Rg2.a = m[4]; Rg2.b = add(m[3], mul(m[5], prev1));
+        Rg2.c = add(add(m[0], mul(m[1], prev1)), mul(mul(m[2], prev1), prev1));
+        fe sRu = add(add(add(Rg1.a, Rg1.b), Rg1.c), Rg1.c);
+        fe sRv = add(add(add(Rg2.a, Rg2.b), Rg2.c), Rg2.c);
+        maskR_sumcu = mul(mul(alpha, Zu), sRu);
+        maskR_sumcv = mul(mul(beta, Zv), sRv);
+        preZu = Zu; preZv = Zv;
+        Zu = fone(); Zv = fone(); Iuv = fone();
+        if (layer_id > 1) {
+            for (int i = 0; i < 6; ++i) m[i] = l.maskR[i];
+            sumRc.a = add(m[2], m[2]);
+            sumRc.b = add(add(m[1], m[1]), m[5]);
+            sumRc.c = add(add(add(m[0], m[0]), m[3]), m[4]);
+        }
+        maskr[0] = l.maskr[0]; maskr[1] = l.maskr[1];
+        if (layer_id == 1) {
+            m[0] = maskr[0]; m[1] = maskr[1];
+            m[2] = m[3] = m[4] = m[5] = fzero();
+            sumRc.a = fzero(); sumRc.b = m[1]; sumRc.c = m[0];
+        }
+    }
+    void maskpolys(const layer_sched &l)                      // zk_prover.cpp:162-188
+    {
+        int length = 2 * l.bl + 1, degree = 2;
+        maskpoly = l.maskpoly;
+        for (auto &x : maskpoly) pri.push_back(x);
+        rho = l.rho;
+        for (auto &x : maskpoly) x = mul(x, rho);
+        fe s = maskpoly[length * degree];
+        for (int i = length * degree; i >= 0; --i) s = add(s, maskpoly[i]);
+        for (int i = 1; i <= 6; ++i) s = add(s, maskpoly[length * degree + i]);
+        for (int i = 1; i < length; ++i) s = add(s, s);
+        maskpoly_sumc = s;
+        maskpoly_sumr = fzero();
+    }
+    // phase-1 round (zk_prover.cpp:596-787), mask part only
+    void p1_round(fe r, int cb, bool last, std::vector<fe> &out, fe *lb)
+    {
+        fe one = fone();
+        Iuv = mul(Iuv, sub(one, r));
+        if (cb > 0) {
+            maskR_sumcu = mul(maskR_sumcu, sub(one, r));
+            maskR_sumcv = mul(maskR_sumcv, sub(one, r));
+            Zu = mul(mul(Zu, sub(one, r)), r);
+        }
+        fe db = neg(add(maskR_sumcu, maskR_sumcv)), dc = add(maskR_sumcu, maskR_sumcv);
+        int k = lu + lv - cb, base = (lu + lv + 1) * 2;
+        fe t1 = dbl_n(maskpoly[cb << 1], k), t2 = dbl_n(maskpoly[(cb << 1) + 1], k);
+        fe t4 = fzero(), t5 = fzero(), t6 = fzero();
+        if (last) {
+            lb[0] = sumRc.a; lb[1] = sumRc.b; lb[2] = sumRc.c; lb[3] = Zu;
+            t4 = dbl_n(maskpoly[base + 1], k); t5 = dbl_n(maskpoly[base + 2], k); t6 = dbl_n(maskpoly[base + 3], k);
+            maskpoly_sumc = mul(sub(sub(sub(sub(sub(maskpoly_sumc, t1), t2), t4), t5), t6), inv2);
+        } else {
+            maskpoly_sumc = mul(sub(sub(maskpoly_sumc, t1), t2), inv2);
+        }
+        fe t3 = fzero();
+        if (cb > 0) {
+            maskpoly_sumr = add(add(maskpoly_sumr, mul(mul(maskpoly[(cb << 1) - 2], r), r)), mul(maskpoly[(cb << 1) - 1], r));
+            t3 = dbl_n(maskpoly_sumr, k);
+        }
+        out.push_back(t1);
+        out.push_back(add(db, t2));
+        out.push_back(add(add(dc, maskpoly_sumc), t3));
+        if (last) { out.push_back(t4); out.push_back(t5); out.push_back(t6); }
+    }
+    fe p2_init(fe r)                                          // zk_prover.cpp:789-983, mask part
+    {
+        fe one = fone();
+        preu1 = r;
+        maskR_sumcu = mul(maskR_sumcu, sub(one, r));
+        maskR_sumcv = mul(maskR_sumcv, sub(one, r));
+        Iuv = mul(Iuv, sub(one, r));
+        Zu = mul(mul(Zu, sub(one, r)), r);
+        fe vconst = mul(Zu, q_eval(sumRc, r));
+        int base = (lu + lv + 1) * 2;
+        fe r2 = mul(r, r), r3 = mul(r2, r), r4 = mul(r3, r), r5 = mul(r4, r);
+        maskpoly_sumr = add(add(maskpoly_sumr, mul(maskpoly[lu * 2 - 2], r2)), mul(maskpoly[lu * 2 - 1], r));
+        maskpoly_sumr = add(maskpoly_sumr, mul(maskpoly[base + 1], r5));
+        maskpoly_sumr = add(maskpoly_sumr, mul(maskpoly[base + 2], r4));
+        maskpoly_sumr = add(maskpoly_sumr, mul(maskpoly[base + 3], r3));
+        return vconst;
+    }
+    // phase-2 round (zk_prover.cpp:988-1171), mask part only
+    void p2_round(fe r, int cb, bool last, std::vector<fe> &out, fe *lb)
+    {
+        fe one = fone();
+        if (cb > 0 || last) Iuv = mul(Iuv, sub(one, r));
+        if (cb > 0) {
+            maskR_sumcu = mul(maskR_sumcu, sub(one, r));
+            maskR_sumcv = mul(maskR_sumcv, sub(one, r));
+            Zv = mul(mul(Zv, sub(one, r)), r);
+        }
+        fe db = neg(add(maskR_sumcu, maskR_sumcv)), dc = add(maskR_sumcu, maskR_sumcv);
+        int cur = cb + lu, k = lu + lv - cur, base = (lu + lv + 1) * 2;
+        fe t1 = dbl_n(maskpoly[cur << 1], k), t2 = dbl_n(maskpoly[(cur << 1) + 1], k);
+        fe t4 = fzero(), t5 = fzero(), t6 = fzero();
+        if (last) {
+            lb[0] = sumRc.a; lb[1] = sumRc.b; lb[2] = sumRc.c; lb[3] = Zv;
+            t4 = dbl_n(maskpoly[base + 4], k); t5 = dbl_n(maskpoly[base + 5], k); t6 = dbl_n(maskpoly[base + 6], k);
+            maskpoly_sumc = mul(sub(sub(sub(sub(sub(maskpoly_sumc, t1), t2), t4), t5), t6), inv2);
+        } else {
+            maskpoly_sumc = mul(sub(sub(maskpoly_sumc, t1), t2), inv2);
+        }
+        maskpoly_sumr = add(add(maskpoly_sumr, mul(mul(maskpoly[(cur << 1) - 2], r), r)), mul(maskpoly[(cur << 1) - 1], r));
+        fe t3 = dbl_n(maskpoly_sumr, k);
+        out.push_back(t1);
+        out.push_back(add(db, t2));
+        out.push_back(add(add(dc, maskpoly_sumc), t3));
+        if (last) { out.push_back(t4); out.push_back(t5); out.push_back(t6); }
+    }
+    fe finalize(fe r)                                          // zk_prover.cpp:1206-1215
+    {
+        fe one = fone();
+        prev1 = r;
+        Iuv = mul(Iuv, sub(one, r));
+        Zv = mul(mul(Zv, sub(one, r)), r);
+        return mul(Zv, q_eval(sumRc, r));
+    }
+    void finalround(fe r, int current, fe *out)                // zk_prover.cpp:1173-1203
+    {
+        fe zu = mul(Iuv, preZu), zv = mul(Iuv, preZv);
+        fe a = add(mul(mul(zu, Rg1.a), alpha), mul(mul(zv, Rg2.a), beta));
+        fe b = add(mul(mul(zu, Rg1.b), alpha), mul(mul(zv, Rg2.b), beta));
+        fe c = add(mul(mul(zu, Rg1.c), alpha), mul(mul(zv, Rg2.c), beta));
+        int k = lu + lv - current, base = (lu + lv + 1) * 2;
+        fe t1 = dbl_n(maskpoly[current << 1], k), t2 = dbl_n(maskpoly[(current << 1) + 1], k);
+        maskpoly_sumc = mul(sub(sub(maskpoly_sumc, t1), t2), inv2);
+        fe r2 = mul(r, r), r3 = mul(r2, r), r4 = mul(r3, r), r5 = mul(r4, r);
+        maskpoly_sumr = add(add(maskpoly_sumr, mul(maskpoly[(current << 1) - 2], r2)), mul(maskpoly[(current << 1) - 1], r));
+        maskpoly_sumr = add(maskpoly_sumr, mul(maskpoly[base + 4], r5));
+        maskpoly_sumr = add(maskpoly_sumr, mul(maskpoly[base + 5], r4));
+        maskpoly_sumr = add(maskpoly_sumr, mul(maskpoly[base + 6], r3));
+        fe t3 = dbl_n(maskpoly_sumr, k);
+        out[0] = add(a, t1);
+        out[1] = add(b, t2);
+        out[2] = add(add(c, maskpoly_sumc), t3);
+    }
+    fe query(const fe *u, const fe *v, fe rc) const            // zk_prover.cpp:284-306
+    {
+        fe res = fzero();
+        int base = 2 * (lu + lv + 1);
+        const fe *m = maskpoly.data();
+        for (int i = 0; i < lu; ++i) {
+            res = add(add(res, mul(mul(m[2 * i], u[i]), u[i])), mul(m[2 * i + 1], u[i]));
+            if (i == lu - 1) {
+                fe x = u[i], x2 = mul(x, x), x3 = mul(x2, x), x4 = mul(x3, x), x5 = mul(x4, x);
+                res = add(add(add(res, mul(m[base + 1], x5)), mul(m[base + 2], x4)), mul(m[base + 3], x3));
+            }
+        }
+        for (int i = 0; i < lv; ++i) {
+            res = add(add(res, mul(mul(m[2 * (i + lu)], v[i]), v[i])), mul(m[2 * (i + lu) + 1], v[i]));
+            if (i == lv - 1) {
+                fe x = v[i], x2 = mul(x, x), x3 = mul(x2, x), x4 = mul(x3, x), x5 = mul(x4, x);
+                res = add(add(add(res, mul(m[base + 4], x5)), mul(m[base + 5], x4)), mul(m[base + 6], x3));
+            }
+        }
+        res = add(add(res, mul(mul(m[2 * (lu + lv)], rc), rc)), mul(m[2 * (lu + lv) + 1], rc));
+        return add(res, m[2 * (lu + lv + 1)]);
+    }
+};
+
+// verifier's public mask coefficients (zk_verifier.cpp:983-1025)
+void push_pub_masks(std::vector<fe> &pub, const layer_sched &l, int i, fe preu1, fe prev1)
+{
+    int ll = l.bl;
+    fe one = fone();
+    fe rc = i == 1 ? fzero() : l.r_c;
+    fe rcb = l.random_combine, lc = l.linear_coeff, lcm = l.linear_combine, rho = l.rho;
+    std::vector<fe> rv(ll);
+    for (int k = 0; k < ll; ++k) rv[k] = i == 1 ? fzero() : l.r_v[k];
+    pub.push_back(add(mul(one, rcb), mul(mul(lc, one), rcb)));
+    pub.push_back(add(mul(preu1, rcb), mul(mul(lc, prev1), rcb)));
+    pub.push_back(add(mul(mul(preu1, preu1), rcb), mul(mul(mul(lc, prev1), prev1), rcb)));
+    pub.push_back(add(mul(rc, rcb), mul(mul(lc, rc), rcb)));
+    pub.push_back(add(mul(mul(rc, rc), rcb), mul(mul(mul(lc, rc), rc), rcb)));
+    pub.push_back(add(mul(mul(preu1, rc), rcb), mul(mul(mul(lc, prev1), rc), rcb)));
+    for (int k = 0; k < ll; ++k) {
+        pub.push_back(mul(mul(mul(lcm, l.r_u[k]), l.r_u[k]), rho));
+        pub.push_back(mul(mul(lcm, l.r_u[k]), rho));
+    }
+    for (int k = 0; k < ll; ++k) {
+        pub.push_back(mul(mul(mul(lcm, rv[k]), rv[k]), rho));
+        pub.push_back(mul(mul(lcm, rv[k]), rho));
+    }
+    pub.push_back(mul(mul(mul(lcm, rc), rc), rho));
+    pub.push_back(mul(mul(lcm, rc), rho));
+    pub.push_back(mul(mul(lcm, one), rho));
+    for (int w = 0; w < 2; ++w) {
+        fe x = w == 0 ? l.r_u[ll - 1] : rv[ll - 1];
+        fe x2 = mul(x, x), x3 = mul(x2, x), x4 = mul(x3, x), x5 = mul(x4, x);
+        pub.push_back(mul(mul(lcm, x5), rho));
+        pub.push_back(mul(mul(lcm, x4), rho));
+        pub.push_back(mul(mul(lcm, x3), rho));
+    }
+}
+
+// last-bit terms of sumcheck_phase{1,2}_updatelastbit (zk_prover.cpp:736-747, 1122-1133)
+void lastbit_terms(const fe *lb, fe d, fe e, fe &A, fe &B, fe &Dd, fe &E, fe &Fv)
+{
+    fe a = lb[0], b = lb[1], c = lb[2], Z = lb[3];
+    Dd = mul(mul(neg(a), d), Z);
+    E = mul(sub(mul(a, sub(d, e)), mul(b, d)), Z);
+    Fv = mul(sub(add(mul(a, e), mul(b, sub(d, e))), mul(c, d)), Z);
+    A = add(A, mul(add(mul(c, sub(d, e)), mul(b, e)), Z));
+    B = add(B, mul(mul(c, e), Z));
+}
+
+struct tbuf {
+    std::vector<uint8_t> b;
+    void put(const void *p, size_t n) { const uint8_t *c = (const uint8_t *)p; b.insert(b.end(), c, c + n); }
+    void f(const fe &x) { put(&x, 16); }
+    void u32(uint32_t x) { put(&x, 4); }
+};
+
+int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// proof context: device arena + per-proof host state
+// ---------------------------------------------------------------------------
+struct vg_ctx {
+    host_circuit C;
+    hipStream_t st = nullptr;
+    bool own_stream = false;
+    uint8_t *arena = nullptr;
+    int64_t arena_bytes = 0;
+    // device views
+    std::vector<uint8_t *> d_ty;
+    std::vector<int32_t *> d_u, d_v;
+    std::vector<fe *> cv;
+    fe *chal = nullptr;         // challenges
+    fe *amb = nullptr, *aVb = nullptr;
+    lpoly *tabs[6] = {};
+    fe *part = nullptr;
+    fe *bt[8] = {};             // beta half tables: g0f g0s g1f g1s uf us vf vs
+    fe *msg = nullptr;          // device results of the GKR part
+    fe *mle_t[2] = {};
+    // commitment
+    int n = 0, log_SS = 0, log_C = 0, log_R = 0;
+    int64_t SS = 0, real = 0, m = 0, C_ = 0, LQ = 0;
+    fe *tw = nullptr, *T12 = nullptr, *coefP = nullptr, *coefQ = nullptr, *l = nullptr, *q = nullptr, *h = nullptr,
+       *vo = nullptr, *qv = nullptr, *eqs = nullptr, *lqc = nullptr, *c0 = nullptr, *all_sum = nullptr, *pri = nullptr,
+       *pubm = nullptr, *cw = nullptr, *gout = nullptr;
+    int *lens = nullptr;
+    uint8_t *treeL = nullptr, *treeH = nullptr, *ctree = nullptr;
+    uint64_t *gsrc = nullptr;
+    int64_t n_gather = 0, gather_cap = 0;
+    std::vector<int64_t> cw_off, ctree_off;
+    // host staging (pinned)
+    fe *h_chal = nullptr;
+    int64_t n_chal = 0;
+    uint64_t *h_gsrc = nullptr;
+    fe *h_gout = nullptr;
+    fe *h_msg = nullptr;
+    int64_t msg_len = 0;
+    // per-proof host state
+    schedule S;
+    std::vector<layer_host> LH;
+    std::vector<int64_t> chal_ru, chal_rv, chal_ab;   // offsets into chal per layer
+    int64_t chal_r0 = 0, chal_r1 = 0, chal_fri = 0, chal_one = 0;
+    std::vector<int64_t> msg_layer;                   // offset of each layer's block in msg
+    int64_t msg_vres = 0, msg_allsum = 0;
+    fe post[5];
+    hipEvent_t ev[8] = {};
+    std::chrono::steady_clock::time_point t_start;
+    bool pending = false;
+};
+
+namespace {
+
+template <class T>
+T *carve(uint8_t *base, int64_t &off, int64_t count)
+{
+    off = (off + 255) & ~(int64_t)255;
+    T *p = base ? (T *)(base + off) : nullptr;
+    off += (int64_t)sizeof(T) * count;
+    return p;
+}
+
+// lays out (base == nullptr: measures) every device buffer of the context
+int64_t layout(vg_ctx *c, uint8_t *base)
+{
+    int64_t off = 0;
+    const host_circuit &C = c->C;
+    int D = C.depth;
+    c->d_ty.assign(D, nullptr); c->d_u.assign(D, nullptr); c->d_v.assign(D, nullptr); c->cv.assign(D, nullptr);
+    int maxbl = 0;
+    for (int i = 0; i < D; ++i) {
+        int64_t n = (int64_t)1 << C.bl[i];
+        if (C.bl[i] > maxbl) maxbl = C.bl[i];
+        c->d_u[i] = carve<int32_t>(base, off, n);
+        if (i > 0) {
+            c->d_ty[i] = carve<uint8_t>(base, off, n);
+            c->d_v[i] = carve<int32_t>(base, off, n);
+        }
+        c->cv[i] = carve<fe>(base, off, n);
+    }
+    c->chal = carve<fe>(base, off, c->n_chal);
+    int64_t T = (int64_t)1 << maxbl;
+    c->amb = carve<fe>(base, off, T);
+    c->aVb = carve<fe>(base, off, T);
+    for (int k = 0; k < 6; ++k) c->tabs[k] = carve<lpoly>(base, off, T / 2 > 0 ? T / 2 : 1);
+    c->part = carve<fe>(base, off, 6 * 1024);
+    int half = (maxbl >> 1) + 1;
+    for (int k = 0; k < 8; ++k) c->bt[k] = carve<fe>(base, off, (int64_t)1 << half);
+    c->msg = carve<fe>(base, off, c->msg_len);
+    int64_t no = (int64_t)1 << C.bl[D - 1];
+    c->mle_t[0] = carve<fe>(base, off, no);
+    c->mle_t[1] = carve<fe>(base, off, no);
+    // commitment
+    int64_t SS = c->SS;
+    c->T12 = carve<fe>(base, off, 1024 + (SS >> 10) + 1);
+    c->tw = carve<fe>(base, off, SS);
+    c->coefP = carve<fe>(base, off, 65 * c->C_);
+    c->coefQ = carve<fe>(base, off, 65 * c->C_);
+    c->l = carve<fe>(base, off, 65 * SS);
+    c->q = carve<fe>(base, off, 65 * SS);
+    c->h = carve<fe>(base, off, 65 * SS);
+    c->vo = carve<fe>(base, off, 65 * SS);
+    c->qv = carve<fe>(base, off, (int64_t)1 << c->n);
+    c->eqs = carve<fe>(base, off, 2 * ((int64_t)1 << (c->n / 2 + 1)));
+    c->lqc = carve<fe>(base, off, 65 * c->LQ);
+    c->c0 = carve<fe>(base, off, 65);
+    c->all_sum = carve<fe>(base, off, 65);
+    c->pri = carve<fe>(base, off, c->m);
+    c->pubm = carve<fe>(base, off, c->m);
+    c->lens = carve<int>(base, off, 65);
+    c->treeL = carve<uint8_t>(base, off, SS * 32);
+    c->treeH = carve<uint8_t>(base, off, SS * 32);
+    int nf = c->n - 6;
+    c->cw_off.assign(nf, 0); c->ctree_off.assign(nf, 0);
+    int64_t cwt = 0, ctt = 0;
+    for (int k = 0, S = (int)SS; k < nf; ++k, S /= 2) {
+        c->cw_off[k] = cwt; cwt += 65LL * (S / 2);
+        c->ctree_off[k] = ctt; ctt += (int64_t)(S / 2) * 32;
+    }
+    c->cw = carve<fe>(base, off, cwt);
+    c->ctree = carve<uint8_t>(base, off, ctt);
+    c->gsrc = carve<uint64_t>(base, off, c->gather_cap);
+    c->gout = carve<fe>(base, off, c->gather_cap);
+    return off + 256;
+}
+
+// number of gathered 16-byte items of the commitment (roots, final codeword, queries)
+int64_t gather_count(int n)
+{
+    int64_t k = 2 * 2 + 2 * (n - 6) + 2048 + 32;
+    int64_t per = 2 * (130 + 2 * (n - 1));
+    for (int lvl = 0; lvl < n - 6; ++lvl) per += 130 + 2 * (n - 2 - lvl);
+    return k + 33 * per;
+}
+
+// host work of a proof that depends only on the shape: schedule, masks,
+// challenge buffer, gather addresses
+int prepare(vg_ctx *c)
+{
+    host_circuit &C = c->C;
+    build_schedule(C, c->S);
+    schedule &S = c->S;
+    int D = C.depth;
+    // challenge buffer
+    std::vector<fe> ch;
+    c->chal_one = (int64_t)ch.size(); ch.push_back(fone()); ch.push_back(fzero());
+    c->chal_r0 = (int64_t)ch.size(); ch.insert(ch.end(), S.r0.begin(), S.r0.end());
+    c->chal_r1 = (int64_t)ch.size(); ch.insert(ch.end(), S.r1.begin(), S.r1.end());
+    c->chal_ru.assign(D, 0); c->chal_rv.assign(D, 0); c->chal_ab.assign(D, 0);
+    for (int i = D - 1; i >= 1; --i) {
+        layer_sched &l = S.L[i];
+        c->chal_ru[i] = (int64_t)ch.size(); ch.insert(ch.end(), l.r_u.begin(), l.r_u.end());
+        c->chal_rv[i] = (int64_t)ch.size();
+        for (int k = 0; k < l.bl; ++k) ch.push_back(i == 1 ? fzero() : l.r_v[k]);
+        c->chal_ab[i] = (int64_t)ch.size(); ch.push_back(l.alpha); ch.push_back(l.beta);
+    }
+    c->chal_fri = (int64_t)ch.size(); ch.insert(ch.end(), S.fri_r.begin(), S.fri_r.end());
+    if ((int64_t)ch.size() + 2 * c->m != c->n_chal) FAILV(-30, "challenge buffer size mismatch");
+    memcpy(c->h_chal, ch.data(), sizeof(fe) * ch.size());
+
+    // host parts of the prover messages (the mask bookkeeping)
+    mask_machine M;
+    std::vector<fe> pub;
+    c->LH.assign(D, layer_host());
+    fe alpha = fone(), beta = fzero();
+    for (int i = D - 1; i >= 1; --i) {
+        layer_sched &l = S.L[i];
+        layer_host &H = c->LH[i];
+        int bl = H.bl = l.bl;
+        M.alpha = alpha; M.beta = beta; M.lu = bl; M.lv = bl;
+        M.gen_maskR(l, i);
+        M.maskpolys(l);
+        H.maskpoly_sumc = M.maskpoly_sumc;
+        push_pub_masks(pub, l, i, M.prepreu1, M.preprev1);
+        std::vector<fe> rv(bl);
+        for (int k = 0; k < bl; ++k) rv[k] = i == 1 ? fzero() : l.r_v[k];
+        for (int j = 0; j < bl; ++j) M.p1_round(j ? l.r_u[j - 1] : fzero(), j, j == bl - 1, H.p1, H.lb1);
+        H.vu_const = M.p2_init(l.r_u[bl - 1]);
+        for (int j = 0; j < bl; ++j) M.p2_round(j ? rv[j - 1] : fzero(), j, j == bl - 1, H.p2, H.lb2);
+        H.vv_const = M.finalize(rv[bl - 1]);
+        M.finalround(rv[bl - 1], bl << 1, H.fin);
+        fe rc = i == 1 ? fzero() : l.r_c;
+        H.query = M.query(l.r_u.data(), rv.data(), rc);
+        H.rg1 = q_eval(M.Rg1, rc);
+        H.rg2 = q_eval(M.Rg2, rc);
+        H.Iuv = M.Iuv; H.preZu = M.preZu; H.preZv = M.preZv;
+        alpha = l.alpha; beta = l.beta;
+    }
+    pub.push_back(M.Zu);
+    pub.push_back(mul(M.Zu, M.preu1));
+    M.pri.push_back(M.maskr[0]);
+    M.pri.push_back(M.maskr[1]);
+    c->post[0] = M.Zu; c->post[1] = M.preu1; c->post[2] = M.sumRc.a; c->post[3] = M.sumRc.b; c->post[4] = M.sumRc.c;
+    // masks padded to m (poly_commit.h:56-69, 136-139)
+    std::vector<fe> prim(c->m, fzero()), pubm(c->m, fzero());
+    for (size_t k = 0; k < M.pri.size() && (int64_t)k < c->m; ++k) prim[k] = M.pri[k];
+    for (size_t k = 0; k < pub.size() && (int64_t)k < c->m; ++k) pubm[k] = pub[k];
+    // pri and pub masks ride along with the challenges
+    memcpy(c->h_chal + ch.size(), prim.data(), sizeof(fe) * c->m);
+    memcpy(c->h_chal + ch.size() + c->m, pubm.data(), sizeof(fe) * c->m);
+    return 0;
+}
+
+// addresses of every gathered item, in transcript order
+int build_gather(vg_ctx *c)
+{
+    int n = c->n;
+    int64_t SS = c->SS, half = SS / 2;
+    int log_R = c->log_R;
+    uint64_t *g = c->h_gsrc;
+    int64_t k = 0;
+    auto dig = [&](const uint8_t *tree, int64_t node) {
+        g[k++] = (uint64_t)(uintptr_t)(tree + node * 32);
+        g[k++] = (uint64_t)(uintptr_t)(tree + node * 32 + 16);
+    };
+    auto val = [&](const fe *p) { g[k++] = (uint64_t)(uintptr_t)p; };
+    dig(c->treeL, 1);
+    dig(c->treeH, 1);
+    int nf = n - 6;
+    for (int lv = 0; lv < nf; ++lv) dig(c->ctree + c->ctree_off[lv], 1);
+    // final codeword (cpd.rs_codeword[last]: [(i<<7)|(j<<1)|h] = slice j at i + 16h)
+    const fe *fin = c->cw + c->cw_off[nf - 1];
+    for (int i = 0; i < 16; ++i)
+        for (int j = 0; j < 64; ++j)
+            for (int h = 0; h < 2; ++h) val(fin + j * 32 + cw_idx(i + 16 * h, 32, log_R));
+    for (int i = 0; i < 16; ++i)
+        for (int h = 0; h < 2; ++h) val(fin + 64 * 32 + cw_idx(i + 16 * h, 32, log_R));
+    for (int rep = 0; rep < 33; ++rep) {
+        int64_t pow = c->S.pow[rep];
+        int64_t s0 = pow / 2;
+        for (int o = 0; o < 2; ++o) {
+            const fe *src = o == 0 ? c->l : c->h;
+            const uint8_t *tree = o == 0 ? c->treeL : c->treeH;
+            for (int j = 0; j <= 64; ++j) {
+                val(src + j * SS + cw_idx(s0, SS, log_R));
+                val(src + j * SS + cw_idx(s0 + half, SS, log_R));
+            }
+            int64_t pos = s0 + half;
+            for (int d = 0; d < n - 2; ++d) { dig(tree, pos ^ 1); pos /= 2; }
+            dig(tree, s0 + half);
+        }
+        for (int lv = 0; lv < nf; ++lv) {
+            int64_t S = SS >> (lv + 1);              // codeword size after step lv
+            int64_t pw = pow % ((int64_t)1 << (n - 1 - lv));
+            int64_t q = (pw / 2) % (S / 2);
+            const fe *cwp = c->cw + c->cw_off[lv];
+            for (int j = 0; j <= 64; ++j) {
+                val(cwp + j * S + cw_idx(q, S, log_R));
+                val(cwp + j * S + cw_idx(q + S / 2, S, log_R));
+            }
+            const uint8_t *tree = c->ctree + c->ctree_off[lv];
+            int64_t p0 = q + S / 2;
+            int64_t leaf = p0;
+            while (p0 != 1) { dig(tree, p0 ^ 1); p0 /= 2; }
+            dig(tree, leaf);
+        }
+    }
+    c->n_gather = k;
+    if (k > c->gather_cap) FAILV(-30, "gather list overflow (%lld > %lld)", (long long)k, (long long)c->gather_cap);
+    return 0;
+}
+
+int enqueue(vg_ctx *c)
+{
+    hipStream_t st = c->st;
+    host_circuit &C = c->C;
+    int D = C.depth;
+    HIPCHK(hipEventRecord(c->ev[0], st));
+    HIPCHK(hipMemcpyAsync(c->chal, c->h_chal, sizeof(fe) * c->n_chal, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->gsrc, c->h_gsrc, sizeof(uint64_t) * c->n_gather, hipMemcpyHostToDevice, st));
+    // ---- evaluate (zk_prover.cpp:54-157) ----
+    launch::eval_input(c->d_u[0], c->cv[0], 1 << C.bl[0], st);
+    for (int i = 1; i < D; ++i) launch::eval_layer({c->d_ty[i], c->d_u[i], c->d_v[i], C.bl[i]}, c->cv[i - 1], c->cv[i], st);
+    HIPCHK(hipEventRecord(c->ev[1], st));
+    // ---- V_res (zk_prover.cpp:30-52) ----
+    launch::mle_eval(c->cv[D - 1], C.bl[D - 1], c->chal + c->chal_r0, c->mle_t[0], c->mle_t[1], c->msg + c->msg_vres, st);
+    // ---- GKR layers ----
+    launch::fold_ws ws;
+    ws.V[0] = c->tabs[0]; ws.aV[0] = c->tabs[1]; ws.am[0] = c->tabs[2];
+    ws.V[1] = c->tabs[3]; ws.aV[1] = c->tabs[4]; ws.am[1] = c->tabs[5];
+    ws.part = c->part; ws.counter = nullptr;
+    const fe *r0 = c->chal + c->chal_r0, *r1 = c->chal + c->chal_r1;
+    const fe *alpha = c->chal + c->chal_one, *beta = c->chal + c->chal_one + 1;
+    for (int i = D - 1; i >= 1; --i) {
+        int bl = C.bl[i - 1], bg = C.bl[i];
+        const fe *ru = c->chal + c->chal_ru[i], *rv = c->chal + c->chal_rv[i];
+        fe *M = c->msg + c->msg_layer[i];
+        fe *sums1 = M, *end1 = M + 3 * bl, *sums2 = M + 3 * bl + 5, *end2 = M + 6 * bl + 5, *gv = M + 6 * bl + 10;
+        gates_dev gd = {c->d_ty[i], c->d_u[i], c->d_v[i], bg};
+        launch::half_tables(c->bt[0], c->bt[1], alpha, r0, bg, st);
+        launch::half_tables(c->bt[2], c->bt[3], beta, r1, bg, st);
+        launch::beta_g BG = {c->bt[0], c->bt[1], c->bt[2], c->bt[3], bg >> 1};
+        int64_t T = (int64_t)1 << bl;
+        launch::zero(c->amb, sizeof(fe) * T, st);
+        launch::zero(c->aVb, sizeof(fe) * T, st);
+        launch::p1_scatter(gd, c->cv[i - 1], BG, c->amb, c->aVb, st);
+        launch::sumcheck_phase(c->cv[i - 1], c->aVb, c->amb, bl, ru, c->LH[i].vu_const, ws, sums1, end1, st);
+        launch::half_tables(c->bt[4], c->bt[5], nullptr, ru, bl, st);
+        launch::beta_u BU = {c->bt[4], c->bt[5], bl >> 1};
+        launch::zero(c->amb, sizeof(fe) * T, st);
+        launch::zero(c->aVb, sizeof(fe) * T, st);
+        launch::p2_scatter(gd, BG, BU, end1 + 4, c->amb, c->aVb, st);
+        launch::sumcheck_phase(c->cv[i - 1], c->aVb, c->amb, bl, rv, c->LH[i].vv_const, ws, sums2, end2, st);
+        if (i > 1) {
+            launch::half_tables(c->bt[6], c->bt[7], nullptr, rv, bl, st);
+            launch::beta_u BV = {c->bt[6], c->bt[7], bl >> 1};
+            launch::general_value(gd, BG, BU, BV, end1 + 4, end2 + 4, c->part, nullptr, gv, st);
+        }
+        r0 = ru; r1 = rv;
+        alpha = c->chal + c->chal_ab[i];
+        beta = alpha + 1;
+    }
+    HIPCHK(hipEventRecord(c->ev[2], st));
+    // ---- commit_private_array (poly_commit.h:42-124) ----
+    int n = c->n;
+    int64_t SS = c->SS, real = c->real, m = c->m, Cc = c->C_;
+    int lr = ilog2(real), lm = ilog2(m);
+    fe *pri_d = c->chal + (c->n_chal - 2 * m), *pub_d = c->chal + (c->n_chal - m);
+    fe inv_real = finv(mk((uint64_t)real, 0)), inv_m = finv(mk((uint64_t)m, 0));
+    fe inv_2real = finv(mk((uint64_t)(2 * real), 0)), inv_2m = finv(mk((uint64_t)(2 * m), 0));
+    launch::intt({0, c->cv[0], real, nullptr, nullptr, 0, 0, 0, 0}, c->coefP, Cc, 64, lr, inv_real, c->tw, SS, st);
+    launch::intt({0, pri_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefP + 64 * Cc, Cc, 1, lm, inv_m, c->tw, SS, st);
+    launch::lde(c->coefP, Cc, c->lens, 65, c->l, c->log_C, c->log_R, c->tw, SS, st);
+    launch::leaf_hash_init(c->l, SS, c->log_R, c->treeL, st);
+    launch::merkle(c->treeL, SS / 2, st);
+    HIPCHK(hipEventRecord(c->ev[3], st));
+    // ---- commit_public_array (poly_commit.h:125-351) ----
+    const fe *rlast = c->chal + c->chal_ru[1];
+    launch::eq_table(c->qv, rlast, n, c->eqs, st);
+    launch::intt({0, c->qv, real, nullptr, nullptr, 0, 0, 0, 0}, c->coefQ, Cc, 64, lr, inv_real, c->tw, SS, st);
+    launch::intt({0, pub_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefQ + 64 * Cc, Cc, 1, lm, inv_m, c->tw, SS, st);
+    launch::lde(c->coefQ, Cc, c->lens, 65, c->q, c->log_C, c->log_R, c->tw, SS, st);
+    launch::intt({1, nullptr, 0, c->l, c->q, SS, SS, SS / (2 * real), c->log_R}, c->lqc, c->LQ, 64, lr + 1, inv_2real, c->tw, SS, st);
+    launch::intt({1, nullptr, 0, c->l + 64 * SS, c->q + 64 * SS, SS, SS, SS / (2 * m), c->log_R}, c->lqc + 64 * c->LQ, c->LQ, 1,
+                 lm + 1, inv_2m, c->tw, SS, st);
+    launch::slice_sums(c->lqc, c->LQ, 65, real, m, c->c0, c->msg + c->msg_allsum, st);
+    launch::lde(c->lqc, c->LQ, c->lens, 65, c->h, c->log_C, c->log_R, c->tw, SS, st, 1);
+    launch::leaf_hash_init(c->h, SS, c->log_R, c->treeH, st);
+    launch::merkle(c->treeH, SS / 2, st);
+    HIPCHK(hipEventRecord(c->ev[4], st));
+    // ---- commit_phase: FRI (vpd_verifier.cpp:46-78, fri.cpp:291-430) ----
+    launch::virtual_oracle(c->l, c->q, c->h, c->c0, 65, SS, c->log_R, real, m, c->tw, c->vo, st);
+    const fe *prev = c->vo;
+    int64_t S = SS;
+    for (int k = 0; k < n - 6; ++k) {
+        fe *next = c->cw + c->cw_off[k];
+        uint8_t *tree = c->ctree + c->ctree_off[k];
+        launch::fri_step(prev, S, c->log_R, c->chal + c->chal_fri + k, c->tw, SS, next, tree, st);
+        launch::merkle(tree, S / 4, st);
+        prev = next;
+        S /= 2;
+    }
+    HIPCHK(hipEventRecord(c->ev[5], st));
+    // ---- query answers (fri.cpp:148-289) ----
+    launch::gather16(c->gsrc, c->n_gather, c->gout, st);
+    HIPCHK(hipMemcpyAsync(c->h_gout, c->gout, sizeof(fe) * c->n_gather, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(c->h_msg, c->msg, sizeof(fe) * c->msg_len, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(c->ev[6], st));
+    return 0;
+}
+
+void assemble(vg_ctx *c, tbuf &T)
+{
+    host_circuit &C = c->C;
+    int D = C.depth;
+    T.put("VGT1", 4);
+    T.u32((uint32_t)D);
+    T.u32((uint32_t)C.bl[0]);
+    for (int i = 0; i < D; ++i) T.u32((uint32_t)C.bl[i]);
+    T.f(c->h_msg[c->msg_vres]);
+    for (int i = D - 1; i >= 1; --i) {
+        layer_host &H = c->LH[i];
+        int bl = H.bl;
+        const fe *M = c->h_msg + c->msg_layer[i];
+        const fe *sums1 = M, *end1 = M + 3 * bl, *sums2 = M + 3 * bl + 5, *end2 = M + 6 * bl + 5, *gv = M + 6 * bl + 10;
+        T.f(H.maskpoly_sumc);
+        for (int ph = 0; ph < 2; ++ph) {
+            const fe *sums = ph == 0 ? sums1 : sums2;
+            const fe *end = ph == 0 ? end1 : end2;
+            const std::vector<fe> &hp = ph == 0 ? H.p1 : H.p2;
+            const fe *lb = ph == 0 ? H.lb1 : H.lb2;
+            for (int j = 0; j < bl - 1; ++j) {
+                T.f(add(sums[3 * j], hp[3 * j]));
+                T.f(add(sums[3 * j + 1], hp[3 * j + 1]));
+                T.f(add(sums[3 * j + 2], hp[3 * j + 2]));
+            }
+            int j = bl - 1;
+            fe A = sums[3 * j], B = sums[3 * j + 1], Cc = sums[3 * j + 2], Dd, E, Fv;
+            lastbit_terms(lb, end[2], end[3], A, B, Dd, E, Fv);
+            const fe *h6 = &hp[3 * j];
+            A = add(A, h6[0]); B = add(B, h6[1]); Cc = add(Cc, h6[2]);
+            Dd = add(Dd, h6[3]); E = add(E, h6[4]); Fv = add(Fv, h6[5]);
+            T.f(Dd); T.f(E); T.f(Fv); T.f(A); T.f(B); T.f(Cc);      // quintuple (d,e,f,a,b,c), zk_prover.cpp:781
+            T.f(end[4]);                                            // v_u, then v_v
+        }
+        fe g = i > 1 ? *gv : fzero();
+        T.f(H.fin[0]); T.f(add(H.fin[1], g)); T.f(H.fin[2]);
+        T.f(H.query); T.f(H.rg1); T.f(H.rg2);
+        T.f(H.Iuv); T.f(H.preZu); T.f(H.preZv);
+    }
+    for (int k = 0; k < 5; ++k) T.f(c->post[k]);
+    const fe *go = c->h_gout;
+    int64_t k = 0;
+    T.put(go + k, 64); k += 4;                                     // roots L, H
+    for (int s = 0; s < 65; ++s) T.f(c->h_msg[c->msg_allsum + s]);
+    T.put(go + k, (size_t)16 * (c->n_gather - k));
+}
+
+int ctx_init(vg_ctx *c)
+{
+    host_circuit &C = c->C;
+    int n = c->n = C.bl[0];
+    if (n > 26) FAILV(-2, "input layer of 2^%d exceeds this build's commitment limit", n);
+    c->SS = (int64_t)1 << (n - 1);
+    c->log_SS = n - 1;
+    c->real = c->SS >> 5;
+    // mask count (4 bl + 15 per layer, + 2) and the padded mask length m (poly_commit.h:56-66)
+    int64_t nm = 2;
+    for (int i = C.depth - 1; i >= 1; --i) nm += 4 * C.bl[i - 1] + 15;
+    int64_t gap = c->SS / nm;
+    for (int j = 0; j < 64; ++j)
+        if ((1LL << j) <= gap && (1LL << (j + 1)) > gap) { gap = 1LL << j; break; }
+    if (gap < 2) FAILV(-2, "too many mask coefficients for the input size");
+    c->m = c->SS / gap;
+    c->C_ = c->real > c->m ? c->real : c->m;
+    c->log_C = ilog2(c->C_);
+    c->log_R = c->log_SS - c->log_C;
+    c->LQ = 2 * c->C_;
+    // challenge buffer size: ones/zero + r0 + r1 + per layer (2 bl + 2) + fri + masks
+    int64_t nc = 2 + 2 * (int64_t)C.bl[C.depth - 1] + (n - 6);
+    for (int i = 1; i < C.depth; ++i) nc += 2 * C.bl[i - 1] + 2;
+    c->n_chal = nc + 2 * c->m;
+    // device message block
+    c->msg_layer.assign(C.depth, 0);
+    int64_t ml = 0;
+    c->msg_vres = ml++;
+    for (int i = C.depth - 1; i >= 1; --i) { c->msg_layer[i] = ml; ml += 6 * C.bl[i - 1] + 11; }
+    c->msg_allsum = ml; ml += 65;
+    c->msg_len = ml;
+    c->gather_cap = gather_count(n);
+    int64_t bytes = layout(c, nullptr);
+    HIPCHK(hipMalloc(&c->arena, bytes));
+    c->arena_bytes = bytes;
+    layout(c, c->arena);
+    HIPCHK(hipHostMalloc(&c->h_chal, sizeof(fe) * c->n_chal, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&c->h_gsrc, sizeof(uint64_t) * c->gather_cap, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&c->h_gout, sizeof(fe) * (c->gather_cap > (1024 + (c->SS >> 10) + 1) ? c->gather_cap : (1024 + (c->SS >> 10) + 1)), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&c->h_msg, sizeof(fe) * c->msg_len, hipHostMallocDefault));
+    for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+    // gates and inputs
+    for (int i = 0; i < C.depth; ++i) {
+        int64_t nn = (int64_t)1 << C.bl[i];
+        HIPCHK(hipMemcpy(c->d_u[i], C.u[i].data(), 4 * nn, hipMemcpyHostToDevice));
+        if (i > 0) {
+            HIPCHK(hipMemcpy(c->d_ty[i], C.ty[i].data(), nn, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(c->d_v[i], C.v[i].data(), 4 * nn, hipMemcpyHostToDevice));
+        }
+    }
+    std::vector<int> lens(65, (int)c->real);
+    lens[64] = (int)c->m;
+    HIPCHK(hipMemcpy(c->lens, lens.data(), sizeof(int) * 65, hipMemcpyHostToDevice));
+    {
+        // twiddle table w_SS^e, e < SS (depends only on the input size)
+        int64_t SS = c->SS;
+        std::vector<fe> T12(1024 + (SS >> 10) + 1);
+        fe w = root_of_unity(c->log_SS), x = fone();
+        for (int j = 0; j < 1024; ++j) { T12[j] = x; x = mul(x, w); }
+        fe w10 = x, y = fone();
+        for (int64_t j = 0; j < (SS >> 10) + 1; ++j) { T12[1024 + j] = y; y = mul(y, w10); }
+        HIPCHK(hipMemcpy(c->T12, T12.data(), sizeof(fe) * T12.size(), hipMemcpyHostToDevice));
+        launch::twiddles(c->tw, SS, c->T12, c->T12 + 1024, 0);
+    }
+    HIPCHK(hipDeviceSynchronize());
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int vg_init(int device)
+{
+    HIPCHK(hipSetDevice(device));
+    return 0;
+}
+const char *vg_last_error(void) { return g_err.c_str(); }
+void vg_free(void *p) { free(p); }
+
+int vg_ctx_create(vg_ctx **out, const vg_circuit_desc *d, void *stream)
+{
+    if (!out || !d) FAILV(-1, "null argument");
+    vg_ctx *c = new vg_ctx();
+    int rc = build_circuit(d, c->C);
+    if (rc) { delete c; return rc; }
+    if (stream) c->st = (hipStream_t)stream;
+    else {
+        hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+        if (e != hipSuccess) { delete c; FAILV(-20, "hipStreamCreate: %s", hipGetErrorString(e)); }
+        c->own_stream = true;
+    }
+    rc = ctx_init(c);
+    if (rc) { vg_ctx_destroy(c); return rc; }
+    *out = c;
+    return 0;
+}
+
+int vg_ctx_destroy(vg_ctx *c)
+{
+    if (!c) return 0;
+    if (c->st) hipStreamSynchronize(c->st);
+    if (c->arena) hipFree(c->arena);
+    if (c->h_chal) hipHostFree(c->h_chal);
+    if (c->h_gsrc) hipHostFree(c->h_gsrc);
+    if (c->h_gout) hipHostFree(c->h_gout);
+    if (c->h_msg) hipHostFree(c->h_msg);
+    for (auto &e : c->ev) if (e) hipEventDestroy(e);
+    if (c->own_stream && c->st) hipStreamDestroy(c->st);
+    delete c;
+    return 0;
+}
+
+int64_t vg_ctx_device_bytes(const vg_ctx *c) { return c ? c->arena_bytes : 0; }
+
+int vg_prove_async(vg_ctx *c)
+{
+    if (!c) FAILV(-1, "null context");
+    if (c->pending) FAILV(-1, "a proof is already in flight on this context");
+    c->t_start = std::chrono::steady_clock::now();
+    int rc = prepare(c);
+    if (rc) return rc;
+    rc = build_gather(c);
+    if (rc) return rc;
+    rc = enqueue(c);
+    if (rc) return rc;
+    c->pending = true;
+    return 0;
+}
+
+int vg_finish(vg_ctx *c, uint8_t **transcript, size_t *len, double *stage_ms)
+{
+    if (!c || !c->pending) FAILV(-1, "no proof in flight");
+    c->pending = false;
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipGetLastError());
+    tbuf T;
+    assemble(c, T);
+    uint8_t *p = (uint8_t *)malloc(T.b.size());
+    if (!p) FAILV(-5, "out of host memory");
+    memcpy(p, T.b.data(), T.b.size());
+    *transcript = p;
+    *len = T.b.size();
+    if (stage_ms) {
+        for (int k = 0; k < 6; ++k) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]);
+            stage_ms[k] = ms;
+        }
+    }
+    return 0;
+}
+
+int vg_prove(vg_ctx *c, uint8_t **transcript, size_t *len, double *stage_ms)
+{
+    int rc = vg_prove_async(c);
+    if (rc) return rc;
+    return vg_finish(c, transcript, len, stage_ms);
+}
+
+int vg_output(vg_ctx *c, vg_fe *out, size_t cap, int *bits)
+{
+    if (!c) FAILV(-1, "null context");
+    int bl = c->C.bl[c->C.depth - 1];
+    *bits = bl;
+    if (((size_t)1 << bl) > cap) FAILV(-1, "output buffer too small");
+    HIPCHK(hipMemcpy(out, c->cv[c->C.depth - 1], sizeof(fe) << bl, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // extern "C"

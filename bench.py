@@ -33,7 +33,7 @@ def parse():
     ap.add_argument("--log-width", type=int, default=20)
     ap.add_argument("--depth", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-log-width", type=int, default=16)
+    ap.add_argument("--cpu-sample-log-width", type=int, default=18)
     return ap.parse_args()
 
 

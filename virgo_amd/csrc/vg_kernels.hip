@@ -11,7 +11,8 @@ namespace {
 
 constexpr int TPB = 256;
 constexpr int NTT_LM = 11;          // LDS-resident NTT block: 2^11 elements = 32 KB
-constexpr int FOLD_TAIL = 4096;     // fold rounds with <= this many outputs run in one block
+constexpr int FOLD_TAIL = 512;      // fold rounds with <= this many outputs run in one block
+constexpr int TAIL_T = 256;         // threads of that block
 constexpr int MAX_PARTS = 1024;     // max blocks of a fold round (partials per round)
 
 __host__ __device__ inline unsigned nblk(int64_t n, int t = TPB) { return (unsigned)((n + t - 1) / t); }
@@ -33,11 +34,29 @@ __device__ __forceinline__ void atomic_add_p(uint64_t *a, uint64_t x)
         old = atomicCAS(p, assumed, (unsigned long long)addp(assumed, x));
     } while (old != assumed);
 }
-__device__ __forceinline__ void atomic_add_fe(fe *a, const fe &x)
+// Table builds accumulate into split limbs: each canonical component x < 2^61
+// is added as (x mod 2^32) and (x >> 32) into two 64-bit counters with
+// non-returning atomicAdd, which cannot overflow before 2^32 terms.  The first
+// fold round folds the limbs back into canonical field elements.
+__device__ __forceinline__ void atomic_add_fe(acc4 *a, const fe &x)
 {
-    atomic_add_p(&a->real, x.real);
-    atomic_add_p(&a->img, x.img);
+    if (x.real) {
+        atomicAdd((unsigned long long *)&a->rl, (unsigned long long)(x.real & 0xFFFFFFFFull));
+        atomicAdd((unsigned long long *)&a->rh, (unsigned long long)(x.real >> 32));
+    }
+    if (x.img) {
+        atomicAdd((unsigned long long *)&a->il, (unsigned long long)(x.img & 0xFFFFFFFFull));
+        atomicAdd((unsigned long long *)&a->ih, (unsigned long long)(x.img >> 32));
+    }
 }
+// (hi * 2^32 + lo) mod p, hi < 2^61, lo < 2^64
+__device__ __forceinline__ uint64_t limbs_to_p(uint64_t lo, uint64_t hi)
+{
+    uint64_t s = ((hi & 0x1FFFFFFFull) << 32) + (hi >> 29) + (lo & P) + (lo >> 61);
+    s = (s & P) + (s >> 61);
+    return s >= P ? s - P : s;
+}
+__device__ __forceinline__ fe acc_value(const acc4 &a) { return mk(limbs_to_p(a.rl, a.rh), limbs_to_p(a.il, a.ih)); }
 
 // ---- reductions ----
 __device__ __forceinline__ fe wave_sum(fe x)
@@ -171,7 +190,7 @@ __device__ __forceinline__ fe beta_u(const bu_t &b, int u)
 // sumcheck phase-1 table build: scatter by u (zk_prover.cpp:420-587)
 // ---------------------------------------------------------------------
 __global__ void k_p1_scatter(const uint8_t *gty, const int32_t *gu, const int32_t *gv, int n, const fe *pv, bg_t bg,
-                             fe *amb, fe *aVb)
+                             acc4 *amb, acc4 *aVb)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -200,7 +219,7 @@ __global__ void k_p1_scatter(const uint8_t *gty, const int32_t *gu, const int32_
 // the block first.
 // ---------------------------------------------------------------------
 __global__ void k_p2_scatter(const uint8_t *gty, const int32_t *gu, const int32_t *gv, int n, bg_t bg, bu_t bu,
-                             const fe *vup, fe *amb, fe *aVb)
+                             const fe *vup, acc4 *amb, acc4 *aVb)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     fe acc0 = fzero();
@@ -236,15 +255,15 @@ __global__ void k_p2_scatter(const uint8_t *gty, const int32_t *gu, const int32_
 struct tabs { lpoly *V, *aV, *am; };
 struct ctabs { const lpoly *V, *aV, *am; };
 
-__device__ __forceinline__ void fold_one(bool first, const fe *Vb, const fe *aVb, const fe *amb, ctabs in, const fe &r,
+__device__ __forceinline__ void fold_one(bool first, const fe *Vb, const acc4 *aVb, const acc4 *amb, ctabs in, const fe &r,
                                          int64_t i, lpoly &V, lpoly &aV, lpoly &am)
 {
     if (first) {
         fe x0 = Vb[2 * i], x1 = Vb[2 * i + 1];
         V.b = x0; V.a = sub(x1, x0);
-        x0 = aVb[2 * i]; x1 = aVb[2 * i + 1];
+        x0 = acc_value(aVb[2 * i]); x1 = acc_value(aVb[2 * i + 1]);
         aV.b = x0; aV.a = sub(x1, x0);
-        x0 = amb[2 * i]; x1 = amb[2 * i + 1];
+        x0 = acc_value(amb[2 * i]); x1 = acc_value(amb[2 * i + 1]);
         am.b = x0; am.a = sub(x1, x0);
     } else {
         lpoly y0 = in.V[2 * i], y1 = in.V[2 * i + 1];
@@ -271,7 +290,7 @@ __device__ void reduce_prev(const fe *prev, int nprev, fe *dst)
     block_sum<3>(acc);
     if (threadIdx.x == 0) { dst[0] = acc[0]; dst[1] = acc[1]; dst[2] = acc[2]; }
 }
-__global__ __launch_bounds__(TPB) void k_fold(int first, const fe *Vb, const fe *aVb, const fe *amb, ctabs in, const fe *rp,
+__global__ __launch_bounds__(TPB) void k_fold(int first, const fe *Vb, const acc4 *aVb, const acc4 *amb, ctabs in, const fe *rp,
                                              tabs out, int64_t n_out, fe *part, const fe *prev_part, int nprev, fe *prev_dst)
 {
     reduce_prev(prev_part, nprev, prev_dst);
@@ -287,7 +306,7 @@ __global__ __launch_bounds__(TPB) void k_fold(int first, const fe *Vb, const fe 
     if (threadIdx.x == 0) { part[3 * blockIdx.x] = acc[0]; part[3 * blockIdx.x + 1] = acc[1]; part[3 * blockIdx.x + 2] = acc[2]; }
 }
 // remaining rounds j0..L-1 in one block; X holds round j0's input (unless j0 == 0)
-__global__ __launch_bounds__(1024) void k_fold_tail(int j0, int L, const fe *Vb, const fe *aVb, const fe *amb, tabs X, tabs Y,
+__global__ __launch_bounds__(TAIL_T) void k_fold_tail(int j0, int L, const fe *Vb, const acc4 *aVb, const acc4 *amb, tabs X, tabs Y,
                                                     const fe *r, fe vconst, const fe *prev_part, int nprev, fe *sums, fe *end)
 {
     if (nprev) {
@@ -424,23 +443,26 @@ __global__ __launch_bounds__(TPB) void k_ntt_lds(ntt_in in, fe *out, int64_t out
     fe *o = out + b * out_stride + (q << log_m);
     for (int i = threadIdx.x; i < M; i += blockDim.x) o[i] = do_scale ? mul(sh[i], scale) : sh[i];
 }
-// stages [s0, s0+g) in registers; thread = (batch, base with bits [s0,s0+g) clear)
-__global__ __launch_bounds__(TPB) void k_ntt_regs(fe *data, int64_t stride, int64_t nb, int log_n, int s0, int g, int inverse,
+// stages [s0, s0+G) in registers; thread = (batch, base with bits [s0,s0+G) clear)
+template <int G>
+__global__ __launch_bounds__(TPB) void k_ntt_regs(fe *data, int64_t stride, int64_t nb, int log_n, int s0, int inverse,
                                                  fe scale, int do_scale, const fe *tw, int64_t SS, int log_SS)
 {
-    int64_t per = (int64_t)1 << (log_n - g);
+    int64_t per = (int64_t)1 << (log_n - G);
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (tid >= nb * per) return;
     int64_t b = tid / per, r = tid % per;
     int64_t lo = r & (((int64_t)1 << s0) - 1), hi = r >> s0;
-    int64_t base = (hi << (s0 + g)) | lo;
+    int64_t base = (hi << (s0 + G)) | lo;
     fe *d = data + b * stride;
-    fe x[16];
-    int G = 1 << g;
-    for (int m = 0; m < G; ++m) x[m] = d[base + ((int64_t)m << s0)];
-    for (int t = 0; t < g; ++t) {
+    fe x[1 << G];
+#pragma unroll
+    for (int m = 0; m < (1 << G); ++m) x[m] = d[base + ((int64_t)m << s0)];
+#pragma unroll
+    for (int t = 0; t < G; ++t) {
         int s = s0 + t;
-        for (int m = 0; m < G; ++m) {
+#pragma unroll
+        for (int m = 0; m < (1 << G); ++m) {
             if (m & (1 << t)) continue;
             int m1 = m | (1 << t);
             int64_t e = lo + ((int64_t)(m & ((1 << t) - 1)) << s0);
@@ -450,7 +472,8 @@ __global__ __launch_bounds__(TPB) void k_ntt_regs(fe *data, int64_t stride, int6
             x[m1] = sub(a, c);
         }
     }
-    for (int m = 0; m < G; ++m) d[base + ((int64_t)m << s0)] = do_scale ? mul(x[m], scale) : x[m];
+#pragma unroll
+    for (int m = 0; m < (1 << G); ++m) d[base + ((int64_t)m << s0)] = do_scale ? mul(x[m], scale) : x[m];
 }
 
 // tw[e] = T1[e & 1023] * T2[e >> 10]
@@ -510,17 +533,6 @@ __device__ __forceinline__ int64_t leaf_of(int64_t tid, int64_t nleaf, int log_R
     int64_t s = tid / cl, t = tid - s * cl;
     return (t << log_R) + s;
 }
-__global__ __launch_bounds__(TPB) void k_leaf_init(const fe *cw, int64_t S, int log_R, uint8_t *tree)
-{
-    int64_t half = S / 2;
-    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (tid >= half) return;
-    int64_t i = leaf_of(tid, half, log_R);
-    int64_t k0 = cw_idx(i, S, log_R), k1 = cw_idx(i + half, S, log_R);
-    uint64_t h[4] = {0, 0, 0, 0};
-    for (int j = 0; j <= 64; ++j) chain_step(h, cw[j * S + k0], cw[j * S + k1]);
-    store_digest(tree, half + i, h);
-}
 __global__ __launch_bounds__(TPB) void k_merkle_level(uint8_t *tree, int64_t start, int64_t n)
 {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -545,45 +557,78 @@ __global__ __launch_bounds__(1024) void k_merkle_tail(uint8_t *tree, int64_t n)
         __syncthreads();
     }
 }
-// FRI step: fold with r (fri.cpp:321-340, 367-379), write the folded codeword,
-// hash each new leaf (fri.cpp:397-420).
-__global__ __launch_bounds__(TPB) void k_fri_step(const fe *prev, int64_t S, int log_R, const fe *rp, const fe *tw,
-                                                 int64_t SS, fe *next, uint8_t *tree)
+// FRI fold of 65 slices (fri.cpp:321-340, 367-379): next[p] = ((a+b) + w_S^-p r (a-b)) / 2,
+// a = prev[p], b = prev[p + S/2]; one thread per (slice, position), row-contiguous.
+__global__ __launch_bounds__(TPB) void k_fri_fold(const fe *prev, int64_t S, int log_R, const fe *rp, const fe *tw, int64_t SS,
+                                                 fe *next)
 {
-    int64_t Sn = S / 2, nleaf = S / 4;
+    int64_t Sn = S / 2;
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (tid >= nleaf) return;
-    int64_t i = leaf_of(tid, nleaf, log_R);
-    fe r = *rp;
+    if (tid >= 65 * Sn) return;
+    int64_t j = tid / Sn, st = tid - j * Sn;
+    int64_t cols = Sn >> log_R;
+    int64_t srow = st / cols, t = st - srow * cols;
+    int64_t p = (t << log_R) + srow;
+    fe a = prev[j * S + cw_idx(p, S, log_R)], b = prev[j * S + cw_idx(p + Sn, S, log_R)];
+    fe imr = mul(tw[(SS - p * (SS / S)) & (SS - 1)], *rp);
     const fe inv2 = mk(1ull << 60, 0);
-    int64_t p[2] = {i, i + nleaf};
-    int64_t ka[2], kb[2], kn[2];
-    fe imr[2];
-    int64_t sh = SS / S;
-    for (int z = 0; z < 2; ++z) {
-        ka[z] = cw_idx(p[z], S, log_R);
-        kb[z] = cw_idx(p[z] + Sn, S, log_R);
-        kn[z] = cw_idx(p[z], Sn, log_R);
-        imr[z] = mul(tw[(SS - p[z] * sh) & (SS - 1)], r);
-    }
+    next[tid] = mul(inv2, add(add(a, b), mul(imr, sub(a, b))));
+}
+struct leaf_segs {
+    int n;
+    int log_R;
+    const fe *cw[24];
+    int64_t S[24];
+    uint8_t *tree[24];
+    int64_t start[25];          // prefix sums of leaf counts
+};
+// Merkle leaves of every commitment at once: a leaf i of a codeword with S
+// positions per slice chains SHA3 over the 64 slice pairs (i, i+S/2) and then
+// the mask pair (fri.cpp:97-126, 397-420); the digest goes to tree[S/2 + i].
+__global__ __launch_bounds__(TPB) void k_leaf_multi(leaf_segs L)
+{
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= L.start[L.n]) return;
+    int sg = 0;
+    while (tid >= L.start[sg + 1]) ++sg;
+    tid -= L.start[sg];
+    int64_t S = L.S[sg], half = S / 2;
+    const fe *cw = L.cw[sg];
+    int64_t i = leaf_of(tid, half, L.log_R);
+    int64_t k0 = cw_idx(i, S, L.log_R), k1 = cw_idx(i + half, S, L.log_R);
     uint64_t h[4] = {0, 0, 0, 0};
-    for (int j = 0; j <= 64; ++j) {
-        fe f[2];
-        for (int z = 0; z < 2; ++z) {
-            fe a = prev[j * S + ka[z]], b = prev[j * S + kb[z]];
-            f[z] = mul(inv2, add(add(a, b), mul(imr[z], sub(a, b))));
-            next[j * Sn + kn[z]] = f[z];
-        }
-        chain_step(h, f[0], f[1]);
+    for (int j = 0; j <= 64; ++j) chain_step(h, cw[j * S + k0], cw[j * S + k1]);
+    store_digest(L.tree[sg], half + i, h);
+}
+struct tree_set {
+    int n;
+    uint8_t *tree[24];
+    int64_t nleaf[24];
+};
+// level h (h >= 1) of every tree that has it: nodes [nleaf>>h, 2*(nleaf>>h))
+__global__ __launch_bounds__(TPB) void k_merkle_multi(tree_set T, int h, int64_t total)
+{
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= total) return;
+    int sg = 0;
+    for (; sg < T.n; ++sg) {
+        int64_t c = T.nleaf[sg] >> h;
+        if (tid < c) break;
+        tid -= c;
     }
-    store_digest(tree, nleaf + i, h);
+    if (sg >= T.n) return;
+    int64_t k = (T.nleaf[sg] >> h) + tid;
+    const uint64_t *c = (const uint64_t *)(T.tree[sg] + 2 * k * 32);
+    uint64_t m[8], hh[4];
+    for (int q = 0; q < 8; ++q) m[q] = c[q];
+    sha3_64(m, hh);
+    store_digest(T.tree[sg], k, hh);
 }
 __global__ void k_gather16(const uint64_t *src, int64_t n, fe *out)
 {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k < n) out[k] = *(const fe *)(uintptr_t)src[k];
 }
-
 __global__ void k_sha3_batch(const uint64_t *in, uint64_t *out, int64_t n)
 {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -624,8 +669,14 @@ void ntt_run(const ntt_in &in, fe *out, int64_t out_stride, int64_t nb, int log_
         int g = rest > 4 ? 4 : rest;
         rest -= g;
         int64_t threads = nb * ((int64_t)1 << (log_n - g));
-        hipLaunchKernelGGL(k_ntt_regs, dim3(nblk(threads)), dim3(TPB), 0, st, out, out_stride, nb, log_n, s0, g, inverse,
-                           scale, rest == 0 ? do_scale : 0, tw, SS, log_SS);
+        int ds = rest == 0 ? do_scale : 0;
+        dim3 gr(nblk(threads)), bk(TPB);
+        switch (g) {
+        case 1: hipLaunchKernelGGL(k_ntt_regs<1>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, tw, SS, log_SS); break;
+        case 2: hipLaunchKernelGGL(k_ntt_regs<2>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, tw, SS, log_SS); break;
+        case 3: hipLaunchKernelGGL(k_ntt_regs<3>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, tw, SS, log_SS); break;
+        default: hipLaunchKernelGGL(k_ntt_regs<4>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, tw, SS, log_SS); break;
+        }
         s0 += g;
     }
 }
@@ -689,20 +740,20 @@ void eq_table(fe *out, const fe *r, int len, fe *scratch, hipStream_t st)
     int64_t n = (int64_t)1 << len;
     hipLaunchKernelGGL(k_eq_expand, dim3(nblk(n)), dim3(TPB), 0, st, out, lo, hi, fh, n);
 }
-void p1_scatter(gates_dev g, const fe *prevV, beta_g bg, fe *amb, fe *aVb, hipStream_t s)
+void p1_scatter(gates_dev g, const fe *prevV, beta_g bg, acc4 *amb, acc4 *aVb, hipStream_t s)
 {
     int n = 1 << g.bl;
     bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
     hipLaunchKernelGGL(k_p1_scatter, dim3(nblk(n)), dim3(TPB), 0, s, g.ty, g.u, g.v, n, prevV, b, amb, aVb);
 }
-void p2_scatter(gates_dev g, beta_g bg, beta_u bu, const fe *vu, fe *amb, fe *aVb, hipStream_t s)
+void p2_scatter(gates_dev g, beta_g bg, beta_u bu, const fe *vu, acc4 *amb, acc4 *aVb, hipStream_t s)
 {
     int n = 1 << g.bl;
     bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
     bu_t u = {bu.f, bu.s, bu.fh};
     hipLaunchKernelGGL(k_p2_scatter, dim3(nblk(n)), dim3(TPB), 0, s, g.ty, g.u, g.v, n, b, u, vu, amb, aVb);
 }
-void sumcheck_phase(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *sums,
+void sumcheck_phase(const fe *Vb, const acc4 *aVb, const acc4 *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *sums,
                     fe *end, hipStream_t s)
 {
     int j = 0;
@@ -725,7 +776,7 @@ void sumcheck_phase(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe 
     }
     tabs X = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
     tabs Y = {ws.V[cur], ws.aV[cur], ws.am[cur]};
-    hipLaunchKernelGGL(k_fold_tail, dim3(1), dim3(1024), 0, s, j, L, Vb, aVb, amb, X, Y, r, vconst, prev_part, nprev, sums, end);
+    hipLaunchKernelGGL(k_fold_tail, dim3(1), dim3(TAIL_T), 0, s, j, L, Vb, aVb, amb, X, Y, r, vconst, prev_part, nprev, sums, end);
 }
 void general_value(gates_dev g, beta_g bg, beta_u bu, beta_u bv, const fe *vu, const fe *vv, fe *part, unsigned *counter,
                    fe *dst, hipStream_t s)
@@ -769,10 +820,6 @@ void slice_sums(const fe *lq_coef, int64_t stride, int nslice, int64_t real, int
 {
     hipLaunchKernelGGL(k_slice_sums, dim3(1), dim3(128), 0, s, lq_coef, stride, nslice, real, m, c0, all_sum);
 }
-void leaf_hash_init(const fe *cw, int64_t S, int log_R, uint8_t *tree, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_leaf_init, dim3(nblk(S / 2)), dim3(TPB), 0, s, cw, S, log_R, tree);
-}
 void merkle(uint8_t *tree, int64_t n_leaf, hipStream_t s)
 {
     int64_t n = n_leaf / 2;
@@ -781,11 +828,6 @@ void merkle(uint8_t *tree, int64_t n_leaf, hipStream_t s)
         n /= 2;
     }
     if (n >= 1) hipLaunchKernelGGL(k_merkle_tail, dim3(1), dim3(1024), 0, s, tree, n);
-}
-void fri_step(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, int64_t SS, fe *next, uint8_t *tree,
-              hipStream_t s)
-{
-    hipLaunchKernelGGL(k_fri_step, dim3(nblk(S / 4)), dim3(TPB), 0, s, prev, S, log_R, r, tw, SS, next, tree);
 }
 void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int inverse, fe scale, int do_scale, const fe *tw,
          int64_t SS, hipStream_t s)
@@ -809,6 +851,35 @@ void fe_mul(const fe *a, const fe *b, fe *o, int64_t n, hipStream_t s)
     if (n) hipLaunchKernelGGL(k_fe_mul, dim3(nblk(n)), dim3(TPB), 0, s, a, b, o, n);
 }
 
+void fri_fold(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, int64_t SS, fe *next, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fri_fold, dim3(nblk(65 * (S / 2))), dim3(TPB), 0, s, prev, S, log_R, r, tw, SS, next);
+}
+void leaf_hash_multi(int nseg, const fe *const *cw, const int64_t *S, uint8_t *const *tree, int log_R, hipStream_t s)
+{
+    leaf_segs L;
+    memset(&L, 0, sizeof L);
+    L.n = nseg; L.log_R = log_R;
+    L.start[0] = 0;
+    for (int k = 0; k < nseg; ++k) {
+        L.cw[k] = cw[k]; L.S[k] = S[k]; L.tree[k] = tree[k];
+        L.start[k + 1] = L.start[k] + S[k] / 2;
+    }
+    hipLaunchKernelGGL(k_leaf_multi, dim3(nblk(L.start[nseg])), dim3(TPB), 0, s, L);
+}
+void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStream_t s)
+{
+    tree_set T;
+    memset(&T, 0, sizeof T);
+    T.n = ntree;
+    int64_t mx = 0;
+    for (int k = 0; k < ntree; ++k) { T.tree[k] = tree[k]; T.nleaf[k] = nleaf[k]; if (nleaf[k] > mx) mx = nleaf[k]; }
+    for (int h = 1; ((int64_t)1 << h) <= mx; ++h) {
+        int64_t total = 0;
+        for (int k = 0; k < ntree; ++k) total += nleaf[k] >> h;
+        hipLaunchKernelGGL(k_merkle_multi, dim3(nblk(total)), dim3(TPB), 0, s, T, h, total);
+    }
+}
 void gather16(const uint64_t *src, int64_t n, fe *out, hipStream_t s)
 {
     if (n) hipLaunchKernelGGL(k_gather16, dim3(nblk(n)), dim3(TPB), 0, s, src, n, out);

@@ -22,6 +22,11 @@ __host__ __device__ inline int64_t cw_idx(int64_t p, int64_t S, int log_R)
     return (p & (R - 1)) * (S >> log_R) + (p >> log_R);
 }
 
+// table-build accumulator: real/img split into 32-bit-limb sums
+struct acc4 {
+    uint64_t rl, rh, il, ih;
+};
+
 struct gates_dev {            // one layer's gates, SoA
     const uint8_t *ty;
     const int32_t *u, *v;
@@ -46,8 +51,8 @@ void eq_table(fe *out, const fe *r, int len, fe *scratch, hipStream_t s);
 struct beta_g { const fe *g0f, *g0s, *g1f, *g1s; int fh; };
 struct beta_u { const fe *f, *s; int fh; };
 
-void p1_scatter(gates_dev g, const fe *prevV, beta_g bg, fe *amb, fe *aVb, hipStream_t s);
-void p2_scatter(gates_dev g, beta_g bg, beta_u bu, const fe *vu, fe *amb, fe *aVb, hipStream_t s);
+void p1_scatter(gates_dev g, const fe *prevV, beta_g bg, acc4 *amb, acc4 *aVb, hipStream_t s);
+void p2_scatter(gates_dev g, beta_g bg, beta_u bu, const fe *vu, acc4 *amb, acc4 *aVb, hipStream_t s);
 
 struct fold_ws {              // workspace of a sumcheck phase
     lpoly *V[2], *aV[2], *am[2];
@@ -58,7 +63,7 @@ struct fold_ws {              // workspace of a sumcheck phase
 // Round j folds with r[j-1] (r[-1] unused); round sums (a,b,c) -> sums[3j..].
 // After the last round: end[0..3] = V[0].a, V[0].b, am[0].a, am[0].b and
 // end[4] = V[0].a * r[L-1] + V[0].b + vconst (= v_u or v_v).
-void sumcheck_phase(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst,
+void sumcheck_phase(const fe *Vb, const acc4 *aVb, const acc4 *amb, int L, const fe *r, fe vconst,
                     fold_ws ws, fe *sums, fe *end, hipStream_t s);
 
 // sum over gates of beta_g(g) beta_u(u) beta_v(v) f_type(v_u, v_v)
@@ -89,15 +94,8 @@ void virtual_oracle(const fe *l, const fe *q, const fe *h, const fe *c0, int nsl
 // all_sum[i] = (lq_coef[0] + lq_coef[len_i]) * len_i; c0[i] = lq_coef[0] + lq_coef[len_i]
 void slice_sums(const fe *lq_coef, int64_t stride, int nslice, int64_t real, int64_t m, fe *c0, fe *all_sum, hipStream_t s);
 
-// Merkle leaves of an initial commitment (fri.cpp:97-126): leaf i hashes the
-// 64 slice pairs (p_i, p_i + S/2) then the mask pair; written to tree[S/2 + i].
-void leaf_hash_init(const fe *cw, int64_t S, int log_R, uint8_t *tree, hipStream_t s);
 // heap Merkle tree over n_leaf leaves already at tree[n_leaf..2n_leaf) (merkle_tree.cpp:46-56)
 void merkle(uint8_t *tree, int64_t n_leaf, hipStream_t s);
-// one FRI step (fri.cpp:291-430): fold 65 slices of size S with r, write the
-// S/2 codeword and the leaves of its Merkle tree (tree[S/4 + i]).
-void fri_step(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, int64_t SS, fe *next,
-              uint8_t *tree, hipStream_t s);
 // generic batched NTT (inverse: w^-1), natural order in and out
 void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int inverse, fe scale, int do_scale, const fe *tw,
          int64_t SS, hipStream_t s);
@@ -105,6 +103,13 @@ void sha3_batch(const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s);
 // fills leaves [n_leaf, pad) of a tree with pad leaves with H(0^64)
 void merkle_pad(uint8_t *tree, int64_t pad, int64_t n_leaf, hipStream_t s);
 void fe_mul(const fe *a, const fe *b, fe *o, int64_t n, hipStream_t s);
+// FRI fold only (no hashing): next (S/2 per slice) from prev (S per slice)
+void fri_fold(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, int64_t SS, fe *next, hipStream_t s);
+// leaf chains of nseg codewords in one launch (codeword k: S[k] positions per
+// slice, leaves S[k]/2 written to tree[k] + S[k]/2 nodes)
+void leaf_hash_multi(int nseg, const fe *const *cw, const int64_t *S, uint8_t *const *tree, int log_R, hipStream_t s);
+// every Merkle tree of the set, level by level across trees
+void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStream_t s);
 // out[k] = 16 bytes at address src[k]
 void gather16(const uint64_t *src, int64_t n, fe *out, hipStream_t s);
 

@@ -470,7 +470,7 @@ struct vg_ctx {
     std::vector<int32_t *> d_u, d_v;
     std::vector<fe *> cv;
     fe *chal = nullptr;         // challenges
-    fe *amb = nullptr, *aVb = nullptr;
+    acc4 *amb = nullptr, *aVb = nullptr;
     lpoly *tabs[6] = {};
     fe *part = nullptr;
     fe *bt[8] = {};             // beta half tables: g0f g0s g1f g1s uf us vf vs
@@ -502,7 +502,8 @@ struct vg_ctx {
     std::vector<int64_t> msg_layer;                   // offset of each layer's block in msg
     int64_t msg_vres = 0, msg_allsum = 0;
     fe post[5];
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[8] = {};      // stage boundaries
+    hipEvent_t ev_leaf[2] = {}; // around the leaf-chain kernel
     std::chrono::steady_clock::time_point t_start;
     bool pending = false;
 };
@@ -538,8 +539,8 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     }
     c->chal = carve<fe>(base, off, c->n_chal);
     int64_t T = (int64_t)1 << maxbl;
-    c->amb = carve<fe>(base, off, T);
-    c->aVb = carve<fe>(base, off, T);
+    c->amb = carve<acc4>(base, off, T);
+    c->aVb = carve<acc4>(base, off, T);
     for (int k = 0; k < 6; ++k) c->tabs[k] = carve<lpoly>(base, off, T / 2 > 0 ? T / 2 : 1);
     c->part = carve<fe>(base, off, 6 * 1024);
     int half = (maxbl >> 1) + 1;
@@ -749,14 +750,14 @@ int enqueue(vg_ctx *c)
         launch::half_tables(c->bt[2], c->bt[3], beta, r1, bg, st);
         launch::beta_g BG = {c->bt[0], c->bt[1], c->bt[2], c->bt[3], bg >> 1};
         int64_t T = (int64_t)1 << bl;
-        launch::zero(c->amb, sizeof(fe) * T, st);
-        launch::zero(c->aVb, sizeof(fe) * T, st);
+        launch::zero(c->amb, sizeof(acc4) * T, st);
+        launch::zero(c->aVb, sizeof(acc4) * T, st);
         launch::p1_scatter(gd, c->cv[i - 1], BG, c->amb, c->aVb, st);
         launch::sumcheck_phase(c->cv[i - 1], c->aVb, c->amb, bl, ru, c->LH[i].vu_const, ws, sums1, end1, st);
         launch::half_tables(c->bt[4], c->bt[5], nullptr, ru, bl, st);
         launch::beta_u BU = {c->bt[4], c->bt[5], bl >> 1};
-        launch::zero(c->amb, sizeof(fe) * T, st);
-        launch::zero(c->aVb, sizeof(fe) * T, st);
+        launch::zero(c->amb, sizeof(acc4) * T, st);
+        launch::zero(c->aVb, sizeof(acc4) * T, st);
         launch::p2_scatter(gd, BG, BU, end1 + 4, c->amb, c->aVb, st);
         launch::sumcheck_phase(c->cv[i - 1], c->aVb, c->amb, bl, rv, c->LH[i].vv_const, ws, sums2, end2, st);
         if (i > 1) {
@@ -779,8 +780,6 @@ int enqueue(vg_ctx *c)
     launch::intt({0, c->cv[0], real, nullptr, nullptr, 0, 0, 0, 0}, c->coefP, Cc, 64, lr, inv_real, c->tw, SS, st);
     launch::intt({0, pri_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefP + 64 * Cc, Cc, 1, lm, inv_m, c->tw, SS, st);
     launch::lde(c->coefP, Cc, c->lens, 65, c->l, c->log_C, c->log_R, c->tw, SS, st);
-    launch::leaf_hash_init(c->l, SS, c->log_R, c->treeL, st);
-    launch::merkle(c->treeL, SS / 2, st);
     HIPCHK(hipEventRecord(c->ev[3], st));
     // ---- commit_public_array (poly_commit.h:125-351) ----
     const fe *rlast = c->chal + c->chal_ru[1];
@@ -793,27 +792,44 @@ int enqueue(vg_ctx *c)
                  lm + 1, inv_2m, c->tw, SS, st);
     launch::slice_sums(c->lqc, c->LQ, 65, real, m, c->c0, c->msg + c->msg_allsum, st);
     launch::lde(c->lqc, c->LQ, c->lens, 65, c->h, c->log_C, c->log_R, c->tw, SS, st, 1);
-    launch::leaf_hash_init(c->h, SS, c->log_R, c->treeH, st);
-    launch::merkle(c->treeH, SS / 2, st);
     HIPCHK(hipEventRecord(c->ev[4], st));
-    // ---- commit_phase: FRI (vpd_verifier.cpp:46-78, fri.cpp:291-430) ----
+    // ---- commit_phase folds (vpd_verifier.cpp:46-78, fri.cpp:321-390).  The
+    // challenges are known in advance, so every fold runs before any hashing.
     launch::virtual_oracle(c->l, c->q, c->h, c->c0, 65, SS, c->log_R, real, m, c->tw, c->vo, st);
     const fe *prev = c->vo;
     int64_t S = SS;
     for (int k = 0; k < n - 6; ++k) {
         fe *next = c->cw + c->cw_off[k];
-        uint8_t *tree = c->ctree + c->ctree_off[k];
-        launch::fri_step(prev, S, c->log_R, c->chal + c->chal_fri + k, c->tw, SS, next, tree, st);
-        launch::merkle(tree, S / 4, st);
+        launch::fri_fold(prev, S, c->log_R, c->chal + c->chal_fri + k, c->tw, SS, next, st);
         prev = next;
         S /= 2;
     }
     HIPCHK(hipEventRecord(c->ev[5], st));
+    // ---- every SHA3 leaf chain (fri.cpp:97-126, 397-420) in one launch, then
+    // every Merkle tree (merkle_tree.cpp:46-56) level by level ----
+    {
+        const fe *cws[24];
+        int64_t Ss[24], nl[24];
+        uint8_t *trees[24];
+        int ns = 0;
+        cws[ns] = c->l; Ss[ns] = SS; trees[ns] = c->treeL; ++ns;
+        cws[ns] = c->h; Ss[ns] = SS; trees[ns] = c->treeH; ++ns;
+        int64_t Sk = SS / 2;
+        for (int k = 0; k < n - 6; ++k, Sk /= 2) {
+            cws[ns] = c->cw + c->cw_off[k]; Ss[ns] = Sk; trees[ns] = c->ctree + c->ctree_off[k]; ++ns;
+        }
+        for (int k = 0; k < ns; ++k) nl[k] = Ss[k] / 2;
+        HIPCHK(hipEventRecord(c->ev_leaf[0], st));
+        launch::leaf_hash_multi(ns, cws, Ss, trees, c->log_R, st);
+        HIPCHK(hipEventRecord(c->ev_leaf[1], st));
+        launch::merkle_multi(ns, trees, nl, st);
+    }
+    HIPCHK(hipEventRecord(c->ev[6], st));
     // ---- query answers (fri.cpp:148-289) ----
     launch::gather16(c->gsrc, c->n_gather, c->gout, st);
     HIPCHK(hipMemcpyAsync(c->h_gout, c->gout, sizeof(fe) * c->n_gather, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(c->h_msg, c->msg, sizeof(fe) * c->msg_len, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipEventRecord(c->ev[6], st));
+    HIPCHK(hipEventRecord(c->ev[7], st));
     return 0;
 }
 
@@ -905,6 +921,7 @@ int ctx_init(vg_ctx *c)
     HIPCHK(hipHostMalloc(&c->h_gout, sizeof(fe) * (c->gather_cap > (1024 + (c->SS >> 10) + 1) ? c->gather_cap : (1024 + (c->SS >> 10) + 1)), hipHostMallocDefault));
     HIPCHK(hipHostMalloc(&c->h_msg, sizeof(fe) * c->msg_len, hipHostMallocDefault));
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+    for (auto &e : c->ev_leaf) HIPCHK(hipEventCreate(&e));
     // gates and inputs
     for (int i = 0; i < C.depth; ++i) {
         int64_t nn = (int64_t)1 << C.bl[i];
@@ -968,14 +985,15 @@ int vg_ctx_create(vg_ctx **out, const vg_circuit_desc *d, void *stream)
 int vg_ctx_destroy(vg_ctx *c)
 {
     if (!c) return 0;
-    if (c->st) hipStreamSynchronize(c->st);
-    if (c->arena) hipFree(c->arena);
-    if (c->h_chal) hipHostFree(c->h_chal);
-    if (c->h_gsrc) hipHostFree(c->h_gsrc);
-    if (c->h_gout) hipHostFree(c->h_gout);
-    if (c->h_msg) hipHostFree(c->h_msg);
-    for (auto &e : c->ev) if (e) hipEventDestroy(e);
-    if (c->own_stream && c->st) hipStreamDestroy(c->st);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    if (c->arena) (void)hipFree(c->arena);
+    if (c->h_chal) (void)hipHostFree(c->h_chal);
+    if (c->h_gsrc) (void)hipHostFree(c->h_gsrc);
+    if (c->h_gout) (void)hipHostFree(c->h_gout);
+    if (c->h_msg) (void)hipHostFree(c->h_msg);
+    for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->ev_leaf) if (e) (void)hipEventDestroy(e);
+    if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return 0;
 }
@@ -1011,11 +1029,14 @@ int vg_finish(vg_ctx *c, uint8_t **transcript, size_t *len, double *stage_ms)
     *transcript = p;
     *len = T.b.size();
     if (stage_ms) {
-        for (int k = 0; k < 6; ++k) {
+        for (int k = 0; k < 7; ++k) {
             float ms = 0;
-            hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]);
+            (void)hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]);
             stage_ms[k] = ms;
         }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev_leaf[0], c->ev_leaf[1]);
+        stage_ms[7] = ms;
     }
     return 0;
 }

@@ -58,10 +58,11 @@ int64_t vg_ctx_device_bytes(const vg_ctx *ctx);
  * query answers of fri::request_init_value_with_merkle /
  * request_step_commit (fri.cpp:148-289).
  * Writes a malloc'd transcript (free with vg_free).  stage_ms (may be NULL)
- * receives 7 device timings in ms: evaluate, gkr (incl. V_res), lde_l,
- * lde_qh (q and h extensions), fri_fold (virtual oracle + folds), hash (all
- * leaf chains and Merkle trees), queries (gathers + copies to the host),
- * then an 8th entry: the SHA3 leaf-chain kernel alone (part of "hash"). */
+ * receives 9 device timings in ms: evaluate, gkr (incl. V_res), then the
+ * commitment, which runs on a second stream beside gkr: lde_l, lde_qh (q and
+ * h extensions), fri_fold (virtual oracle + folds), hash (all leaf chains
+ * and Merkle trees); then queries (gathers + copies to the host), the SHA3
+ * leaf-chain kernel alone (part of hash), and the whole device schedule. */
 int vg_prove(vg_ctx *ctx, uint8_t **transcript, size_t *len, double *stage_ms);
 /* Same, but only enqueues the device work of the proof on the context's
  * stream and returns; vg_finish waits and assembles the transcript.  Lets

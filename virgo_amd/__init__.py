@@ -85,7 +85,7 @@ class _Desc(ctypes.Structure):
                 ("u", ctypes.c_void_p), ("v", ctypes.c_void_p)]
 
 
-STAGES = ["evaluate", "gkr", "lde_l", "lde_qh", "fri_fold", "hash", "queries", "leaf_kernel"]
+STAGES = ["evaluate", "gkr", "lde_l", "lde_qh", "fri_fold", "hash", "queries", "leaf_kernel", "device_total"]
 
 
 class Prover:
@@ -101,7 +101,7 @@ class Prover:
         d = _Desc(len(n), _p(n), _p(ty), _p(u), _p(v))
         self._ctx = ctypes.c_void_p()
         _check(lib().vg_ctx_create(ctypes.byref(self._ctx), ctypes.byref(d), stream))
-        self.stage_ms = np.zeros(8, dtype=np.float64)
+        self.stage_ms = np.zeros(9, dtype=np.float64)
 
     def device_bytes(self):
         return int(lib().vg_ctx_device_bytes(self._ctx))

@@ -338,6 +338,234 @@ __global__ __launch_bounds__(TAIL_T) void k_fold_tail(int j0, int L, const fe *V
     }
 }
 
+
+// ---------------------------------------------------------------------
+// Table builds by gather over CSR transposes (owner computes, no atomics).
+// Phase 1 (zk_prover.cpp:420-587): every table entry t sums the gates that
+// feed it through u (a sum gate feeds every t in its range).  Phase 2
+// (:845-972): entry t sums the gates whose v is t; the gates that feed entry
+// 0 unconditionally (sum, exp-sum, NOT, relay) are reduced separately.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_p1_gather(const uint32_t *off, const int32_t *ent, const uint8_t *gty,
+                                                  const int32_t *gu, const int32_t *gv, int nt, const fe *pv, bg_t bg,
+                                                  fe *amb, fe *aVb)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    fe am = fzero(), aV = fzero();
+    for (uint32_t e = off[t]; e < off[t + 1]; ++e) {
+        int g = ent[e];
+        int ty = gty[g], u = gu[g], v = gv[g];
+        if (ty == 4) { am = add(am, beta_g(bg, u)); continue; }
+        fe tmp = beta_g(bg, g);
+        switch (ty) {
+        case 0: aV = add(aV, mul(pv[v], tmp)); am = add(am, tmp); break;
+        case 1: am = add(am, mul(pv[v], tmp)); break;
+        case 5: am = add(am, tmp); break;
+        case 12: am = add(am, mul_base(tmp, 1ull << (t - u))); break;
+        case 6: am = sub(am, tmp); aV = add(aV, tmp); break;
+        case 7: aV = sub(aV, mul(pv[v], tmp)); am = add(am, tmp); break;
+        case 8: { fe tv = mul(tmp, pv[v]); aV = add(aV, tv); am = sub(add(am, tmp), dbl(tv)); break; }
+        case 13: { fe tv = mul(tmp, pv[v]); am = add(sub(am, tv), tmp); break; }
+        case 9: { fe tv = mul(tmp, pv[v]); aV = add(aV, tv); am = sub(am, tv); break; }
+        case 10: am = add(am, tmp); break;
+        default: break;
+        }
+    }
+    amb[t] = am;
+    aVb[t] = aV;
+}
+__global__ __launch_bounds__(TPB) void k_p2_gather(const uint32_t *off, const int32_t *ent, const uint8_t *gty,
+                                                  const int32_t *gu, int nt, bg_t bg, bu_t bu, const fe *vup, fe *amb,
+                                                  fe *aVb)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    fe vu = *vup;
+    fe am = fzero(), aV = fzero();
+    for (uint32_t e = off[t]; e < off[t + 1]; ++e) {
+        int g = ent[e];
+        int ty = gty[g], u = gu[g];
+        fe tgu = mul(beta_g(bg, g), beta_u(bu, u));
+        switch (ty) {
+        case 1: am = add(am, mul(tgu, vu)); break;
+        case 0: am = add(am, tgu); aV = add(aV, mul(tgu, vu)); break;
+        case 7: am = sub(am, tgu); aV = add(aV, mul(tgu, vu)); break;
+        case 8: { fe tv = mul(tgu, vu); am = sub(add(am, tgu), dbl(tv)); aV = add(aV, tv); break; }
+        case 13: { fe tv = mul(tgu, vu); am = sub(am, tv); aV = add(aV, tv); break; }
+        case 9: am = add(am, sub(tgu, mul(vu, tgu))); break;
+        default: break;
+        }
+    }
+    amb[t] = am;
+    aVb[t] = aV;
+}
+// gates adding into addV[0] in phase 2; per-block partials
+__global__ __launch_bounds__(TPB) void k_p2_zero(const int32_t *list, int n, const uint8_t *gty, const int32_t *gu,
+                                                const int32_t *gv, bg_t bg, bu_t bu, const fe *vup, fe *part)
+{
+    fe acc = fzero();
+    fe vu = *vup;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int g = list[k];
+        int ty = gty[g], u = gu[g], v = gv[g];
+        fe tg = beta_g(bg, g);
+        switch (ty) {
+        case 5: { fe t = mul(tg, vu); for (int j = u; j < v; ++j) acc = add(acc, mul(t, beta_u(bu, j))); break; }
+        case 12: { fe t = mul(tg, vu); for (int j = u; j <= v; ++j) { acc = add(acc, mul(t, beta_u(bu, j))); t = dbl(t); } break; }
+        case 6: { fe t = mul(tg, beta_u(bu, u)); acc = add(acc, sub(t, mul(t, vu))); break; }
+        case 10: { fe t = mul(tg, beta_u(bu, u)); acc = add(acc, mul(t, vu)); break; }
+        default: break;
+        }
+    }
+    block_sum<1>(&acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+__global__ void k_add_parts(const fe *part, int n, fe *dst)
+{
+    fe acc = fzero();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) acc = add(acc, part[k]);
+    block_sum<1>(&acc);
+    if (threadIdx.x == 0) *dst = add(*dst, acc);
+}
+
+// ---------------------------------------------------------------------
+// K fold rounds per launch: a thread reads 2^K consecutive entries of the
+// three tables, folds K times in registers and writes one entry; each round's
+// (a, b, c) sums are block-reduced into per-block partials [block][K][3].
+// ---------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(TPB) void k_fold_k(int j0, const fe *Vb, const fe *aVb, const fe *amb, ctabs in,
+                                               const fe *r, tabs out, int64_t n_out, fe *part, const fe *prev_part,
+                                               int nprev, int kprev, fe *prev_dst)
+{
+    if (blockIdx.x == 0 && nprev) {           // finish the previous launch's round sums
+        for (int q = 0; q < kprev; ++q) {
+            fe acc[3] = {fzero(), fzero(), fzero()};
+            for (int k = threadIdx.x; k < nprev; k += blockDim.x)
+                for (int c = 0; c < 3; ++c) acc[c] = add(acc[c], prev_part[(k * kprev + q) * 3 + c]);
+            block_sum<3>(acc);
+            if (threadIdx.x == 0) { prev_dst[3 * q] = acc[0]; prev_dst[3 * q + 1] = acc[1]; prev_dst[3 * q + 2] = acc[2]; }
+        }
+    }
+    const bool first = j0 == 0;
+    fe acc[K][3];
+#pragma unroll
+    for (int q = 0; q < K; ++q) acc[q][0] = acc[q][1] = acc[q][2] = fzero();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_out; i += (int64_t)gridDim.x * blockDim.x) {
+        lpoly V[1 << K], aV[1 << K], am[1 << K];
+        int64_t base = i << K;
+        if (first) {
+#pragma unroll
+            for (int m = 0; m < (1 << K); ++m) {
+                V[m].a = fzero(); V[m].b = Vb[base + m];
+                aV[m].a = fzero(); aV[m].b = aVb[base + m];
+                am[m].a = fzero(); am[m].b = amb[base + m];
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < (1 << K); ++m) { V[m] = in.V[base + m]; aV[m] = in.aV[base + m]; am[m] = in.am[base + m]; }
+        }
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            bool flat = first && q == 0;
+            fe rr = flat ? fzero() : r[j0 + q - 1];      // round j0+q folds with r[j0+q-1]
+#pragma unroll
+            for (int m = 0; m < (1 << (K - 1 - q)); ++m) {
+                lpoly y0, y1, z;
+#define VG_FOLD(T) \
+                y0 = T[2 * m]; y1 = T[2 * m + 1]; \
+                if (flat) { z.b = y0.b; z.a = sub(y1.b, y0.b); } \
+                else { z.b = add(mul(y0.a, rr), y0.b); z.a = sub(add(mul(y1.a, rr), y1.b), z.b); } \
+                T[m] = z;
+                VG_FOLD(V) VG_FOLD(aV) VG_FOLD(am)
+#undef VG_FOLD
+                fold_acc(acc[q], V[m], aV[m], am[m]);
+            }
+        }
+        out.V[i] = V[0]; out.aV[i] = aV[0]; out.am[i] = am[0];
+    }
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        block_sum<3>(acc[q]);
+        if (threadIdx.x == 0)
+            for (int c = 0; c < 3; ++c) part[(blockIdx.x * K + q) * 3 + c] = acc[q][c];
+    }
+}
+// Remaining rounds of a phase in one block, tables in LDS.  X holds the
+// input of round j0 (n_in entries, n_in <= TAIL_IN), or the flat tables if j0 == 0.
+constexpr int TAIL_IN = 512;
+__global__ __launch_bounds__(256) void k_fold_tail_lds(int j0, int L, const fe *Vb, const fe *aVb, const fe *amb, ctabs X,
+                                                      const fe *r, fe vconst, const fe *prev_part, int nprev, int kprev,
+                                                      fe *prev_dst, fe *sums, fe *end)
+{
+    __shared__ lpoly sV[TAIL_IN], saV[TAIL_IN], sam[TAIL_IN];
+    if (nprev) {
+        for (int q = 0; q < kprev; ++q) {
+            fe acc[3] = {fzero(), fzero(), fzero()};
+            for (int k = threadIdx.x; k < nprev; k += blockDim.x)
+                for (int c = 0; c < 3; ++c) acc[c] = add(acc[c], prev_part[(k * kprev + q) * 3 + c]);
+            block_sum<3>(acc);
+            if (threadIdx.x == 0) { prev_dst[3 * q] = acc[0]; prev_dst[3 * q + 1] = acc[1]; prev_dst[3 * q + 2] = acc[2]; }
+        }
+    }
+    int n_in = 1 << (L - j0);
+    for (int i = threadIdx.x; i < n_in; i += blockDim.x) {
+        if (j0 == 0) {
+            sV[i].a = fzero(); sV[i].b = Vb[i];
+            saV[i].a = fzero(); saV[i].b = aVb[i];
+            sam[i].a = fzero(); sam[i].b = amb[i];
+        } else {
+            sV[i] = X.V[i]; saV[i] = X.aV[i]; sam[i] = X.am[i];
+        }
+    }
+    __syncthreads();
+    for (int j = j0; j < L; ++j) {
+        int n_out = 1 << (L - 1 - j);
+        bool flat = j == 0;
+        fe rr = flat ? fzero() : r[j - 1];
+        fe acc[3] = {fzero(), fzero(), fzero()};
+        lpoly nV, naV, nam;
+        bool act = threadIdx.x < n_out;
+        if (act) {
+            int i = threadIdx.x;
+#define VG_TFOLD(S, N) { lpoly y0 = S[2 * i], y1 = S[2 * i + 1]; \
+                if (flat) { N.b = y0.b; N.a = sub(y1.b, y0.b); } \
+                else { N.b = add(mul(y0.a, rr), y0.b); N.a = sub(add(mul(y1.a, rr), y1.b), N.b); } }
+            VG_TFOLD(sV, nV) VG_TFOLD(saV, naV) VG_TFOLD(sam, nam)
+#undef VG_TFOLD
+            fold_acc(acc, nV, naV, nam);
+        }
+        __syncthreads();
+        if (act) { sV[threadIdx.x] = nV; saV[threadIdx.x] = naV; sam[threadIdx.x] = nam; }
+        block_sum<3>(acc);
+        if (threadIdx.x == 0) { sums[3 * j] = acc[0]; sums[3 * j + 1] = acc[1]; sums[3 * j + 2] = acc[2]; }
+    }
+    if (threadIdx.x == 0) {
+        lpoly V0 = sV[0], am0 = sam[0];
+        end[0] = V0.a; end[1] = V0.b; end[2] = am0.a; end[3] = am0.b;
+        end[4] = add(add(mul(V0.a, r[L - 1]), V0.b), vconst);
+    }
+}
+
+// every eq half table of the proof in one launch: out[j] = f0 * prod_{i<nb} eq(bit_i(j), r[i])
+__global__ void k_half_multi(const launch::half_desc *d, int nd, int64_t total)
+{
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= total) return;
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {                         // last descriptor with start <= tid
+        int mid = (lo + hi + 1) >> 1;
+        if (d[mid].start <= tid) lo = mid; else hi = mid - 1;
+    }
+    const launch::half_desc &h = d[lo];
+    int64_t j = tid - h.start;
+    fe one = fone();
+    fe acc = h.f0 ? *h.f0 : one;
+    for (int i = 0; i < h.nbits; ++i) acc = mul(acc, ((j >> i) & 1) ? h.r[i] : sub(one, h.r[i]));
+    h.out[j] = acc;
+}
+
 // ---------------------------------------------------------------------
 // wiring predicates combined with v_u, v_v (zk_verifier.cpp:219-606, :1149)
 // ---------------------------------------------------------------------
@@ -740,6 +968,10 @@ void eq_table(fe *out, const fe *r, int len, fe *scratch, hipStream_t st)
     int64_t n = (int64_t)1 << len;
     hipLaunchKernelGGL(k_eq_expand, dim3(nblk(n)), dim3(TPB), 0, st, out, lo, hi, fh, n);
 }
+void eq_expand(fe *out, const fe *lo, const fe *hi, int fh, int64_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_eq_expand, dim3(nblk(n)), dim3(TPB), 0, st, out, lo, hi, fh, n);
+}
 void p1_scatter(gates_dev g, const fe *prevV, beta_g bg, acc4 *amb, acc4 *aVb, hipStream_t s)
 {
     int n = 1 << g.bl;
@@ -777,6 +1009,63 @@ void sumcheck_phase(const fe *Vb, const acc4 *aVb, const acc4 *amb, int L, const
     tabs X = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
     tabs Y = {ws.V[cur], ws.aV[cur], ws.am[cur]};
     hipLaunchKernelGGL(k_fold_tail, dim3(1), dim3(TAIL_T), 0, s, j, L, Vb, aVb, amb, X, Y, r, vconst, prev_part, nprev, sums, end);
+}
+void p1_gather(const uint32_t *off, const int32_t *ent, gates_dev g, int nt, const fe *prevV, beta_g bg, fe *amb, fe *aVb,
+               hipStream_t s)
+{
+    bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
+    hipLaunchKernelGGL(k_p1_gather, dim3(nblk(nt)), dim3(TPB), 0, s, off, ent, g.ty, g.u, g.v, nt, prevV, b, amb, aVb);
+}
+void p2_gather(const uint32_t *off, const int32_t *ent, const int32_t *zlist, int nz, gates_dev g, int nt, beta_g bg,
+               beta_u bu, const fe *vu, fe *amb, fe *aVb, fe *part, hipStream_t s)
+{
+    bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
+    bu_t u = {bu.f, bu.s, bu.fh};
+    hipLaunchKernelGGL(k_p2_gather, dim3(nblk(nt)), dim3(TPB), 0, s, off, ent, g.ty, g.u, nt, b, u, vu, amb, aVb);
+    if (nz > 0) {
+        unsigned blocks = nblk(nz);
+        if (blocks > MAX_PARTS) blocks = MAX_PARTS;
+        hipLaunchKernelGGL(k_p2_zero, dim3(blocks), dim3(TPB), 0, s, zlist, nz, g.ty, g.u, g.v, b, u, vu, part);
+        hipLaunchKernelGGL(k_add_parts, dim3(1), dim3(1024), 0, s, part, (int)blocks, aVb);
+    }
+}
+void sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *sums,
+                     fe *end, hipStream_t s)
+{
+    int j = 0, cur = 0;
+    const fe *prev_part = nullptr;
+    int nprev = 0, kprev = 0;
+    fe *part[2] = {ws.part, ws.part + 9 * MAX_PARTS};
+    int launch_no = 0;
+    while ((1 << (L - j)) > TAIL_IN) {
+        int K = L - j - 9;                       // rounds until the table fits the tail
+        if (K > 3) K = 3;
+        if (K < 1) K = 1;
+        int64_t n_out = (int64_t)1 << (L - j - K);
+        unsigned g = nblk(n_out);
+        if (g > MAX_PARTS) g = MAX_PARTS;
+        ctabs in = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
+        tabs out = {ws.V[cur], ws.aV[cur], ws.am[cur]};
+        fe *pp = part[launch_no & 1];
+        fe *pdst = j > 0 ? sums + 3 * (j - kprev) : sums;
+        switch (K) {
+        case 1: hipLaunchKernelGGL(k_fold_k<1>, dim3(g), dim3(TPB), 0, s, j, Vb, aVb, amb, in, r, out, n_out, pp, prev_part, nprev, kprev, pdst); break;
+        case 2: hipLaunchKernelGGL(k_fold_k<2>, dim3(g), dim3(TPB), 0, s, j, Vb, aVb, amb, in, r, out, n_out, pp, prev_part, nprev, kprev, pdst); break;
+        default: hipLaunchKernelGGL(k_fold_k<3>, dim3(g), dim3(TPB), 0, s, j, Vb, aVb, amb, in, r, out, n_out, pp, prev_part, nprev, kprev, pdst); break;
+        }
+        prev_part = pp; nprev = (int)g; kprev = K;
+        cur ^= 1;
+        j += K;
+        ++launch_no;
+    }
+    ctabs X = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
+    fe *pdst = j > 0 ? sums + 3 * (j - kprev) : sums;
+    hipLaunchKernelGGL(k_fold_tail_lds, dim3(1), dim3(256), 0, s, j, L, Vb, aVb, amb, X, r, vconst, prev_part, nprev, kprev,
+                       pdst, sums, end);
+}
+void half_multi(const half_desc *d, int nd, int64_t total, hipStream_t s)
+{
+    if (total) hipLaunchKernelGGL(k_half_multi, dim3(nblk(total)), dim3(TPB), 0, s, d, nd, total);
 }
 void general_value(gates_dev g, beta_g bg, beta_u bu, beta_u bv, const fe *vu, const fe *vv, fe *part, unsigned *counter,
                    fe *dst, hipStream_t s)

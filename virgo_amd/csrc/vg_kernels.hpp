@@ -45,6 +45,8 @@ void mle_eval(const fe *in, int log_n, const fe *r, fe *tmp0, fe *tmp1, fe *dst,
 // f[j] = f0 * prod_{i<fh} eq(bit_i(j), r[i]), s[j] = prod_{i<sh} eq(bit_i(j), r[fh+i]),
 // fh = len/2, sh = len - fh (zk_prover.cpp:391-418).  f0 may be null (= 1).
 void half_tables(fe *f, fe *s, const fe *f0, const fe *r, int len, hipStream_t st);
+// out[x] = lo[x mod 2^fh] * hi[x >> fh]
+void eq_expand(fe *out, const fe *lo, const fe *hi, int fh, int64_t n, hipStream_t st);
 // full eq table out[x] = prod_i eq(bit_i(x), r[i]) (zk_verifier.cpp:838-847)
 void eq_table(fe *out, const fe *r, int len, fe *scratch, hipStream_t s);
 
@@ -65,6 +67,27 @@ struct fold_ws {              // workspace of a sumcheck phase
 // end[4] = V[0].a * r[L-1] + V[0].b + vconst (= v_u or v_v).
 void sumcheck_phase(const fe *Vb, const acc4 *aVb, const acc4 *amb, int L, const fe *r, fe vconst,
                     fold_ws ws, fe *sums, fe *end, hipStream_t s);
+
+// table builds by gather over CSR transposes (built once per circuit).
+// p1: off/ent list, per table entry t, the gates feeding t through u.
+void p1_gather(const uint32_t *off, const int32_t *ent, gates_dev g, int nt, const fe *prevV, beta_g bg, fe *amb, fe *aVb,
+               hipStream_t s);
+// p2: off/ent by v for add/mult/minus/xor/naab/bit-test gates; zlist: the
+// sum, exp-sum, NOT and relay gates, which all add into addV[0].
+void p2_gather(const uint32_t *off, const int32_t *ent, const int32_t *zlist, int nz, gates_dev g, int nt, beta_g bg,
+               beta_u bu, const fe *vu, fe *amb, fe *aVb, fe *part, hipStream_t s);
+// all rounds of a phase on plain tables (as sumcheck_phase), folding up to 3
+// rounds per launch; ws.part needs 2 * 9 * 1024 entries.
+void sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *sums,
+                     fe *end, hipStream_t s);
+struct half_desc {            // one eq half table: out[j] = f0 * prod_{i<nbits} eq(bit_i(j), r[i])
+    fe *out;
+    const fe *f0;
+    const fe *r;
+    int nbits;
+    int64_t start;            // prefix sum of table sizes
+};
+void half_multi(const half_desc *d, int nd, int64_t total, hipStream_t s);
 
 // sum over gates of beta_g(g) beta_u(u) beta_v(v) f_type(v_u, v_v)
 // (zk_verifier.cpp:219-606 and the combination at :1149)

@@ -461,7 +461,8 @@ int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
 // ---------------------------------------------------------------------------
 struct vg_ctx {
     host_circuit C;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;   // evaluation, sumcheck, queries
+    hipStream_t st2 = nullptr;  // commitment (runs beside the sumcheck)
     bool own_stream = false;
     uint8_t *arena = nullptr;
     int64_t arena_bytes = 0;
@@ -470,7 +471,18 @@ struct vg_ctx {
     std::vector<int32_t *> d_u, d_v;
     std::vector<fe *> cv;
     fe *chal = nullptr;         // challenges
-    acc4 *amb = nullptr, *aVb = nullptr;
+    fe *amb = nullptr, *aVb = nullptr;
+    // CSR transposes of every gate layer (circuit structure only, built once)
+    std::vector<std::vector<uint32_t>> h_off1, h_off2;
+    std::vector<std::vector<int32_t>> h_ent1, h_ent2, h_z2;
+    std::vector<uint32_t *> off1, off2;
+    std::vector<int32_t *> ent1, ent2, z2;
+    // eq half tables of every layer: 8 per layer (g0f g0s g1f g1s uf us vf vs) + 2 for q
+    fe *ht = nullptr;
+    int64_t ht_stride = 0;
+    launch::half_desc *hdesc = nullptr;
+    int n_hdesc = 0;
+    int64_t ht_total = 0;
     lpoly *tabs[6] = {};
     fe *part = nullptr;
     fe *bt[8] = {};             // beta half tables: g0f g0s g1f g1s uf us vf vs
@@ -502,7 +514,7 @@ struct vg_ctx {
     std::vector<int64_t> msg_layer;                   // offset of each layer's block in msg
     int64_t msg_vres = 0, msg_allsum = 0;
     fe post[5];
-    hipEvent_t ev[8] = {};      // stage boundaries
+    hipEvent_t ev[10] = {};     // stage boundaries
     hipEvent_t ev_leaf[2] = {}; // around the leaf-chain kernel
     std::chrono::steady_clock::time_point t_start;
     bool pending = false;
@@ -539,10 +551,22 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     }
     c->chal = carve<fe>(base, off, c->n_chal);
     int64_t T = (int64_t)1 << maxbl;
-    c->amb = carve<acc4>(base, off, T);
-    c->aVb = carve<acc4>(base, off, T);
+    c->amb = carve<fe>(base, off, T);
+    c->aVb = carve<fe>(base, off, T);
+    c->off1.assign(D, nullptr); c->off2.assign(D, nullptr);
+    c->ent1.assign(D, nullptr); c->ent2.assign(D, nullptr); c->z2.assign(D, nullptr);
+    for (int i = 1; i < D; ++i) {
+        c->off1[i] = carve<uint32_t>(base, off, (int64_t)c->h_off1[i].size());
+        c->off2[i] = carve<uint32_t>(base, off, (int64_t)c->h_off2[i].size());
+        c->ent1[i] = carve<int32_t>(base, off, (int64_t)c->h_ent1[i].size() + 1);
+        c->ent2[i] = carve<int32_t>(base, off, (int64_t)c->h_ent2[i].size() + 1);
+        c->z2[i] = carve<int32_t>(base, off, (int64_t)c->h_z2[i].size() + 1);
+    }
+    c->ht_stride = (int64_t)1 << ((maxbl >> 1) + 1);
+    c->ht = carve<fe>(base, off, (int64_t)D * 8 * c->ht_stride + 2 * ((int64_t)1 << (C.bl[0] / 2 + 1)));
+    c->hdesc = carve<launch::half_desc>(base, off, (int64_t)D * 8 + 2);
     for (int k = 0; k < 6; ++k) c->tabs[k] = carve<lpoly>(base, off, T / 2 > 0 ? T / 2 : 1);
-    c->part = carve<fe>(base, off, 6 * 1024);
+    c->part = carve<fe>(base, off, 18 * 1024);
     int half = (maxbl >> 1) + 1;
     for (int k = 0; k < 8; ++k) c->bt[k] = carve<fe>(base, off, (int64_t)1 << half);
     c->msg = carve<fe>(base, off, c->msg_len);
@@ -600,23 +624,19 @@ int prepare(vg_ctx *c)
     build_schedule(C, c->S);
     schedule &S = c->S;
     int D = C.depth;
-    // challenge buffer
-    std::vector<fe> ch;
-    c->chal_one = (int64_t)ch.size(); ch.push_back(fone()); ch.push_back(fzero());
-    c->chal_r0 = (int64_t)ch.size(); ch.insert(ch.end(), S.r0.begin(), S.r0.end());
-    c->chal_r1 = (int64_t)ch.size(); ch.insert(ch.end(), S.r1.begin(), S.r1.end());
-    c->chal_ru.assign(D, 0); c->chal_rv.assign(D, 0); c->chal_ab.assign(D, 0);
+    // challenge buffer (offsets fixed by chal_layout)
+    fe *ch = c->h_chal;
+    ch[c->chal_one] = fone(); ch[c->chal_one + 1] = fzero();
+    for (size_t k = 0; k < S.r0.size(); ++k) { ch[c->chal_r0 + k] = S.r0[k]; ch[c->chal_r1 + k] = S.r1[k]; }
     for (int i = D - 1; i >= 1; --i) {
         layer_sched &l = S.L[i];
-        c->chal_ru[i] = (int64_t)ch.size(); ch.insert(ch.end(), l.r_u.begin(), l.r_u.end());
-        c->chal_rv[i] = (int64_t)ch.size();
-        for (int k = 0; k < l.bl; ++k) ch.push_back(i == 1 ? fzero() : l.r_v[k]);
-        c->chal_ab[i] = (int64_t)ch.size(); ch.push_back(l.alpha); ch.push_back(l.beta);
+        for (int k = 0; k < l.bl; ++k) {
+            ch[c->chal_ru[i] + k] = l.r_u[k];
+            ch[c->chal_rv[i] + k] = i == 1 ? fzero() : l.r_v[k];
+        }
+        ch[c->chal_ab[i]] = l.alpha; ch[c->chal_ab[i] + 1] = l.beta;
     }
-    c->chal_fri = (int64_t)ch.size(); ch.insert(ch.end(), S.fri_r.begin(), S.fri_r.end());
-    if ((int64_t)ch.size() + 2 * c->m != c->n_chal) FAILV(-30, "challenge buffer size mismatch");
-    memcpy(c->h_chal, ch.data(), sizeof(fe) * ch.size());
-
+    for (size_t k = 0; k < S.fri_r.size(); ++k) ch[c->chal_fri + k] = S.fri_r[k];
     // host parts of the prover messages (the mask bookkeeping)
     mask_machine M;
     std::vector<fe> pub;
@@ -655,8 +675,8 @@ int prepare(vg_ctx *c)
     for (size_t k = 0; k < M.pri.size() && (int64_t)k < c->m; ++k) prim[k] = M.pri[k];
     for (size_t k = 0; k < pub.size() && (int64_t)k < c->m; ++k) pubm[k] = pub[k];
     // pri and pub masks ride along with the challenges
-    memcpy(c->h_chal + ch.size(), prim.data(), sizeof(fe) * c->m);
-    memcpy(c->h_chal + ch.size() + c->m, pubm.data(), sizeof(fe) * c->m);
+    memcpy(c->h_chal + c->n_chal - 2 * c->m, prim.data(), sizeof(fe) * c->m);
+    memcpy(c->h_chal + c->n_chal - c->m, pubm.data(), sizeof(fe) * c->m);
     return 0;
 }
 
@@ -721,16 +741,22 @@ int build_gather(vg_ctx *c)
 
 int enqueue(vg_ctx *c)
 {
-    hipStream_t st = c->st;
+    hipStream_t st = c->st, st2 = c->st2;
     host_circuit &C = c->C;
     int D = C.depth;
     HIPCHK(hipEventRecord(c->ev[0], st));
     HIPCHK(hipMemcpyAsync(c->chal, c->h_chal, sizeof(fe) * c->n_chal, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(c->gsrc, c->h_gsrc, sizeof(uint64_t) * c->n_gather, hipMemcpyHostToDevice, st));
+    // every eq half table of the proof, from the uploaded challenges
+    launch::half_multi(c->hdesc, c->n_hdesc, c->ht_total, st);
     // ---- evaluate (zk_prover.cpp:54-157) ----
     launch::eval_input(c->d_u[0], c->cv[0], 1 << C.bl[0], st);
     for (int i = 1; i < D; ++i) launch::eval_layer({c->d_ty[i], c->d_u[i], c->d_v[i], C.bl[i]}, c->cv[i - 1], c->cv[i], st);
     HIPCHK(hipEventRecord(c->ev[1], st));
+    // The commitment of the input layer needs only the input values, the
+    // host-known masks and challenges, never a sumcheck result: it runs on the
+    // second stream beside the latency-bound sumcheck rounds.
+    HIPCHK(hipStreamWaitEvent(st2, c->ev[1], 0));
     // ---- V_res (zk_prover.cpp:30-52) ----
     launch::mle_eval(c->cv[D - 1], C.bl[D - 1], c->chal + c->chal_r0, c->mle_t[0], c->mle_t[1], c->msg + c->msg_vres, st);
     // ---- GKR layers ----
@@ -746,30 +772,24 @@ int enqueue(vg_ctx *c)
         fe *M = c->msg + c->msg_layer[i];
         fe *sums1 = M, *end1 = M + 3 * bl, *sums2 = M + 3 * bl + 5, *end2 = M + 6 * bl + 5, *gv = M + 6 * bl + 10;
         gates_dev gd = {c->d_ty[i], c->d_u[i], c->d_v[i], bg};
-        launch::half_tables(c->bt[0], c->bt[1], alpha, r0, bg, st);
-        launch::half_tables(c->bt[2], c->bt[3], beta, r1, bg, st);
-        launch::beta_g BG = {c->bt[0], c->bt[1], c->bt[2], c->bt[3], bg >> 1};
-        int64_t T = (int64_t)1 << bl;
-        launch::zero(c->amb, sizeof(acc4) * T, st);
-        launch::zero(c->aVb, sizeof(acc4) * T, st);
-        launch::p1_scatter(gd, c->cv[i - 1], BG, c->amb, c->aVb, st);
-        launch::sumcheck_phase(c->cv[i - 1], c->aVb, c->amb, bl, ru, c->LH[i].vu_const, ws, sums1, end1, st);
-        launch::half_tables(c->bt[4], c->bt[5], nullptr, ru, bl, st);
-        launch::beta_u BU = {c->bt[4], c->bt[5], bl >> 1};
-        launch::zero(c->amb, sizeof(acc4) * T, st);
-        launch::zero(c->aVb, sizeof(acc4) * T, st);
-        launch::p2_scatter(gd, BG, BU, end1 + 4, c->amb, c->aVb, st);
-        launch::sumcheck_phase(c->cv[i - 1], c->aVb, c->amb, bl, rv, c->LH[i].vv_const, ws, sums2, end2, st);
-        if (i > 1) {
-            launch::half_tables(c->bt[6], c->bt[7], nullptr, rv, bl, st);
-            launch::beta_u BV = {c->bt[6], c->bt[7], bl >> 1};
-            launch::general_value(gd, BG, BU, BV, end1 + 4, end2 + 4, c->part, nullptr, gv, st);
-        }
+        fe *t = c->ht + (int64_t)i * 8 * c->ht_stride;
+        int64_t hs = c->ht_stride;
+        launch::beta_g BG = {t, t + hs, t + 2 * hs, t + 3 * hs, bg >> 1};
+        launch::beta_u BU = {t + 4 * hs, t + 5 * hs, bl >> 1};
+        launch::beta_u BV = {t + 6 * hs, t + 7 * hs, bl >> 1};
+        int nt = 1 << bl;
+        launch::p1_gather(c->off1[i], c->ent1[i], gd, nt, c->cv[i - 1], BG, c->amb, c->aVb, st);
+        launch::sumcheck_phase2(c->cv[i - 1], c->aVb, c->amb, bl, ru, c->LH[i].vu_const, ws, sums1, end1, st);
+        launch::p2_gather(c->off2[i], c->ent2[i], c->z2[i], (int)c->h_z2[i].size(), gd, nt, BG, BU, end1 + 4, c->amb, c->aVb,
+                          c->part, st);
+        launch::sumcheck_phase2(c->cv[i - 1], c->aVb, c->amb, bl, rv, c->LH[i].vv_const, ws, sums2, end2, st);
+        if (i > 1) launch::general_value(gd, BG, BU, BV, end1 + 4, end2 + 4, c->part, nullptr, gv, st);
         r0 = ru; r1 = rv;
         alpha = c->chal + c->chal_ab[i];
         beta = alpha + 1;
     }
     HIPCHK(hipEventRecord(c->ev[2], st));
+    st = st2;
     // ---- commit_private_array (poly_commit.h:42-124) ----
     int n = c->n;
     int64_t SS = c->SS, real = c->real, m = c->m, Cc = c->C_;
@@ -782,8 +802,10 @@ int enqueue(vg_ctx *c)
     launch::lde(c->coefP, Cc, c->lens, 65, c->l, c->log_C, c->log_R, c->tw, SS, st);
     HIPCHK(hipEventRecord(c->ev[3], st));
     // ---- commit_public_array (poly_commit.h:125-351) ----
-    const fe *rlast = c->chal + c->chal_ru[1];
-    launch::eq_table(c->qv, rlast, n, c->eqs, st);
+    {
+        fe *q = c->ht + (int64_t)D * 8 * c->ht_stride;
+        launch::eq_expand(c->qv, q, q + ((int64_t)1 << (n >> 1)), n >> 1, (int64_t)1 << n, st);
+    }
     launch::intt({0, c->qv, real, nullptr, nullptr, 0, 0, 0, 0}, c->coefQ, Cc, 64, lr, inv_real, c->tw, SS, st);
     launch::intt({0, pub_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefQ + 64 * Cc, Cc, 1, lm, inv_m, c->tw, SS, st);
     launch::lde(c->coefQ, Cc, c->lens, 65, c->q, c->log_C, c->log_R, c->tw, SS, st);
@@ -825,6 +847,9 @@ int enqueue(vg_ctx *c)
         launch::merkle_multi(ns, trees, nl, st);
     }
     HIPCHK(hipEventRecord(c->ev[6], st));
+    st = c->st;
+    HIPCHK(hipStreamWaitEvent(st, c->ev[6], 0));
+    HIPCHK(hipEventRecord(c->ev[8], st));
     // ---- query answers (fri.cpp:148-289) ----
     launch::gather16(c->gsrc, c->n_gather, c->gout, st);
     HIPCHK(hipMemcpyAsync(c->h_gout, c->gout, sizeof(fe) * c->n_gather, hipMemcpyDeviceToHost, st));
@@ -880,6 +905,49 @@ void assemble(vg_ctx *c, tbuf &T)
     T.put(go + k, (size_t)16 * (c->n_gather - k));
 }
 
+// CSR transposes of one gate layer (the structure of zk_prover.cpp:420-587
+// and :845-972, computed once per circuit): phase 1 lists, for every entry u
+// of the previous layer, the gates that add into it through u (a sum gate
+// into every u of its range); phase 2 lists by v the gates that add into
+// entry v, and separately the gates that always add into entry 0.
+void build_csr(vg_ctx *c, int i)
+{
+    const host_circuit &C = c->C;
+    int64_t nt = (int64_t)1 << C.bl[i - 1];
+    int64_t ng = (int64_t)C.ty[i].size();
+    std::vector<uint32_t> &o1 = c->h_off1[i], &o2 = c->h_off2[i];
+    std::vector<int32_t> &e1 = c->h_ent1[i], &e2 = c->h_ent2[i], &z = c->h_z2[i];
+    o1.assign(nt + 1, 0); o2.assign(nt + 1, 0);
+    const uint8_t *ty = C.ty[i].data();
+    const int32_t *u = C.u[i].data(), *v = C.v[i].data();
+    auto p1_range = [&](int64_t g, int64_t &lo, int64_t &hi) {
+        switch (ty[g]) {
+        case 0: case 1: case 4: case 6: case 7: case 8: case 9: case 10: case 13: lo = u[g]; hi = u[g] + 1; break;
+        case 5: lo = u[g]; hi = v[g]; break;
+        case 12: lo = u[g]; hi = (int64_t)v[g] + 1; break;
+        default: lo = hi = 0; break;
+        }
+    };
+    auto p2_by_v = [&](int64_t g) { int t = ty[g]; return t == 0 || t == 1 || t == 7 || t == 8 || t == 9 || t == 13; };
+    auto p2_zero = [&](int64_t g) { int t = ty[g]; return t == 5 || t == 12 || t == 6 || t == 10; };
+    for (int64_t g = 0; g < ng; ++g) {
+        int64_t lo, hi;
+        p1_range(g, lo, hi);
+        for (int64_t t = lo; t < hi; ++t) o1[t + 1]++;
+        if (p2_by_v(g)) o2[v[g] + 1]++;
+    }
+    for (int64_t t = 0; t < nt; ++t) { o1[t + 1] += o1[t]; o2[t + 1] += o2[t]; }
+    e1.assign(o1[nt], 0); e2.assign(o2[nt], 0); z.clear();
+    std::vector<uint32_t> c1(o1.begin(), o1.end() - 1), c2(o2.begin(), o2.end() - 1);
+    for (int64_t g = 0; g < ng; ++g) {
+        int64_t lo, hi;
+        p1_range(g, lo, hi);
+        for (int64_t t = lo; t < hi; ++t) e1[c1[t]++] = (int32_t)g;
+        if (p2_by_v(g)) e2[c2[v[g]]++] = (int32_t)g;
+        if (p2_zero(g)) z.push_back((int32_t)g);
+    }
+}
+
 int ctx_init(vg_ctx *c)
 {
     host_circuit &C = c->C;
@@ -900,10 +968,22 @@ int ctx_init(vg_ctx *c)
     c->log_C = ilog2(c->C_);
     c->log_R = c->log_SS - c->log_C;
     c->LQ = 2 * c->C_;
-    // challenge buffer size: ones/zero + r0 + r1 + per layer (2 bl + 2) + fri + masks
-    int64_t nc = 2 + 2 * (int64_t)C.bl[C.depth - 1] + (n - 6);
-    for (int i = 1; i < C.depth; ++i) nc += 2 * C.bl[i - 1] + 2;
-    c->n_chal = nc + 2 * c->m;
+    // challenge buffer layout: 1, 0 | r_0 | r_1 | per layer r_u, r_v, alpha, beta | fri | pri masks | pub masks
+    {
+        int64_t k = 0;
+        int bl_out = C.bl[C.depth - 1];
+        c->chal_one = k; k += 2;
+        c->chal_r0 = k; k += bl_out;
+        c->chal_r1 = k; k += bl_out;
+        c->chal_ru.assign(C.depth, 0); c->chal_rv.assign(C.depth, 0); c->chal_ab.assign(C.depth, 0);
+        for (int i = C.depth - 1; i >= 1; --i) {
+            c->chal_ru[i] = k; k += C.bl[i - 1];
+            c->chal_rv[i] = k; k += C.bl[i - 1];
+            c->chal_ab[i] = k; k += 2;
+        }
+        c->chal_fri = k; k += n - 6;
+        c->n_chal = k + 2 * c->m;
+    }
     // device message block
     c->msg_layer.assign(C.depth, 0);
     int64_t ml = 0;
@@ -912,6 +992,9 @@ int ctx_init(vg_ctx *c)
     c->msg_allsum = ml; ml += 65;
     c->msg_len = ml;
     c->gather_cap = gather_count(n);
+    c->h_off1.assign(C.depth, {}); c->h_off2.assign(C.depth, {});
+    c->h_ent1.assign(C.depth, {}); c->h_ent2.assign(C.depth, {}); c->h_z2.assign(C.depth, {});
+    for (int i = 1; i < C.depth; ++i) build_csr(c, i);
     int64_t bytes = layout(c, nullptr);
     HIPCHK(hipMalloc(&c->arena, bytes));
     c->arena_bytes = bytes;
@@ -930,6 +1013,47 @@ int ctx_init(vg_ctx *c)
             HIPCHK(hipMemcpy(c->d_ty[i], C.ty[i].data(), nn, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(c->d_v[i], C.v[i].data(), 4 * nn, hipMemcpyHostToDevice));
         }
+    }
+    for (int i = 1; i < C.depth; ++i) {
+        HIPCHK(hipMemcpy(c->off1[i], c->h_off1[i].data(), 4 * c->h_off1[i].size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->off2[i], c->h_off2[i].data(), 4 * c->h_off2[i].size(), hipMemcpyHostToDevice));
+        if (!c->h_ent1[i].empty()) HIPCHK(hipMemcpy(c->ent1[i], c->h_ent1[i].data(), 4 * c->h_ent1[i].size(), hipMemcpyHostToDevice));
+        if (!c->h_ent2[i].empty()) HIPCHK(hipMemcpy(c->ent2[i], c->h_ent2[i].data(), 4 * c->h_ent2[i].size(), hipMemcpyHostToDevice));
+        if (!c->h_z2[i].empty()) HIPCHK(hipMemcpy(c->z2[i], c->h_z2[i].data(), 4 * c->h_z2[i].size(), hipMemcpyHostToDevice));
+    }
+    {
+        // eq half tables of every layer, computed in one launch per proof
+        // from the uploaded challenges (zk_prover.cpp:391-418, 802-822)
+        std::vector<launch::half_desc> hd;
+        int64_t tot = 0;
+        auto addd = [&](fe *out, const fe *f0, const fe *r, int nbits) {
+            launch::half_desc d; d.out = out; d.f0 = f0; d.r = r; d.nbits = nbits; d.start = tot;
+            hd.push_back(d); tot += (int64_t)1 << nbits;
+        };
+        const fe *r0 = c->chal + c->chal_r0, *r1 = c->chal + c->chal_r1;
+        const fe *al = c->chal + c->chal_one, *be = al + 1;
+        for (int i = C.depth - 1; i >= 1; --i) {
+            int bg = C.bl[i], bu = C.bl[i - 1];
+            fe *t = c->ht + (int64_t)i * 8 * c->ht_stride;
+            const fe *ru = c->chal + c->chal_ru[i], *rv = c->chal + c->chal_rv[i];
+            addd(t + 0 * c->ht_stride, al, r0, bg >> 1);
+            addd(t + 1 * c->ht_stride, nullptr, r0 + (bg >> 1), bg - (bg >> 1));
+            addd(t + 2 * c->ht_stride, be, r1, bg >> 1);
+            addd(t + 3 * c->ht_stride, nullptr, r1 + (bg >> 1), bg - (bg >> 1));
+            addd(t + 4 * c->ht_stride, nullptr, ru, bu >> 1);
+            addd(t + 5 * c->ht_stride, nullptr, ru + (bu >> 1), bu - (bu >> 1));
+            addd(t + 6 * c->ht_stride, nullptr, rv, bu >> 1);
+            addd(t + 7 * c->ht_stride, nullptr, rv + (bu >> 1), bu - (bu >> 1));
+            r0 = ru; r1 = rv;
+            al = c->chal + c->chal_ab[i]; be = al + 1;
+        }
+        fe *q = c->ht + (int64_t)C.depth * 8 * c->ht_stride;
+        const fe *rl = c->chal + c->chal_ru[1];
+        addd(q, nullptr, rl, n >> 1);
+        addd(q + ((int64_t)1 << (n >> 1)), nullptr, rl + (n >> 1), n - (n >> 1));
+        c->n_hdesc = (int)hd.size();
+        c->ht_total = tot;
+        HIPCHK(hipMemcpy(c->hdesc, hd.data(), sizeof(launch::half_desc) * hd.size(), hipMemcpyHostToDevice));
     }
     std::vector<int> lens(65, (int)c->real);
     lens[64] = (int)c->m;
@@ -976,6 +1100,10 @@ int vg_ctx_create(vg_ctx **out, const vg_circuit_desc *d, void *stream)
         if (e != hipSuccess) { delete c; FAILV(-20, "hipStreamCreate: %s", hipGetErrorString(e)); }
         c->own_stream = true;
     }
+    {
+        hipError_t e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
+        if (e != hipSuccess) { vg_ctx_destroy(c); FAILV(-20, "hipStreamCreate: %s", hipGetErrorString(e)); }
+    }
     rc = ctx_init(c);
     if (rc) { vg_ctx_destroy(c); return rc; }
     *out = c;
@@ -986,6 +1114,7 @@ int vg_ctx_destroy(vg_ctx *c)
 {
     if (!c) return 0;
     if (c->st) (void)hipStreamSynchronize(c->st);
+    if (c->st2) (void)hipStreamSynchronize(c->st2);
     if (c->arena) (void)hipFree(c->arena);
     if (c->h_chal) (void)hipHostFree(c->h_chal);
     if (c->h_gsrc) (void)hipHostFree(c->h_gsrc);
@@ -994,6 +1123,7 @@ int vg_ctx_destroy(vg_ctx *c)
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto &e : c->ev_leaf) if (e) (void)hipEventDestroy(e);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
+    if (c->st2) (void)hipStreamDestroy(c->st2);
     delete c;
     return 0;
 }
@@ -1029,14 +1159,14 @@ int vg_finish(vg_ctx *c, uint8_t **transcript, size_t *len, double *stage_ms)
     *transcript = p;
     *len = T.b.size();
     if (stage_ms) {
-        for (int k = 0; k < 7; ++k) {
+        // evaluate, gkr | lde_l, lde_qh, fri_fold, hash (second stream) | queries, leaf kernel, total
+        static const int pairs[9][2] = {{0, 1}, {1, 2}, {1, 3}, {3, 4}, {4, 5}, {5, 6}, {8, 7}, {-1, -1}, {0, 7}};
+        for (int k = 0; k < 9; ++k) {
             float ms = 0;
-            (void)hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]);
+            if (pairs[k][0] < 0) (void)hipEventElapsedTime(&ms, c->ev_leaf[0], c->ev_leaf[1]);
+            else (void)hipEventElapsedTime(&ms, c->ev[pairs[k][0]], c->ev[pairs[k][1]]);
             stage_ms[k] = ms;
         }
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, c->ev_leaf[0], c->ev_leaf[1]);
-        stage_ms[7] = ms;
     }
     return 0;
 }

@@ -13,7 +13,7 @@ constexpr int TPB = 256;
 constexpr int NTT_LM = 11;          // LDS-resident NTT block: 2^11 elements = 32 KB
 constexpr int FOLD_TAIL = 512;      // fold rounds with <= this many outputs run in one block
 constexpr int TAIL_T = 256;         // threads of that block
-constexpr int MAX_PARTS = 1024;     // max blocks of a fold round (partials per round)
+constexpr int MAX_PARTS = 1024;     // max blocks of a reduction with per-block partials
 
 __host__ __device__ inline unsigned nblk(int64_t n, int t = TPB) { return (unsigned)((n + t - 1) / t); }
 
@@ -492,23 +492,107 @@ __global__ __launch_bounds__(TPB) void k_fold_k(int j0, const fe *Vb, const fe *
             for (int c = 0; c < 3; ++c) part[(blockIdx.x * K + q) * 3 + c] = acc[q][c];
     }
 }
-// Remaining rounds of a phase in one block, tables in LDS.  X holds the
-// input of round j0 (n_in entries, n_in <= TAIL_IN), or the flat tables if j0 == 0.
-constexpr int TAIL_IN = 512;
-__global__ __launch_bounds__(256) void k_fold_tail_lds(int j0, int L, const fe *Vb, const fe *aVb, const fe *amb, ctabs X,
-                                                      const fe *r, fe vconst, const fe *prev_part, int nprev, int kprev,
-                                                      fe *prev_dst, fe *sums, fe *end)
+
+// ---------------------------------------------------------------------
+// Wave-cooperative fold of K rounds (K <= 6): lane gi folds the input pair
+// (2gi, 2gi+1) of round j0; for each later round the lanes combine their
+// neighbours' results with __shfl_down, so only round j0+K-1's table is
+// written.  Per-round (a, b, c) sums go to per-block partials [block][K][3].
+// ---------------------------------------------------------------------
+__device__ __forceinline__ fe shfl_down_fe(const fe &x, int d)
 {
-    __shared__ lpoly sV[TAIL_IN], saV[TAIL_IN], sam[TAIL_IN];
-    if (nprev) {
-        for (int q = 0; q < kprev; ++q) {
-            fe acc[3] = {fzero(), fzero(), fzero()};
-            for (int k = threadIdx.x; k < nprev; k += blockDim.x)
-                for (int c = 0; c < 3; ++c) acc[c] = add(acc[c], prev_part[(k * kprev + q) * 3 + c]);
-            block_sum<3>(acc);
-            if (threadIdx.x == 0) { prev_dst[3 * q] = acc[0]; prev_dst[3 * q + 1] = acc[1]; prev_dst[3 * q + 2] = acc[2]; }
+    fe y;
+    y.real = __shfl_down((unsigned long long)x.real, d);
+    y.img = __shfl_down((unsigned long long)x.img, d);
+    return y;
+}
+__device__ __forceinline__ lpoly shfl_down_lp(const lpoly &x, int d)
+{
+    lpoly y;
+    y.a = shfl_down_fe(x.a, d);
+    y.b = shfl_down_fe(x.b, d);
+    return y;
+}
+__device__ __forceinline__ lpoly fold_pair(const lpoly &y0, const lpoly &y1, const fe &r)
+{
+    lpoly z;
+    z.b = add(mul(y0.a, r), y0.b);
+    z.a = sub(add(mul(y1.a, r), y1.b), z.b);
+    return z;
+}
+template <int K>
+__global__ __launch_bounds__(TPB) void k_fold_wave(int j0, const fe *Vb, const fe *aVb, const fe *amb, ctabs in,
+                                                  const fe *r, tabs out, int64_t n0, fe *part)
+{
+    int lane = threadIdx.x & 63;
+    int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    int64_t nchunks = n0 >> 6;
+    fe acc[K][3];
+#pragma unroll
+    for (int q = 0; q < K; ++q) acc[q][0] = acc[q][1] = acc[q][2] = fzero();
+    fe r0 = j0 ? r[j0 - 1] : fzero();
+    for (int64_t chunk = wave; chunk < nchunks; chunk += nwaves) {     // uniform per wave
+        int64_t gi = (chunk << 6) + lane;
+        lpoly V, aV, am;
+        if (j0 == 0) {
+            fe x0 = Vb[2 * gi], x1 = Vb[2 * gi + 1];
+            V.b = x0; V.a = sub(x1, x0);
+            x0 = aVb[2 * gi]; x1 = aVb[2 * gi + 1];
+            aV.b = x0; aV.a = sub(x1, x0);
+            x0 = amb[2 * gi]; x1 = amb[2 * gi + 1];
+            am.b = x0; am.a = sub(x1, x0);
+        } else {
+            V = fold_pair(in.V[2 * gi], in.V[2 * gi + 1], r0);
+            aV = fold_pair(in.aV[2 * gi], in.aV[2 * gi + 1], r0);
+            am = fold_pair(in.am[2 * gi], in.am[2 * gi + 1], r0);
+        }
+        fold_acc(acc[0], V, aV, am);
+#pragma unroll
+        for (int q = 1; q < K; ++q) {
+            lpoly V2 = shfl_down_lp(V, 1 << (q - 1));
+            lpoly aV2 = shfl_down_lp(aV, 1 << (q - 1));
+            lpoly am2 = shfl_down_lp(am, 1 << (q - 1));
+            if ((lane & ((1 << q) - 1)) == 0) {
+                fe rr = r[j0 + q - 1];
+                V = fold_pair(V, V2, rr);
+                aV = fold_pair(aV, aV2, rr);
+                am = fold_pair(am, am2, rr);
+                fold_acc(acc[q], V, aV, am);
+            }
+        }
+        if ((lane & ((1 << (K - 1)) - 1)) == 0) {
+            int64_t o = gi >> (K - 1);
+            out.V[o] = V; out.aV[o] = aV; out.am[o] = am;
         }
     }
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        block_sum<3>(acc[q]);
+        if (threadIdx.x == 0)
+            for (int c = 0; c < 3; ++c) part[(int64_t)(q * 3 + c) * launch::FW_BLOCKS + blockIdx.x] = acc[q][c];
+    }
+}
+// round sums of every wave-fold launch of the proof: slot R (a round) holds
+// 3 x FW_BLOCKS block partials; nb[R] of them are live; dst[R] indexes msg.
+__global__ void k_reduce_slots(const fe *part, const int *nb, const int64_t *dst, fe *msg)
+{
+    int R = blockIdx.x;
+    fe acc[3] = {fzero(), fzero(), fzero()};
+    for (int b = threadIdx.x; b < nb[R]; b += blockDim.x)
+        for (int c = 0; c < 3; ++c) acc[c] = add(acc[c], part[((int64_t)R * 3 + c) * launch::FW_BLOCKS + b]);
+    block_sum<3>(acc);
+    if (threadIdx.x == 0) { msg[dst[R]] = acc[0]; msg[dst[R] + 1] = acc[1]; msg[dst[R] + 2] = acc[2]; }
+}
+// Remaining rounds of a phase in one block, tables in LDS (input of round j0:
+// 2^(L-j0) <= TAIL_IN entries, or the flat tables if j0 == 0).  Rounds with
+// more than 64 outputs use the block; the last ones run in wave 0 alone,
+// folding through shuffles with no barriers.
+constexpr int TAIL_IN = 512;
+__global__ __launch_bounds__(256) void k_fold_tail_lds(int j0, int L, const fe *Vb, const fe *aVb, const fe *amb, ctabs X,
+                                                      const fe *r, fe vconst, fe *sums, fe *end)
+{
+    __shared__ lpoly sV[TAIL_IN], saV[TAIL_IN], sam[TAIL_IN];
     int n_in = 1 << (L - j0);
     for (int i = threadIdx.x; i < n_in; i += blockDim.x) {
         if (j0 == 0) {
@@ -520,7 +604,8 @@ __global__ __launch_bounds__(256) void k_fold_tail_lds(int j0, int L, const fe *
         }
     }
     __syncthreads();
-    for (int j = j0; j < L; ++j) {
+    int j = j0;
+    for (; j < L && (1 << (L - 1 - j)) > 64; ++j) {
         int n_out = 1 << (L - 1 - j);
         bool flat = j == 0;
         fe rr = flat ? fzero() : r[j - 1];
@@ -531,7 +616,7 @@ __global__ __launch_bounds__(256) void k_fold_tail_lds(int j0, int L, const fe *
             int i = threadIdx.x;
 #define VG_TFOLD(S, N) { lpoly y0 = S[2 * i], y1 = S[2 * i + 1]; \
                 if (flat) { N.b = y0.b; N.a = sub(y1.b, y0.b); } \
-                else { N.b = add(mul(y0.a, rr), y0.b); N.a = sub(add(mul(y1.a, rr), y1.b), N.b); } }
+                else N = fold_pair(y0, y1, rr); }
             VG_TFOLD(sV, nV) VG_TFOLD(saV, naV) VG_TFOLD(sam, nam)
 #undef VG_TFOLD
             fold_acc(acc, nV, naV, nam);
@@ -541,10 +626,42 @@ __global__ __launch_bounds__(256) void k_fold_tail_lds(int j0, int L, const fe *
         block_sum<3>(acc);
         if (threadIdx.x == 0) { sums[3 * j] = acc[0]; sums[3 * j + 1] = acc[1]; sums[3 * j + 2] = acc[2]; }
     }
-    if (threadIdx.x == 0) {
-        lpoly V0 = sV[0], am0 = sam[0];
-        end[0] = V0.a; end[1] = V0.b; end[2] = am0.a; end[3] = am0.b;
-        end[4] = add(add(mul(V0.a, r[L - 1]), V0.b), vconst);
+    if (threadIdx.x >= 64) return;
+    // wave rounds: after local round q lane i holds entry i >> q (valid if i % 2^q == 0)
+    int lane = threadIdx.x;
+    lpoly V, aV, am;
+    for (int q = 0; j < L; ++j, ++q) {
+        int n_out = 1 << (L - 1 - j);
+        bool flat = j == 0;
+        fe rr = flat ? fzero() : r[j - 1];
+        fe acc[3] = {fzero(), fzero(), fzero()};
+        if (q == 0) {
+            if (lane < n_out) {
+                lpoly y0 = sV[2 * lane], y1 = sV[2 * lane + 1];
+                if (flat) { V.b = y0.b; V.a = sub(y1.b, y0.b); } else V = fold_pair(y0, y1, rr);
+                y0 = saV[2 * lane]; y1 = saV[2 * lane + 1];
+                if (flat) { aV.b = y0.b; aV.a = sub(y1.b, y0.b); } else aV = fold_pair(y0, y1, rr);
+                y0 = sam[2 * lane]; y1 = sam[2 * lane + 1];
+                if (flat) { am.b = y0.b; am.a = sub(y1.b, y0.b); } else am = fold_pair(y0, y1, rr);
+                fold_acc(acc, V, aV, am);
+            }
+        } else {
+            lpoly V2 = shfl_down_lp(V, 1 << (q - 1));
+            lpoly aV2 = shfl_down_lp(aV, 1 << (q - 1));
+            lpoly am2 = shfl_down_lp(am, 1 << (q - 1));
+            if ((lane & ((1 << q) - 1)) == 0 && (lane >> q) < n_out) {
+                V = fold_pair(V, V2, rr);
+                aV = fold_pair(aV, aV2, rr);
+                am = fold_pair(am, am2, rr);
+                fold_acc(acc, V, aV, am);
+            }
+        }
+        for (int c = 0; c < 3; ++c) acc[c] = wave_sum(acc[c]);
+        if (lane == 0) { sums[3 * j] = acc[0]; sums[3 * j + 1] = acc[1]; sums[3 * j + 2] = acc[2]; }
+    }
+    if (lane == 0) {
+        end[0] = V.a; end[1] = V.b; end[2] = am.a; end[3] = am.b;
+        end[4] = add(add(mul(V.a, r[L - 1]), V.b), vconst);
     }
 }
 
@@ -1029,39 +1146,52 @@ void p2_gather(const uint32_t *off, const int32_t *ent, const int32_t *zlist, in
         hipLaunchKernelGGL(k_add_parts, dim3(1), dim3(1024), 0, s, part, (int)blocks, aVb);
     }
 }
-void sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *sums,
-                     fe *end, hipStream_t s)
+int fold_plan(int L, int *js, int *ks)
 {
-    int j = 0, cur = 0;
-    const fe *prev_part = nullptr;
-    int nprev = 0, kprev = 0;
-    fe *part[2] = {ws.part, ws.part + 9 * MAX_PARTS};
-    int launch_no = 0;
+    int n = 0, j = 0;
     while ((1 << (L - j)) > TAIL_IN) {
-        int K = L - j - 9;                       // rounds until the table fits the tail
-        if (K > 3) K = 3;
-        if (K < 1) K = 1;
-        int64_t n_out = (int64_t)1 << (L - j - K);
-        unsigned g = nblk(n_out);
-        if (g > MAX_PARTS) g = MAX_PARTS;
+        int K = L - j - 9;
+        if (K > 6) K = 6;
+        js[n] = j; ks[n] = K; ++n;
+        j += K;
+    }
+    return n;
+}
+int sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *slots,
+                    fe *sums, fe *end, hipStream_t s)
+{
+    int js[32], ks[32];
+    int nl = fold_plan(L, js, ks);
+    int cur = 0, rounds = 0;
+    for (int k = 0; k < nl; ++k) {
+        int j = js[k], K = ks[k];
+        int64_t n0 = (int64_t)1 << (L - j - 1);
+        int64_t g64 = n0 / TPB;
+        unsigned g = (unsigned)(g64 > FW_BLOCKS ? FW_BLOCKS : g64);
         ctabs in = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
         tabs out = {ws.V[cur], ws.aV[cur], ws.am[cur]};
-        fe *pp = part[launch_no & 1];
-        fe *pdst = j > 0 ? sums + 3 * (j - kprev) : sums;
+        fe *pp = slots + (int64_t)rounds * 3 * FW_BLOCKS;
         switch (K) {
-        case 1: hipLaunchKernelGGL(k_fold_k<1>, dim3(g), dim3(TPB), 0, s, j, Vb, aVb, amb, in, r, out, n_out, pp, prev_part, nprev, kprev, pdst); break;
-        case 2: hipLaunchKernelGGL(k_fold_k<2>, dim3(g), dim3(TPB), 0, s, j, Vb, aVb, amb, in, r, out, n_out, pp, prev_part, nprev, kprev, pdst); break;
-        default: hipLaunchKernelGGL(k_fold_k<3>, dim3(g), dim3(TPB), 0, s, j, Vb, aVb, amb, in, r, out, n_out, pp, prev_part, nprev, kprev, pdst); break;
+#define VG_FW(k) case k: hipLaunchKernelGGL(k_fold_wave<k>, dim3(g), dim3(TPB), 0, s, j, Vb, aVb, amb, in, r, out, n0, pp); break;
+        VG_FW(1) VG_FW(2) VG_FW(3) VG_FW(4) VG_FW(5) VG_FW(6)
+#undef VG_FW
         }
-        prev_part = pp; nprev = (int)g; kprev = K;
         cur ^= 1;
-        j += K;
-        ++launch_no;
+        rounds += K;
     }
+    int j = nl ? js[nl - 1] + ks[nl - 1] : 0;
     ctabs X = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
-    fe *pdst = j > 0 ? sums + 3 * (j - kprev) : sums;
-    hipLaunchKernelGGL(k_fold_tail_lds, dim3(1), dim3(256), 0, s, j, L, Vb, aVb, amb, X, r, vconst, prev_part, nprev, kprev,
-                       pdst, sums, end);
+    hipLaunchKernelGGL(k_fold_tail_lds, dim3(1), dim3(256), 0, s, j, L, Vb, aVb, amb, X, r, vconst, sums, end);
+    return rounds;
+}
+int fold_blocks(int L, int j)
+{
+    int64_t g = ((int64_t)1 << (L - j - 1)) / TPB;
+    return (int)(g > FW_BLOCKS ? FW_BLOCKS : g);
+}
+void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s)
+{
+    if (nslots) hipLaunchKernelGGL(k_reduce_slots, dim3(nslots), dim3(256), 0, s, slots, nb, dst, msg);
 }
 void half_multi(const half_desc *d, int nd, int64_t total, hipStream_t s)
 {

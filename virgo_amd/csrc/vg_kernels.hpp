@@ -58,7 +58,8 @@ void p2_scatter(gates_dev g, beta_g bg, beta_u bu, const fe *vu, acc4 *amb, acc4
 
 struct fold_ws {              // workspace of a sumcheck phase
     lpoly *V[2], *aV[2], *am[2];
-    fe *part;                 // per-block partial sums (3 per block)
+    fe *part;                 // two halves of per-block partial sums
+    int64_t part_half;        // >= 18 * (2^(L-1) / 256) for the widest layer
     unsigned *counter;
 };
 // Runs every round of one sumcheck phase on tables of 2^L entries.
@@ -76,10 +77,18 @@ void p1_gather(const uint32_t *off, const int32_t *ent, gates_dev g, int nt, con
 // sum, exp-sum, NOT and relay gates, which all add into addV[0].
 void p2_gather(const uint32_t *off, const int32_t *ent, const int32_t *zlist, int nz, gates_dev g, int nt, beta_g bg,
                beta_u bu, const fe *vu, fe *amb, fe *aVb, fe *part, hipStream_t s);
-// all rounds of a phase on plain tables (as sumcheck_phase), folding up to 3
-// rounds per launch; ws.part needs 2 * 9 * 1024 entries.
-void sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *sums,
-                     fe *end, hipStream_t s);
+// All rounds of a phase on plain tables: up to 6 rounds per launch with wave
+// shuffles, then the last <= 9 rounds in one block.  Launch k of the plan
+// (fold_plan) writes its block partials of round q to
+// slots[(q*3 + c) * FW_BLOCKS + block]; the caller reduces every slot of the
+// proof with reduce_slots.  Tail round sums go straight to sums[3j..].
+// Returns the number of slot rounds used.
+constexpr int FW_BLOCKS = 512;
+int fold_plan(int L, int *js, int *ks);
+int fold_blocks(int L, int j);
+int sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *slots,
+                    fe *sums, fe *end, hipStream_t s);
+void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s);
 struct half_desc {            // one eq half table: out[j] = f0 * prod_{i<nbits} eq(bit_i(j), r[i])
     fe *out;
     const fe *f0;

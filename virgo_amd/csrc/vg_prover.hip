@@ -485,6 +485,14 @@ struct vg_ctx {
     int64_t ht_total = 0;
     lpoly *tabs[6] = {};
     fe *part = nullptr;
+    int64_t part_half = 0;
+    // round-sum slots of the wave folds (see launch::sumcheck_phase2)
+    fe *slots = nullptr;
+    int n_slots = 0;
+    int *slot_nb = nullptr;
+    int64_t *slot_dst = nullptr;
+    std::vector<int> h_slot_nb;
+    std::vector<int64_t> h_slot_dst;
     fe *bt[8] = {};             // beta half tables: g0f g0s g1f g1s uf us vf vs
     fe *msg = nullptr;          // device results of the GKR part
     fe *mle_t[2] = {};
@@ -518,6 +526,7 @@ struct vg_ctx {
     hipEvent_t ev_leaf[2] = {}; // around the leaf-chain kernel
     std::chrono::steady_clock::time_point t_start;
     bool pending = false;
+    bool serial = false;        // VIRGO_SERIAL=1: commitment after the sumcheck on one stream
 };
 
 namespace {
@@ -566,7 +575,11 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     c->ht = carve<fe>(base, off, (int64_t)D * 8 * c->ht_stride + 2 * ((int64_t)1 << (C.bl[0] / 2 + 1)));
     c->hdesc = carve<launch::half_desc>(base, off, (int64_t)D * 8 + 2);
     for (int k = 0; k < 6; ++k) c->tabs[k] = carve<lpoly>(base, off, T / 2 > 0 ? T / 2 : 1);
-    c->part = carve<fe>(base, off, 18 * 1024);
+    c->part_half = 18 * 1024;
+    c->part = carve<fe>(base, off, 2 * c->part_half);
+    c->slots = carve<fe>(base, off, (int64_t)(c->n_slots > 0 ? c->n_slots : 1) * 3 * launch::FW_BLOCKS);
+    c->slot_nb = carve<int>(base, off, c->n_slots + 1);
+    c->slot_dst = carve<int64_t>(base, off, c->n_slots + 1);
     int half = (maxbl >> 1) + 1;
     for (int k = 0; k < 8; ++k) c->bt[k] = carve<fe>(base, off, (int64_t)1 << half);
     c->msg = carve<fe>(base, off, c->msg_len);
@@ -763,9 +776,10 @@ int enqueue(vg_ctx *c)
     launch::fold_ws ws;
     ws.V[0] = c->tabs[0]; ws.aV[0] = c->tabs[1]; ws.am[0] = c->tabs[2];
     ws.V[1] = c->tabs[3]; ws.aV[1] = c->tabs[4]; ws.am[1] = c->tabs[5];
-    ws.part = c->part; ws.counter = nullptr;
+    ws.part = c->part; ws.part_half = c->part_half; ws.counter = nullptr;
     const fe *r0 = c->chal + c->chal_r0, *r1 = c->chal + c->chal_r1;
     const fe *alpha = c->chal + c->chal_one, *beta = c->chal + c->chal_one + 1;
+    int slot = 0;
     for (int i = D - 1; i >= 1; --i) {
         int bl = C.bl[i - 1], bg = C.bl[i];
         const fe *ru = c->chal + c->chal_ru[i], *rv = c->chal + c->chal_rv[i];
@@ -779,17 +793,21 @@ int enqueue(vg_ctx *c)
         launch::beta_u BV = {t + 6 * hs, t + 7 * hs, bl >> 1};
         int nt = 1 << bl;
         launch::p1_gather(c->off1[i], c->ent1[i], gd, nt, c->cv[i - 1], BG, c->amb, c->aVb, st);
-        launch::sumcheck_phase2(c->cv[i - 1], c->aVb, c->amb, bl, ru, c->LH[i].vu_const, ws, sums1, end1, st);
+        slot += launch::sumcheck_phase2(c->cv[i - 1], c->aVb, c->amb, bl, ru, c->LH[i].vu_const, ws,
+                                        c->slots + (int64_t)slot * 3 * launch::FW_BLOCKS, sums1, end1, st);
         launch::p2_gather(c->off2[i], c->ent2[i], c->z2[i], (int)c->h_z2[i].size(), gd, nt, BG, BU, end1 + 4, c->amb, c->aVb,
                           c->part, st);
-        launch::sumcheck_phase2(c->cv[i - 1], c->aVb, c->amb, bl, rv, c->LH[i].vv_const, ws, sums2, end2, st);
+        slot += launch::sumcheck_phase2(c->cv[i - 1], c->aVb, c->amb, bl, rv, c->LH[i].vv_const, ws,
+                                        c->slots + (int64_t)slot * 3 * launch::FW_BLOCKS, sums2, end2, st);
         if (i > 1) launch::general_value(gd, BG, BU, BV, end1 + 4, end2 + 4, c->part, nullptr, gv, st);
         r0 = ru; r1 = rv;
         alpha = c->chal + c->chal_ab[i];
         beta = alpha + 1;
     }
+    if (slot != c->n_slots) FAILV(-30, "fold slot plan mismatch");
+    launch::reduce_slots(c->slots, c->n_slots, c->slot_nb, c->slot_dst, c->msg, st);
     HIPCHK(hipEventRecord(c->ev[2], st));
-    st = st2;
+    if (!c->serial) st = st2;
     // ---- commit_private_array (poly_commit.h:42-124) ----
     int n = c->n;
     int64_t SS = c->SS, real = c->real, m = c->m, Cc = c->C_;
@@ -991,6 +1009,22 @@ int ctx_init(vg_ctx *c)
     for (int i = C.depth - 1; i >= 1; --i) { c->msg_layer[i] = ml; ml += 6 * C.bl[i - 1] + 11; }
     c->msg_allsum = ml; ml += 65;
     c->msg_len = ml;
+    // slot tables of the wave folds, in launch order (layer from the top, phase 1 then 2)
+    c->h_slot_nb.clear(); c->h_slot_dst.clear();
+    for (int i = C.depth - 1; i >= 1; --i) {
+        int bl = C.bl[i - 1];
+        int js[32], ks[32];
+        int nl = launch::fold_plan(bl, js, ks);
+        for (int ph = 0; ph < 2; ++ph) {
+            int64_t sums = c->msg_layer[i] + (ph == 0 ? 0 : 3 * bl + 5);
+            for (int k = 0; k < nl; ++k)
+                for (int q = 0; q < ks[k]; ++q) {
+                    c->h_slot_nb.push_back(launch::fold_blocks(bl, js[k]));
+                    c->h_slot_dst.push_back(sums + 3 * (js[k] + q));
+                }
+        }
+    }
+    c->n_slots = (int)c->h_slot_nb.size();
     c->gather_cap = gather_count(n);
     c->h_off1.assign(C.depth, {}); c->h_off2.assign(C.depth, {});
     c->h_ent1.assign(C.depth, {}); c->h_ent2.assign(C.depth, {}); c->h_z2.assign(C.depth, {});
@@ -1013,6 +1047,10 @@ int ctx_init(vg_ctx *c)
             HIPCHK(hipMemcpy(c->d_ty[i], C.ty[i].data(), nn, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(c->d_v[i], C.v[i].data(), 4 * nn, hipMemcpyHostToDevice));
         }
+    }
+    if (c->n_slots) {
+        HIPCHK(hipMemcpy(c->slot_nb, c->h_slot_nb.data(), sizeof(int) * c->n_slots, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->slot_dst, c->h_slot_dst.data(), sizeof(int64_t) * c->n_slots, hipMemcpyHostToDevice));
     }
     for (int i = 1; i < C.depth; ++i) {
         HIPCHK(hipMemcpy(c->off1[i], c->h_off1[i].data(), 4 * c->h_off1[i].size(), hipMemcpyHostToDevice));
@@ -1103,6 +1141,10 @@ int vg_ctx_create(vg_ctx **out, const vg_circuit_desc *d, void *stream)
     {
         hipError_t e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
         if (e != hipSuccess) { vg_ctx_destroy(c); FAILV(-20, "hipStreamCreate: %s", hipGetErrorString(e)); }
+    }
+    {
+        const char *e = getenv("VIRGO_SERIAL");
+        c->serial = e && e[0] == '1';
     }
     rc = ctx_init(c);
     if (rc) { vg_ctx_destroy(c); return rc; }

@@ -11,8 +11,7 @@ namespace {
 
 constexpr int TPB = 256;
 constexpr int NTT_LM = 11;          // LDS-resident NTT block: 2^11 elements = 32 KB
-constexpr int FOLD_TAIL = 512;      // fold rounds with <= this many outputs run in one block
-constexpr int TAIL_T = 256;         // threads of that block
+constexpr int FOLD_TAIL = 512;      // MLE rounds with <= this many outputs run in one block
 constexpr int MAX_PARTS = 1024;     // max blocks of a reduction with per-block partials
 
 __host__ __device__ inline unsigned nblk(int64_t n, int t = TPB) { return (unsigned)((n + t - 1) / t); }
@@ -23,40 +22,6 @@ __device__ __forceinline__ uint64_t red64(uint64_t x)       // x mod p for any 6
     return r >= P ? r - P : r;
 }
 
-// ---- exact modular atomics (CAS on each 64-bit component) ----
-__device__ __forceinline__ void atomic_add_p(uint64_t *a, uint64_t x)
-{
-    if (x == 0) return;
-    unsigned long long *p = (unsigned long long *)a;
-    unsigned long long old = *p, assumed;
-    do {
-        assumed = old;
-        old = atomicCAS(p, assumed, (unsigned long long)addp(assumed, x));
-    } while (old != assumed);
-}
-// Table builds accumulate into split limbs: each canonical component x < 2^61
-// is added as (x mod 2^32) and (x >> 32) into two 64-bit counters with
-// non-returning atomicAdd, which cannot overflow before 2^32 terms.  The first
-// fold round folds the limbs back into canonical field elements.
-__device__ __forceinline__ void atomic_add_fe(acc4 *a, const fe &x)
-{
-    if (x.real) {
-        atomicAdd((unsigned long long *)&a->rl, (unsigned long long)(x.real & 0xFFFFFFFFull));
-        atomicAdd((unsigned long long *)&a->rh, (unsigned long long)(x.real >> 32));
-    }
-    if (x.img) {
-        atomicAdd((unsigned long long *)&a->il, (unsigned long long)(x.img & 0xFFFFFFFFull));
-        atomicAdd((unsigned long long *)&a->ih, (unsigned long long)(x.img >> 32));
-    }
-}
-// (hi * 2^32 + lo) mod p, hi < 2^61, lo < 2^64
-__device__ __forceinline__ uint64_t limbs_to_p(uint64_t lo, uint64_t hi)
-{
-    uint64_t s = ((hi & 0x1FFFFFFFull) << 32) + (hi >> 29) + (lo & P) + (lo >> 61);
-    s = (s & P) + (s >> 61);
-    return s >= P ? s - P : s;
-}
-__device__ __forceinline__ fe acc_value(const acc4 &a) { return mk(limbs_to_p(a.rl, a.rh), limbs_to_p(a.il, a.ih)); }
 
 // ---- reductions ----
 __device__ __forceinline__ fe wave_sum(fe x)
@@ -82,12 +47,13 @@ __device__ __forceinline__ void block_sum(fe *v)
     if (lane == 0)
         for (int k = 0; k < K; ++k) red[k][w] = v[k];
     __syncthreads();
-    if (threadIdx.x == 0)
+    if (threadIdx.x < 64) {
         for (int k = 0; k < K; ++k) {
-            fe s = red[k][0];
-            for (int i = 1; i < nw; ++i) s = add(s, red[k][i]);
-            v[k] = s;
+            fe s = lane < nw ? red[k][lane] : fzero();
+            s = wave_sum(s);
+            if (threadIdx.x == 0) v[k] = s;
         }
+    }
     __syncthreads();
 }
 
@@ -148,26 +114,6 @@ __global__ __launch_bounds__(1024) void k_mle_tail(fe *a, fe *b, int64_t n, cons
     if (threadIdx.x == 0) *dst = a[0];
 }
 
-// ---------------------------------------------------------------------
-// eq / beta half tables (zk_prover.cpp:391-418, 802-822)
-// ---------------------------------------------------------------------
-__global__ void k_half_tables(fe *f, fe *s, const fe *f0, const fe *r, int fh, int sh)
-{
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    fe one = fone();
-    if (j < (1 << fh)) {
-        fe acc = f0 ? *f0 : one;
-        for (int i = 0; i < fh; ++i) acc = mul(acc, ((j >> i) & 1) ? r[i] : sub(one, r[i]));
-        f[j] = acc;
-    } else {
-        j -= 1 << fh;
-        if (j < (1 << sh)) {
-            fe acc = one;
-            for (int i = 0; i < sh; ++i) acc = mul(acc, ((j >> i) & 1) ? r[fh + i] : sub(one, r[fh + i]));
-            s[j] = acc;
-        }
-    }
-}
 __global__ void k_eq_expand(fe *out, const fe *lo, const fe *hi, int fh, int64_t n)
 {
     int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -187,99 +133,11 @@ __device__ __forceinline__ fe beta_u(const bu_t &b, int u)
 }
 
 // ---------------------------------------------------------------------
-// sumcheck phase-1 table build: scatter by u (zk_prover.cpp:420-587)
-// ---------------------------------------------------------------------
-__global__ void k_p1_scatter(const uint8_t *gty, const int32_t *gu, const int32_t *gv, int n, const fe *pv, bg_t bg,
-                             acc4 *amb, acc4 *aVb)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int ty = gty[i], u = gu[i], v = gv[i];
-    if (ty == 2 || ty == 3) return;
-    fe tmp = beta_g(bg, i);
-    switch (ty) {
-    case 0: atomic_add_fe(&aVb[u], mul(pv[v], tmp)); atomic_add_fe(&amb[u], tmp); break;
-    case 1: atomic_add_fe(&amb[u], mul(pv[v], tmp)); break;
-    case 5: for (int j = u; j < v; ++j) atomic_add_fe(&amb[j], tmp); break;
-    case 12: for (int j = u; j <= v; ++j) { atomic_add_fe(&amb[j], tmp); tmp = dbl(tmp); } break;
-    case 4: atomic_add_fe(&amb[u], beta_g(bg, u)); break;
-    case 6: atomic_add_fe(&amb[u], neg(tmp)); atomic_add_fe(&aVb[u], tmp); break;
-    case 7: atomic_add_fe(&aVb[u], neg(mul(pv[v], tmp))); atomic_add_fe(&amb[u], tmp); break;
-    case 8: { fe tv = mul(tmp, pv[v]); atomic_add_fe(&aVb[u], tv); atomic_add_fe(&amb[u], sub(tmp, dbl(tv))); break; }
-    case 13: { fe tv = mul(tmp, pv[v]); atomic_add_fe(&amb[u], sub(tmp, tv)); break; }
-    case 9: { fe tv = mul(tmp, pv[v]); atomic_add_fe(&aVb[u], tv); atomic_add_fe(&amb[u], neg(tv)); break; }
-    case 10: atomic_add_fe(&amb[u], tmp); break;
-    default: break;
-    }
-}
-
-// ---------------------------------------------------------------------
-// sumcheck phase-2 table build: scatter by v (zk_prover.cpp:845-972).
-// Gates whose target is index 0 (sum, exp-sum, NOT, relay) are reduced in
-// the block first.
-// ---------------------------------------------------------------------
-__global__ void k_p2_scatter(const uint8_t *gty, const int32_t *gu, const int32_t *gv, int n, bg_t bg, bu_t bu,
-                             const fe *vup, acc4 *amb, acc4 *aVb)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    fe acc0 = fzero();
-    if (i < n) {
-        int ty = gty[i], u = gu[i], v = gv[i];
-        if (ty != 2 && ty != 3 && ty != 4) {
-            fe vu = *vup;
-            fe tg = beta_g(bg, i);
-            switch (ty) {
-            case 1: atomic_add_fe(&amb[v], mul(mul(tg, beta_u(bu, u)), vu)); break;
-            case 0: { fe t = mul(tg, beta_u(bu, u)); atomic_add_fe(&amb[v], t); atomic_add_fe(&aVb[v], mul(t, vu)); break; }
-            case 5: { fe t = mul(tg, vu); for (int j = u; j < v; ++j) acc0 = add(acc0, mul(t, beta_u(bu, j))); break; }
-            case 12: { fe t = mul(tg, vu); for (int j = u; j <= v; ++j) { acc0 = add(acc0, mul(t, beta_u(bu, j))); t = dbl(t); } break; }
-            case 6: { fe t = mul(tg, beta_u(bu, u)); acc0 = add(acc0, sub(t, mul(t, vu))); break; }
-            case 7: { fe t = mul(tg, beta_u(bu, u)); atomic_add_fe(&amb[v], neg(t)); atomic_add_fe(&aVb[v], mul(t, vu)); break; }
-            case 8: { fe t = mul(tg, beta_u(bu, u)), tv = mul(t, vu); atomic_add_fe(&amb[v], sub(t, dbl(tv))); atomic_add_fe(&aVb[v], tv); break; }
-            case 13: { fe t = mul(tg, beta_u(bu, u)), tv = mul(t, vu); atomic_add_fe(&amb[v], neg(tv)); atomic_add_fe(&aVb[v], tv); break; }
-            case 9: { fe t = mul(tg, beta_u(bu, u)); atomic_add_fe(&amb[v], sub(t, mul(vu, t))); break; }
-            case 10: { fe t = mul(tg, beta_u(bu, u)); acc0 = add(acc0, mul(t, vu)); break; }
-            default: break;
-            }
-        }
-    }
-    int any = __syncthreads_or(!is_zero(acc0));
-    if (!any) return;
-    block_sum<1>(&acc0);
-    if (threadIdx.x == 0) atomic_add_fe(&aVb[0], acc0);
-}
-
-// ---------------------------------------------------------------------
 // bookkeeping fold (zk_prover.cpp:600-633, 992-1025)
 // ---------------------------------------------------------------------
 struct tabs { lpoly *V, *aV, *am; };
 struct ctabs { const lpoly *V, *aV, *am; };
 
-__device__ __forceinline__ void fold_one(bool first, const fe *Vb, const acc4 *aVb, const acc4 *amb, ctabs in, const fe &r,
-                                         int64_t i, lpoly &V, lpoly &aV, lpoly &am)
-{
-    if (first) {
-        fe x0 = Vb[2 * i], x1 = Vb[2 * i + 1];
-        V.b = x0; V.a = sub(x1, x0);
-        x0 = acc_value(aVb[2 * i]); x1 = acc_value(aVb[2 * i + 1]);
-        aV.b = x0; aV.a = sub(x1, x0);
-        x0 = acc_value(amb[2 * i]); x1 = acc_value(amb[2 * i + 1]);
-        am.b = x0; am.a = sub(x1, x0);
-    } else {
-        lpoly y0 = in.V[2 * i], y1 = in.V[2 * i + 1];
-        V.b = add(mul(y0.a, r), y0.b); V.a = sub(add(mul(y1.a, r), y1.b), V.b);
-        y0 = in.aV[2 * i]; y1 = in.aV[2 * i + 1];
-        aV.b = add(mul(y0.a, r), y0.b); aV.a = sub(add(mul(y1.a, r), y1.b), aV.b);
-        y0 = in.am[2 * i]; y1 = in.am[2 * i + 1];
-        am.b = add(mul(y0.a, r), y0.b); am.a = sub(add(mul(y1.a, r), y1.b), am.b);
-    }
-}
-__device__ __forceinline__ void fold_acc(fe *acc, const lpoly &V, const lpoly &aV, const lpoly &am)
-{
-    acc[0] = add(acc[0], mul(am.a, V.a));
-    acc[1] = add(add(acc[1], add(mul(am.a, V.b), mul(am.b, V.a))), aV.a);
-    acc[2] = add(add(acc[2], mul(am.b, V.b)), aV.b);
-}
 // block 0 first reduces the previous round's per-block partials
 __device__ void reduce_prev(const fe *prev, int nprev, fe *dst)
 {
@@ -290,54 +148,6 @@ __device__ void reduce_prev(const fe *prev, int nprev, fe *dst)
     block_sum<3>(acc);
     if (threadIdx.x == 0) { dst[0] = acc[0]; dst[1] = acc[1]; dst[2] = acc[2]; }
 }
-__global__ __launch_bounds__(TPB) void k_fold(int first, const fe *Vb, const acc4 *aVb, const acc4 *amb, ctabs in, const fe *rp,
-                                             tabs out, int64_t n_out, fe *part, const fe *prev_part, int nprev, fe *prev_dst)
-{
-    reduce_prev(prev_part, nprev, prev_dst);
-    fe r = first ? fzero() : *rp;
-    fe acc[3] = {fzero(), fzero(), fzero()};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_out; i += (int64_t)gridDim.x * blockDim.x) {
-        lpoly V, aV, am;
-        fold_one(first, Vb, aVb, amb, in, r, i, V, aV, am);
-        out.V[i] = V; out.aV[i] = aV; out.am[i] = am;
-        fold_acc(acc, V, aV, am);
-    }
-    block_sum<3>(acc);
-    if (threadIdx.x == 0) { part[3 * blockIdx.x] = acc[0]; part[3 * blockIdx.x + 1] = acc[1]; part[3 * blockIdx.x + 2] = acc[2]; }
-}
-// remaining rounds j0..L-1 in one block; X holds round j0's input (unless j0 == 0)
-__global__ __launch_bounds__(TAIL_T) void k_fold_tail(int j0, int L, const fe *Vb, const acc4 *aVb, const acc4 *amb, tabs X, tabs Y,
-                                                    const fe *r, fe vconst, const fe *prev_part, int nprev, fe *sums, fe *end)
-{
-    if (nprev) {
-        fe acc[3] = {fzero(), fzero(), fzero()};
-        for (int k = threadIdx.x; k < nprev; k += blockDim.x)
-            for (int c = 0; c < 3; ++c) acc[c] = add(acc[c], prev_part[3 * k + c]);
-        block_sum<3>(acc);
-        if (threadIdx.x == 0) { sums[3 * (j0 - 1)] = acc[0]; sums[3 * (j0 - 1) + 1] = acc[1]; sums[3 * (j0 - 1) + 2] = acc[2]; }
-    }
-    for (int j = j0; j < L; ++j) {
-        int64_t n_out = (int64_t)1 << (L - 1 - j);
-        fe rr = j == 0 ? fzero() : r[j - 1];
-        ctabs in = {X.V, X.aV, X.am};
-        fe acc[3] = {fzero(), fzero(), fzero()};
-        for (int64_t i = threadIdx.x; i < n_out; i += blockDim.x) {
-            lpoly V, aV, am;
-            fold_one(j == 0, Vb, aVb, amb, in, rr, i, V, aV, am);
-            Y.V[i] = V; Y.aV[i] = aV; Y.am[i] = am;
-            fold_acc(acc, V, aV, am);
-        }
-        block_sum<3>(acc);      // contains __syncthreads: Y complete before the swap
-        if (threadIdx.x == 0) { sums[3 * j] = acc[0]; sums[3 * j + 1] = acc[1]; sums[3 * j + 2] = acc[2]; }
-        tabs t = X; X = Y; Y = t;
-    }
-    if (threadIdx.x == 0) {
-        lpoly V0 = X.V[0], am0 = X.am[0];
-        end[0] = V0.a; end[1] = V0.b; end[2] = am0.a; end[3] = am0.b;
-        end[4] = add(add(mul(V0.a, r[L - 1]), V0.b), vconst);
-    }
-}
-
 
 // ---------------------------------------------------------------------
 // Table builds by gather over CSR transposes (owner computes, no atomics).
@@ -430,70 +240,6 @@ __global__ void k_add_parts(const fe *part, int n, fe *dst)
 }
 
 // ---------------------------------------------------------------------
-// K fold rounds per launch: a thread reads 2^K consecutive entries of the
-// three tables, folds K times in registers and writes one entry; each round's
-// (a, b, c) sums are block-reduced into per-block partials [block][K][3].
-// ---------------------------------------------------------------------
-template <int K>
-__global__ __launch_bounds__(TPB) void k_fold_k(int j0, const fe *Vb, const fe *aVb, const fe *amb, ctabs in,
-                                               const fe *r, tabs out, int64_t n_out, fe *part, const fe *prev_part,
-                                               int nprev, int kprev, fe *prev_dst)
-{
-    if (blockIdx.x == 0 && nprev) {           // finish the previous launch's round sums
-        for (int q = 0; q < kprev; ++q) {
-            fe acc[3] = {fzero(), fzero(), fzero()};
-            for (int k = threadIdx.x; k < nprev; k += blockDim.x)
-                for (int c = 0; c < 3; ++c) acc[c] = add(acc[c], prev_part[(k * kprev + q) * 3 + c]);
-            block_sum<3>(acc);
-            if (threadIdx.x == 0) { prev_dst[3 * q] = acc[0]; prev_dst[3 * q + 1] = acc[1]; prev_dst[3 * q + 2] = acc[2]; }
-        }
-    }
-    const bool first = j0 == 0;
-    fe acc[K][3];
-#pragma unroll
-    for (int q = 0; q < K; ++q) acc[q][0] = acc[q][1] = acc[q][2] = fzero();
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_out; i += (int64_t)gridDim.x * blockDim.x) {
-        lpoly V[1 << K], aV[1 << K], am[1 << K];
-        int64_t base = i << K;
-        if (first) {
-#pragma unroll
-            for (int m = 0; m < (1 << K); ++m) {
-                V[m].a = fzero(); V[m].b = Vb[base + m];
-                aV[m].a = fzero(); aV[m].b = aVb[base + m];
-                am[m].a = fzero(); am[m].b = amb[base + m];
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < (1 << K); ++m) { V[m] = in.V[base + m]; aV[m] = in.aV[base + m]; am[m] = in.am[base + m]; }
-        }
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            bool flat = first && q == 0;
-            fe rr = flat ? fzero() : r[j0 + q - 1];      // round j0+q folds with r[j0+q-1]
-#pragma unroll
-            for (int m = 0; m < (1 << (K - 1 - q)); ++m) {
-                lpoly y0, y1, z;
-#define VG_FOLD(T) \
-                y0 = T[2 * m]; y1 = T[2 * m + 1]; \
-                if (flat) { z.b = y0.b; z.a = sub(y1.b, y0.b); } \
-                else { z.b = add(mul(y0.a, rr), y0.b); z.a = sub(add(mul(y1.a, rr), y1.b), z.b); } \
-                T[m] = z;
-                VG_FOLD(V) VG_FOLD(aV) VG_FOLD(am)
-#undef VG_FOLD
-                fold_acc(acc[q], V[m], aV[m], am[m]);
-            }
-        }
-        out.V[i] = V[0]; out.aV[i] = aV[0]; out.am[i] = am[0];
-    }
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-        block_sum<3>(acc[q]);
-        if (threadIdx.x == 0)
-            for (int c = 0; c < 3; ++c) part[(blockIdx.x * K + q) * 3 + c] = acc[q][c];
-    }
-}
-
-// ---------------------------------------------------------------------
 // Wave-cooperative fold of K rounds (K <= 6): lane gi folds the input pair
 // (2gi, 2gi+1) of round j0; for each later round the lanes combine their
 // neighbours' results with __shfl_down, so only round j0+K-1's table is
@@ -506,13 +252,6 @@ __device__ __forceinline__ fe shfl_down_fe(const fe &x, int d)
     y.img = __shfl_down((unsigned long long)x.img, d);
     return y;
 }
-__device__ __forceinline__ lpoly shfl_down_lp(const lpoly &x, int d)
-{
-    lpoly y;
-    y.a = shfl_down_fe(x.a, d);
-    y.b = shfl_down_fe(x.b, d);
-    return y;
-}
 __device__ __forceinline__ lpoly fold_pair(const lpoly &y0, const lpoly &y1, const fe &r)
 {
     lpoly z;
@@ -520,59 +259,102 @@ __device__ __forceinline__ lpoly fold_pair(const lpoly &y0, const lpoly &y1, con
     z.a = sub(add(mul(y1.a, r), y1.b), z.b);
     return z;
 }
-template <int K>
-__global__ __launch_bounds__(TPB) void k_fold_wave(int j0, const fe *Vb, const fe *aVb, const fe *amb, ctabs in,
-                                                  const fe *r, tabs out, int64_t n0, fe *part)
+// round contributions a = am.a V.a, b = am.a V.b + am.b V.a + aV.a, c = am.b V.b + aV.b
+__device__ __forceinline__ void fold_terms(const lpoly &V, const lpoly &aV, const lpoly &am, fe *t)
 {
-    int lane = threadIdx.x & 63;
-    int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-    int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    int64_t nchunks = n0 >> 6;
-    fe acc[K][3];
-#pragma unroll
-    for (int q = 0; q < K; ++q) acc[q][0] = acc[q][1] = acc[q][2] = fzero();
-    fe r0 = j0 ? r[j0 - 1] : fzero();
-    for (int64_t chunk = wave; chunk < nchunks; chunk += nwaves) {     // uniform per wave
-        int64_t gi = (chunk << 6) + lane;
-        lpoly V, aV, am;
-        if (j0 == 0) {
-            fe x0 = Vb[2 * gi], x1 = Vb[2 * gi + 1];
-            V.b = x0; V.a = sub(x1, x0);
-            x0 = aVb[2 * gi]; x1 = aVb[2 * gi + 1];
-            aV.b = x0; aV.a = sub(x1, x0);
-            x0 = amb[2 * gi]; x1 = amb[2 * gi + 1];
-            am.b = x0; am.a = sub(x1, x0);
-        } else {
-            V = fold_pair(in.V[2 * gi], in.V[2 * gi + 1], r0);
-            aV = fold_pair(in.aV[2 * gi], in.aV[2 * gi + 1], r0);
-            am = fold_pair(in.am[2 * gi], in.am[2 * gi + 1], r0);
-        }
-        fold_acc(acc[0], V, aV, am);
-#pragma unroll
-        for (int q = 1; q < K; ++q) {
-            lpoly V2 = shfl_down_lp(V, 1 << (q - 1));
-            lpoly aV2 = shfl_down_lp(aV, 1 << (q - 1));
-            lpoly am2 = shfl_down_lp(am, 1 << (q - 1));
-            if ((lane & ((1 << q) - 1)) == 0) {
-                fe rr = r[j0 + q - 1];
-                V = fold_pair(V, V2, rr);
-                aV = fold_pair(aV, aV2, rr);
-                am = fold_pair(am, am2, rr);
-                fold_acc(acc[q], V, aV, am);
+    t[0] = mul(am.a, V.a);
+    t[1] = add(add(mul(am.a, V.b), mul(am.b, V.a)), aV.a);
+    t[2] = add(mul(am.b, V.b), aV.b);
+}
+// ---------------------------------------------------------------------
+// Block fold: a block folds a chunk of 2^b consecutive entries of round j0's
+// tables through b rounds in LDS (every fold computed once); the round sums
+// are wave-reduced into LDS.  With several blocks, each block's chunk result
+// goes to out[chunk] and the round sums to per-block partials
+// part[(q*3 + c) * FB_MAXB + block]; with a single block (the phase tail) the
+// sums go to sums[3(j0+q)..] and the final entry to end[].
+// ---------------------------------------------------------------------
+constexpr int FB_LOG = 10;
+constexpr int FB_MAXB = 1024;
+__global__ __launch_bounds__(256) void k_fold_blk(int j0, int b, int L, const fe *Vb, const fe *aVb, const fe *amb, ctabs X,
+                                                 const fe *r, tabs out, int64_t nchunks, fe *part, fe *sums, fe *end,
+                                                 fe vconst)
+{
+    __shared__ lpoly sV[1 << (FB_LOG - 1)], saV[1 << (FB_LOG - 1)], sam[1 << (FB_LOG - 1)];
+    __shared__ fe red[FB_LOG][4][3];
+    int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int k = tid; k < FB_LOG * 4 * 3; k += blockDim.x) (&red[0][0][0])[k] = fzero();
+    __syncthreads();
+    int n0 = 1 << (b - 1);
+    bool flat = j0 == 0;
+    fe r0 = flat ? fzero() : r[j0 - 1];
+    for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        int64_t base = ch << b;
+        // round j0: fold input pairs straight from global memory
+        fe acc[3] = {fzero(), fzero(), fzero()};
+        for (int t = tid; t < n0; t += blockDim.x) {
+            int64_t e0 = base + 2 * t;
+            lpoly V, aV, am;
+            if (flat) {
+                fe x0 = Vb[e0], x1 = Vb[e0 + 1];
+                V.b = x0; V.a = sub(x1, x0);
+                x0 = aVb[e0]; x1 = aVb[e0 + 1];
+                aV.b = x0; aV.a = sub(x1, x0);
+                x0 = amb[e0]; x1 = amb[e0 + 1];
+                am.b = x0; am.a = sub(x1, x0);
+            } else {
+                V = fold_pair(X.V[e0], X.V[e0 + 1], r0);
+                aV = fold_pair(X.aV[e0], X.aV[e0 + 1], r0);
+                am = fold_pair(X.am[e0], X.am[e0 + 1], r0);
             }
+            sV[t] = V; saV[t] = aV; sam[t] = am;
+            fe tt[3];
+            fold_terms(V, aV, am, tt);
+            acc[0] = add(acc[0], tt[0]); acc[1] = add(acc[1], tt[1]); acc[2] = add(acc[2], tt[2]);
         }
-        if ((lane & ((1 << (K - 1)) - 1)) == 0) {
-            int64_t o = gi >> (K - 1);
-            out.V[o] = V; out.aV[o] = aV; out.am[o] = am;
+        if (w * 64 < n0) {
+            for (int c = 0; c < 3; ++c) acc[c] = wave_sum(acc[c]);
+            if (lane == 0) for (int c = 0; c < 3; ++c) red[0][w][c] = add(red[0][w][c], acc[c]);
         }
+        __syncthreads();
+        // rounds j0+1 .. j0+b-1 in LDS: n <= 256 outputs, one per thread
+        for (int q = 1; q < b; ++q) {
+            int n = n0 >> q;
+            fe rr = r[j0 + q - 1];
+            lpoly V, aV, am;
+            fe tt[3] = {fzero(), fzero(), fzero()};
+            bool act = tid < n;
+            if (act) {
+                V = fold_pair(sV[2 * tid], sV[2 * tid + 1], rr);
+                aV = fold_pair(saV[2 * tid], saV[2 * tid + 1], rr);
+                am = fold_pair(sam[2 * tid], sam[2 * tid + 1], rr);
+                fold_terms(V, aV, am, tt);
+            }
+            __syncthreads();
+            if (act) { sV[tid] = V; saV[tid] = aV; sam[tid] = am; }
+            if (w * 64 < n) {
+                for (int c = 0; c < 3; ++c) tt[c] = wave_sum(tt[c]);
+                if (lane == 0) for (int c = 0; c < 3; ++c) red[q][w][c] = add(red[q][w][c], tt[c]);
+            }
+            __syncthreads();
+        }
+        if (tid == 0 && nchunks > 1) { out.V[ch] = sV[0]; out.aV[ch] = saV[0]; out.am[ch] = sam[0]; }
+        __syncthreads();
     }
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-        block_sum<3>(acc[q]);
-        if (threadIdx.x == 0)
-            for (int c = 0; c < 3; ++c) part[(int64_t)(q * 3 + c) * launch::FW_BLOCKS + blockIdx.x] = acc[q][c];
+    // round sums of this block
+    for (int k = tid; k < b * 3; k += blockDim.x) {
+        int q = k / 3, c = k - 3 * q;
+        fe v = add(add(red[q][0][c], red[q][1][c]), add(red[q][2][c], red[q][3][c]));
+        if (nchunks > 1) part[(int64_t)(q * 3 + c) * FB_MAXB + blockIdx.x] = v;
+        else sums[3 * (j0 + q) + c] = v;
+    }
+    if (nchunks == 1 && tid == 0) {
+        lpoly V0 = sV[0], am0 = sam[0];
+        end[0] = V0.a; end[1] = V0.b; end[2] = am0.a; end[3] = am0.b;
+        end[4] = add(add(mul(V0.a, r[L - 1]), V0.b), vconst);
     }
 }
+
 // round sums of every wave-fold launch of the proof: slot R (a round) holds
 // 3 x FW_BLOCKS block partials; nb[R] of them are live; dst[R] indexes msg.
 __global__ void k_reduce_slots(const fe *part, const int *nb, const int64_t *dst, fe *msg)
@@ -580,91 +362,10 @@ __global__ void k_reduce_slots(const fe *part, const int *nb, const int64_t *dst
     int R = blockIdx.x;
     fe acc[3] = {fzero(), fzero(), fzero()};
     for (int b = threadIdx.x; b < nb[R]; b += blockDim.x)
-        for (int c = 0; c < 3; ++c) acc[c] = add(acc[c], part[((int64_t)R * 3 + c) * launch::FW_BLOCKS + b]);
+        for (int c = 0; c < 3; ++c) acc[c] = add(acc[c], part[((int64_t)R * 3 + c) * FB_MAXB + b]);
     block_sum<3>(acc);
     if (threadIdx.x == 0) { msg[dst[R]] = acc[0]; msg[dst[R] + 1] = acc[1]; msg[dst[R] + 2] = acc[2]; }
 }
-// Remaining rounds of a phase in one block, tables in LDS (input of round j0:
-// 2^(L-j0) <= TAIL_IN entries, or the flat tables if j0 == 0).  Rounds with
-// more than 64 outputs use the block; the last ones run in wave 0 alone,
-// folding through shuffles with no barriers.
-constexpr int TAIL_IN = 512;
-__global__ __launch_bounds__(256) void k_fold_tail_lds(int j0, int L, const fe *Vb, const fe *aVb, const fe *amb, ctabs X,
-                                                      const fe *r, fe vconst, fe *sums, fe *end)
-{
-    __shared__ lpoly sV[TAIL_IN], saV[TAIL_IN], sam[TAIL_IN];
-    int n_in = 1 << (L - j0);
-    for (int i = threadIdx.x; i < n_in; i += blockDim.x) {
-        if (j0 == 0) {
-            sV[i].a = fzero(); sV[i].b = Vb[i];
-            saV[i].a = fzero(); saV[i].b = aVb[i];
-            sam[i].a = fzero(); sam[i].b = amb[i];
-        } else {
-            sV[i] = X.V[i]; saV[i] = X.aV[i]; sam[i] = X.am[i];
-        }
-    }
-    __syncthreads();
-    int j = j0;
-    for (; j < L && (1 << (L - 1 - j)) > 64; ++j) {
-        int n_out = 1 << (L - 1 - j);
-        bool flat = j == 0;
-        fe rr = flat ? fzero() : r[j - 1];
-        fe acc[3] = {fzero(), fzero(), fzero()};
-        lpoly nV, naV, nam;
-        bool act = threadIdx.x < n_out;
-        if (act) {
-            int i = threadIdx.x;
-#define VG_TFOLD(S, N) { lpoly y0 = S[2 * i], y1 = S[2 * i + 1]; \
-                if (flat) { N.b = y0.b; N.a = sub(y1.b, y0.b); } \
-                else N = fold_pair(y0, y1, rr); }
-            VG_TFOLD(sV, nV) VG_TFOLD(saV, naV) VG_TFOLD(sam, nam)
-#undef VG_TFOLD
-            fold_acc(acc, nV, naV, nam);
-        }
-        __syncthreads();
-        if (act) { sV[threadIdx.x] = nV; saV[threadIdx.x] = naV; sam[threadIdx.x] = nam; }
-        block_sum<3>(acc);
-        if (threadIdx.x == 0) { sums[3 * j] = acc[0]; sums[3 * j + 1] = acc[1]; sums[3 * j + 2] = acc[2]; }
-    }
-    if (threadIdx.x >= 64) return;
-    // wave rounds: after local round q lane i holds entry i >> q (valid if i % 2^q == 0)
-    int lane = threadIdx.x;
-    lpoly V, aV, am;
-    for (int q = 0; j < L; ++j, ++q) {
-        int n_out = 1 << (L - 1 - j);
-        bool flat = j == 0;
-        fe rr = flat ? fzero() : r[j - 1];
-        fe acc[3] = {fzero(), fzero(), fzero()};
-        if (q == 0) {
-            if (lane < n_out) {
-                lpoly y0 = sV[2 * lane], y1 = sV[2 * lane + 1];
-                if (flat) { V.b = y0.b; V.a = sub(y1.b, y0.b); } else V = fold_pair(y0, y1, rr);
-                y0 = saV[2 * lane]; y1 = saV[2 * lane + 1];
-                if (flat) { aV.b = y0.b; aV.a = sub(y1.b, y0.b); } else aV = fold_pair(y0, y1, rr);
-                y0 = sam[2 * lane]; y1 = sam[2 * lane + 1];
-                if (flat) { am.b = y0.b; am.a = sub(y1.b, y0.b); } else am = fold_pair(y0, y1, rr);
-                fold_acc(acc, V, aV, am);
-            }
-        } else {
-            lpoly V2 = shfl_down_lp(V, 1 << (q - 1));
-            lpoly aV2 = shfl_down_lp(aV, 1 << (q - 1));
-            lpoly am2 = shfl_down_lp(am, 1 << (q - 1));
-            if ((lane & ((1 << q) - 1)) == 0 && (lane >> q) < n_out) {
-                V = fold_pair(V, V2, rr);
-                aV = fold_pair(aV, aV2, rr);
-                am = fold_pair(am, am2, rr);
-                fold_acc(acc, V, aV, am);
-            }
-        }
-        for (int c = 0; c < 3; ++c) acc[c] = wave_sum(acc[c]);
-        if (lane == 0) { sums[3 * j] = acc[0]; sums[3 * j + 1] = acc[1]; sums[3 * j + 2] = acc[2]; }
-    }
-    if (lane == 0) {
-        end[0] = V.a; end[1] = V.b; end[2] = am.a; end[3] = am.b;
-        end[4] = add(add(mul(V.a, r[L - 1]), V.b), vconst);
-    }
-}
-
 // every eq half table of the proof in one launch: out[j] = f0 * prod_{i<nb} eq(bit_i(j), r[i])
 __global__ void k_half_multi(const launch::half_desc *d, int nd, int64_t total)
 {
@@ -1071,61 +772,9 @@ void mle_eval(const fe *in, int log_n, const fe *r, fe *t0, fe *t1, fe *dst, hip
     }
     hipLaunchKernelGGL(k_mle_tail, dim3(1), dim3(1024), 0, s, (fe *)src, bufs[cur], n, r + k, log_n - k, dst);
 }
-void half_tables(fe *f, fe *s, const fe *f0, const fe *r, int len, hipStream_t st)
-{
-    int fh = len >> 1, sh = len - fh;
-    int n = (1 << fh) + (1 << sh);
-    hipLaunchKernelGGL(k_half_tables, dim3(nblk(n)), dim3(TPB), 0, st, f, s, f0, r, fh, sh);
-}
-void eq_table(fe *out, const fe *r, int len, fe *scratch, hipStream_t st)
-{
-    int fh = len >> 1;
-    fe *lo = scratch, *hi = scratch + (1 << fh);
-    half_tables(lo, hi, nullptr, r, len, st);
-    int64_t n = (int64_t)1 << len;
-    hipLaunchKernelGGL(k_eq_expand, dim3(nblk(n)), dim3(TPB), 0, st, out, lo, hi, fh, n);
-}
 void eq_expand(fe *out, const fe *lo, const fe *hi, int fh, int64_t n, hipStream_t st)
 {
     hipLaunchKernelGGL(k_eq_expand, dim3(nblk(n)), dim3(TPB), 0, st, out, lo, hi, fh, n);
-}
-void p1_scatter(gates_dev g, const fe *prevV, beta_g bg, acc4 *amb, acc4 *aVb, hipStream_t s)
-{
-    int n = 1 << g.bl;
-    bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
-    hipLaunchKernelGGL(k_p1_scatter, dim3(nblk(n)), dim3(TPB), 0, s, g.ty, g.u, g.v, n, prevV, b, amb, aVb);
-}
-void p2_scatter(gates_dev g, beta_g bg, beta_u bu, const fe *vu, acc4 *amb, acc4 *aVb, hipStream_t s)
-{
-    int n = 1 << g.bl;
-    bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
-    bu_t u = {bu.f, bu.s, bu.fh};
-    hipLaunchKernelGGL(k_p2_scatter, dim3(nblk(n)), dim3(TPB), 0, s, g.ty, g.u, g.v, n, b, u, vu, amb, aVb);
-}
-void sumcheck_phase(const fe *Vb, const acc4 *aVb, const acc4 *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *sums,
-                    fe *end, hipStream_t s)
-{
-    int j = 0;
-    int cur = 0;
-    const fe *prev_part = nullptr;
-    int nprev = 0;
-    fe *part[2] = {ws.part, ws.part + 3 * MAX_PARTS};
-    while (j < L && ((int64_t)1 << (L - 1 - j)) > FOLD_TAIL) {
-        int64_t n_out = (int64_t)1 << (L - 1 - j);
-        unsigned g = nblk(n_out);
-        if (g > MAX_PARTS) g = MAX_PARTS;
-        ctabs in = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
-        tabs out = {ws.V[cur], ws.aV[cur], ws.am[cur]};
-        hipLaunchKernelGGL(k_fold, dim3(g), dim3(TPB), 0, s, j == 0 ? 1 : 0, Vb, aVb, amb, in, j == 0 ? r : r + j - 1, out,
-                           n_out, part[j & 1], prev_part, nprev, j > 0 ? sums + 3 * (j - 1) : sums);
-        prev_part = part[j & 1];
-        nprev = (int)g;
-        cur ^= 1;
-        ++j;
-    }
-    tabs X = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
-    tabs Y = {ws.V[cur], ws.aV[cur], ws.am[cur]};
-    hipLaunchKernelGGL(k_fold_tail, dim3(1), dim3(TAIL_T), 0, s, j, L, Vb, aVb, amb, X, Y, r, vconst, prev_part, nprev, sums, end);
 }
 void p1_gather(const uint32_t *off, const int32_t *ent, gates_dev g, int nt, const fe *prevV, beta_g bg, fe *amb, fe *aVb,
                hipStream_t s)
@@ -1149,13 +798,24 @@ void p2_gather(const uint32_t *off, const int32_t *ent, const int32_t *zlist, in
 int fold_plan(int L, int *js, int *ks)
 {
     int n = 0, j = 0;
-    while ((1 << (L - j)) > TAIL_IN) {
-        int K = L - j - 9;
-        if (K > 6) K = 6;
-        js[n] = j; ks[n] = K; ++n;
-        j += K;
+    while (L - j > FB_LOG) {
+        int b = L - j - FB_LOG;
+        if (b > FB_LOG) b = FB_LOG;
+        js[n] = j; ks[n] = b; ++n;
+        j += b;
     }
     return n;
+}
+int fold_blocks(int L, int j)
+{
+    int js[32], ks[32];
+    int n = fold_plan(L, js, ks);
+    for (int k = 0; k < n; ++k)
+        if (js[k] == j) {
+            int64_t ch = (int64_t)1 << (L - j - ks[k]);
+            return (int)(ch > FB_MAXB ? FB_MAXB : ch);
+        }
+    return 0;
 }
 int sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *slots,
                     fe *sums, fe *end, hipStream_t s)
@@ -1164,30 +824,22 @@ int sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe 
     int nl = fold_plan(L, js, ks);
     int cur = 0, rounds = 0;
     for (int k = 0; k < nl; ++k) {
-        int j = js[k], K = ks[k];
-        int64_t n0 = (int64_t)1 << (L - j - 1);
-        int64_t g64 = n0 / TPB;
-        unsigned g = (unsigned)(g64 > FW_BLOCKS ? FW_BLOCKS : g64);
+        int j = js[k], b = ks[k];
+        int64_t nch = (int64_t)1 << (L - j - b);
+        unsigned g = (unsigned)(nch > FB_MAXB ? FB_MAXB : nch);
         ctabs in = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
         tabs out = {ws.V[cur], ws.aV[cur], ws.am[cur]};
-        fe *pp = slots + (int64_t)rounds * 3 * FW_BLOCKS;
-        switch (K) {
-#define VG_FW(k) case k: hipLaunchKernelGGL(k_fold_wave<k>, dim3(g), dim3(TPB), 0, s, j, Vb, aVb, amb, in, r, out, n0, pp); break;
-        VG_FW(1) VG_FW(2) VG_FW(3) VG_FW(4) VG_FW(5) VG_FW(6)
-#undef VG_FW
-        }
+        hipLaunchKernelGGL(k_fold_blk, dim3(g), dim3(256), 0, s, j, b, L, Vb, aVb, amb, in, r, out, nch,
+                           slots + (int64_t)rounds * 3 * FB_MAXB, (fe *)nullptr, (fe *)nullptr, fzero());
         cur ^= 1;
-        rounds += K;
+        rounds += b;
     }
     int j = nl ? js[nl - 1] + ks[nl - 1] : 0;
     ctabs X = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
-    hipLaunchKernelGGL(k_fold_tail_lds, dim3(1), dim3(256), 0, s, j, L, Vb, aVb, amb, X, r, vconst, sums, end);
+    tabs none = {nullptr, nullptr, nullptr};
+    hipLaunchKernelGGL(k_fold_blk, dim3(1), dim3(256), 0, s, j, L - j, L, Vb, aVb, amb, X, r, none, (int64_t)1,
+                       (fe *)nullptr, sums, end, vconst);
     return rounds;
-}
-int fold_blocks(int L, int j)
-{
-    int64_t g = ((int64_t)1 << (L - j - 1)) / TPB;
-    return (int)(g > FW_BLOCKS ? FW_BLOCKS : g);
 }
 void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s)
 {
@@ -1197,10 +849,8 @@ void half_multi(const half_desc *d, int nd, int64_t total, hipStream_t s)
 {
     if (total) hipLaunchKernelGGL(k_half_multi, dim3(nblk(total)), dim3(TPB), 0, s, d, nd, total);
 }
-void general_value(gates_dev g, beta_g bg, beta_u bu, beta_u bv, const fe *vu, const fe *vv, fe *part, unsigned *counter,
-                   fe *dst, hipStream_t s)
+void general_value(gates_dev g, beta_g bg, beta_u bu, beta_u bv, const fe *vu, const fe *vv, fe *part, fe *dst, hipStream_t s)
 {
-    (void)counter;
     int n = 1 << g.bl;
     unsigned blocks = nblk(n);
     if (blocks > MAX_PARTS) blocks = MAX_PARTS;

@@ -22,11 +22,6 @@ __host__ __device__ inline int64_t cw_idx(int64_t p, int64_t S, int log_R)
     return (p & (R - 1)) * (S >> log_R) + (p >> log_R);
 }
 
-// table-build accumulator: real/img split into 32-bit-limb sums
-struct acc4 {
-    uint64_t rl, rh, il, ih;
-};
-
 struct gates_dev {            // one layer's gates, SoA
     const uint8_t *ty;
     const int32_t *u, *v;
@@ -42,32 +37,17 @@ void eval_layer(gates_dev g, const fe *prev, fe *out, hipStream_t s);
 // out[j] = in[2j](1-r[k]) + in[2j+1] r[k] for all rounds k < nr, result -> dst
 void mle_eval(const fe *in, int log_n, const fe *r, fe *tmp0, fe *tmp1, fe *dst, hipStream_t s);
 
-// f[j] = f0 * prod_{i<fh} eq(bit_i(j), r[i]), s[j] = prod_{i<sh} eq(bit_i(j), r[fh+i]),
-// fh = len/2, sh = len - fh (zk_prover.cpp:391-418).  f0 may be null (= 1).
-void half_tables(fe *f, fe *s, const fe *f0, const fe *r, int len, hipStream_t st);
 // out[x] = lo[x mod 2^fh] * hi[x >> fh]
 void eq_expand(fe *out, const fe *lo, const fe *hi, int fh, int64_t n, hipStream_t st);
-// full eq table out[x] = prod_i eq(bit_i(x), r[i]) (zk_verifier.cpp:838-847)
-void eq_table(fe *out, const fe *r, int len, fe *scratch, hipStream_t s);
 
 struct beta_g { const fe *g0f, *g0s, *g1f, *g1s; int fh; };
 struct beta_u { const fe *f, *s; int fh; };
-
-void p1_scatter(gates_dev g, const fe *prevV, beta_g bg, acc4 *amb, acc4 *aVb, hipStream_t s);
-void p2_scatter(gates_dev g, beta_g bg, beta_u bu, const fe *vu, acc4 *amb, acc4 *aVb, hipStream_t s);
 
 struct fold_ws {              // workspace of a sumcheck phase
     lpoly *V[2], *aV[2], *am[2];
     fe *part;                 // two halves of per-block partial sums
     int64_t part_half;        // >= 18 * (2^(L-1) / 256) for the widest layer
-    unsigned *counter;
 };
-// Runs every round of one sumcheck phase on tables of 2^L entries.
-// Round j folds with r[j-1] (r[-1] unused); round sums (a,b,c) -> sums[3j..].
-// After the last round: end[0..3] = V[0].a, V[0].b, am[0].a, am[0].b and
-// end[4] = V[0].a * r[L-1] + V[0].b + vconst (= v_u or v_v).
-void sumcheck_phase(const fe *Vb, const acc4 *aVb, const acc4 *amb, int L, const fe *r, fe vconst,
-                    fold_ws ws, fe *sums, fe *end, hipStream_t s);
 
 // table builds by gather over CSR transposes (built once per circuit).
 // p1: off/ent list, per table entry t, the gates feeding t through u.
@@ -77,13 +57,17 @@ void p1_gather(const uint32_t *off, const int32_t *ent, gates_dev g, int nt, con
 // sum, exp-sum, NOT and relay gates, which all add into addV[0].
 void p2_gather(const uint32_t *off, const int32_t *ent, const int32_t *zlist, int nz, gates_dev g, int nt, beta_g bg,
                beta_u bu, const fe *vu, fe *amb, fe *aVb, fe *part, hipStream_t s);
-// All rounds of a phase on plain tables: up to 6 rounds per launch with wave
-// shuffles, then the last <= 9 rounds in one block.  Launch k of the plan
-// (fold_plan) writes its block partials of round q to
-// slots[(q*3 + c) * FW_BLOCKS + block]; the caller reduces every slot of the
-// proof with reduce_slots.  Tail round sums go straight to sums[3j..].
-// Returns the number of slot rounds used.
-constexpr int FW_BLOCKS = 512;
+// Runs every round of one sumcheck phase on tables of 2^L entries
+// (zk_prover.cpp:600-633, 992-1025).  Round j folds with r[j-1] (r[-1]
+// unused).  After the last round: end[0..3] = V[0].a, V[0].b, am[0].a,
+// am[0].b and end[4] = V[0].a * r[L-1] + V[0].b + vconst (= v_u or v_v).
+// All rounds of a phase on plain tables: block folds of up to 10 rounds per
+// launch (chunks of 1024 entries in LDS), then the last <= 10 rounds in one
+// block.  Launch k of the plan (fold_plan) writes its block partials of
+// round q to slots[(q*3 + c) * FW_BLOCKS + block]; the caller reduces every
+// slot of the proof with reduce_slots.  Tail round sums go straight to
+// sums[3j..].  Returns the number of slot rounds used.
+constexpr int FW_BLOCKS = 1024;
 int fold_plan(int L, int *js, int *ks);
 int fold_blocks(int L, int j);
 int sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *slots,
@@ -101,7 +85,7 @@ void half_multi(const half_desc *d, int nd, int64_t total, hipStream_t s);
 // sum over gates of beta_g(g) beta_u(u) beta_v(v) f_type(v_u, v_v)
 // (zk_verifier.cpp:219-606 and the combination at :1149)
 void general_value(gates_dev g, beta_g bg, beta_u bu, beta_u bv, const fe *vu, const fe *vv,
-                   fe *part, unsigned *counter, fe *dst, hipStream_t s);
+                   fe *part, fe *dst, hipStream_t s);
 
 // tw[e] = w^e, e < n, from T1[j] = w^j (j < 1024) and T2[j] = w^(1024 j)
 void twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2, hipStream_t s);

@@ -471,7 +471,12 @@ struct vg_ctx {
     std::vector<int32_t *> d_u, d_v;
     std::vector<fe *> cv;
     fe *chal = nullptr;         // challenges
-    fe *amb = nullptr, *aVb = nullptr;
+    static constexpr int NGS = 4;             // sumcheck streams (layers run concurrently)
+    fe *amb_s[NGS] = {}, *aVb_s[NGS] = {};
+    lpoly *tabs_s[NGS][6] = {};
+    fe *part_s[NGS] = {};
+    hipStream_t gst[NGS] = {};
+    hipEvent_t gev[NGS] = {};
     // CSR transposes of every gate layer (circuit structure only, built once)
     std::vector<std::vector<uint32_t>> h_off1, h_off2;
     std::vector<std::vector<int32_t>> h_ent1, h_ent2, h_z2;
@@ -483,8 +488,6 @@ struct vg_ctx {
     launch::half_desc *hdesc = nullptr;
     int n_hdesc = 0;
     int64_t ht_total = 0;
-    lpoly *tabs[6] = {};
-    fe *part = nullptr;
     int64_t part_half = 0;
     // round-sum slots of the wave folds (see launch::sumcheck_phase2)
     fe *slots = nullptr;
@@ -560,8 +563,10 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     }
     c->chal = carve<fe>(base, off, c->n_chal);
     int64_t T = (int64_t)1 << maxbl;
-    c->amb = carve<fe>(base, off, T);
-    c->aVb = carve<fe>(base, off, T);
+    for (int w = 0; w < vg_ctx::NGS; ++w) {
+        c->amb_s[w] = carve<fe>(base, off, T);
+        c->aVb_s[w] = carve<fe>(base, off, T);
+    }
     c->off1.assign(D, nullptr); c->off2.assign(D, nullptr);
     c->ent1.assign(D, nullptr); c->ent2.assign(D, nullptr); c->z2.assign(D, nullptr);
     for (int i = 1; i < D; ++i) {
@@ -574,9 +579,10 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     c->ht_stride = (int64_t)1 << ((maxbl >> 1) + 1);
     c->ht = carve<fe>(base, off, (int64_t)D * 8 * c->ht_stride + 2 * ((int64_t)1 << (C.bl[0] / 2 + 1)));
     c->hdesc = carve<launch::half_desc>(base, off, (int64_t)D * 8 + 2);
-    for (int k = 0; k < 6; ++k) c->tabs[k] = carve<lpoly>(base, off, T / 2 > 0 ? T / 2 : 1);
+    for (int w = 0; w < vg_ctx::NGS; ++w)
+        for (int k = 0; k < 6; ++k) c->tabs_s[w][k] = carve<lpoly>(base, off, T / 2 > 0 ? T / 2 : 1);
     c->part_half = 18 * 1024;
-    c->part = carve<fe>(base, off, 2 * c->part_half);
+    for (int w = 0; w < vg_ctx::NGS; ++w) c->part_s[w] = carve<fe>(base, off, 2 * c->part_half);
     c->slots = carve<fe>(base, off, (int64_t)(c->n_slots > 0 ? c->n_slots : 1) * 3 * launch::FW_BLOCKS);
     c->slot_nb = carve<int>(base, off, c->n_slots + 1);
     c->slot_dst = carve<int64_t>(base, off, c->n_slots + 1);
@@ -773,14 +779,20 @@ int enqueue(vg_ctx *c)
     // ---- V_res (zk_prover.cpp:30-52) ----
     launch::mle_eval(c->cv[D - 1], C.bl[D - 1], c->chal + c->chal_r0, c->mle_t[0], c->mle_t[1], c->msg + c->msg_vres, st);
     // ---- GKR layers ----
-    launch::fold_ws ws;
-    ws.V[0] = c->tabs[0]; ws.aV[0] = c->tabs[1]; ws.am[0] = c->tabs[2];
-    ws.V[1] = c->tabs[3]; ws.aV[1] = c->tabs[4]; ws.am[1] = c->tabs[5];
-    ws.part = c->part; ws.part_half = c->part_half; ws.counter = nullptr;
+    // Every layer's sumcheck depends only on the evaluated circuit and on
+    // challenges known in advance, so layers run concurrently on NGS streams.
+    for (int w = 0; w < vg_ctx::NGS; ++w) HIPCHK(hipStreamWaitEvent(c->gst[w], c->ev[1], 0));
     const fe *r0 = c->chal + c->chal_r0, *r1 = c->chal + c->chal_r1;
     const fe *alpha = c->chal + c->chal_one, *beta = c->chal + c->chal_one + 1;
     int slot = 0;
     for (int i = D - 1; i >= 1; --i) {
+        int w = (D - 1 - i) % vg_ctx::NGS;
+        hipStream_t gs = c->gst[w];
+        launch::fold_ws ws;
+        ws.V[0] = c->tabs_s[w][0]; ws.aV[0] = c->tabs_s[w][1]; ws.am[0] = c->tabs_s[w][2];
+        ws.V[1] = c->tabs_s[w][3]; ws.aV[1] = c->tabs_s[w][4]; ws.am[1] = c->tabs_s[w][5];
+        ws.part = c->part_s[w]; ws.part_half = c->part_half;
+        fe *amb = c->amb_s[w], *aVb = c->aVb_s[w];
         int bl = C.bl[i - 1], bg = C.bl[i];
         const fe *ru = c->chal + c->chal_ru[i], *rv = c->chal + c->chal_rv[i];
         fe *M = c->msg + c->msg_layer[i];
@@ -792,17 +804,19 @@ int enqueue(vg_ctx *c)
         launch::beta_u BU = {t + 4 * hs, t + 5 * hs, bl >> 1};
         launch::beta_u BV = {t + 6 * hs, t + 7 * hs, bl >> 1};
         int nt = 1 << bl;
-        launch::p1_gather(c->off1[i], c->ent1[i], gd, nt, c->cv[i - 1], BG, c->amb, c->aVb, st);
-        slot += launch::sumcheck_phase2(c->cv[i - 1], c->aVb, c->amb, bl, ru, c->LH[i].vu_const, ws,
-                                        c->slots + (int64_t)slot * 3 * launch::FW_BLOCKS, sums1, end1, st);
-        launch::p2_gather(c->off2[i], c->ent2[i], c->z2[i], (int)c->h_z2[i].size(), gd, nt, BG, BU, end1 + 4, c->amb, c->aVb,
-                          c->part, st);
-        slot += launch::sumcheck_phase2(c->cv[i - 1], c->aVb, c->amb, bl, rv, c->LH[i].vv_const, ws,
-                                        c->slots + (int64_t)slot * 3 * launch::FW_BLOCKS, sums2, end2, st);
-        if (i > 1) launch::general_value(gd, BG, BU, BV, end1 + 4, end2 + 4, c->part, nullptr, gv, st);
-        r0 = ru; r1 = rv;
-        alpha = c->chal + c->chal_ab[i];
-        beta = alpha + 1;
+        launch::p1_gather(c->off1[i], c->ent1[i], gd, nt, c->cv[i - 1], BG, amb, aVb, gs);
+        slot += launch::sumcheck_phase2(c->cv[i - 1], aVb, amb, bl, ru, c->LH[i].vu_const, ws,
+                                        c->slots + (int64_t)slot * 3 * launch::FW_BLOCKS, sums1, end1, gs);
+        launch::p2_gather(c->off2[i], c->ent2[i], c->z2[i], (int)c->h_z2[i].size(), gd, nt, BG, BU, end1 + 4, amb, aVb,
+                          ws.part, gs);
+        slot += launch::sumcheck_phase2(c->cv[i - 1], aVb, amb, bl, rv, c->LH[i].vv_const, ws,
+                                        c->slots + (int64_t)slot * 3 * launch::FW_BLOCKS, sums2, end2, gs);
+        if (i > 1) launch::general_value(gd, BG, BU, BV, end1 + 4, end2 + 4, ws.part, gv, gs);
+        (void)r0; (void)r1; (void)alpha; (void)beta;
+    }
+    for (int w = 0; w < vg_ctx::NGS; ++w) {
+        HIPCHK(hipEventRecord(c->gev[w], c->gst[w]));
+        HIPCHK(hipStreamWaitEvent(st, c->gev[w], 0));
     }
     if (slot != c->n_slots) FAILV(-30, "fold slot plan mismatch");
     launch::reduce_slots(c->slots, c->n_slots, c->slot_nb, c->slot_dst, c->msg, st);
@@ -1140,6 +1154,8 @@ int vg_ctx_create(vg_ctx **out, const vg_circuit_desc *d, void *stream)
     }
     {
         hipError_t e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
+        for (int w = 0; w < vg_ctx::NGS && e == hipSuccess; ++w) e = hipStreamCreateWithFlags(&c->gst[w], hipStreamNonBlocking);
+        for (int w = 0; w < vg_ctx::NGS && e == hipSuccess; ++w) e = hipEventCreateWithFlags(&c->gev[w], hipEventDisableTiming);
         if (e != hipSuccess) { vg_ctx_destroy(c); FAILV(-20, "hipStreamCreate: %s", hipGetErrorString(e)); }
     }
     {
@@ -1157,6 +1173,8 @@ int vg_ctx_destroy(vg_ctx *c)
     if (!c) return 0;
     if (c->st) (void)hipStreamSynchronize(c->st);
     if (c->st2) (void)hipStreamSynchronize(c->st2);
+    for (int w = 0; w < vg_ctx::NGS; ++w)
+        if (c->gst[w]) (void)hipStreamSynchronize(c->gst[w]);
     if (c->arena) (void)hipFree(c->arena);
     if (c->h_chal) (void)hipHostFree(c->h_chal);
     if (c->h_gsrc) (void)hipHostFree(c->h_gsrc);
@@ -1166,6 +1184,10 @@ int vg_ctx_destroy(vg_ctx *c)
     for (auto &e : c->ev_leaf) if (e) (void)hipEventDestroy(e);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
     if (c->st2) (void)hipStreamDestroy(c->st2);
+    for (int w = 0; w < vg_ctx::NGS; ++w) {
+        if (c->gst[w]) (void)hipStreamDestroy(c->gst[w]);
+        if (c->gev[w]) (void)hipEventDestroy(c->gev[w]);
+    }
     delete c;
     return 0;
 }

@@ -37,6 +37,16 @@ def parse():
     return ap.parse_args()
 
 
+def leaf_traffic(n):
+    """HBM bytes per k_leaf_multi launch, from rocprofv3 FETCH_SIZE and
+    WRITE_SIZE of the 2^20-gate proof scaled by the same counters for a 1 GiB
+    copy (FETCH_SIZE x 2048 B/KiB, WRITE_SIZE x 1024 B/KiB): 799009.9 KiB
+    read, 24575.5 KiB written.  Other widths: null (not measured)."""
+    if n != 20:
+        return None
+    return int(799009.9 * 2048 + 24575.5 * 1024)
+
+
 def cpu_baseline(args):
     """Times the CPU oracle (single-threaded C restatement of the reference
     prover) on a bounded sample: the same circuit family at 2^sample gates per
@@ -104,7 +114,7 @@ def main():
                    "transcript_bytes": len(tr),
                    "stage_ms": {k: round(float(v), 3) for k, v in zip(names, st)}},
         "roofline": {"bound": "valu", "achieved": round(achieved, 1), "peak": PEAK_VALU_GOPS, "unit": "Gop/s",
-                     "frac": round(achieved / PEAK_VALU_GOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_VALU_GOPS, 4), "traffic": leaf_traffic(n),
                      "kernel": "k_leaf_multi (SHA3-256 leaf chains, %d chains x 65 Keccak-f)" % chains,
                      "kernel_ms": round(leaf_ms, 3)},
     }

@@ -57,12 +57,16 @@ int64_t vg_ctx_device_bytes(const vg_ctx *ctx);
  * commit_phase (poly_commit.h:42-351, vpd_verifier.cpp:46-78) plus the
  * query answers of fri::request_init_value_with_merkle /
  * request_step_commit (fri.cpp:148-289).
+ * The transcript ends with the fft_gkr section of the commitment (as
+ * vg_fft_gkr with lg = n - 6; vpd_verifier.cpp:90-99 runs that program and
+ * adds its prover time to the proof's).
  * Writes a malloc'd transcript (free with vg_free).  stage_ms (may be NULL)
- * receives 9 device timings in ms: evaluate, gkr (incl. V_res), then the
+ * receives 10 device timings in ms: evaluate, gkr (incl. V_res), then the
  * commitment, which runs on a second stream beside gkr: lde_l, lde_qh (q and
  * h extensions), fri_fold (virtual oracle + folds), hash (all leaf chains
  * and Merkle trees); then queries (gathers + copies to the host), the SHA3
- * leaf-chain kernel alone (part of hash), and the whole device schedule. */
+ * leaf-chain kernel alone (part of hash), the whole device schedule, and
+ * fft_gkr (on its own stream). */
 int vg_prove(vg_ctx *ctx, uint8_t **transcript, size_t *len, double *stage_ms);
 /* Same, but only enqueues the device work of the proof on the context's
  * stream and returns; vg_finish waits and assembles the transcript.  Lets
@@ -87,6 +91,18 @@ int vg_sha3_64(const uint8_t *in, uint8_t *out, size_t n);
 int vg_merkle_tree(const uint8_t *leaves, int n_leaf, uint8_t *tree);
 /* element-wise F_{p^2} product (prime_field.h:125-151) */
 int vg_fe_mul(const vg_fe *a, const vg_fe *b, vg_fe *out, size_t n);
+
+/* fft_gkr (src/VPD/fft_circuit_GKR.cpp:827-845; the program that
+ * vpd_verifier.cpp:90-99 runs as "./fft_gkr lg log_fftgkr.txt"): the GKR
+ * proof of the FFT/evaluation circuit of a commitment with 2^lg
+ * coefficients, 1 <= lg <= 20, with its own challenge stream (rand() seed 1).
+ * Writes the malloc'd "VGF1" transcript section (free with vg_free): F claim
+ * on the 64 outputs; addition layer (lg+6) x {a,b,c} then v_u; mult layer
+ * lg x {a,b,c} then v_u; per inverse-DFT layer lg x {a,b,c}, v_u,
+ * lg x {a,b,c}, v_v.  accepted: the program's inline verdict (its `result`);
+ * proof_size: bytes as the program counts them; p_ms: device time.  Any of
+ * the last three may be NULL. */
+int vg_fft_gkr(int lg, uint8_t **section, size_t *len, int *accepted, int *proof_size, double *p_ms);
 
 #ifdef __cplusplus
 }

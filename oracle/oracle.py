@@ -50,6 +50,11 @@ def lib():
         L.vgo_leaf_hashes.argtypes = [vp, vp, ctypes.c_int, vp]
         L.vgo_fri_fold.argtypes = [vp, ctypes.c_int, vp, vp]
         L.vgo_eq_table.argtypes = [vp, ctypes.c_int, vp]
+        L.vgo_fft_gkr.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
+                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_double)]
+        L.vgo_fft_gkr_verify.argtypes = [ctypes.c_int, vp, ctypes.c_size_t]
+        L.vgo_fft_gkr_layer.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -74,12 +79,12 @@ def _desc(circ):
 
 
 def prove(circ):
-    """Returns (transcript bytes, times[evaluate, gkr, commit, queries])."""
+    """Returns (transcript bytes, times[evaluate, gkr, commit, queries, fft_gkr])."""
     L = lib()
     d, keep = _desc(circ)
     out = ctypes.c_void_p()
     ln = ctypes.c_size_t()
-    times = np.zeros(4, dtype=np.float64)
+    times = np.zeros(5, dtype=np.float64)
     rc = L.vgo_prove(ctypes.byref(d), ctypes.byref(out), ctypes.byref(ln), _p(times))
     if rc != 0:
         raise RuntimeError("oracle prove failed (%d): %s" % (rc, L.vgo_last_error().decode()))
@@ -94,6 +99,35 @@ def verify(circ, transcript):
     buf = np.frombuffer(transcript, dtype=np.uint8)
     rc = L.vgo_verify(ctypes.byref(d), _p(buf), len(transcript))
     return rc == 1, L.vgo_last_error().decode()
+
+
+def fft_gkr(lg):
+    """fft_gkr prover (fresh seed-1 stream): returns (transcript section,
+    accepted, proof_size, prover seconds)."""
+    L = lib()
+    out = ctypes.c_void_p()
+    ln = ctypes.c_size_t()
+    acc, ps, sec = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    rc = L.vgo_fft_gkr(lg, ctypes.byref(out), ctypes.byref(ln), ctypes.byref(acc), ctypes.byref(ps), ctypes.byref(sec))
+    if rc != 0:
+        raise RuntimeError(L.vgo_last_error().decode())
+    data = ctypes.string_at(out, ln.value)
+    L.vgo_free(out)
+    return data, bool(acc.value), ps.value, sec.value
+
+
+def fft_gkr_verify(lg, section):
+    buf = np.frombuffer(section, dtype=np.uint8)
+    return lib().vgo_fft_gkr_verify(lg, _p(buf), len(section)) == 1
+
+
+def fft_gkr_layer(lg, idx):
+    cap = 64 << lg
+    out = np.zeros(cap, dtype=FE)
+    n = lib().vgo_fft_gkr_layer(lg, idx, _p(out), cap)
+    if n < 0:
+        raise RuntimeError(lib().vgo_last_error().decode())
+    return out[:n].copy()
 
 
 def evaluate(circ):

@@ -27,6 +27,8 @@
  *   33 x { 65 x (F,F) + (n-1) D   (L init query, fri.cpp:148-201)
  *          65 x (F,F) + (n-1) D   (H init query)
  *          for lvl < n-6: 65 x (F,F) + (n-2-lvl) D  (fri.cpp:227-289) }
+ *   fft_gkr section for lg = n-6 ("VGF1", see vgo_fft_gkr below;
+ *   vpd_verifier.cpp:90-99 runs that program during the commitment)
  */
 #include "vg_oracle.h"
 #include <stdlib.h>
@@ -1446,6 +1448,8 @@ static void push_pub_masks(fvec *pub, const lsched *l, int i, fe preu1, fe prev1
 static void tb_q(tbuf *t, qpoly p) { tb_fe(t, p.a); tb_fe(t, p.b); tb_fe(t, p.c); }
 static void tb_q5(tbuf *t, q5poly p) { tb_fe(t, p.a); tb_fe(t, p.b); tb_fe(t, p.c); tb_fe(t, p.d); tb_fe(t, p.e); tb_fe(t, p.f); }
 
+static void fg_prove(int lg, tbuf *t, double *p_sec);
+
 int vgo_prove(const vgo_circuit_desc *dsc, uint8_t **out, size_t *out_len, double *times)
 {
     circuit C;
@@ -1568,7 +1572,10 @@ int vgo_prove(const vgo_circuit_desc *dsc, uint8_t **out, size_t *out_len, doubl
         }
     }
     double t4 = now_s();
-    if (times) { times[0] = t1 - t0; times[1] = t2 - t1; times[2] = t3 - t2; times[3] = t4 - t3; }
+    /* fft_gkr section (vpd_verifier.cpp:90-99: "./fft_gkr n-6", own rand stream) */
+    fg_prove(n - 6, &T, NULL);
+    double t5 = now_s();
+    if (times) { times[0] = t1 - t0; times[1] = t2 - t1; times[2] = t3 - t2; times[3] = t4 - t3; times[4] = t5 - t4; }
     *out = T.p; *out_len = T.n; T.p = NULL;
 done:
     free(q); free(pubm); free(pub.v);
@@ -1813,7 +1820,10 @@ int vgo_verify(const vgo_circuit_desc *dsc, const uint8_t *tr, size_t len)
         for (int j = 1; j < 32; ++j) if (!feq(fin[2048 + j], fin[2048 + j - 1])) REJ("final mask rs code not constant");
     }
     free(fin);
-    if (R.bad || R.off != R.n) REJ("transcript length mismatch (%zu of %zu bytes read)", R.off, R.n);
+    fin = NULL;
+    if (R.bad || R.off > R.n) REJ("transcript truncated");
+    if (!vgo_fft_gkr_verify(n - 6, R.p + R.off, R.n - R.off)) REJ("fft_gkr section rejected");
+    R.off = R.n;
 vdone:
     free(qcoef); free(pubcoef); free(vals); free(path); free(allpath);
     free(pub.v); free(r0); free(r1); free(o0); free(o1);
@@ -1894,3 +1904,456 @@ int vgo_evaluate(const vgo_circuit_desc *dsc, fe *out, size_t cap, int *bits)
     return 0;
 }
 void vgo_free(void *p) { free(p); }
+
+/* ===================================================================== */
+/* fft_gkr: the GKR proof of the commitment's FFT/evaluation circuit      */
+/* (fft_circuit_GKR.cpp:22-825).  The reference runs it as its own        */
+/* process (vpd_verifier.cpp:90-99) with lg = n - 6, so it draws from a   */
+/* fresh rand() stream (seed 1).  Prover messages plus the program's own  */
+/* inline checks (its `result`).                                          */
+/*                                                                        */
+/* Transcript section: "VGF1", u32 lg, F a_0 (claim on the 64 outputs),   */
+/*   addition layer: (lg+6) x {F a,b,c}, F v_u                            */
+/*   mult layer:      lg x {F a,b,c}, F v_u                               */
+/*   per dep = 0..lg-1: lg x {F a,b,c}, F v_u, lg x {F a,b,c}, F v_v      */
+/* ===================================================================== */
+typedef struct {
+    int lg, nl;
+    fe *val[64];
+    int64_t size[64];
+    fe eval_points[64];
+    fe rou, inv_rou, inv_n;
+} fgc;
+
+static void fg_free(fgc *c)
+{
+    for (int i = 0; i < c->nl; ++i) free(c->val[i]);
+    c->nl = 0;
+}
+static fe *fg_push(fgc *c, int64_t n)
+{
+    fe *p = (fe *)calloc((size_t)n, sizeof(fe));
+    c->val[c->nl] = p;
+    c->size[c->nl] = n;
+    c->nl++;
+    return p;
+}
+
+/* build_circuit (fft_circuit_GKR.cpp:22-112) */
+static void fg_build(fgc *c, int lg, const fe *r, grand_t *g)
+{
+    c->lg = lg; c->nl = 0;
+    fe *l0 = fg_push(c, 1);
+    l0[0] = ONE();
+    for (int i = 0; i < lg; ++i) {
+        fe *prev = c->val[i], *cur = fg_push(c, (int64_t)2 << i);
+        for (int j = 0; j < (1 << i); ++j) {
+            cur[j << 1] = mul(prev[j], r[i]);
+            cur[j << 1 | 1] = mul(prev[j], sub(ONE(), r[i]));
+        }
+    }
+    c->rou = root_of_unity(lg);
+    c->inv_rou = finv(c->rou);
+    c->inv_n = fpow(fe_u((uint64_t)1 << lg), P61 - 2);
+    fe rot[62];
+    rot[0] = c->inv_rou;
+    for (int i = 1; i < 62; ++i) rot[i] = mul(rot[i - 1], rot[i - 1]);
+    int64_t N = (int64_t)1 << lg;
+    fe *x_arr = (fe *)malloc(sizeof(fe) * N);
+    for (int dep = lg - 1; dep >= 0; --dep) {          /* :44-75 */
+        int blk = 1 << (lg - dep), half = blk >> 1;
+        fe *pre = c->val[c->nl - 1], *cur = fg_push(c, N);
+        x_arr[0] = ONE();
+        for (int j = 1; j < blk; ++j) x_arr[j] = mul(x_arr[j - 1], rot[dep]);
+        for (int k = 0; k < half; ++k) {
+            int dk = k & (half - 1);
+            for (int j = 0; j < (1 << dep); ++j) {
+                fe lv = pre[(int64_t)dk << (dep + 1) | j];
+                fe rv = mul(x_arr[k], pre[(int64_t)dk << (dep + 1) | (1 << dep) | j]);
+                cur[(int64_t)k << dep | j] = add(lv, rv);
+                cur[(int64_t)(k + half) << dep | j] = sub(lv, rv);
+            }
+        }
+    }
+    free(x_arr);
+    {                                                  /* :76-82 */
+        fe *pre = c->val[c->nl - 1], *cur = fg_push(c, N);
+        for (int64_t i = 0; i < N; ++i) cur[i] = mul(pre[i], c->inv_n);
+    }
+    {                                                  /* :84-102 */
+        fe *pre = c->val[c->nl - 1], *cur = fg_push(c, 64 * N);
+        for (int i = 0; i < 64; ++i) {
+            fe ep = grand_fe(g), x = ONE();
+            c->eval_points[i] = ep;
+            for (int64_t j = 0; j < N; ++j) {
+                cur[j + ((int64_t)i << lg)] = mul(pre[j], x);
+                x = mul(x, ep);
+            }
+        }
+    }
+    {                                                  /* :103-111 */
+        fe *pre = c->val[c->nl - 1], *cur = fg_push(c, 64);
+        for (int i = 0; i < 64; ++i) {
+            fe s = ZERO;
+            for (int64_t j = 0; j < N; ++j) s = add(s, pre[j + (int64_t)i * N]);
+            cur[i] = s;
+        }
+    }
+}
+
+typedef struct {
+    lpoly *V, *aV, *am;
+    fe *b0f, *b0s, *b1f, *b1s, *buf, *bus;
+    fe alpha, beta;
+} fgp;
+
+/* sumcheck_phase1_update / phase2_update (fft_circuit_GKR.cpp:162-246):
+ * the same fold and round sums on the three tables */
+static qpoly fg_update(fgp *p, fe r, int cb, int64_t total)
+{
+    qpoly q = {ZERO, ZERO, ZERO};
+    for (int64_t i = 0; i < (total >> 1); ++i) {
+        int64_t g0 = i << 1, g1 = g0 | 1;
+        lpoly *T[3] = {p->V, p->aV, p->am};
+        for (int t = 0; t < 3; ++t) {
+            lpoly *X = T[t];
+            if (cb == 0) {
+                X[i].b = X[g0].b; X[i].a = sub(X[g1].b, X[i].b);
+            } else {
+                X[i].b = add(mul(X[g0].a, r), X[g0].b);
+                X[i].a = sub(add(mul(X[g1].a, r), X[g1].b), X[i].b);
+            }
+        }
+        q.a = add(q.a, mul(p->am[i].a, p->V[i].a));
+        q.b = add(add(add(q.b, mul(p->am[i].a, p->V[i].b)), mul(p->am[i].b, p->V[i].a)), p->aV[i].a);
+        q.c = add(add(q.c, mul(p->am[i].b, p->V[i].b)), p->aV[i].b);
+    }
+    return q;
+}
+static void fg_refresh(grand_t *g, fe *r, fe *omr, int n)        /* :114-121 */
+{
+    for (int i = 0; i < n; ++i) { r[i] = grand_fe(g); omr[i] = sub(ONE(), r[i]); }
+}
+static inline fe fg_bg(const fgp *p, int64_t g, int fh)
+{
+    int64_t m = ((int64_t)1 << fh) - 1;
+    return add(mul(p->b0f[g & m], p->b0s[g >> fh]), mul(p->b1f[g & m], p->b1s[g >> fh]));
+}
+static void fg_tables_init(fgp *p, const fe *V, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        p->V[i].a = ZERO; p->V[i].b = V[i];
+        p->aV[i].a = p->aV[i].b = ZERO;
+        p->am[i].a = p->am[i].b = ZERO;
+    }
+}
+/* a phase of `bits` rounds (:330-340, :426-436, :604-614, :660-670);
+ * returns V[0] at the last challenge (v_u or v_v) */
+static fe fg_phase(fgp *p, tbuf *t, const fe *rr, int bits)
+{
+    fe prev = ZERO;
+    for (int i = 0; i < bits; ++i) {
+        qpoly q = fg_update(p, prev, i, (int64_t)1 << (bits - i));
+        tb_q(t, q);
+        prev = rr[i];
+    }
+    return add(mul(p->V[0].a, prev), p->V[0].b);
+}
+static void fg_alloc(fgp *p, int lg)
+{
+    int64_t maxn = (int64_t)64 << lg;
+    p->V = (lpoly *)malloc(sizeof(lpoly) * maxn);
+    p->aV = (lpoly *)malloc(sizeof(lpoly) * maxn);
+    p->am = (lpoly *)malloc(sizeof(lpoly) * maxn);
+    int hl = ((lg + 6) >> 1) + 2;
+    p->b0f = (fe *)malloc(sizeof(fe) << hl); p->b0s = (fe *)malloc(sizeof(fe) << hl);
+    p->b1f = (fe *)malloc(sizeof(fe) << hl); p->b1s = (fe *)malloc(sizeof(fe) << hl);
+    p->buf = (fe *)malloc(sizeof(fe) << hl); p->bus = (fe *)malloc(sizeof(fe) << hl);
+}
+static void fg_dealloc(fgp *p)
+{
+    free(p->V); free(p->aV); free(p->am);
+    free(p->b0f); free(p->b0s); free(p->b1f); free(p->b1s); free(p->buf); free(p->bus);
+}
+static void fg_rot(fe inv_rou, fe *rot)
+{
+    rot[0] = inv_rou;
+    for (int i = 1; i < 62; ++i) rot[i] = mul(rot[i - 1], rot[i - 1]);
+}
+/* proof_size as the program counts it: sizeof(quadratic_poly) per round,
+ * plus the extension part it does not prove (extension_gkr :745-754) */
+static int fg_proof_size(int lg)
+{
+    int q = (lg + 6) + lg + 2 * lg * lg;
+    for (int i = 1; i <= lg; ++i) q += i;
+    return 48 * q;
+}
+
+/* engage_gkr (:756-806) and main (:827-845): the prover side */
+static void fg_prove(int lg, tbuf *t, double *p_sec)
+{
+    grand_t g;
+    grand_init(&g, 1);
+    fe r[64];
+    for (int i = 0; i < lg; ++i) r[i] = grand_fe(&g);
+    fgc C;
+    fg_build(&C, lg, r, &g);
+    double t0 = now_s();
+    int64_t N = (int64_t)1 << lg;
+    fgp p;
+    fg_alloc(&p, lg);
+    fe r0[80], omr0[80], r1[80], omr1[80], ru[80], omru[80], rv[80], omrv[80];
+    p.alpha = ONE(); p.beta = ZERO;
+    fg_refresh(&g, r0, omr0, lg + 10);
+    fg_refresh(&g, r1, omr1, lg + 10);
+    tb_put(t, "VGF1", 4);
+    tb_u32(t, (uint32_t)lg);
+    tb_fe(t, V_res(omr0, r0, C.val[C.nl - 1], 6, 64));          /* V_output :134-150 */
+    {                                                            /* addition_layer :248-369 */
+        half_tables(p.b0f, p.b0s, p.alpha, r0, omr0, 6);
+        half_tables(p.b1f, p.b1s, p.beta, r1, omr1, 6);
+        fg_tables_init(&p, C.val[C.nl - 2], 64 * N);
+        for (int i = 0; i < 64; ++i) {
+            fe tmp = fg_bg(&p, i, 3);
+            for (int64_t j = (int64_t)i * N; j < (int64_t)(i + 1) * N; ++j) p.am[j].b = add(p.am[j].b, tmp);
+        }
+        fg_refresh(&g, ru, omru, lg + 6);
+        fg_refresh(&g, rv, omrv, lg + 6);
+        tb_fe(t, fg_phase(&p, t, ru, lg + 6));
+        for (int i = 0; i < lg + 6; ++i) { r0[i] = ru[i]; r1[i] = rv[i]; omr0[i] = omru[i]; omr1[i] = omrv[i]; }
+    }
+    {                                                            /* mult_layer :371-495 */
+        int length_g = lg + 6, fh = length_g >> 1;
+        half_tables(p.b0f, p.b0s, p.alpha, r0, omr0, length_g);
+        half_tables(p.b1f, p.b1s, p.beta, r1, omr1, length_g);
+        fg_tables_init(&p, C.val[C.nl - 3], N);
+        fe x_eval[64];
+        for (int j = 0; j < 64; ++j) x_eval[j] = ONE();
+        for (int64_t i = 0; i < N; ++i)
+            for (int j = 0; j < 64; ++j) {
+                fe tmp = fg_bg(&p, (int64_t)j * N + i, fh);
+                p.am[i].b = add(p.am[i].b, mul(tmp, x_eval[j]));
+                x_eval[j] = mul(x_eval[j], C.eval_points[j]);
+            }
+        fg_refresh(&g, ru, omru, lg);
+        fg_refresh(&g, rv, omrv, lg);
+        tb_fe(t, fg_phase(&p, t, ru, lg));
+        for (int i = 0; i < lg; ++i) { r0[i] = ru[i]; r1[i] = rv[i]; omr0[i] = omru[i]; omr1[i] = omrv[i]; }
+    }
+    fe rot[62];                                                  /* ifft_gkr :504-743 */
+    fg_rot(C.inv_rou, rot);
+    int fh = lg >> 1;
+    int64_t mf = ((int64_t)1 << fh) - 1;
+    for (int dep = 0; dep < lg; ++dep) {
+        int half = 1 << (lg - dep - 1);
+        const fe *pre = C.val[2 * lg - dep - 1];
+        half_tables(p.b0f, p.b0s, p.alpha, r0, omr0, lg);
+        half_tables(p.b1f, p.b1s, p.beta, r1, omr1, lg);
+        fg_tables_init(&p, pre, N);
+        fe x = ONE();
+        for (int k = 0; k < half; ++k) {
+            for (int j = 0; j < (1 << dep); ++j) {
+                int64_t gid = (int64_t)k << dep | j, uid = (int64_t)k << (dep + 1) | j, vid = uid | (1 << dep);
+                fe tmp = fg_bg(&p, gid, fh);
+                p.am[uid].b = add(p.am[uid].b, tmp);
+                p.aV[uid].b = add(p.aV[uid].b, mul(mul(tmp, x), pre[vid]));
+                gid = (int64_t)(k + half) << dep | j;
+                tmp = fg_bg(&p, gid, fh);
+                p.am[uid].b = add(p.am[uid].b, tmp);
+                p.aV[uid].b = sub(p.aV[uid].b, mul(mul(tmp, x), pre[vid]));
+            }
+            x = mul(x, rot[dep]);
+        }
+        fg_refresh(&g, ru, omru, lg);
+        fg_refresh(&g, rv, omrv, lg);
+        fe v_u = fg_phase(&p, t, ru, lg);
+        tb_fe(t, v_u);
+        half_tables(p.buf, p.bus, ONE(), ru, omru, lg);          /* phase 2 init :615-656 */
+        fg_tables_init(&p, pre, N);
+        x = ONE();
+        for (int k = 0; k < half; ++k) {
+            for (int j = 0; j < (1 << dep); ++j) {
+                int64_t gid = (int64_t)k << dep | j, uid = (int64_t)k << (dep + 1) | j, vid = uid | (1 << dep);
+                fe tu = mul(p.buf[uid & mf], p.bus[uid >> fh]);
+                fe tgu = mul(fg_bg(&p, gid, fh), tu);
+                p.am[vid].b = add(p.am[vid].b, mul(tgu, x));
+                p.aV[vid].b = add(p.aV[vid].b, mul(tgu, v_u));
+                gid = (int64_t)(k + half) << dep | j;
+                tgu = mul(fg_bg(&p, gid, fh), tu);
+                p.am[vid].b = sub(p.am[vid].b, mul(tgu, x));
+                p.aV[vid].b = add(p.aV[vid].b, mul(tgu, v_u));
+            }
+            x = mul(x, rot[dep]);
+        }
+        tb_fe(t, fg_phase(&p, t, rv, lg));
+        for (int i = 0; i < lg; ++i) { r0[i] = ru[i]; r1[i] = rv[i]; omr0[i] = omru[i]; omr1[i] = omrv[i]; }
+        p.alpha = grand_fe(&g);
+        p.beta = grand_fe(&g);
+    }
+    if (p_sec) *p_sec = now_s() - t0;
+    fg_dealloc(&p);
+    fg_free(&C);
+}
+
+/* the program's inline verifier checks, replayed on a transcript section:
+ * round sums (:342-352 and like), the closed forms that end each layer
+ * (:327-352, :456-484, :677-731) and the claim chaining between layers */
+static int fg_rounds(trd *rd, const fe *rr, int bits, fe *claim)
+{
+    int ok = 1;
+    for (int i = 0; i < bits; ++i) {
+        qpoly q = tr_q(rd);
+        if (!feq(add(q_eval(q, ZERO), q_eval(q, ONE())), *claim)) ok = 0;
+        *claim = q_eval(q, rr[i]);
+    }
+    return ok;
+}
+int vgo_fft_gkr_verify(int lg, const uint8_t *tr, size_t len)
+{
+    if (lg < 1 || lg > 24) return 0;
+    trd rd = {tr, len, 0, 0};
+    char magic[4];
+    tr_get(&rd, magic, 4);
+    uint32_t l32;
+    tr_get(&rd, &l32, 4);
+    if (memcmp(magic, "VGF1", 4) || (int)l32 != lg) return 0;
+    grand_t g;
+    grand_init(&g, 1);
+    fe r[64];
+    for (int i = 0; i < lg; ++i) r[i] = grand_fe(&g);
+    fgc C;
+    fg_build(&C, lg, r, &g);
+    fgp p;
+    int hl = ((lg + 6) >> 1) + 2;
+    p.b0f = (fe *)malloc(sizeof(fe) << hl); p.b0s = (fe *)malloc(sizeof(fe) << hl);
+    p.b1f = (fe *)malloc(sizeof(fe) << hl); p.b1s = (fe *)malloc(sizeof(fe) << hl);
+    fe r0[80], omr0[80], r1[80], omr1[80], ru[80], omru[80], rv[80], omrv[80];
+    fe one = ONE();
+    int ok = 1;
+    p.alpha = one; p.beta = ZERO;
+    fg_refresh(&g, r0, omr0, lg + 10);
+    fg_refresh(&g, r1, omr1, lg + 10);
+    fe claim = tr_fe(&rd);
+    if (!feq(claim, V_res(omr0, r0, C.val[C.nl - 1], 6, 64))) ok = 0;
+    {                                                            /* addition layer */
+        half_tables(p.b0f, p.b0s, p.alpha, r0, omr0, 6);
+        half_tables(p.b1f, p.b1s, p.beta, r1, omr1, 6);
+        int log_uv = lg + 6;
+        fg_refresh(&g, ru, omru, log_uv);
+        fg_refresh(&g, rv, omrv, log_uv);
+        ok &= fg_rounds(&rd, ru, log_uv, &claim);
+        fe v_u = tr_fe(&rd);
+        fe sv = ZERO;
+        for (int i = 0; i < 64; ++i) {
+            fe tg = fg_bg(&p, i, 3), tu = one;
+            for (int j = 0; j < 6; ++j)
+                tu = mul(tu, ((i >> j) & 1) ? ru[log_uv - 6 + j] : sub(one, ru[log_uv - 6 + j]));
+            sv = add(sv, mul(tg, tu));
+        }
+        for (int i = 0; i < log_uv; ++i) { r0[i] = ru[i]; r1[i] = rv[i]; omr0[i] = omru[i]; omr1[i] = omrv[i]; }
+        if (!feq(claim, mul(sv, v_u))) ok = 0;
+        claim = mul(p.alpha, v_u);
+    }
+    {                                                            /* mult layer */
+        int length_g = lg + 6;
+        fg_refresh(&g, ru, omru, lg);
+        fg_refresh(&g, rv, omrv, lg);
+        ok &= fg_rounds(&rd, ru, lg, &claim);
+        fe v_u = tr_fe(&rd);
+        fe sm = ZERO;
+        for (int i = 0; i < 64; ++i) {
+            fe tg0 = p.alpha, tg1 = p.beta;
+            for (int j = 0; j < 6; ++j) {
+                int b = (i >> j) & 1;
+                tg0 = mul(tg0, b ? r0[length_g - 6 + j] : sub(one, r0[length_g - 6 + j]));
+                tg1 = mul(tg1, b ? r1[length_g - 6 + j] : sub(one, r1[length_g - 6 + j]));
+            }
+            fe tu0 = one, tu1 = one, x = C.eval_points[i];
+            for (int j = 0; j < lg; ++j) {
+                tu0 = mul(tu0, add(mul(mul(r0[j], ru[j]), x), mul(sub(one, r0[j]), sub(one, ru[j]))));
+                tu1 = mul(tu1, add(mul(mul(r1[j], ru[j]), x), mul(sub(one, r1[j]), sub(one, ru[j]))));
+                x = mul(x, x);
+            }
+            sm = add(add(sm, mul(tg0, tu0)), mul(tg1, tu1));
+        }
+        for (int i = 0; i < lg; ++i) { r0[i] = ru[i]; r1[i] = rv[i]; omr0[i] = omru[i]; omr1[i] = omrv[i]; }
+        if (!feq(claim, mul(sm, v_u))) ok = 0;
+        claim = mul(p.alpha, v_u);
+    }
+    claim = mul(claim, fe_u((uint64_t)1 << lg));                 /* intermediate_layer */
+    fe rot[62];
+    fg_rot(C.inv_rou, rot);
+    for (int dep = 0; dep < lg; ++dep) {                          /* ifft layers */
+        fg_refresh(&g, ru, omru, lg);
+        fg_refresh(&g, rv, omrv, lg);
+        ok &= fg_rounds(&rd, ru, lg, &claim);
+        fe v_u = tr_fe(&rd);
+        ok &= fg_rounds(&rd, rv, lg, &claim);
+        fe v_v = tr_fe(&rd);
+        fe xr = rot[dep];
+        int log_k = lg - dep - 1, log_j = dep;
+        fe uv = mul(sub(one, ru[log_j]), rv[log_j]);
+        fe u0A = mul(mul(sub(one, r0[lg - 1]), uv), p.alpha), u1A = mul(mul(sub(one, r1[lg - 1]), uv), p.beta);
+        fe v0A = u0A, v1A = u1A;
+        fe u0B = mul(mul(r0[lg - 1], uv), p.alpha), u1B = mul(mul(r1[lg - 1], uv), p.beta);
+        fe v0B = u0B, v1B = u1B;
+        for (int i = 0; i < log_k; ++i) {
+            fe a0 = r0[log_j + i], a1 = r1[log_j + i], bu = ru[log_j + 1 + i], bv = rv[log_j + 1 + i];
+            fe nn = mul(sub(one, bu), sub(one, bv)), pp = mul(bu, bv);
+            fe t0 = add(mul(a0, pp), mul(sub(one, a0), nn));
+            fe t1 = add(mul(a1, pp), mul(sub(one, a1), nn));
+            fe t0x = add(mul(mul(a0, pp), xr), mul(sub(one, a0), nn));
+            fe t1x = add(mul(mul(a1, pp), xr), mul(sub(one, a1), nn));
+            u0A = mul(u0A, t0); u1A = mul(u1A, t1); v0A = mul(v0A, t0x); v1A = mul(v1A, t1x);
+            u0B = mul(u0B, t0); u1B = mul(u1B, t1); v0B = mul(v0B, t0x); v1B = mul(v1B, t1x);
+            xr = mul(xr, xr);
+        }
+        for (int i = 0; i < log_j; ++i) {
+            fe nn = mul(sub(one, ru[i]), sub(one, rv[i])), pp = mul(ru[i], rv[i]);
+            fe t0 = add(mul(r0[i], pp), mul(sub(one, r0[i]), nn));
+            fe t1 = add(mul(r1[i], pp), mul(sub(one, r1[i]), nn));
+            u0A = mul(u0A, t0); u1A = mul(u1A, t1); v0A = mul(v0A, t0); v1A = mul(v1A, t1);
+            u0B = mul(u0B, t0); u1B = mul(u1B, t1); v0B = mul(v0B, t0); v1B = mul(v1B, t1);
+        }
+        fe want = add(mul(add(add(add(u0A, u1A), u0B), u1B), v_u), mul(sub(sub(add(v0A, v1A), v0B), v1B), v_v));
+        if (!feq(claim, want)) ok = 0;
+        for (int i = 0; i < lg; ++i) { r0[i] = ru[i]; r1[i] = rv[i]; omr0[i] = omru[i]; omr1[i] = omrv[i]; }
+        p.alpha = grand_fe(&g);
+        p.beta = grand_fe(&g);
+        claim = add(mul(p.alpha, v_u), mul(p.beta, v_v));
+    }
+    if (rd.bad || rd.off != rd.n) ok = 0;
+    free(p.b0f); free(p.b0s); free(p.b1f); free(p.b1s);
+    fg_free(&C);
+    return ok;
+}
+
+int vgo_fft_gkr(int lg, uint8_t **out, size_t *out_len, int *accepted, int *proof_size, double *p_sec)
+{
+    if (lg < 1 || lg > 24) FAIL(-1, "fft_gkr: lg %d out of range", lg);
+    tbuf t = {0};
+    fg_prove(lg, &t, p_sec);
+    if (accepted) *accepted = vgo_fft_gkr_verify(lg, t.p, t.n);
+    if (proof_size) *proof_size = fg_proof_size(lg);
+    *out = t.p;
+    *out_len = t.n;
+    return 0;
+}
+
+/* one layer of the FFT circuit (for closed-form tests): index as in
+ * C.circuit_val (0 = the constant 1 ... 2lg+3 = the 64 outputs) */
+int vgo_fft_gkr_layer(int lg, int idx, fe *out, size_t cap)
+{
+    if (lg < 1 || lg > 20 || idx < 0 || idx > 2 * lg + 3) FAIL(-1, "bad layer");
+    grand_t g;
+    grand_init(&g, 1);
+    fe r[64];
+    for (int i = 0; i < lg; ++i) r[i] = grand_fe(&g);
+    fgc C;
+    fg_build(&C, lg, r, &g);
+    int64_t n = C.size[idx];
+    if ((size_t)n > cap) { fg_free(&C); FAIL(-2, "capacity"); }
+    memcpy(out, C.val[idx], sizeof(fe) * n);
+    fg_free(&C);
+    return (int)n;
+}

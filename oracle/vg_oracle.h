@@ -78,14 +78,24 @@ void vgo_eq_table(const vgo_fe *r, int len, vgo_fe *out);
 /* ---- whole protocol ---- */
 /* Proves the circuit with the reference's challenge stream (glibc rand, seed
  * 1).  Writes a malloc'd transcript.  Returns 0 on success, else an error
- * code (the message is in vgo_last_error()).  times[0..3] (seconds):
- * evaluate, gkr sumcheck, commitment (LDE + Merkle + FRI), queries. */
+ * code (the message is in vgo_last_error()).  times[0..4] (seconds):
+ * evaluate, gkr sumcheck, commitment (LDE + Merkle + FRI), queries, fft_gkr
+ * (whose section ends the transcript). */
 int  vgo_prove(const vgo_circuit_desc *c, uint8_t **transcript, size_t *len, double *times);
 /* Restated verifier (zk_verifier.cpp:910-1253, vpd_verifier.cpp:80-335) run
  * on a transcript.  Returns 1 = accept, 0 = reject, <0 = malformed input. */
 int  vgo_verify(const vgo_circuit_desc *c, const uint8_t *transcript, size_t len);
 /* Evaluates the circuit and returns the output layer (for cross-checks). */
 int  vgo_evaluate(const vgo_circuit_desc *c, vgo_fe *out, size_t out_cap, int *out_bits);
+
+/* fft_gkr (fft_circuit_GKR.cpp:22-845): prover of the FFT/evaluation circuit
+ * with lg = log2(coefficient count); fresh rand() stream (seed 1).  Writes
+ * the "VGF1" transcript section, the program's verdict (its inline checks,
+ * replayed by vgo_fft_gkr_verify), its proof_size and prover seconds. */
+int  vgo_fft_gkr(int lg, uint8_t **out, size_t *out_len, int *accepted, int *proof_size, double *p_sec);
+int  vgo_fft_gkr_verify(int lg, const uint8_t *tr, size_t len);
+/* layer idx of the circuit (0 .. 2lg+3); returns its length */
+int  vgo_fft_gkr_layer(int lg, int idx, vgo_fe *out, size_t cap);
 void vgo_free(void *p);
 const char *vgo_last_error(void);
 

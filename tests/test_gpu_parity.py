@@ -108,6 +108,18 @@ def test_full_size_c3_accepted():
     p.close()
 
 
+@pytest.mark.parametrize("lg", [1, 4, 9, 14])
+def test_fft_gkr_bit_exact(lg):
+    """fft_gkr section (lg = 9: the 32x32 matmul commitment, 14: the 2^20
+    one): bytes, verdict and proof size equal the oracle's."""
+    print("fft_gkr lg", lg, flush=True)
+    sec, ok, psize, ms = V.fft_gkr(lg)
+    ref, rok, rsize, _ = O.fft_gkr(lg)
+    assert ok and rok and psize == rsize
+    assert sec == ref
+    assert O.fft_gkr_verify(lg, sec)
+
+
 def test_rejects_bad_circuit():
     circ = W.random_circuit(10, 3, seed=1)
     circ["u"][int(circ["n"][0]) + 5] = 1 << 20          # out-of-range wire

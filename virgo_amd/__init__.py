@@ -23,7 +23,7 @@ FE = np.dtype([("img", "<u8"), ("real", "<u8")])     # prime_field.h:93 layout
 
 EXPORTS = ["vg_init", "vg_last_error", "vg_free", "vg_ctx_create", "vg_ctx_destroy", "vg_ctx_device_bytes",
            "vg_prove", "vg_prove_async", "vg_finish", "vg_output", "vg_fft", "vg_ifft", "vg_sha3_64",
-           "vg_merkle_tree", "vg_fe_mul"]
+           "vg_merkle_tree", "vg_fe_mul", "vg_fft_gkr"]
 
 
 class VirgoError(RuntimeError):
@@ -62,6 +62,9 @@ def lib():
         L.vg_sha3_64.argtypes = [vp, vp, ctypes.c_size_t]
         L.vg_merkle_tree.argtypes = [vp, ctypes.c_int, vp]
         L.vg_fe_mul.argtypes = [vp, vp, vp, ctypes.c_size_t]
+        L.vg_fft_gkr.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t),
+                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                 ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -90,7 +93,8 @@ class _Desc(ctypes.Structure):
                 ("u", ctypes.c_void_p), ("v", ctypes.c_void_p)]
 
 
-STAGES = ["evaluate", "gkr", "lde_l", "lde_qh", "fri_fold", "hash", "queries", "leaf_kernel", "device_total"]
+STAGES = ["evaluate", "gkr", "lde_l", "lde_qh", "fri_fold", "hash", "queries", "leaf_kernel", "device_total",
+          "fft_gkr"]
 
 
 class Prover:
@@ -106,7 +110,7 @@ class Prover:
         d = _Desc(len(n), _p(n), _p(ty), _p(u), _p(v))
         self._ctx = ctypes.c_void_p()
         _check(lib().vg_ctx_create(ctypes.byref(self._ctx), ctypes.byref(d), stream))
-        self.stage_ms = np.zeros(9, dtype=np.float64)
+        self.stage_ms = np.zeros(len(STAGES), dtype=np.float64)
 
     def device_bytes(self):
         return int(lib().vg_ctx_device_bytes(self._ctx))
@@ -155,6 +159,20 @@ def prove(circ):
         return p.prove()
     finally:
         p.close()
+
+
+def fft_gkr(lg):
+    """fft_gkr proof of the FFT/evaluation circuit with 2^lg coefficients:
+    returns (transcript section, accepted, proof_size, device ms)."""
+    init()
+    out = ctypes.c_void_p()
+    ln = ctypes.c_size_t()
+    acc, ps, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    _check(lib().vg_fft_gkr(lg, ctypes.byref(out), ctypes.byref(ln), ctypes.byref(acc), ctypes.byref(ps),
+                            ctypes.byref(ms)))
+    data = ctypes.string_at(out, ln.value)
+    lib().vg_free(out)
+    return data, bool(acc.value), ps.value, ms.value
 
 
 def fe_mul(a, b):

@@ -1,0 +1,26 @@
+// fft_gkr on the device: the GKR proof of the commitment's FFT/evaluation
+// circuit (src/VPD/fft_circuit_GKR.cpp:22-845).  The reference runs it as a
+// separate program ("./fft_gkr lg out", vpd_verifier.cpp:90-99), so it draws
+// its challenges from a fresh rand() stream (seed 1) and depends on lg only.
+#pragma once
+#include <stdint.h>
+#include <vector>
+#include <hip/hip_runtime.h>
+
+namespace vg {
+
+struct fg_ctx;
+
+// Allocates every device buffer of one fft_gkr proof with 2^lg coefficients;
+// work runs on `stream` (nullptr: a stream owned by the context).
+int fg_create(fg_ctx **out, int lg, hipStream_t stream);
+void fg_destroy(fg_ctx *c);
+int64_t fg_device_bytes(const fg_ctx *c);
+// Draws the challenge stream and enqueues the whole proof (upload, circuit,
+// sumchecks, copy of the messages to the host); returns without waiting.
+int fg_enqueue(fg_ctx *c);
+// Waits for the proof, appends the "VGF1" transcript section to `out` and
+// replays the program's inline checks (its verdict).  p_ms: device time.
+int fg_finish(fg_ctx *c, std::vector<uint8_t> &out, int *accepted, int *proof_size, double *p_ms);
+
+}  // namespace vg

@@ -1,0 +1,55 @@
+// Host-side helpers shared by the prover drivers: the reference's challenge
+// generator and a byte buffer for transcripts.
+#pragma once
+#include <stdint.h>
+#include <vector>
+#include "vg_field.hpp"
+
+namespace vg {
+
+// ---------------------------------------------------------------------------
+// glibc rand() TYPE_3 (additive feedback, r[i] = r[i-3] + r[i-31]), seed 1;
+// prime_field::random (prime_field.cpp:35-41) draws real, then img.
+// ---------------------------------------------------------------------------
+struct glibc_rand {
+    uint32_t s[34];
+    uint64_t k;
+    explicit glibc_rand(uint32_t seed)
+    {
+        int32_t r[34];
+        r[0] = (int32_t)(seed ? seed : 1);
+        for (int i = 1; i < 31; ++i) {
+            int64_t t = (16807LL * r[i - 1]) % 2147483647LL;
+            if (t < 0) t += 2147483647LL;
+            r[i] = (int32_t)t;
+        }
+        for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
+        for (int i = 0; i < 34; ++i) s[i] = (uint32_t)r[i];
+        k = 34;
+        for (int i = 0; i < 310; ++i) next();
+    }
+    uint32_t next()
+    {
+        uint32_t v = s[(k - 31) % 34] + s[(k - 3) % 34];
+        s[k % 34] = v;
+        ++k;
+        return v >> 1;
+    }
+    fe field()
+    {
+        fe r;
+        r.real = next() % P;
+        r.img = next() % P;
+        return r;
+    }
+};
+
+// transcript bytes: F = 16 bytes {img, real}, little-endian u32
+struct tbuf {
+    std::vector<uint8_t> b;
+    void put(const void *p, size_t n) { const uint8_t *c = (const uint8_t *)p; b.insert(b.end(), c, c + n); }
+    void f(const fe &x) { put(&x, 16); }
+    void u32(uint32_t x) { put(&x, 4); }
+};
+
+}  // namespace vg

@@ -276,9 +276,8 @@ __device__ __forceinline__ void fold_terms(const lpoly &V, const lpoly &aV, cons
 // ---------------------------------------------------------------------
 constexpr int FB_LOG = 10;
 constexpr int FB_MAXB = 1024;
-__global__ __launch_bounds__(256) void k_fold_blk(int j0, int b, int L, const fe *Vb, const fe *aVb, const fe *amb, ctabs X,
-                                                 const fe *r, tabs out, int64_t nchunks, fe *part, fe *sums, fe *end,
-                                                 fe vconst)
+__device__ __forceinline__ void fold_body(int j0, int b, int L, const fe *Vb, const fe *aVb, const fe *amb, ctabs X,
+                                          const fe *r, tabs out, int64_t nchunks, fe *part, fe *sums, fe *end, fe vconst)
 {
     __shared__ lpoly sV[1 << (FB_LOG - 1)], saV[1 << (FB_LOG - 1)], sam[1 << (FB_LOG - 1)];
     __shared__ fe red[FB_LOG][4][3];
@@ -353,6 +352,25 @@ __global__ __launch_bounds__(256) void k_fold_blk(int j0, int b, int L, const fe
         end[0] = V0.a; end[1] = V0.b; end[2] = am0.a; end[3] = am0.b;
         end[4] = add(add(mul(V0.a, r[L - 1]), V0.b), vconst);
     }
+}
+__global__ __launch_bounds__(256) void k_fold_blk(int j0, int b, int L, const fe *Vb, const fe *aVb, const fe *amb, ctabs X,
+                                                 const fe *r, tabs out, int64_t nchunks, fe *part, fe *sums, fe *end,
+                                                 fe vconst)
+{
+    fold_body(j0, b, L, Vb, aVb, amb, X, r, out, nchunks, part, sums, end, vconst);
+}
+// the same launch for blockIdx.y = instance of a batch of phases with equal L;
+// launch k of the plan reads ws[cur ^ 1] (or the flat tables if j0 == 0) and
+// writes ws[cur]; part = this launch's first slot round of the instance
+__global__ __launch_bounds__(256) void k_fold_multi(const launch::fold_inst *I, int cur, int j0, int b, int L,
+                                                   int64_t nchunks, int slot_round)
+{
+    const launch::fold_inst &t = I[blockIdx.y];
+    ctabs X = {t.V[cur ^ 1], t.aV[cur ^ 1], t.am[cur ^ 1]};
+    tabs out = {t.V[cur], t.aV[cur], t.am[cur]};
+    fe vc = t.vconst ? *t.vconst : fzero();
+    fold_body(j0, b, L, t.Vb, t.aVb, t.amb, X, t.r, out, nchunks, t.slots + (int64_t)slot_round * 3 * FB_MAXB, t.sums,
+              t.end, vc);
 }
 
 // round sums of every wave-fold launch of the proof: slot R (a round) holds
@@ -727,6 +745,133 @@ void ntt_run(const ntt_in &in, fe *out, int64_t out_stride, int64_t nb, int log_
     }
 }
 
+
+// ---------------------------------------------------------------------
+// fft_gkr: the FFT/evaluation circuit of the commitment and the table builds
+// of its GKR layers (fft_circuit_GKR.cpp:22-743).  N = 2^lg coefficients.
+// ---------------------------------------------------------------------
+// layer lg of the circuit: out[x] = prod_i (bit (lg-1-i) of x ? 1 - r_i : r_i)
+__global__ void k_fg_eq(fe *out, const fe *r, int lg)
+{
+    int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (x >= ((int64_t)1 << lg)) return;
+    fe one = fone(), v = one;
+    for (int i = 0; i < lg; ++i) v = mul(v, ((x >> (lg - 1 - i)) & 1) ? sub(one, r[i]) : r[i]);
+    out[x] = v;
+}
+// power tables of npts points: lo[i][t] = x_i^t (t < 2^lo), hi[i][s] = x_i^(s 2^lo)
+__global__ void k_fg_pow(const fe *pts, int npts, int lo, int hi, fe *plo, fe *phi)
+{
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t nlo = (int64_t)npts << lo, nhi = (int64_t)npts << hi;
+    if (tid >= nlo + nhi) return;
+    bool isl = tid < nlo;
+    int64_t k = isl ? tid : tid - nlo;
+    int bits = isl ? lo : hi;
+    int i = (int)(k >> bits);
+    uint64_t e = (uint64_t)(k & (((int64_t)1 << bits) - 1));
+    if (!isl) e <<= lo;
+    fe x = pts[i], acc = fone();
+    while (e) { if (e & 1) acc = mul(acc, x); x = mul(x, x); e >>= 1; }
+    (isl ? plo : phi)[k] = acc;
+}
+__device__ __forceinline__ fe fg_pw(const fe *plo, const fe *phi, int lo, int hi, int i, int64_t e)
+{
+    return mul(plo[((int64_t)i << lo) + (e & (((int64_t)1 << lo) - 1))], phi[((int64_t)i << hi) + (e >> lo)]);
+}
+// one butterfly layer of the circuit's inverse DFT (:44-75); the x_arr[k]
+// twiddle is inv_rou^(k 2^dep); scale (the 1/N of :76-82) when do_scale
+__global__ void k_fg_butterfly(const fe *pre, fe *cur, int dep, int lg, const fe *plo, const fe *phi, int lo, int hi,
+                               int tw_pt, fe scale, int do_scale)
+{
+    int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t half = (int64_t)1 << (lg - 1);
+    if (idx >= half) return;
+    int64_t k = idx >> dep, j = idx & (((int64_t)1 << dep) - 1);
+    fe x = fg_pw(plo, phi, lo, hi, tw_pt, k << dep);
+    fe l = pre[k << (dep + 1) | j], r = mul(x, pre[k << (dep + 1) | ((int64_t)1 << dep) | j]);
+    fe a = add(l, r), b = sub(l, r);
+    int64_t h2 = (int64_t)1 << (lg - dep - 1);
+    if (do_scale) { a = mul(a, scale); b = mul(b, scale); }
+    cur[k << dep | j] = a;
+    cur[(k + h2) << dep | j] = b;
+}
+// the 64 x N evaluation layer (:84-102): out[j + (i << lg)] = coef[j] x_i^j
+__global__ void k_fg_eval(const fe *coef, const fe *plo, const fe *phi, int lo, int hi, int lg, fe *out)
+{
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= ((int64_t)64 << lg)) return;
+    int i = (int)(tid >> lg);
+    int64_t j = tid & (((int64_t)1 << lg) - 1);
+    out[tid] = mul(coef[j], fg_pw(plo, phi, lo, hi, i, j));
+}
+// the 64 outputs (:103-111): one block per row
+__global__ __launch_bounds__(TPB) void k_fg_rowsum(const fe *ev, int lg, fe *outs)
+{
+    int64_t N = (int64_t)1 << lg;
+    const fe *row = ev + blockIdx.x * N;
+    fe acc = fzero();
+    for (int64_t j = threadIdx.x; j < N; j += blockDim.x) acc = add(acc, row[j]);
+    block_sum<1>(&acc);
+    if (threadIdx.x == 0) outs[blockIdx.x] = acc;
+}
+// addition layer tables (:289-298): amb[j] = beta_g(j >> lg), aVb = 0
+__global__ void k_fg_add_tables(bg_t bg, int lg, fe *amb, fe *aVb)
+{
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= ((int64_t)64 << lg)) return;
+    amb[j] = beta_g(bg, (int)(j >> lg));
+    aVb[j] = fzero();
+}
+// mult layer tables (:411-421): amb[i] = sum_j beta_g(j N + i) x_j^i, aVb = 0
+__global__ void k_fg_mult_tables(bg_t bg, const fe *plo, const fe *phi, int lo, int hi, int lg, fe *amb, fe *aVb)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= ((int64_t)1 << lg)) return;
+    fe acc = fzero();
+    for (int j = 0; j < 64; ++j)
+        acc = add(acc, mul(beta_g(bg, (int)(((int64_t)j << lg) + i)), fg_pw(plo, phi, lo, hi, j, i)));
+    amb[i] = acc;
+    aVb[i] = fzero();
+}
+// ifft layers, phase 1 (:583-605), blockIdx.y = layer: gate g = k 2^dep + j
+// and g + N/2 both read u = k 2^(dep+1) + j and v = u + 2^dep
+__global__ void k_fg_ifft_p1m(const launch::fg_layer_dev *D, const fe *plo, const fe *phi, int lo, int hi, int tw_pt,
+                              int lg)
+{
+    const launch::fg_layer_dev &d = D[blockIdx.y];
+    int dep = d.dep;
+    int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (u >= ((int64_t)1 << lg)) return;
+    if ((u >> dep) & 1) { d.amb[u] = fzero(); d.aVb[u] = fzero(); return; }
+    bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
+    int64_t k = u >> (dep + 1), j = u & (((int64_t)1 << dep) - 1);
+    int g = (int)(k << dep | j);
+    fe b1 = beta_g(bg, g), b2 = beta_g(bg, g + (1 << (lg - 1)));
+    fe xv = mul(fg_pw(plo, phi, lo, hi, tw_pt, k << dep), d.pre[u | ((int64_t)1 << dep)]);
+    d.amb[u] = add(b1, b2);
+    d.aVb[u] = sub(mul(b1, xv), mul(b2, xv));
+}
+// ifft layers, phase 2 (:638-665), blockIdx.y = layer: entries v = u + 2^dep
+__global__ void k_fg_ifft_p2m(const launch::fg_layer_dev *D, const fe *plo, const fe *phi, int lo, int hi, int tw_pt,
+                              int lg)
+{
+    const launch::fg_layer_dev &d = D[blockIdx.y];
+    int dep = d.dep;
+    int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= ((int64_t)1 << lg)) return;
+    if (!((v >> dep) & 1)) { d.amb[v] = fzero(); d.aVb[v] = fzero(); return; }
+    bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
+    bu_t bu = {d.bu.f, d.bu.s, d.bu.fh};
+    int64_t u = v ^ ((int64_t)1 << dep);
+    int64_t k = u >> (dep + 1), j = u & (((int64_t)1 << dep) - 1);
+    int g = (int)(k << dep | j);
+    fe tu = beta_u(bu, (int)u);
+    fe t1 = mul(beta_g(bg, g), tu), t2 = mul(beta_g(bg, g + (1 << (lg - 1))), tu);
+    fe x = fg_pw(plo, phi, lo, hi, tw_pt, k << dep), vu = *d.vu;
+    d.amb[v] = sub(mul(t1, x), mul(t2, x));
+    d.aVb[v] = add(mul(t1, vu), mul(t2, vu));
+}
 }  // namespace
 
 namespace launch {
@@ -839,6 +984,23 @@ int sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe 
     tabs none = {nullptr, nullptr, nullptr};
     hipLaunchKernelGGL(k_fold_blk, dim3(1), dim3(256), 0, s, j, L - j, L, Vb, aVb, amb, X, r, none, (int64_t)1,
                        (fe *)nullptr, sums, end, vconst);
+    return rounds;
+}
+int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s)
+{
+    int js[32], ks[32];
+    int nl = fold_plan(L, js, ks);
+    int cur = 0, rounds = 0;
+    for (int k = 0; k < nl; ++k) {
+        int j = js[k], b = ks[k];
+        int64_t nch = (int64_t)1 << (L - j - b);
+        unsigned g = (unsigned)(nch > FB_MAXB ? FB_MAXB : nch);
+        hipLaunchKernelGGL(k_fold_multi, dim3(g, ninst), dim3(256), 0, s, I, cur, j, b, L, nch, rounds);
+        cur ^= 1;
+        rounds += b;
+    }
+    int j = nl ? js[nl - 1] + ks[nl - 1] : 0;
+    hipLaunchKernelGGL(k_fold_multi, dim3(1, ninst), dim3(256), 0, s, I, cur, j, L - j, L, (int64_t)1, rounds);
     return rounds;
 }
 void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s)
@@ -954,5 +1116,49 @@ void gather16(const uint64_t *src, int64_t n, fe *out, hipStream_t s)
     if (n) hipLaunchKernelGGL(k_gather16, dim3(nblk(n)), dim3(TPB), 0, s, src, n, out);
 }
 
+
+// ---- fft_gkr ----
+void fg_eq(fe *out, const fe *r, int lg, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fg_eq, dim3(nblk((int64_t)1 << lg)), dim3(TPB), 0, s, out, r, lg);
+}
+void fg_pow(const fe *pts, int npts, int lo, int hi, fe *plo, fe *phi, hipStream_t s)
+{
+    int64_t n = ((int64_t)npts << lo) + ((int64_t)npts << hi);
+    hipLaunchKernelGGL(k_fg_pow, dim3(nblk(n)), dim3(TPB), 0, s, pts, npts, lo, hi, plo, phi);
+}
+void fg_butterfly(const fe *pre, fe *cur, int dep, int lg, fg_pw_t pw, fe scale, int do_scale, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fg_butterfly, dim3(nblk((int64_t)1 << (lg - 1))), dim3(TPB), 0, s, pre, cur, dep, lg, pw.plo,
+                       pw.phi, pw.lo, pw.hi, pw.tw_pt, scale, do_scale);
+}
+void fg_eval(const fe *coef, fg_pw_t pw, int lg, fe *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fg_eval, dim3(nblk((int64_t)64 << lg)), dim3(TPB), 0, s, coef, pw.plo, pw.phi, pw.lo, pw.hi,
+                       lg, out);
+}
+void fg_rowsum(const fe *ev, int lg, fe *outs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fg_rowsum, dim3(64), dim3(TPB), 0, s, ev, lg, outs);
+}
+void fg_add_tables(beta_g bg, int lg, fe *amb, fe *aVb, hipStream_t s)
+{
+    bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
+    hipLaunchKernelGGL(k_fg_add_tables, dim3(nblk((int64_t)64 << lg)), dim3(TPB), 0, s, b, lg, amb, aVb);
+}
+void fg_mult_tables(beta_g bg, fg_pw_t pw, int lg, fe *amb, fe *aVb, hipStream_t s)
+{
+    bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
+    hipLaunchKernelGGL(k_fg_mult_tables, dim3(nblk((int64_t)1 << lg)), dim3(TPB), 0, s, b, pw.plo, pw.phi, pw.lo,
+                       pw.hi, lg, amb, aVb);
+}
+void fg_ifft_tables(int phase, const fg_layer_dev *D, int nlayer, fg_pw_t pw, int lg, hipStream_t s)
+{
+    dim3 grid(nblk((int64_t)1 << lg), nlayer);
+    if (phase == 0)
+        hipLaunchKernelGGL(k_fg_ifft_p1m, grid, dim3(TPB), 0, s, D, pw.plo, pw.phi, pw.lo, pw.hi, pw.tw_pt, lg);
+    else
+        hipLaunchKernelGGL(k_fg_ifft_p2m, grid, dim3(TPB), 0, s, D, pw.plo, pw.phi, pw.lo, pw.hi, pw.tw_pt, lg);
+}
 }  // namespace launch
 }  // namespace vg

@@ -57,22 +57,38 @@ void p1_gather(const uint32_t *off, const int32_t *ent, gates_dev g, int nt, con
 // sum, exp-sum, NOT and relay gates, which all add into addV[0].
 void p2_gather(const uint32_t *off, const int32_t *ent, const int32_t *zlist, int nz, gates_dev g, int nt, beta_g bg,
                beta_u bu, const fe *vu, fe *amb, fe *aVb, fe *part, hipStream_t s);
-// Runs every round of one sumcheck phase on tables of 2^L entries
-// (zk_prover.cpp:600-633, 992-1025).  Round j folds with r[j-1] (r[-1]
-// unused).  After the last round: end[0..3] = V[0].a, V[0].b, am[0].a,
-// am[0].b and end[4] = V[0].a * r[L-1] + V[0].b + vconst (= v_u or v_v).
-// All rounds of a phase on plain tables: block folds of up to 10 rounds per
+// Runs every round of one sumcheck phase on flat tables of 2^L entries
+// (zk_prover.cpp:600-633, 992-1025): block folds of up to 10 rounds per
 // launch (chunks of 1024 entries in LDS), then the last <= 10 rounds in one
-// block.  Launch k of the plan (fold_plan) writes its block partials of
-// round q to slots[(q*3 + c) * FW_BLOCKS + block]; the caller reduces every
-// slot of the proof with reduce_slots.  Tail round sums go straight to
-// sums[3j..].  Returns the number of slot rounds used.
+// block.  Round j folds with r[j-1] (r[-1] unused).  Launch k of the plan
+// (fold_plan) writes its block partials of round q to
+// slots[(q*3 + c) * FW_BLOCKS + block]; the caller reduces every slot of the
+// proof with reduce_slots.  Tail round sums go straight to sums[3j..].  After
+// the last round: end[0..3] = V[0].a, V[0].b, am[0].a, am[0].b and
+// end[4] = V[0].a * r[L-1] + V[0].b + vconst (= v_u or v_v).  Returns the
+// number of slot rounds used.
 constexpr int FW_BLOCKS = 1024;
+// entries of each folded table a phase on 2^L entries writes (the chunk
+// results of its first block-fold launch)
+inline int64_t fold_ws_len(int L) { return L > 20 ? ((int64_t)1 << (L - 10)) : 1024; }
 int fold_plan(int L, int *js, int *ks);
 int fold_blocks(int L, int j);
 int sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *slots,
                     fe *sums, fe *end, hipStream_t s);
 void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s);
+// one phase of each of a batch of sumchecks with equal L, as sumcheck_phase2
+// but every pointer per instance (device array I[ninst]); instance t's slot
+// rounds are t.slots[(round*3 + c) * FW_BLOCKS + block].  Returns the slot
+// rounds per instance.
+struct fold_inst {
+    const fe *Vb, *aVb, *amb;       // flat tables
+    lpoly *V[2], *aV[2], *am[2];    // chunk results, fold_ws_len(L) entries each
+    const fe *r;                    // the phase's challenges
+    fe *slots;                      // the instance's slot rounds
+    fe *sums, *end;                 // tail round sums, end values
+    const fe *vconst;               // added to end[4]; null = 0
+};
+int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s);
 struct half_desc {            // one eq half table: out[j] = f0 * prod_{i<nbits} eq(bit_i(j), r[i])
     fe *out;
     const fe *f0;
@@ -128,6 +144,34 @@ void leaf_hash_multi(int nseg, const fe *const *cw, const int64_t *S, uint8_t *c
 void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStream_t s);
 // out[k] = 16 bytes at address src[k]
 void gather16(const uint64_t *src, int64_t n, fe *out, hipStream_t s);
+
+// ---- fft_gkr (fft_circuit_GKR.cpp:22-743), N = 2^lg ----
+struct fg_pw_t {              // power tables of 65 points: 64 evaluation points, then inv_rou
+    const fe *plo, *phi;      // x_i^t (t < 2^lo) and x_i^(s 2^lo) (s < 2^hi), row i
+    int lo, hi;
+    int tw_pt;                // row of inv_rou
+};
+// layer lg of the circuit: out[x] = prod_i (bit (lg-1-i) of x ? 1 - r_i : r_i)
+void fg_eq(fe *out, const fe *r, int lg, hipStream_t s);
+void fg_pow(const fe *pts, int npts, int lo, int hi, fe *plo, fe *phi, hipStream_t s);
+// one butterfly layer of the circuit's inverse DFT, times scale if do_scale
+void fg_butterfly(const fe *pre, fe *cur, int dep, int lg, fg_pw_t pw, fe scale, int do_scale, hipStream_t s);
+// out[j + (i << lg)] = coef[j] x_i^j for the 64 evaluation points
+void fg_eval(const fe *coef, fg_pw_t pw, int lg, fe *out, hipStream_t s);
+void fg_rowsum(const fe *ev, int lg, fe *outs, hipStream_t s);
+// sumcheck tables of the addition, mult and inverse-DFT layers
+void fg_add_tables(beta_g bg, int lg, fe *amb, fe *aVb, hipStream_t s);
+void fg_mult_tables(beta_g bg, fg_pw_t pw, int lg, fe *amb, fe *aVb, hipStream_t s);
+struct fg_layer_dev {          // one inverse-DFT layer of the circuit
+    beta_g bg;                // beta_g half tables
+    beta_u bu;                // beta_u half tables (phase 2)
+    const fe *pre;            // the layer's input values
+    fe *amb, *aVb;            // its flat tables
+    const fe *vu;             // v_u (end[4] of its phase 1)
+    int dep;
+};
+// phase (0 or 1) tables of every inverse-DFT layer in one launch
+void fg_ifft_tables(int phase, const fg_layer_dev *D, int nlayer, fg_pw_t pw, int lg, hipStream_t s);
 
 }  // namespace launch
 }  // namespace vg

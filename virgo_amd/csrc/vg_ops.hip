@@ -30,7 +30,7 @@ int fail(const char *what, hipError_t e)
 
 struct dbuf {
     void *p = nullptr;
-    ~dbuf() { if (p) hipFree(p); }
+    ~dbuf() { if (p) (void)hipFree(p); }
 };
 int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
 bool pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }

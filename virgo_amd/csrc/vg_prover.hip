@@ -16,6 +16,8 @@
 #include <vector>
 
 #include "vg_kernels.hpp"
+#include "vg_host.hpp"
+#include "vg_fftgkr.hpp"
 #include "../../include/virgo.h"
 
 using namespace vg;
@@ -26,43 +28,6 @@ void vg_set_error(const std::string &msg) { g_err = msg; }
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) FAILV(-20, "%s: %s", #x, hipGetErrorString(e_)); } while (0)
 
 namespace {
-
-// ---------------------------------------------------------------------------
-// glibc rand() TYPE_3 (additive feedback, r[i] = r[i-3] + r[i-31]), seed 1;
-// prime_field::random (prime_field.cpp:35-41) draws real, then img.
-// ---------------------------------------------------------------------------
-struct glibc_rand {
-    uint32_t s[34];
-    uint64_t k;
-    explicit glibc_rand(uint32_t seed)
-    {
-        int32_t r[34];
-        r[0] = (int32_t)(seed ? seed : 1);
-        for (int i = 1; i < 31; ++i) {
-            int64_t t = (16807LL * r[i - 1]) % 2147483647LL;
-            if (t < 0) t += 2147483647LL;
-            r[i] = (int32_t)t;
-        }
-        for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
-        for (int i = 0; i < 34; ++i) s[i] = (uint32_t)r[i];
-        k = 34;
-        for (int i = 0; i < 310; ++i) next();
-    }
-    uint32_t next()
-    {
-        uint32_t v = s[(k - 31) % 34] + s[(k - 3) % 34];
-        s[k % 34] = v;
-        ++k;
-        return v >> 1;
-    }
-    fe field()
-    {
-        fe r;
-        r.real = next() % P;
-        r.img = next() % P;
-        return r;
-    }
-};
 
 // ---------------------------------------------------------------------------
 // circuit, as zk_verifier::read_circuit builds it (zk_verifier.cpp:15-217)
@@ -445,13 +410,6 @@ void lastbit_terms(const fe *lb, fe d, fe e, fe &A, fe &B, fe &Dd, fe &E, fe &Fv
     B = add(B, mul(mul(c, e), Z));
 }
 
-struct tbuf {
-    std::vector<uint8_t> b;
-    void put(const void *p, size_t n) { const uint8_t *c = (const uint8_t *)p; b.insert(b.end(), c, c + n); }
-    void f(const fe &x) { put(&x, 16); }
-    void u32(uint32_t x) { put(&x, 4); }
-};
-
 int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
 
 }  // namespace
@@ -530,6 +488,7 @@ struct vg_ctx {
     std::chrono::steady_clock::time_point t_start;
     bool pending = false;
     bool serial = false;        // VIRGO_SERIAL=1: commitment after the sumcheck on one stream
+    fg_ctx *fg = nullptr;       // fft_gkr of the commitment (lg = n - 6), on its own stream
 };
 
 namespace {
@@ -580,7 +539,7 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     c->ht = carve<fe>(base, off, (int64_t)D * 8 * c->ht_stride + 2 * ((int64_t)1 << (C.bl[0] / 2 + 1)));
     c->hdesc = carve<launch::half_desc>(base, off, (int64_t)D * 8 + 2);
     for (int w = 0; w < vg_ctx::NGS; ++w)
-        for (int k = 0; k < 6; ++k) c->tabs_s[w][k] = carve<lpoly>(base, off, T / 2 > 0 ? T / 2 : 1);
+        for (int k = 0; k < 6; ++k) c->tabs_s[w][k] = carve<lpoly>(base, off, launch::fold_ws_len(maxbl));
     c->part_half = 18 * 1024;
     for (int w = 0; w < vg_ctx::NGS; ++w) c->part_s[w] = carve<fe>(base, off, 2 * c->part_half);
     c->slots = carve<fe>(base, off, (int64_t)(c->n_slots > 0 ? c->n_slots : 1) * 3 * launch::FW_BLOCKS);
@@ -1164,6 +1123,8 @@ int vg_ctx_create(vg_ctx **out, const vg_circuit_desc *d, void *stream)
     }
     rc = ctx_init(c);
     if (rc) { vg_ctx_destroy(c); return rc; }
+    rc = fg_create(&c->fg, c->n - 6, c->serial ? c->st : nullptr);
+    if (rc) { vg_ctx_destroy(c); return rc; }
     *out = c;
     return 0;
 }
@@ -1175,6 +1136,7 @@ int vg_ctx_destroy(vg_ctx *c)
     if (c->st2) (void)hipStreamSynchronize(c->st2);
     for (int w = 0; w < vg_ctx::NGS; ++w)
         if (c->gst[w]) (void)hipStreamSynchronize(c->gst[w]);
+    fg_destroy(c->fg);
     if (c->arena) (void)hipFree(c->arena);
     if (c->h_chal) (void)hipHostFree(c->h_chal);
     if (c->h_gsrc) (void)hipHostFree(c->h_gsrc);
@@ -1192,7 +1154,7 @@ int vg_ctx_destroy(vg_ctx *c)
     return 0;
 }
 
-int64_t vg_ctx_device_bytes(const vg_ctx *c) { return c ? c->arena_bytes : 0; }
+int64_t vg_ctx_device_bytes(const vg_ctx *c) { return c ? c->arena_bytes + fg_device_bytes(c->fg) : 0; }
 
 int vg_prove_async(vg_ctx *c)
 {
@@ -1204,6 +1166,8 @@ int vg_prove_async(vg_ctx *c)
     rc = build_gather(c);
     if (rc) return rc;
     rc = enqueue(c);
+    if (rc) return rc;
+    rc = fg_enqueue(c->fg);
     if (rc) return rc;
     c->pending = true;
     return 0;
@@ -1217,6 +1181,9 @@ int vg_finish(vg_ctx *c, uint8_t **transcript, size_t *len, double *stage_ms)
     HIPCHK(hipGetLastError());
     tbuf T;
     assemble(c, T);
+    double fg_ms = 0;
+    int rc = fg_finish(c->fg, T.b, nullptr, nullptr, &fg_ms);
+    if (rc) return rc;
     uint8_t *p = (uint8_t *)malloc(T.b.size());
     if (!p) FAILV(-5, "out of host memory");
     memcpy(p, T.b.data(), T.b.size());
@@ -1231,6 +1198,7 @@ int vg_finish(vg_ctx *c, uint8_t **transcript, size_t *len, double *stage_ms)
             else (void)hipEventElapsedTime(&ms, c->ev[pairs[k][0]], c->ev[pairs[k][1]]);
             stage_ms[k] = ms;
         }
+        stage_ms[9] = fg_ms;
     }
     return 0;
 }

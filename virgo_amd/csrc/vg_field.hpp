@@ -6,9 +6,13 @@
 // the reference does, so any exact evaluation order (parallel reductions,
 // atomics, different FFT factorisations) produces the reference's bits.
 //
-// Multiplication is built for the CDNA integer VALU: a 64x64 product is
-// split into four 32x32->64 partial products (v_mad_u64_u32) and folded with
-// 2^61 == 1 and 2^64 == 8 (mod p); there is no 128-bit intermediate.
+// Multiplication is built for the CDNA integer VALU: operands (< 2^61) are
+// split into 31- and 30-bit limbs, so the 32x32->64 partial products
+// (v_mad_u64_u32, which accumulates a 64-bit addend for free) of up to two
+// products sum in 64 bits without overflow, and are folded once with
+// 2^61 == 1 (mod p).  An F_{p^2} product is two such two-term dot products:
+// re = xr yr + (p - xi) yi, im = xr yi + xi yr (16 v_mad_u64_u32, one
+// reduction per component, no 128-bit intermediate).
 #pragma once
 #include <stdint.h>
 #include <hip/hip_runtime.h>
@@ -34,19 +38,34 @@ VG_HD uint64_t addp(uint64_t a, uint64_t b) { uint64_t s = a + b; return s >= P 
 VG_HD uint64_t subp(uint64_t a, uint64_t b) { return a >= b ? a - b : a + P - b; }
 VG_HD uint64_t negp(uint64_t a) { return a ? P - a : 0; }
 
-// a, b < 2^62 (so Karatsuba sums of canonical values are allowed).
+// s mod p for any 64-bit s
+VG_HD uint64_t fold61(uint64_t s)
+{
+    uint64_t r = (s & P) + (s >> 61);
+    return r >= P ? r - P : r;
+}
+// a*b mod p for a, b < 2^61
 VG_HD uint64_t mulp(uint64_t a, uint64_t b)
 {
-    uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
-    uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
-    uint64_t t0 = (uint64_t)a0 * b0;                          // < 2^64
-    uint64_t t1 = (uint64_t)a1 * b0 + (uint64_t)a0 * b1;      // < 2^63
+    uint32_t a0 = (uint32_t)a & 0x7FFFFFFFu, a1 = (uint32_t)(a >> 31);
+    uint32_t b0 = (uint32_t)b & 0x7FFFFFFFu, b1 = (uint32_t)(b >> 31);
+    uint64_t t0 = (uint64_t)a0 * b0;                          // < 2^62
+    uint64_t t1 = (uint64_t)a1 * b0 + (uint64_t)a0 * b1;      // < 2^62
     uint64_t t2 = (uint64_t)a1 * b1;                          // < 2^60
-    // a*b = t0 + t1*2^32 + t2*2^64
-    //    == (t0 mod 2^61) + (t0 >> 61) + (t1 mod 2^29)*2^32 + (t1 >> 29) + 8*t2
-    uint64_t s = (t0 & P) + (t0 >> 61) + ((t1 & 0x1FFFFFFFull) << 32) + (t1 >> 29) + (t2 << 3);
-    s = (s & P) + (s >> 61);
-    return s >= P ? s - P : s;
+    // a*b = t0 + t1 2^31 + t2 2^62 == t0 + (t1 mod 2^30) 2^31 + (t1 >> 30) + 2 t2
+    return fold61(t0 + (t2 << 1) + ((t1 & 0x3FFFFFFFull) << 31) + (t1 >> 30));
+}
+// a*b + c*d mod p for a, b, c, d < 2^61 (each partial sum < 2^63, total < 2^64)
+VG_HD uint64_t dot2p(uint64_t a, uint64_t b, uint64_t c, uint64_t d)
+{
+    uint32_t a0 = (uint32_t)a & 0x7FFFFFFFu, a1 = (uint32_t)(a >> 31);
+    uint32_t b0 = (uint32_t)b & 0x7FFFFFFFu, b1 = (uint32_t)(b >> 31);
+    uint32_t c0 = (uint32_t)c & 0x7FFFFFFFu, c1 = (uint32_t)(c >> 31);
+    uint32_t d0 = (uint32_t)d & 0x7FFFFFFFu, d1 = (uint32_t)(d >> 31);
+    uint64_t t0 = (uint64_t)a0 * b0 + (uint64_t)c0 * d0;
+    uint64_t t1 = (uint64_t)a1 * b0 + (uint64_t)a0 * b1 + (uint64_t)c1 * d0 + (uint64_t)c0 * d1;
+    uint64_t t2 = (uint64_t)a1 * b1 + (uint64_t)c1 * d1;
+    return fold61(t0 + (t2 << 1) + ((t1 & 0x3FFFFFFFull) << 31) + (t1 >> 30));
 }
 
 VG_HD fe add(const fe &a, const fe &b) { return mk(addp(a.real, b.real), addp(a.img, b.img)); }
@@ -54,13 +73,11 @@ VG_HD fe sub(const fe &a, const fe &b) { return mk(subp(a.real, b.real), subp(a.
 VG_HD fe neg(const fe &a) { return mk(negp(a.real), negp(a.img)); }
 VG_HD fe dbl(const fe &a) { return add(a, a); }
 
-// (a + b i)(c + d i) with three base products (prime_field.h:125-151)
+// (a + b i)(c + d i) = (ac - bd) + (ad + bc) i (prime_field.h:125-151),
+// as two dot products with -b = p - b (p itself is a valid limb input)
 VG_HD fe mul(const fe &x, const fe &y)
 {
-    uint64_t ac = mulp(x.real, y.real);
-    uint64_t bd = mulp(x.img, y.img);
-    uint64_t t = mulp(x.real + x.img, y.real + y.img);
-    return mk(subp(ac, bd), subp(t, addp(ac, bd)));
+    return mk(dot2p(x.real, y.real, P - x.img, y.img), dot2p(x.real, y.img, x.img, y.real));
 }
 // multiply by an element of the base field F_p (img == 0)
 VG_HD fe mul_base(const fe &x, uint64_t c) { return mk(mulp(x.real, c), mulp(x.img, c)); }

@@ -12,7 +12,6 @@ namespace {
 constexpr int TPB = 256;
 constexpr int NTT_LM = 11;          // LDS-resident NTT block: 2^11 elements = 32 KB
 constexpr int FOLD_TAIL = 512;      // MLE rounds with <= this many outputs run in one block
-constexpr int MAX_PARTS = 1024;     // max blocks of a reduction with per-block partials
 
 __host__ __device__ inline unsigned nblk(int64_t n, int t = TPB) { return (unsigned)((n + t - 1) / t); }
 
@@ -85,7 +84,7 @@ __global__ void k_eval_layer(const uint8_t *ty, const int32_t *gu, const int32_t
     case 7: r = sub(p[u], p[v]); break;
     case 8: { fe x = p[u], y = p[v]; r = sub(add(x, y), mul(mul(mk(2, 0), x), y)); break; }
     case 9: { fe x = p[u], y = p[v]; r = sub(y, mul(x, y)); break; }
-    case 12: { r = fzero(); for (int k = u; k <= v; ++k) r = add(r, mul_base(p[k], 1ull << (k - u))); break; }
+    case 12: { r = fzero(); for (int k = u; k <= v; ++k) r = add(r, mul_base(p[k], 1ull << ((k - u) % 61))); break; }
     case 13: r = mul(p[u], sub(fone(), p[v])); break;
     default: r = fzero(); break;
     }
@@ -156,15 +155,22 @@ __device__ void reduce_prev(const fe *prev, int nprev, fe *dst)
 // (:845-972): entry t sums the gates whose v is t; the gates that feed entry
 // 0 unconditionally (sum, exp-sum, NOT, relay) are reduced separately.
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(TPB) void k_p1_gather(const uint32_t *off, const int32_t *ent, const uint8_t *gty,
-                                                  const int32_t *gu, const int32_t *gv, int nt, const fe *pv, bg_t bg,
-                                                  fe *amb, fe *aVb)
+// Table builds of one sumcheck phase for a batch of layers (blockIdx.y =
+// layer), by gather over the CSR transposes built once per circuit: entry t
+// of the tables sums the contributions of the gates that reach it through u
+// (phase 1, zk_prover.cpp:375-592) or v (phase 2, :789-983).
+__global__ __launch_bounds__(TPB) void k_p1_gather(const launch::layer_dev *D)
 {
+    const launch::layer_dev &d = D[blockIdx.y];
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
+    if (t >= d.nt) return;
+    bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
+    const uint8_t *gty = d.ty;
+    const int32_t *gu = d.u, *gv = d.v;
+    const fe *pv = d.pv;
     fe am = fzero(), aV = fzero();
-    for (uint32_t e = off[t]; e < off[t + 1]; ++e) {
-        int g = ent[e];
+    for (uint32_t e = d.off1[t]; e < d.off1[t + 1]; ++e) {
+        int g = d.ent1[e];
         int ty = gty[g], u = gu[g], v = gv[g];
         if (ty == 4) { am = add(am, beta_g(bg, u)); continue; }
         fe tmp = beta_g(bg, g);
@@ -172,7 +178,7 @@ __global__ __launch_bounds__(TPB) void k_p1_gather(const uint32_t *off, const in
         case 0: aV = add(aV, mul(pv[v], tmp)); am = add(am, tmp); break;
         case 1: am = add(am, mul(pv[v], tmp)); break;
         case 5: am = add(am, tmp); break;
-        case 12: am = add(am, mul_base(tmp, 1ull << (t - u))); break;
+        case 12: am = add(am, mul_base(tmp, 1ull << ((t - u) % 61))); break;   // 2^61 == 1
         case 6: am = sub(am, tmp); aV = add(aV, tmp); break;
         case 7: aV = sub(aV, mul(pv[v], tmp)); am = add(am, tmp); break;
         case 8: { fe tv = mul(tmp, pv[v]); aV = add(aV, tv); am = sub(add(am, tmp), dbl(tv)); break; }
@@ -182,20 +188,21 @@ __global__ __launch_bounds__(TPB) void k_p1_gather(const uint32_t *off, const in
         default: break;
         }
     }
-    amb[t] = am;
-    aVb[t] = aV;
+    d.amb[t] = am;
+    d.aVb[t] = aV;
 }
-__global__ __launch_bounds__(TPB) void k_p2_gather(const uint32_t *off, const int32_t *ent, const uint8_t *gty,
-                                                  const int32_t *gu, int nt, bg_t bg, bu_t bu, const fe *vup, fe *amb,
-                                                  fe *aVb)
+__global__ __launch_bounds__(TPB) void k_p2_gather(const launch::layer_dev *D)
 {
+    const launch::layer_dev &d = D[blockIdx.y];
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    fe vu = *vup;
+    if (t >= d.nt) return;
+    bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
+    bu_t bu = {d.bu.f, d.bu.s, d.bu.fh};
+    fe vu = *d.vu;
     fe am = fzero(), aV = fzero();
-    for (uint32_t e = off[t]; e < off[t + 1]; ++e) {
-        int g = ent[e];
-        int ty = gty[g], u = gu[g];
+    for (uint32_t e = d.off2[t]; e < d.off2[t + 1]; ++e) {
+        int g = d.ent2[e];
+        int ty = d.ty[g], u = d.u[g];
         fe tgu = mul(beta_g(bg, g), beta_u(bu, u));
         switch (ty) {
         case 1: am = add(am, mul(tgu, vu)); break;
@@ -207,18 +214,20 @@ __global__ __launch_bounds__(TPB) void k_p2_gather(const uint32_t *off, const in
         default: break;
         }
     }
-    amb[t] = am;
-    aVb[t] = aV;
+    d.amb[t] = am;
+    d.aVb[t] = aV;
 }
-// gates adding into addV[0] in phase 2; per-block partials
-__global__ __launch_bounds__(TPB) void k_p2_zero(const int32_t *list, int n, const uint8_t *gty, const int32_t *gu,
-                                                const int32_t *gv, bg_t bg, bu_t bu, const fe *vup, fe *part)
+// gates adding into addV[0] in phase 2 (sum, exp-sum, NOT, relay); per-block partials
+__global__ __launch_bounds__(TPB) void k_p2_zero(const launch::layer_dev *D)
 {
+    const launch::layer_dev &d = D[blockIdx.y];
+    bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
+    bu_t bu = {d.bu.f, d.bu.s, d.bu.fh};
     fe acc = fzero();
-    fe vu = *vup;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        int g = list[k];
-        int ty = gty[g], u = gu[g], v = gv[g];
+    fe vu = *d.vu;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < d.nz; k += gridDim.x * blockDim.x) {
+        int g = d.z2[k];
+        int ty = d.ty[g], u = d.u[g], v = d.v[g];
         fe tg = beta_g(bg, g);
         switch (ty) {
         case 5: { fe t = mul(tg, vu); for (int j = u; j < v; ++j) acc = add(acc, mul(t, beta_u(bu, j))); break; }
@@ -229,14 +238,15 @@ __global__ __launch_bounds__(TPB) void k_p2_zero(const int32_t *list, int n, con
         }
     }
     block_sum<1>(&acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    if (threadIdx.x == 0) d.part[blockIdx.x] = acc;
 }
-__global__ void k_add_parts(const fe *part, int n, fe *dst)
+__global__ void k_add_parts(const launch::layer_dev *D, int n)
 {
+    const launch::layer_dev &d = D[blockIdx.y];
     fe acc = fzero();
-    for (int k = threadIdx.x; k < n; k += blockDim.x) acc = add(acc, part[k]);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) acc = add(acc, d.part[k]);
     block_sum<1>(&acc);
-    if (threadIdx.x == 0) *dst = add(*dst, acc);
+    if (threadIdx.x == 0) d.aVb[0] = add(d.aVb[0], acc);
 }
 
 // ---------------------------------------------------------------------
@@ -353,12 +363,6 @@ __device__ __forceinline__ void fold_body(int j0, int b, int L, const fe *Vb, co
         end[4] = add(add(mul(V0.a, r[L - 1]), V0.b), vconst);
     }
 }
-__global__ __launch_bounds__(256) void k_fold_blk(int j0, int b, int L, const fe *Vb, const fe *aVb, const fe *amb, ctabs X,
-                                                 const fe *r, tabs out, int64_t nchunks, fe *part, fe *sums, fe *end,
-                                                 fe vconst)
-{
-    fold_body(j0, b, L, Vb, aVb, amb, X, r, out, nchunks, part, sums, end, vconst);
-}
 // the same launch for blockIdx.y = instance of a batch of phases with equal L;
 // launch k of the plan reads ws[cur ^ 1] (or the flat tables if j0 == 0) and
 // writes ws[cur]; part = this launch's first slot round of the instance
@@ -405,14 +409,17 @@ __global__ void k_half_multi(const launch::half_desc *d, int nd, int64_t total)
 // ---------------------------------------------------------------------
 // wiring predicates combined with v_u, v_v (zk_verifier.cpp:219-606, :1149)
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(TPB) void k_general(const uint8_t *gty, const int32_t *gu, const int32_t *gv, int n, bg_t bg,
-                                                bu_t bu, bu_t bv, const fe *vup, const fe *vvp, fe *part)
+__global__ __launch_bounds__(TPB) void k_general(const launch::layer_dev *D)
 {
-    fe vu = *vup, vv = *vvp, one = fone();
+    const launch::layer_dev &d = D[blockIdx.y];
+    if (!d.gv) return;                                   // layer 1: no general value
+    bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
+    bu_t bu = {d.bu.f, d.bu.s, d.bu.fh}, bv = {d.bv.f, d.bv.s, d.bv.fh};
+    fe vu = *d.vu, vv = *d.vv, one = fone();
     fe uv = mul(vu, vv);
     fe acc = fzero();
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int ty = gty[i], u = gu[i], v = gv[i];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += gridDim.x * blockDim.x) {
+        int ty = d.ty[i], u = d.u[i], v = d.v[i];
         if (ty == 2 || ty == 3 || ty == 4) continue;
         fe tg = beta_g(bg, i);
         fe f;
@@ -440,14 +447,16 @@ __global__ __launch_bounds__(TPB) void k_general(const uint8_t *gty, const int32
         }
     }
     block_sum<1>(&acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    if (threadIdx.x == 0) d.part[blockIdx.x] = acc;
 }
-__global__ void k_reduce(const fe *part, int n, fe *dst)
+__global__ void k_reduce(const launch::layer_dev *D, int n)
 {
+    const launch::layer_dev &d = D[blockIdx.y];
+    if (!d.gv) return;
     fe acc = fzero();
-    for (int k = threadIdx.x; k < n; k += blockDim.x) acc = add(acc, part[k]);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) acc = add(acc, d.part[k]);
     block_sum<1>(&acc);
-    if (threadIdx.x == 0) *dst = acc;
+    if (threadIdx.x == 0) *d.gv = acc;
 }
 
 // ---------------------------------------------------------------------
@@ -921,23 +930,19 @@ void eq_expand(fe *out, const fe *lo, const fe *hi, int fh, int64_t n, hipStream
 {
     hipLaunchKernelGGL(k_eq_expand, dim3(nblk(n)), dim3(TPB), 0, st, out, lo, hi, fh, n);
 }
-void p1_gather(const uint32_t *off, const int32_t *ent, gates_dev g, int nt, const fe *prevV, beta_g bg, fe *amb, fe *aVb,
-               hipStream_t s)
+void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipStream_t s)
 {
-    bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
-    hipLaunchKernelGGL(k_p1_gather, dim3(nblk(nt)), dim3(TPB), 0, s, off, ent, g.ty, g.u, g.v, nt, prevV, b, amb, aVb);
-}
-void p2_gather(const uint32_t *off, const int32_t *ent, const int32_t *zlist, int nz, gates_dev g, int nt, beta_g bg,
-               beta_u bu, const fe *vu, fe *amb, fe *aVb, fe *part, hipStream_t s)
-{
-    bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
-    bu_t u = {bu.f, bu.s, bu.fh};
-    hipLaunchKernelGGL(k_p2_gather, dim3(nblk(nt)), dim3(TPB), 0, s, off, ent, g.ty, g.u, nt, b, u, vu, amb, aVb);
-    if (nz > 0) {
-        unsigned blocks = nblk(nz);
+    dim3 grid(nblk(nt), n);
+    if (phase == 0) {
+        hipLaunchKernelGGL(k_p1_gather, grid, dim3(TPB), 0, s, D);
+        return;
+    }
+    hipLaunchKernelGGL(k_p2_gather, grid, dim3(TPB), 0, s, D);
+    if (nz_max > 0) {
+        unsigned blocks = nblk(nz_max);
         if (blocks > MAX_PARTS) blocks = MAX_PARTS;
-        hipLaunchKernelGGL(k_p2_zero, dim3(blocks), dim3(TPB), 0, s, zlist, nz, g.ty, g.u, g.v, b, u, vu, part);
-        hipLaunchKernelGGL(k_add_parts, dim3(1), dim3(1024), 0, s, part, (int)blocks, aVb);
+        hipLaunchKernelGGL(k_p2_zero, dim3(blocks, n), dim3(TPB), 0, s, D);
+        hipLaunchKernelGGL(k_add_parts, dim3(1, n), dim3(1024), 0, s, D, (int)blocks);
     }
 }
 int fold_plan(int L, int *js, int *ks)
@@ -961,30 +966,6 @@ int fold_blocks(int L, int j)
             return (int)(ch > FB_MAXB ? FB_MAXB : ch);
         }
     return 0;
-}
-int sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *slots,
-                    fe *sums, fe *end, hipStream_t s)
-{
-    int js[32], ks[32];
-    int nl = fold_plan(L, js, ks);
-    int cur = 0, rounds = 0;
-    for (int k = 0; k < nl; ++k) {
-        int j = js[k], b = ks[k];
-        int64_t nch = (int64_t)1 << (L - j - b);
-        unsigned g = (unsigned)(nch > FB_MAXB ? FB_MAXB : nch);
-        ctabs in = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
-        tabs out = {ws.V[cur], ws.aV[cur], ws.am[cur]};
-        hipLaunchKernelGGL(k_fold_blk, dim3(g), dim3(256), 0, s, j, b, L, Vb, aVb, amb, in, r, out, nch,
-                           slots + (int64_t)rounds * 3 * FB_MAXB, (fe *)nullptr, (fe *)nullptr, fzero());
-        cur ^= 1;
-        rounds += b;
-    }
-    int j = nl ? js[nl - 1] + ks[nl - 1] : 0;
-    ctabs X = {ws.V[cur ^ 1], ws.aV[cur ^ 1], ws.am[cur ^ 1]};
-    tabs none = {nullptr, nullptr, nullptr};
-    hipLaunchKernelGGL(k_fold_blk, dim3(1), dim3(256), 0, s, j, L - j, L, Vb, aVb, amb, X, r, none, (int64_t)1,
-                       (fe *)nullptr, sums, end, vconst);
-    return rounds;
 }
 int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s)
 {
@@ -1011,15 +992,12 @@ void half_multi(const half_desc *d, int nd, int64_t total, hipStream_t s)
 {
     if (total) hipLaunchKernelGGL(k_half_multi, dim3(nblk(total)), dim3(TPB), 0, s, d, nd, total);
 }
-void general_value(gates_dev g, beta_g bg, beta_u bu, beta_u bv, const fe *vu, const fe *vv, fe *part, fe *dst, hipStream_t s)
+void general_multi(const layer_dev *D, int n, int ng_max, hipStream_t s)
 {
-    int n = 1 << g.bl;
-    unsigned blocks = nblk(n);
+    unsigned blocks = nblk(ng_max);
     if (blocks > MAX_PARTS) blocks = MAX_PARTS;
-    bg_t b = {bg.g0f, bg.g0s, bg.g1f, bg.g1s, bg.fh};
-    bu_t u = {bu.f, bu.s, bu.fh}, v = {bv.f, bv.s, bv.fh};
-    hipLaunchKernelGGL(k_general, dim3(blocks), dim3(TPB), 0, s, g.ty, g.u, g.v, n, b, u, v, vu, vv, part);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, part, (int)blocks, dst);
+    hipLaunchKernelGGL(k_general, dim3(blocks, n), dim3(TPB), 0, s, D);
+    hipLaunchKernelGGL(k_reduce, dim3(1, n), dim3(1024), 0, s, D, (int)blocks);
 }
 void twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2, hipStream_t s)
 {

@@ -43,43 +43,46 @@ void eq_expand(fe *out, const fe *lo, const fe *hi, int fh, int64_t n, hipStream
 struct beta_g { const fe *g0f, *g0s, *g1f, *g1s; int fh; };
 struct beta_u { const fe *f, *s; int fh; };
 
-struct fold_ws {              // workspace of a sumcheck phase
-    lpoly *V[2], *aV[2], *am[2];
-    fe *part;                 // two halves of per-block partial sums
-    int64_t part_half;        // >= 18 * (2^(L-1) / 256) for the widest layer
-};
 
-// table builds by gather over CSR transposes (built once per circuit).
-// p1: off/ent list, per table entry t, the gates feeding t through u.
-void p1_gather(const uint32_t *off, const int32_t *ent, gates_dev g, int nt, const fe *prevV, beta_g bg, fe *amb, fe *aVb,
-               hipStream_t s);
-// p2: off/ent by v for add/mult/minus/xor/naab/bit-test gates; zlist: the
-// sum, exp-sum, NOT and relay gates, which all add into addV[0].
-void p2_gather(const uint32_t *off, const int32_t *ent, const int32_t *zlist, int nz, gates_dev g, int nt, beta_g bg,
-               beta_u bu, const fe *vu, fe *amb, fe *aVb, fe *part, hipStream_t s);
-// Runs every round of one sumcheck phase on flat tables of 2^L entries
-// (zk_prover.cpp:600-633, 992-1025): block folds of up to 10 rounds per
-// launch (chunks of 1024 entries in LDS), then the last <= 10 rounds in one
-// block.  Round j folds with r[j-1] (r[-1] unused).  Launch k of the plan
-// (fold_plan) writes its block partials of round q to
-// slots[(q*3 + c) * FW_BLOCKS + block]; the caller reduces every slot of the
-// proof with reduce_slots.  Tail round sums go straight to sums[3j..].  After
+// One GKR layer of the main prover on the device (all pointers fixed per
+// context).  Batches of layers with equal table size run in one launch.
+struct layer_dev {
+    const uint8_t *ty;
+    const int32_t *u, *v;                 // the layer's gates
+    int nt, ng;                           // table entries 2^bl(i-1), gates 2^bl(i)
+    const uint32_t *off1, *off2;          // CSR by u (phase 1), by v (phase 2)
+    const int32_t *ent1, *ent2;
+    const int32_t *z2;                    // phase-2 gates adding into addV[0]
+    int nz;
+    const fe *pv;                         // values of layer i-1
+    beta_g bg;
+    beta_u bu, bv;
+    fe *amb, *aVb;                        // flat tables
+    const fe *vu, *vv;                    // end[4] of the two phases
+    fe *part;                             // MAX_PARTS block partials
+    fe *gv;                               // general value (null for layer 1)
+};
+constexpr int MAX_PARTS = 1024;
+// phase 0: tables by u (zk_prover.cpp:375-592); phase 1: by v (:789-983),
+// including the gates that add into addV[0] (nz_max: largest z2 list)
+void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipStream_t s);
+// Sumcheck phases (zk_prover.cpp:600-633, 992-1025) of a batch of ninst
+// instances with flat tables of 2^L entries each, blockIdx.y = instance:
+// block folds of up to 10 rounds per launch (chunks of 1024 entries in
+// LDS), then the last <= 10 rounds in one block per instance.  Round j folds
+// with r[j-1] (r[-1] unused).  Each multi-block launch writes its block
+// partials of round q to slots[(q*3 + c) * FW_BLOCKS + block] (q counted
+// over the instance's launches); the caller reduces every slot of the proof
+// once with reduce_slots.  Tail round sums go straight to sums[3j..].  After
 // the last round: end[0..3] = V[0].a, V[0].b, am[0].a, am[0].b and
-// end[4] = V[0].a * r[L-1] + V[0].b + vconst (= v_u or v_v).  Returns the
-// number of slot rounds used.
+// end[4] = V[0].a * r[L-1] + V[0].b + *vconst (= v_u or v_v).  Returns the
+// slot rounds per instance.
 constexpr int FW_BLOCKS = 1024;
 // entries of each folded table a phase on 2^L entries writes (the chunk
 // results of its first block-fold launch)
 inline int64_t fold_ws_len(int L) { return L > 20 ? ((int64_t)1 << (L - 10)) : 1024; }
 int fold_plan(int L, int *js, int *ks);
 int fold_blocks(int L, int j);
-int sumcheck_phase2(const fe *Vb, const fe *aVb, const fe *amb, int L, const fe *r, fe vconst, fold_ws ws, fe *slots,
-                    fe *sums, fe *end, hipStream_t s);
-void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s);
-// one phase of each of a batch of sumchecks with equal L, as sumcheck_phase2
-// but every pointer per instance (device array I[ninst]); instance t's slot
-// rounds are t.slots[(round*3 + c) * FW_BLOCKS + block].  Returns the slot
-// rounds per instance.
 struct fold_inst {
     const fe *Vb, *aVb, *amb;       // flat tables
     lpoly *V[2], *aV[2], *am[2];    // chunk results, fold_ws_len(L) entries each
@@ -89,6 +92,8 @@ struct fold_inst {
     const fe *vconst;               // added to end[4]; null = 0
 };
 int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s);
+// msg[dst[R]..+3] = sum over the nb[R] block partials of slot round R
+void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s);
 struct half_desc {            // one eq half table: out[j] = f0 * prod_{i<nbits} eq(bit_i(j), r[i])
     fe *out;
     const fe *f0;
@@ -98,10 +103,9 @@ struct half_desc {            // one eq half table: out[j] = f0 * prod_{i<nbits}
 };
 void half_multi(const half_desc *d, int nd, int64_t total, hipStream_t s);
 
-// sum over gates of beta_g(g) beta_u(u) beta_v(v) f_type(v_u, v_v)
-// (zk_verifier.cpp:219-606 and the combination at :1149)
-void general_value(gates_dev g, beta_g bg, beta_u bu, beta_u bv, const fe *vu, const fe *vv,
-                   fe *part, fe *dst, hipStream_t s);
+// per layer: sum over gates of beta_g(g) beta_u(u) beta_v(v) f_type(v_u, v_v)
+// (zk_verifier.cpp:219-606 and the combination at :1149) -> *gv
+void general_multi(const layer_dev *D, int n, int ng_max, hipStream_t s);
 
 // tw[e] = w^e, e < n, from T1[j] = w^j (j < 1024) and T2[j] = w^(1024 j)
 void twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2, hipStream_t s);

@@ -430,9 +430,20 @@ struct vg_ctx {
     std::vector<fe *> cv;
     fe *chal = nullptr;         // challenges
     static constexpr int NGS = 4;             // sumcheck streams (layers run concurrently)
-    fe *amb_s[NGS] = {}, *aVb_s[NGS] = {};
-    lpoly *tabs_s[NGS][6] = {};
-    fe *part_s[NGS] = {};
+    // per layer i: flat tables, fold chunk results (6), block partials
+    std::vector<fe *> amb_l, aVb_l, part_l;
+    std::vector<lpoly *> ws_l;
+    // layers grouped by table size; one launch per step covers a group
+    struct batch {
+        int bl = 0;
+        std::vector<int> layers;
+        int nz_max = 0, ng_max = 0;
+        int d0 = 0;                       // first entry in d_layers / d_inst
+    };
+    std::vector<batch> batches;
+    launch::layer_dev *d_layers = nullptr;    // in batch order
+    launch::fold_inst *d_inst = nullptr;      // batch order: phase 1 of its layers, then phase 2
+    std::vector<int> slot0;                   // first slot round of (layer entry, phase)
     hipStream_t gst[NGS] = {};
     hipEvent_t gev[NGS] = {};
     // CSR transposes of every gate layer (circuit structure only, built once)
@@ -446,8 +457,7 @@ struct vg_ctx {
     launch::half_desc *hdesc = nullptr;
     int n_hdesc = 0;
     int64_t ht_total = 0;
-    int64_t part_half = 0;
-    // round-sum slots of the wave folds (see launch::sumcheck_phase2)
+    // round-sum slots of the block folds (see launch::sumcheck_multi)
     fe *slots = nullptr;
     int n_slots = 0;
     int *slot_nb = nullptr;
@@ -479,7 +489,7 @@ struct vg_ctx {
     schedule S;
     std::vector<layer_host> LH;
     std::vector<int64_t> chal_ru, chal_rv, chal_ab;   // offsets into chal per layer
-    int64_t chal_r0 = 0, chal_r1 = 0, chal_fri = 0, chal_one = 0;
+    int64_t chal_r0 = 0, chal_r1 = 0, chal_fri = 0, chal_one = 0, chal_vc = 0;
     std::vector<int64_t> msg_layer;                   // offset of each layer's block in msg
     int64_t msg_vres = 0, msg_allsum = 0;
     fe post[5];
@@ -521,11 +531,17 @@ int64_t layout(vg_ctx *c, uint8_t *base)
         c->cv[i] = carve<fe>(base, off, n);
     }
     c->chal = carve<fe>(base, off, c->n_chal);
-    int64_t T = (int64_t)1 << maxbl;
-    for (int w = 0; w < vg_ctx::NGS; ++w) {
-        c->amb_s[w] = carve<fe>(base, off, T);
-        c->aVb_s[w] = carve<fe>(base, off, T);
+    c->amb_l.assign(D, nullptr); c->aVb_l.assign(D, nullptr); c->part_l.assign(D, nullptr);
+    c->ws_l.assign(6 * D, nullptr);
+    for (int i = 1; i < D; ++i) {
+        int64_t T = (int64_t)1 << C.bl[i - 1];
+        c->amb_l[i] = carve<fe>(base, off, T);
+        c->aVb_l[i] = carve<fe>(base, off, T);
+        c->part_l[i] = carve<fe>(base, off, launch::MAX_PARTS);
+        for (int k = 0; k < 6; ++k) c->ws_l[6 * i + k] = carve<lpoly>(base, off, launch::fold_ws_len(C.bl[i - 1]));
     }
+    c->d_layers = carve<launch::layer_dev>(base, off, D);
+    c->d_inst = carve<launch::fold_inst>(base, off, 2 * D);
     c->off1.assign(D, nullptr); c->off2.assign(D, nullptr);
     c->ent1.assign(D, nullptr); c->ent2.assign(D, nullptr); c->z2.assign(D, nullptr);
     for (int i = 1; i < D; ++i) {
@@ -538,10 +554,6 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     c->ht_stride = (int64_t)1 << ((maxbl >> 1) + 1);
     c->ht = carve<fe>(base, off, (int64_t)D * 8 * c->ht_stride + 2 * ((int64_t)1 << (C.bl[0] / 2 + 1)));
     c->hdesc = carve<launch::half_desc>(base, off, (int64_t)D * 8 + 2);
-    for (int w = 0; w < vg_ctx::NGS; ++w)
-        for (int k = 0; k < 6; ++k) c->tabs_s[w][k] = carve<lpoly>(base, off, launch::fold_ws_len(maxbl));
-    c->part_half = 18 * 1024;
-    for (int w = 0; w < vg_ctx::NGS; ++w) c->part_s[w] = carve<fe>(base, off, 2 * c->part_half);
     c->slots = carve<fe>(base, off, (int64_t)(c->n_slots > 0 ? c->n_slots : 1) * 3 * launch::FW_BLOCKS);
     c->slot_nb = carve<int>(base, off, c->n_slots + 1);
     c->slot_dst = carve<int64_t>(base, off, c->n_slots + 1);
@@ -635,6 +647,8 @@ int prepare(vg_ctx *c)
         H.vu_const = M.p2_init(l.r_u[bl - 1]);
         for (int j = 0; j < bl; ++j) M.p2_round(j ? rv[j - 1] : fzero(), j, j == bl - 1, H.p2, H.lb2);
         H.vv_const = M.finalize(rv[bl - 1]);
+        ch[c->chal_vc + 2 * i] = H.vu_const;
+        ch[c->chal_vc + 2 * i + 1] = H.vv_const;
         M.finalround(rv[bl - 1], bl << 1, H.fin);
         fe rc = i == 1 ? fzero() : l.r_c;
         H.query = M.query(l.r_u.data(), rv.data(), rc);
@@ -741,37 +755,18 @@ int enqueue(vg_ctx *c)
     // Every layer's sumcheck depends only on the evaluated circuit and on
     // challenges known in advance, so layers run concurrently on NGS streams.
     for (int w = 0; w < vg_ctx::NGS; ++w) HIPCHK(hipStreamWaitEvent(c->gst[w], c->ev[1], 0));
-    const fe *r0 = c->chal + c->chal_r0, *r1 = c->chal + c->chal_r1;
-    const fe *alpha = c->chal + c->chal_one, *beta = c->chal + c->chal_one + 1;
     int slot = 0;
-    for (int i = D - 1; i >= 1; --i) {
-        int w = (D - 1 - i) % vg_ctx::NGS;
-        hipStream_t gs = c->gst[w];
-        launch::fold_ws ws;
-        ws.V[0] = c->tabs_s[w][0]; ws.aV[0] = c->tabs_s[w][1]; ws.am[0] = c->tabs_s[w][2];
-        ws.V[1] = c->tabs_s[w][3]; ws.aV[1] = c->tabs_s[w][4]; ws.am[1] = c->tabs_s[w][5];
-        ws.part = c->part_s[w]; ws.part_half = c->part_half;
-        fe *amb = c->amb_s[w], *aVb = c->aVb_s[w];
-        int bl = C.bl[i - 1], bg = C.bl[i];
-        const fe *ru = c->chal + c->chal_ru[i], *rv = c->chal + c->chal_rv[i];
-        fe *M = c->msg + c->msg_layer[i];
-        fe *sums1 = M, *end1 = M + 3 * bl, *sums2 = M + 3 * bl + 5, *end2 = M + 6 * bl + 5, *gv = M + 6 * bl + 10;
-        gates_dev gd = {c->d_ty[i], c->d_u[i], c->d_v[i], bg};
-        fe *t = c->ht + (int64_t)i * 8 * c->ht_stride;
-        int64_t hs = c->ht_stride;
-        launch::beta_g BG = {t, t + hs, t + 2 * hs, t + 3 * hs, bg >> 1};
-        launch::beta_u BU = {t + 4 * hs, t + 5 * hs, bl >> 1};
-        launch::beta_u BV = {t + 6 * hs, t + 7 * hs, bl >> 1};
-        int nt = 1 << bl;
-        launch::p1_gather(c->off1[i], c->ent1[i], gd, nt, c->cv[i - 1], BG, amb, aVb, gs);
-        slot += launch::sumcheck_phase2(c->cv[i - 1], aVb, amb, bl, ru, c->LH[i].vu_const, ws,
-                                        c->slots + (int64_t)slot * 3 * launch::FW_BLOCKS, sums1, end1, gs);
-        launch::p2_gather(c->off2[i], c->ent2[i], c->z2[i], (int)c->h_z2[i].size(), gd, nt, BG, BU, end1 + 4, amb, aVb,
-                          ws.part, gs);
-        slot += launch::sumcheck_phase2(c->cv[i - 1], aVb, amb, bl, rv, c->LH[i].vv_const, ws,
-                                        c->slots + (int64_t)slot * 3 * launch::FW_BLOCKS, sums2, end2, gs);
-        if (i > 1) launch::general_value(gd, BG, BU, BV, end1 + 4, end2 + 4, ws.part, gv, gs);
-        (void)r0; (void)r1; (void)alpha; (void)beta;
+    for (size_t bi = 0; bi < c->batches.size(); ++bi) {
+        const vg_ctx::batch &b = c->batches[bi];
+        hipStream_t gs = c->gst[bi % vg_ctx::NGS];
+        int n = (int)b.layers.size();
+        const launch::layer_dev *Dl = c->d_layers + b.d0;
+        const launch::fold_inst *I = c->d_inst + 2 * b.d0;
+        launch::layer_tables(0, Dl, n, 1 << b.bl, 0, gs);
+        slot += n * launch::sumcheck_multi(I, n, b.bl, gs);
+        launch::layer_tables(1, Dl, n, 1 << b.bl, b.nz_max, gs);
+        slot += n * launch::sumcheck_multi(I + n, n, b.bl, gs);
+        launch::general_multi(Dl, n, b.ng_max, gs);
     }
     for (int w = 0; w < vg_ctx::NGS; ++w) {
         HIPCHK(hipEventRecord(c->gev[w], c->gst[w]));
@@ -973,6 +968,7 @@ int ctx_init(vg_ctx *c)
             c->chal_ab[i] = k; k += 2;
         }
         c->chal_fri = k; k += n - 6;
+        c->chal_vc = k; k += 2 * C.depth;      // per layer: the host parts of v_u, v_v
         c->n_chal = k + 2 * c->m;
     }
     // device message block
@@ -982,20 +978,38 @@ int ctx_init(vg_ctx *c)
     for (int i = C.depth - 1; i >= 1; --i) { c->msg_layer[i] = ml; ml += 6 * C.bl[i - 1] + 11; }
     c->msg_allsum = ml; ml += 65;
     c->msg_len = ml;
-    // slot tables of the wave folds, in launch order (layer from the top, phase 1 then 2)
-    c->h_slot_nb.clear(); c->h_slot_dst.clear();
+    // layer batches (equal table size 2^bl(i-1)), from the top layer down
+    c->batches.clear();
     for (int i = C.depth - 1; i >= 1; --i) {
         int bl = C.bl[i - 1];
+        vg_ctx::batch *b = nullptr;
+        for (auto &x : c->batches) if (x.bl == bl) b = &x;
+        if (!b) { c->batches.push_back(vg_ctx::batch()); b = &c->batches.back(); b->bl = bl; }
+        b->layers.push_back(i);
+        if ((1 << C.bl[i]) > b->ng_max) b->ng_max = 1 << C.bl[i];
+    }
+    {
+        int d0 = 0;
+        for (auto &b : c->batches) { b.d0 = d0; d0 += (int)b.layers.size(); }
+    }
+    // slot rounds of the block folds: batch by batch, phase 1 of its layers,
+    // then phase 2; each instance's rounds contiguous
+    c->h_slot_nb.clear(); c->h_slot_dst.clear();
+    c->slot0.assign(2 * C.depth, 0);
+    for (auto &b : c->batches) {
         int js[32], ks[32];
-        int nl = launch::fold_plan(bl, js, ks);
-        for (int ph = 0; ph < 2; ++ph) {
-            int64_t sums = c->msg_layer[i] + (ph == 0 ? 0 : 3 * bl + 5);
-            for (int k = 0; k < nl; ++k)
-                for (int q = 0; q < ks[k]; ++q) {
-                    c->h_slot_nb.push_back(launch::fold_blocks(bl, js[k]));
-                    c->h_slot_dst.push_back(sums + 3 * (js[k] + q));
-                }
-        }
+        int nl = launch::fold_plan(b.bl, js, ks);
+        for (int ph = 0; ph < 2; ++ph)
+            for (size_t q = 0; q < b.layers.size(); ++q) {
+                int i = b.layers[q];
+                c->slot0[2 * (b.d0 + q) + ph] = (int)c->h_slot_nb.size();
+                int64_t sums = c->msg_layer[i] + (ph == 0 ? 0 : 3 * b.bl + 5);
+                for (int k = 0; k < nl; ++k)
+                    for (int t = 0; t < ks[k]; ++t) {
+                        c->h_slot_nb.push_back(launch::fold_blocks(b.bl, js[k]));
+                        c->h_slot_dst.push_back(sums + 3 * (js[k] + t));
+                    }
+            }
     }
     c->n_slots = (int)c->h_slot_nb.size();
     c->gather_cap = gather_count(n);
@@ -1065,6 +1079,48 @@ int ctx_init(vg_ctx *c)
         c->n_hdesc = (int)hd.size();
         c->ht_total = tot;
         HIPCHK(hipMemcpy(c->hdesc, hd.data(), sizeof(launch::half_desc) * hd.size(), hipMemcpyHostToDevice));
+    }
+    {
+        // static per-layer descriptors of the batched sumcheck launches
+        int nd = C.depth - 1;
+        std::vector<launch::layer_dev> LD(nd > 0 ? nd : 1);
+        std::vector<launch::fold_inst> FI(nd > 0 ? 2 * nd : 1);
+        for (auto &b : c->batches)
+            for (size_t q = 0; q < b.layers.size(); ++q) {
+                int i = b.layers[q], bl = C.bl[i - 1], bg = C.bl[i];
+                int e = b.d0 + (int)q;
+                fe *M = c->msg + c->msg_layer[i];
+                fe *t = c->ht + (int64_t)i * 8 * c->ht_stride;
+                int64_t hs = c->ht_stride;
+                launch::layer_dev &d = LD[e];
+                d.ty = c->d_ty[i]; d.u = c->d_u[i]; d.v = c->d_v[i];
+                d.nt = 1 << bl; d.ng = 1 << bg;
+                d.off1 = c->off1[i]; d.off2 = c->off2[i]; d.ent1 = c->ent1[i]; d.ent2 = c->ent2[i];
+                d.z2 = c->z2[i]; d.nz = (int)c->h_z2[i].size();
+                if (d.nz > b.nz_max) b.nz_max = d.nz;
+                d.pv = c->cv[i - 1];
+                d.bg = {t, t + hs, t + 2 * hs, t + 3 * hs, bg >> 1};
+                d.bu = {t + 4 * hs, t + 5 * hs, bl >> 1};
+                d.bv = {t + 6 * hs, t + 7 * hs, bl >> 1};
+                d.amb = c->amb_l[i]; d.aVb = c->aVb_l[i];
+                d.vu = M + 3 * bl + 4; d.vv = M + 6 * bl + 9;
+                d.part = c->part_l[i];
+                d.gv = i > 1 ? M + 6 * bl + 10 : nullptr;
+                for (int ph = 0; ph < 2; ++ph) {
+                    launch::fold_inst &x = FI[2 * b.d0 + ph * (int)b.layers.size() + (int)q];
+                    x.Vb = c->cv[i - 1]; x.aVb = d.aVb; x.amb = d.amb;
+                    for (int k = 0; k < 2; ++k) {
+                        x.V[k] = c->ws_l[6 * i + 3 * k]; x.aV[k] = c->ws_l[6 * i + 3 * k + 1]; x.am[k] = c->ws_l[6 * i + 3 * k + 2];
+                    }
+                    x.r = c->chal + (ph == 0 ? c->chal_ru[i] : c->chal_rv[i]);
+                    x.slots = c->slots + (int64_t)c->slot0[2 * e + ph] * 3 * launch::FW_BLOCKS;
+                    x.sums = M + (ph == 0 ? 0 : 3 * bl + 5);
+                    x.end = M + (ph == 0 ? 3 * bl : 6 * bl + 5);
+                    x.vconst = c->chal + c->chal_vc + 2 * i + ph;
+                }
+            }
+        HIPCHK(hipMemcpy(c->d_layers, LD.data(), sizeof(launch::layer_dev) * LD.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_inst, FI.data(), sizeof(launch::fold_inst) * FI.size(), hipMemcpyHostToDevice));
     }
     std::vector<int> lens(65, (int)c->real);
     lens[64] = (int)c->m;

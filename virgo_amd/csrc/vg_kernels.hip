@@ -100,17 +100,24 @@ __global__ void k_mle_fold(const fe *in, fe *out, int64_t n_out, const fe *rp)
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_out; i += (int64_t)gridDim.x * blockDim.x)
         out[i] = add(mul(in[2 * i], om), mul(in[2 * i + 1], r));
 }
-__global__ __launch_bounds__(1024) void k_mle_tail(fe *a, fe *b, int64_t n, const fe *r, int nr, fe *dst)
+// the last rounds of an MLE evaluation (n <= 2 FOLD_TAIL entries) in LDS
+__global__ __launch_bounds__(FOLD_TAIL) void k_mle_tail(const fe *in, int n, const fe *r, int nr, fe *dst)
 {
+    __shared__ fe buf[2 * FOLD_TAIL];
+    int t = threadIdx.x;
+    for (int i = t; i < n; i += blockDim.x) buf[i] = in[i];
+    __syncthreads();
     for (int k = 0; k < nr; ++k) {
         fe rr = r[k], om = sub(fone(), rr);
-        int64_t no = n >> 1;
-        for (int64_t i = threadIdx.x; i < no; i += blockDim.x) b[i] = add(mul(a[2 * i], om), mul(a[2 * i + 1], rr));
+        int no = n >> 1;
+        fe v = fzero();
+        if (t < no) v = add(mul(buf[2 * t], om), mul(buf[2 * t + 1], rr));
         __syncthreads();
-        fe *t = a; a = b; b = t;
+        if (t < no) buf[t] = v;
+        __syncthreads();
         n = no;
     }
-    if (threadIdx.x == 0) *dst = a[0];
+    if (t == 0) *dst = buf[0];
 }
 
 __global__ void k_eq_expand(fe *out, const fe *lo, const fe *hi, int fh, int64_t n)
@@ -165,13 +172,11 @@ __global__ __launch_bounds__(TPB) void k_p1_gather(const launch::layer_dev *D)
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.nt) return;
     bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
-    const uint8_t *gty = d.ty;
-    const int32_t *gu = d.u, *gv = d.v;
     const fe *pv = d.pv;
     fe am = fzero(), aV = fzero();
     for (uint32_t e = d.off1[t]; e < d.off1[t + 1]; ++e) {
-        int g = d.ent1[e];
-        int ty = gty[g], u = gu[g], v = gv[g];
+        launch::gate_rec R = d.ent1[e];
+        int g = R.g, ty = R.ty, u = R.u, v = R.v;
         if (ty == 4) { am = add(am, beta_g(bg, u)); continue; }
         fe tmp = beta_g(bg, g);
         switch (ty) {
@@ -201,8 +206,8 @@ __global__ __launch_bounds__(TPB) void k_p2_gather(const launch::layer_dev *D)
     fe vu = *d.vu;
     fe am = fzero(), aV = fzero();
     for (uint32_t e = d.off2[t]; e < d.off2[t + 1]; ++e) {
-        int g = d.ent2[e];
-        int ty = d.ty[g], u = d.u[g];
+        launch::gate_rec R = d.ent2[e];
+        int g = R.g, ty = R.ty, u = R.u;
         fe tgu = mul(beta_g(bg, g), beta_u(bu, u));
         switch (ty) {
         case 1: am = add(am, mul(tgu, vu)); break;
@@ -919,12 +924,7 @@ void mle_eval(const fe *in, int log_n, const fe *r, fe *t0, fe *t1, fe *dst, hip
         n = no;
         ++k;
     }
-    // tail: copy into the ping-pong buffers if still reading the input
-    if (src == in) {
-        (void)hipMemcpyAsync(t0, in, sizeof(fe) * n, hipMemcpyDeviceToDevice, s);
-        src = t0; cur = 1;
-    }
-    hipLaunchKernelGGL(k_mle_tail, dim3(1), dim3(1024), 0, s, (fe *)src, bufs[cur], n, r + k, log_n - k, dst);
+    hipLaunchKernelGGL(k_mle_tail, dim3(1), dim3(FOLD_TAIL), 0, s, src, (int)n, r + k, log_n - k, dst);
 }
 void eq_expand(fe *out, const fe *lo, const fe *hi, int fh, int64_t n, hipStream_t st)
 {
@@ -947,12 +947,12 @@ void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipS
 }
 int fold_plan(int L, int *js, int *ks)
 {
+    // launches of 10 rounds on chunks of 1024 entries while more than 10
+    // rounds remain; the last <= 10 rounds run in the single-block tail
     int n = 0, j = 0;
     while (L - j > FB_LOG) {
-        int b = L - j - FB_LOG;
-        if (b > FB_LOG) b = FB_LOG;
-        js[n] = j; ks[n] = b; ++n;
-        j += b;
+        js[n] = j; ks[n] = FB_LOG; ++n;
+        j += FB_LOG;
     }
     return n;
 }

@@ -44,6 +44,11 @@ struct beta_g { const fe *g0f, *g0s, *g1f, *g1s; int fh; };
 struct beta_u { const fe *f, *s; int fh; };
 
 
+// a gate as listed in the CSR transposes: everything a table build reads of
+// it, in one 16-byte load
+struct __align__(16) gate_rec {
+    int32_t g, u, v, ty;
+};
 // One GKR layer of the main prover on the device (all pointers fixed per
 // context).  Batches of layers with equal table size run in one launch.
 struct layer_dev {
@@ -51,7 +56,7 @@ struct layer_dev {
     const int32_t *u, *v;                 // the layer's gates
     int nt, ng;                           // table entries 2^bl(i-1), gates 2^bl(i)
     const uint32_t *off1, *off2;          // CSR by u (phase 1), by v (phase 2)
-    const int32_t *ent1, *ent2;
+    const gate_rec *ent1, *ent2;
     const int32_t *z2;                    // phase-2 gates adding into addV[0]
     int nz;
     const fe *pv;                         // values of layer i-1

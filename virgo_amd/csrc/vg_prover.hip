@@ -448,9 +448,11 @@ struct vg_ctx {
     hipEvent_t gev[NGS] = {};
     // CSR transposes of every gate layer (circuit structure only, built once)
     std::vector<std::vector<uint32_t>> h_off1, h_off2;
-    std::vector<std::vector<int32_t>> h_ent1, h_ent2, h_z2;
+    std::vector<std::vector<launch::gate_rec>> h_ent1, h_ent2;
+    std::vector<std::vector<int32_t>> h_z2;
     std::vector<uint32_t *> off1, off2;
-    std::vector<int32_t *> ent1, ent2, z2;
+    std::vector<launch::gate_rec *> ent1, ent2;
+    std::vector<int32_t *> z2;
     // eq half tables of every layer: 8 per layer (g0f g0s g1f g1s uf us vf vs) + 2 for q
     fe *ht = nullptr;
     int64_t ht_stride = 0;
@@ -547,8 +549,8 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     for (int i = 1; i < D; ++i) {
         c->off1[i] = carve<uint32_t>(base, off, (int64_t)c->h_off1[i].size());
         c->off2[i] = carve<uint32_t>(base, off, (int64_t)c->h_off2[i].size());
-        c->ent1[i] = carve<int32_t>(base, off, (int64_t)c->h_ent1[i].size() + 1);
-        c->ent2[i] = carve<int32_t>(base, off, (int64_t)c->h_ent2[i].size() + 1);
+        c->ent1[i] = carve<launch::gate_rec>(base, off, (int64_t)c->h_ent1[i].size() + 1);
+        c->ent2[i] = carve<launch::gate_rec>(base, off, (int64_t)c->h_ent2[i].size() + 1);
         c->z2[i] = carve<int32_t>(base, off, (int64_t)c->h_z2[i].size() + 1);
     }
     c->ht_stride = (int64_t)1 << ((maxbl >> 1) + 1);
@@ -902,7 +904,8 @@ void build_csr(vg_ctx *c, int i)
     int64_t nt = (int64_t)1 << C.bl[i - 1];
     int64_t ng = (int64_t)C.ty[i].size();
     std::vector<uint32_t> &o1 = c->h_off1[i], &o2 = c->h_off2[i];
-    std::vector<int32_t> &e1 = c->h_ent1[i], &e2 = c->h_ent2[i], &z = c->h_z2[i];
+    std::vector<launch::gate_rec> &e1 = c->h_ent1[i], &e2 = c->h_ent2[i];
+    std::vector<int32_t> &z = c->h_z2[i];
     o1.assign(nt + 1, 0); o2.assign(nt + 1, 0);
     const uint8_t *ty = C.ty[i].data();
     const int32_t *u = C.u[i].data(), *v = C.v[i].data();
@@ -923,13 +926,14 @@ void build_csr(vg_ctx *c, int i)
         if (p2_by_v(g)) o2[v[g] + 1]++;
     }
     for (int64_t t = 0; t < nt; ++t) { o1[t + 1] += o1[t]; o2[t + 1] += o2[t]; }
-    e1.assign(o1[nt], 0); e2.assign(o2[nt], 0); z.clear();
+    e1.assign(o1[nt], launch::gate_rec()); e2.assign(o2[nt], launch::gate_rec()); z.clear();
     std::vector<uint32_t> c1(o1.begin(), o1.end() - 1), c2(o2.begin(), o2.end() - 1);
     for (int64_t g = 0; g < ng; ++g) {
         int64_t lo, hi;
         p1_range(g, lo, hi);
-        for (int64_t t = lo; t < hi; ++t) e1[c1[t]++] = (int32_t)g;
-        if (p2_by_v(g)) e2[c2[v[g]]++] = (int32_t)g;
+        launch::gate_rec rec = {(int32_t)g, u[g], v[g], ty[g]};
+        for (int64_t t = lo; t < hi; ++t) e1[c1[t]++] = rec;
+        if (p2_by_v(g)) e2[c2[v[g]]++] = rec;
         if (p2_zero(g)) z.push_back((int32_t)g);
     }
 }
@@ -1042,8 +1046,10 @@ int ctx_init(vg_ctx *c)
     for (int i = 1; i < C.depth; ++i) {
         HIPCHK(hipMemcpy(c->off1[i], c->h_off1[i].data(), 4 * c->h_off1[i].size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->off2[i], c->h_off2[i].data(), 4 * c->h_off2[i].size(), hipMemcpyHostToDevice));
-        if (!c->h_ent1[i].empty()) HIPCHK(hipMemcpy(c->ent1[i], c->h_ent1[i].data(), 4 * c->h_ent1[i].size(), hipMemcpyHostToDevice));
-        if (!c->h_ent2[i].empty()) HIPCHK(hipMemcpy(c->ent2[i], c->h_ent2[i].data(), 4 * c->h_ent2[i].size(), hipMemcpyHostToDevice));
+        if (!c->h_ent1[i].empty())
+            HIPCHK(hipMemcpy(c->ent1[i], c->h_ent1[i].data(), sizeof(launch::gate_rec) * c->h_ent1[i].size(), hipMemcpyHostToDevice));
+        if (!c->h_ent2[i].empty())
+            HIPCHK(hipMemcpy(c->ent2[i], c->h_ent2[i].data(), sizeof(launch::gate_rec) * c->h_ent2[i].size(), hipMemcpyHostToDevice));
         if (!c->h_z2[i].empty()) HIPCHK(hipMemcpy(c->z2[i], c->h_z2[i].data(), 4 * c->h_z2[i].size(), hipMemcpyHostToDevice));
     }
     {
@@ -1221,9 +1227,10 @@ int vg_prove_async(vg_ctx *c)
     if (rc) return rc;
     rc = build_gather(c);
     if (rc) return rc;
-    rc = enqueue(c);
-    if (rc) return rc;
+    // fft_gkr first: its small latency-bound launches overlap the evaluation
     rc = fg_enqueue(c->fg);
+    if (rc) return rc;
+    rc = enqueue(c);
     if (rc) return rc;
     c->pending = true;
     return 0;

@@ -62,7 +62,7 @@ struct fg_ctx {
     // 1, 2 + lg + dep = its phase 2 (the order of the slot rounds too)
     fe *amb = nullptr, *aVb = nullptr;      // addition / mult tables (64 N)
     std::vector<fe *> amb_d, aVb_d;         // per inverse-DFT layer (N)
-    std::vector<lpoly *> ws;                // 6 chunk-result tables per instance
+    std::vector<fe *> ws;                   // 6 chunk-result tables per instance
     std::vector<int> slot0;                 // first slot round per instance
     launch::fold_inst *inst = nullptr;
     launch::fg_layer_dev *ldev = nullptr;
@@ -187,7 +187,7 @@ int64_t layout(fg_ctx *c, uint8_t *base)
     c->ws.assign(6 * ninst, nullptr);
     for (int t = 0; t < ninst; ++t)
         for (int k = 0; k < 6; ++k)
-            c->ws[6 * t + k] = carve<lpoly>(base, off, launch::fold_ws_len(t == 0 ? c->lg + 6 : c->lg));
+            c->ws[6 * t + k] = carve<fe>(base, off, launch::fold_ws_len(t == 0 ? c->lg + 6 : c->lg));
     c->inst = carve<launch::fold_inst>(base, off, ninst);
     c->ldev = carve<launch::fg_layer_dev>(base, off, c->lg);
     int ns = (int)c->h_slot_nb.size();

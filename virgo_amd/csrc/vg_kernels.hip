@@ -282,104 +282,158 @@ __device__ __forceinline__ void fold_terms(const lpoly &V, const lpoly &aV, cons
     t[2] = add(mul(am.b, V.b), aV.b);
 }
 // ---------------------------------------------------------------------
-// Block fold: a block folds a chunk of 2^b consecutive entries of round j0's
-// tables through b rounds in LDS (every fold computed once); the round sums
-// are wave-reduced into LDS.  With several blocks, each block's chunk result
-// goes to out[chunk] and the round sums to per-block partials
-// part[(q*3 + c) * FB_MAXB + block]; with a single block (the phase tail) the
-// sums go to sums[3(j0+q)..] and the final entry to end[].
+// Sumcheck fold (zk_prover.cpp:600-633, 992-1025).  A phase is a sequence of
+// rounds on three value tables V, addV, amsum: round j pairs (x0, x1) =
+// (T[2i], T[2i+1]) of each table into the linear polynomial x0 + X (x1 - x0),
+// adds the pair's terms to the round polynomial, and the next round's table
+// is that polynomial at r[j].  Per pair, with d = x1 - x0:
+//   a += d_am d_V,  c += am0 V0 + aV0,  p(1) += am1 V1 + aV1,  b = p(1) - a - c
+// (three products; the reference's (a, b, c) follow exactly).
+//
+// A launch runs b rounds on chunks of 2^b consecutive values: each of the
+// 256 threads folds 2^(b-8) consecutive values through the first b - 8
+// rounds in registers, then 256 values run the last 8 rounds in LDS.  Round
+// sums of the register rounds accumulate per thread, those of the LDS rounds
+// in per-thread LDS slots, over every chunk the block processes; they are
+// reduced once when the block ends.  A chunk's result is its last value at
+// r[j0+b-1], so every launch reads plain value tables.  With a single chunk
+// (the tail) the final pair gives end[] instead.
 // ---------------------------------------------------------------------
-constexpr int FB_LOG = 10;
-constexpr int FB_MAXB = 1024;
-__device__ __forceinline__ void fold_body(int j0, int b, int L, const fe *Vb, const fe *aVb, const fe *amb, ctabs X,
-                                          const fe *r, tabs out, int64_t nchunks, fe *part, fe *sums, fe *end, fe vconst)
+constexpr int FB_B = 11;              // rounds of a multi-block launch (chunks of 2048 values)
+constexpr int FB_MAXB = 1024;         // block partials per slot round
+constexpr int FB_LDS = 8;             // rounds in LDS (256 values)
+struct ftab { const fe *V, *aV, *am; };
+struct fsum { fe a, c, p1; };
+__device__ __forceinline__ void fsum_add(fsum &s, const fe &V0, const fe &V1, const fe &aV0, const fe &aV1,
+                                         const fe &am0, const fe &am1)
 {
-    __shared__ lpoly sV[1 << (FB_LOG - 1)], saV[1 << (FB_LOG - 1)], sam[1 << (FB_LOG - 1)];
-    __shared__ fe red[FB_LOG][4][3];
-    int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int k = tid; k < FB_LOG * 4 * 3; k += blockDim.x) (&red[0][0][0])[k] = fzero();
-    __syncthreads();
-    int n0 = 1 << (b - 1);
-    bool flat = j0 == 0;
-    fe r0 = flat ? fzero() : r[j0 - 1];
-    for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-        int64_t base = ch << b;
-        // round j0: fold input pairs straight from global memory
-        fe acc[3] = {fzero(), fzero(), fzero()};
-        for (int t = tid; t < n0; t += blockDim.x) {
-            int64_t e0 = base + 2 * t;
-            lpoly V, aV, am;
-            if (flat) {
-                fe x0 = Vb[e0], x1 = Vb[e0 + 1];
-                V.b = x0; V.a = sub(x1, x0);
-                x0 = aVb[e0]; x1 = aVb[e0 + 1];
-                aV.b = x0; aV.a = sub(x1, x0);
-                x0 = amb[e0]; x1 = amb[e0 + 1];
-                am.b = x0; am.a = sub(x1, x0);
-            } else {
-                V = fold_pair(X.V[e0], X.V[e0 + 1], r0);
-                aV = fold_pair(X.aV[e0], X.aV[e0 + 1], r0);
-                am = fold_pair(X.am[e0], X.am[e0 + 1], r0);
-            }
-            sV[t] = V; saV[t] = aV; sam[t] = am;
-            fe tt[3];
-            fold_terms(V, aV, am, tt);
-            acc[0] = add(acc[0], tt[0]); acc[1] = add(acc[1], tt[1]); acc[2] = add(acc[2], tt[2]);
-        }
-        if (w * 64 < n0) {
-            for (int c = 0; c < 3; ++c) acc[c] = wave_sum(acc[c]);
-            if (lane == 0) for (int c = 0; c < 3; ++c) red[0][w][c] = add(red[0][w][c], acc[c]);
-        }
-        __syncthreads();
-        // rounds j0+1 .. j0+b-1 in LDS: n <= 256 outputs, one per thread
-        for (int q = 1; q < b; ++q) {
-            int n = n0 >> q;
-            fe rr = r[j0 + q - 1];
-            lpoly V, aV, am;
-            fe tt[3] = {fzero(), fzero(), fzero()};
-            bool act = tid < n;
-            if (act) {
-                V = fold_pair(sV[2 * tid], sV[2 * tid + 1], rr);
-                aV = fold_pair(saV[2 * tid], saV[2 * tid + 1], rr);
-                am = fold_pair(sam[2 * tid], sam[2 * tid + 1], rr);
-                fold_terms(V, aV, am, tt);
-            }
-            __syncthreads();
-            if (act) { sV[tid] = V; saV[tid] = aV; sam[tid] = am; }
-            if (w * 64 < n) {
-                for (int c = 0; c < 3; ++c) tt[c] = wave_sum(tt[c]);
-                if (lane == 0) for (int c = 0; c < 3; ++c) red[q][w][c] = add(red[q][w][c], tt[c]);
-            }
-            __syncthreads();
-        }
-        if (tid == 0 && nchunks > 1) { out.V[ch] = sV[0]; out.aV[ch] = saV[0]; out.am[ch] = sam[0]; }
-        __syncthreads();
-    }
-    // round sums of this block
-    for (int k = tid; k < b * 3; k += blockDim.x) {
-        int q = k / 3, c = k - 3 * q;
-        fe v = add(add(red[q][0][c], red[q][1][c]), add(red[q][2][c], red[q][3][c]));
-        if (nchunks > 1) part[(int64_t)(q * 3 + c) * FB_MAXB + blockIdx.x] = v;
-        else sums[3 * (j0 + q) + c] = v;
-    }
-    if (nchunks == 1 && tid == 0) {
-        lpoly V0 = sV[0], am0 = sam[0];
-        end[0] = V0.a; end[1] = V0.b; end[2] = am0.a; end[3] = am0.b;
-        end[4] = add(add(mul(V0.a, r[L - 1]), V0.b), vconst);
-    }
+    s.a = add(s.a, mul(sub(am1, am0), sub(V1, V0)));
+    s.c = add(s.c, add(mul(am0, V0), aV0));
+    s.p1 = add(s.p1, add(mul(am1, V1), aV1));
 }
-// the same launch for blockIdx.y = instance of a batch of phases with equal L;
-// launch k of the plan reads ws[cur ^ 1] (or the flat tables if j0 == 0) and
-// writes ws[cur]; part = this launch's first slot round of the instance
-__global__ __launch_bounds__(256) void k_fold_multi(const launch::fold_inst *I, int cur, int j0, int b, int L,
-                                                   int64_t nchunks, int slot_round)
+__device__ __forceinline__ fe lerp(const fe &x0, const fe &x1, const fe &r) { return add(x0, mul(sub(x1, x0), r)); }
+
+__global__ __launch_bounds__(256) void k_fold_multi(const launch::fold_inst *I, int cur, int j0, int b, int64_t nchunks,
+                                                   int slot_round)
 {
     const launch::fold_inst &t = I[blockIdx.y];
-    ctabs X = {t.V[cur ^ 1], t.aV[cur ^ 1], t.am[cur ^ 1]};
-    tabs out = {t.V[cur], t.aV[cur], t.am[cur]};
-    fe vc = t.vconst ? *t.vconst : fzero();
-    fold_body(j0, b, L, t.Vb, t.aVb, t.amb, X, t.r, out, nchunks, t.slots + (int64_t)slot_round * 3 * FB_MAXB, t.sums,
-              t.end, vc);
+    __shared__ fe sV[256], saV[256], sam[256];
+    __shared__ fsum lsum[255];
+    int tid = threadIdx.x;
+    int Lr = b > FB_LDS ? b - FB_LDS : 0;            // register rounds (<= 3)
+    int nl = 1 << (b - Lr);                           // values entering the LDS rounds
+    ftab X;
+    if (j0 == 0) X = {t.Vb, t.aVb, t.amb};
+    else X = {t.V[cur ^ 1], t.aV[cur ^ 1], t.am[cur ^ 1]};
+    const fe *r = t.r;
+    for (int k = tid; k < 255; k += blockDim.x) lsum[k] = {fzero(), fzero(), fzero()};
+    fsum acc[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) acc[q] = {fzero(), fzero(), fzero()};
+    __shared__ lpoly last[2];                         // final pair's V, am (single-chunk launches)
+    __syncthreads();
+    for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        int64_t base = ch << b;
+        if (Lr > 0) {
+            // register rounds: 2^Lr consecutive values per thread
+            fe v[3][4];
+            int E = 1 << Lr;
+            int64_t o = base + (int64_t)tid * E;
+            fe r0 = r[j0];
+#pragma unroll
+            for (int p2 = 0; p2 < 4; ++p2) {
+                if (2 * p2 < E) {
+                    fe V0 = X.V[o + 2 * p2], V1 = X.V[o + 2 * p2 + 1];
+                    fe aV0 = X.aV[o + 2 * p2], aV1 = X.aV[o + 2 * p2 + 1];
+                    fe am0 = X.am[o + 2 * p2], am1 = X.am[o + 2 * p2 + 1];
+                    fsum_add(acc[0], V0, V1, aV0, aV1, am0, am1);
+                    v[0][p2] = lerp(V0, V1, r0); v[1][p2] = lerp(aV0, aV1, r0); v[2][p2] = lerp(am0, am1, r0);
+                }
+            }
+#pragma unroll
+            for (int q = 1; q < 3; ++q) {
+                if (q < Lr) {
+                    fe rq = r[j0 + q];
+#pragma unroll
+                    for (int p2 = 0; p2 < 2; ++p2) {
+                        if (2 * p2 < (E >> q)) {
+                            fsum_add(acc[q], v[0][2 * p2], v[0][2 * p2 + 1], v[1][2 * p2], v[1][2 * p2 + 1], v[2][2 * p2],
+                                     v[2][2 * p2 + 1]);
+                            v[0][p2] = lerp(v[0][2 * p2], v[0][2 * p2 + 1], rq);
+                            v[1][p2] = lerp(v[1][2 * p2], v[1][2 * p2 + 1], rq);
+                            v[2][p2] = lerp(v[2][2 * p2], v[2][2 * p2 + 1], rq);
+                        }
+                    }
+                }
+            }
+            sV[tid] = v[0][0]; saV[tid] = v[1][0]; sam[tid] = v[2][0];
+        } else if (tid < nl) {
+            sV[tid] = X.V[base + tid]; saV[tid] = X.aV[base + tid]; sam[tid] = X.am[base + tid];
+        }
+        __syncthreads();
+        // LDS rounds Lr .. b-1
+        int n = nl, slot = 0;
+        for (int q = Lr; q < b; ++q) {
+            int no = n >> 1;
+            bool act = tid < no;
+            fe nv0, nv1, nv2;
+            if (act) {
+                fe V0 = sV[2 * tid], V1 = sV[2 * tid + 1], aV0 = saV[2 * tid], aV1 = saV[2 * tid + 1];
+                fe am0 = sam[2 * tid], am1 = sam[2 * tid + 1];
+                fsum ls = lsum[slot + tid];
+                fsum_add(ls, V0, V1, aV0, aV1, am0, am1);
+                lsum[slot + tid] = ls;
+                fe rq = r[j0 + q];
+                nv0 = lerp(V0, V1, rq); nv1 = lerp(aV0, aV1, rq); nv2 = lerp(am0, am1, rq);
+                if (no == 1) { last[0] = {sub(V1, V0), V0}; last[1] = {sub(am1, am0), am0}; }
+            }
+            __syncthreads();
+            if (act) { sV[tid] = nv0; saV[tid] = nv1; sam[tid] = nv2; }
+            __syncthreads();
+            slot += no;
+            n = no;
+        }
+        if (tid == 0 && nchunks > 1) { t.V[cur][ch] = sV[0]; t.aV[cur][ch] = saV[0]; t.am[cur][ch] = sam[0]; }
+        __syncthreads();
+    }
+    // round sums of this block: (a, b, c) with b = p(1) - a - c
+    auto emit = [&](int q, const fsum &sm) {
+        fe v3[3] = {sm.a, sub(sub(sm.p1, sm.a), sm.c), sm.c};
+        for (int c = 0; c < 3; ++c) {
+            if (nchunks > 1) t.slots[((int64_t)(slot_round + q) * 3 + c) * FB_MAXB + blockIdx.x] = v3[c];
+            else t.sums[3 * (j0 + q) + c] = v3[c];
+        }
+    };
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (q < Lr) {
+            fe v3[3] = {acc[q].a, acc[q].c, acc[q].p1};
+            block_sum<3>(v3);
+            if (tid == 0) emit(q, {v3[0], v3[1], v3[2]});
+        }
+    }
+    {
+        int n = nl, slot = 0;
+        for (int q = Lr; q < b; ++q) {
+            int no = n >> 1;
+            if (tid == q - Lr) {
+                fsum sm = {fzero(), fzero(), fzero()};
+                for (int k = 0; k < no; ++k) {
+                    const fsum &x = lsum[slot + k];
+                    sm.a = add(sm.a, x.a); sm.c = add(sm.c, x.c); sm.p1 = add(sm.p1, x.p1);
+                }
+                emit(q, sm);
+            }
+            slot += no;
+            n = no;
+        }
+    }
+    if (nchunks == 1 && tid == 0) {
+        lpoly lv = last[0], la = last[1];
+        t.end[0] = lv.a; t.end[1] = lv.b; t.end[2] = la.a; t.end[3] = la.b;
+        fe vc = t.vconst ? *t.vconst : fzero();
+        t.end[4] = add(add(mul(lv.a, r[j0 + b - 1]), lv.b), vc);
+    }
 }
 
 // round sums of every wave-fold launch of the proof: slot R (a round) holds
@@ -947,23 +1001,25 @@ void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipS
 }
 int fold_plan(int L, int *js, int *ks)
 {
-    // launches of 10 rounds on chunks of 1024 entries while more than 10
-    // rounds remain; the last <= 10 rounds run in the single-block tail
+    // launches of FB_B rounds while more than FB_B rounds remain; the last
+    // <= FB_B rounds run in the single-block tail
     int n = 0, j = 0;
-    while (L - j > FB_LOG) {
-        js[n] = j; ks[n] = FB_LOG; ++n;
-        j += FB_LOG;
+    while (L - j > FB_B) {
+        js[n] = j; ks[n] = FB_B; ++n;
+        j += FB_B;
     }
     return n;
 }
 int fold_blocks(int L, int j)
 {
+    // a few chunks per block, so the per-block reductions amortize
     int js[32], ks[32];
     int n = fold_plan(L, js, ks);
     for (int k = 0; k < n; ++k)
         if (js[k] == j) {
             int64_t ch = (int64_t)1 << (L - j - ks[k]);
-            return (int)(ch > FB_MAXB ? FB_MAXB : ch);
+            int64_t g = (ch + 3) / 4;
+            return (int)(g > FB_MAXB ? FB_MAXB : g);
         }
     return 0;
 }
@@ -975,13 +1031,13 @@ int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s)
     for (int k = 0; k < nl; ++k) {
         int j = js[k], b = ks[k];
         int64_t nch = (int64_t)1 << (L - j - b);
-        unsigned g = (unsigned)(nch > FB_MAXB ? FB_MAXB : nch);
-        hipLaunchKernelGGL(k_fold_multi, dim3(g, ninst), dim3(256), 0, s, I, cur, j, b, L, nch, rounds);
+        unsigned g = (unsigned)fold_blocks(L, j);
+        hipLaunchKernelGGL(k_fold_multi, dim3(g, ninst), dim3(256), 0, s, I, cur, j, b, nch, rounds);
         cur ^= 1;
         rounds += b;
     }
     int j = nl ? js[nl - 1] + ks[nl - 1] : 0;
-    hipLaunchKernelGGL(k_fold_multi, dim3(1, ninst), dim3(256), 0, s, I, cur, j, L - j, L, (int64_t)1, rounds);
+    hipLaunchKernelGGL(k_fold_multi, dim3(1, ninst), dim3(256), 0, s, I, cur, j, L - j, (int64_t)1, rounds);
     return rounds;
 }
 void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s)

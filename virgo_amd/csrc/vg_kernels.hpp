@@ -73,24 +73,25 @@ constexpr int MAX_PARTS = 1024;
 void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipStream_t s);
 // Sumcheck phases (zk_prover.cpp:600-633, 992-1025) of a batch of ninst
 // instances with flat tables of 2^L entries each, blockIdx.y = instance:
-// block folds of up to 10 rounds per launch (chunks of 1024 entries in
-// LDS), then the last <= 10 rounds in one block per instance.  Round j folds
-// with r[j-1] (r[-1] unused).  Each multi-block launch writes its block
-// partials of round q to slots[(q*3 + c) * FW_BLOCKS + block] (q counted
-// over the instance's launches); the caller reduces every slot of the proof
-// once with reduce_slots.  Tail round sums go straight to sums[3j..].  After
-// the last round: end[0..3] = V[0].a, V[0].b, am[0].a, am[0].b and
-// end[4] = V[0].a * r[L-1] + V[0].b + *vconst (= v_u or v_v).  Returns the
-// slot rounds per instance.
+// launches of 11 rounds on chunks of 2048 values (8 values per thread in
+// registers for the first 3 rounds, the last 8 in LDS), then the last <= 11
+// rounds in one block per instance.  Round j's pair polynomials are
+// evaluated at r[j].  Each multi-block launch writes its block partials of
+// round q to slots[(q*3 + c) * FW_BLOCKS + block] (q counted over the
+// instance's launches); the caller reduces every slot of the proof once with
+// reduce_slots.  Tail round sums go straight to sums[3j..].  After the last
+// round: end[0..3] = the final pair's V.a, V.b, am.a, am.b and
+// end[4] = V.a * r[L-1] + V.b + *vconst (= v_u or v_v).  Returns the slot
+// rounds per instance.
 constexpr int FW_BLOCKS = 1024;
-// entries of each folded table a phase on 2^L entries writes (the chunk
-// results of its first block-fold launch)
-inline int64_t fold_ws_len(int L) { return L > 20 ? ((int64_t)1 << (L - 10)) : 1024; }
+// values of each table a phase on 2^L entries hands from one launch to the
+// next (the chunk results of its first launch, chunks of 2^11)
+inline int64_t fold_ws_len(int L) { return L > 21 ? ((int64_t)1 << (L - 11)) : 1024; }
 int fold_plan(int L, int *js, int *ks);
 int fold_blocks(int L, int j);
 struct fold_inst {
     const fe *Vb, *aVb, *amb;       // flat tables
-    lpoly *V[2], *aV[2], *am[2];    // chunk results, fold_ws_len(L) entries each
+    fe *V[2], *aV[2], *am[2];       // chunk results, fold_ws_len(L) values each
     const fe *r;                    // the phase's challenges
     fe *slots;                      // the instance's slot rounds
     fe *sums, *end;                 // tail round sums, end values

@@ -432,7 +432,7 @@ struct vg_ctx {
     static constexpr int NGS = 4;             // sumcheck streams (layers run concurrently)
     // per layer i: flat tables, fold chunk results (6), block partials
     std::vector<fe *> amb_l, aVb_l, part_l;
-    std::vector<lpoly *> ws_l;
+    std::vector<fe *> ws_l;
     // layers grouped by table size; one launch per step covers a group
     struct batch {
         int bl = 0;
@@ -540,7 +540,7 @@ int64_t layout(vg_ctx *c, uint8_t *base)
         c->amb_l[i] = carve<fe>(base, off, T);
         c->aVb_l[i] = carve<fe>(base, off, T);
         c->part_l[i] = carve<fe>(base, off, launch::MAX_PARTS);
-        for (int k = 0; k < 6; ++k) c->ws_l[6 * i + k] = carve<lpoly>(base, off, launch::fold_ws_len(C.bl[i - 1]));
+        for (int k = 0; k < 6; ++k) c->ws_l[6 * i + k] = carve<fe>(base, off, launch::fold_ws_len(C.bl[i - 1]));
     }
     c->d_layers = carve<launch::layer_dev>(base, off, D);
     c->d_inst = carve<launch::fold_inst>(base, off, 2 * D);

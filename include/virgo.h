@@ -61,12 +61,13 @@ int64_t vg_ctx_device_bytes(const vg_ctx *ctx);
  * vg_fft_gkr with lg = n - 6; vpd_verifier.cpp:90-99 runs that program and
  * adds its prover time to the proof's).
  * Writes a malloc'd transcript (free with vg_free).  stage_ms (may be NULL)
- * receives 10 device timings in ms: evaluate, gkr (incl. V_res), then the
+ * receives 13 timings in ms.  Device: evaluate, gkr (incl. V_res), then the
  * commitment, which runs on a second stream beside gkr: lde_l, lde_qh (q and
  * h extensions), fri_fold (virtual oracle + folds), hash (all leaf chains
  * and Merkle trees); then queries (gathers + copies to the host), the SHA3
  * leaf-chain kernel alone (part of hash), the whole device schedule, and
- * fft_gkr (on its own stream). */
+ * fft_gkr (on its own stream).  Host: prepare (challenge stream, mask
+ * bookkeeping, query gather table), enqueue, assemble (transcript). */
 int vg_prove(vg_ctx *ctx, uint8_t **transcript, size_t *len, double *stage_ms);
 /* Same, but only enqueues the device work of the proof on the context's
  * stream and returns; vg_finish waits and assembles the transcript.  Lets

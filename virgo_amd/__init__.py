@@ -94,7 +94,7 @@ class _Desc(ctypes.Structure):
 
 
 STAGES = ["evaluate", "gkr", "lde_l", "lde_qh", "fri_fold", "hash", "queries", "leaf_kernel", "device_total",
-          "fft_gkr"]
+          "fft_gkr", "host_prepare", "host_enqueue", "host_assemble"]
 
 
 class Prover:

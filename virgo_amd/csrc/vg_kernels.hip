@@ -530,7 +530,7 @@ struct ntt_in {
     const fe *l, *q; int64_t cw_S, cw_stride, step; int log_R;
     const int *lens; int log_Rl; int hmode;
 };
-__device__ __forceinline__ fe ntt_fetch(const ntt_in &in, int64_t b, int64_t j, const fe *tw, int64_t SS)
+__device__ __forceinline__ fe ntt_fetch(const ntt_in &in, int64_t b, int64_t j, const launch::ntt_tabs &T)
 {
     if (in.kind == 0) return in.src[b * in.stride + j];
     if (in.kind == 1) {
@@ -541,14 +541,17 @@ __device__ __forceinline__ fe ntt_fetch(const ntt_in &in, int64_t b, int64_t j, 
     int len = in.lens[slice];
     if (j >= len) return fzero();
     fe c = in.src[slice * in.stride + (in.hmode ? len : 0) + j];
-    return mul(c, tw[(s * j) & (SS - 1)]);
+    // coset twist w_SS^(s j) from two small tables
+    fe x = mul(T.TA[(s << T.th) + (j & ((1 << T.th) - 1))], T.TB[(s << T.tb) + (j >> T.th)]);
+    return mul(c, x);
 }
-__device__ __forceinline__ fe twid(const fe *tw, int64_t e, int64_t SS, int inverse)
+// stage s twiddle w_{2^(s+1)}^k (inverse: w^-k)
+__device__ __forceinline__ fe stage_tw(const launch::ntt_tabs &T, int s, int64_t k, int inverse)
 {
-    return inverse ? tw[(SS - e) & (SS - 1)] : tw[e];
+    return (inverse ? T.Wi : T.W)[((int64_t)1 << s) + k];
 }
 __global__ __launch_bounds__(TPB) void k_ntt_lds(ntt_in in, fe *out, int64_t out_stride, int log_n, int log_m, int inverse,
-                                                fe scale, int do_scale, const fe *tw, int64_t SS, int log_SS)
+                                                fe scale, int do_scale, launch::ntt_tabs T)
 {
     __shared__ fe sh[1 << NTT_LM];
     int64_t b = blockIdx.y;
@@ -557,7 +560,7 @@ __global__ __launch_bounds__(TPB) void k_ntt_lds(ntt_in in, fe *out, int64_t out
     for (int i = threadIdx.x; i < M; i += blockDim.x) {
         int64_t pos = (q << log_m) + i;
         int64_t j = log_n ? (int64_t)(__brevll((unsigned long long)pos) >> (64 - log_n)) : 0;
-        sh[i] = ntt_fetch(in, b, j, tw, SS);
+        sh[i] = ntt_fetch(in, b, j, T);
     }
     __syncthreads();
     for (int s = 0; s < log_m; ++s) {
@@ -565,7 +568,7 @@ __global__ __launch_bounds__(TPB) void k_ntt_lds(ntt_in in, fe *out, int64_t out
         for (int t = threadIdx.x; t < M / 2; t += blockDim.x) {
             int k = t & (half - 1);
             int i0 = ((t >> s) << (s + 1)) | k, i1 = i0 + half;
-            fe w = twid(tw, (int64_t)k << (log_SS - s - 1), SS, inverse);
+            fe w = stage_tw(T, s, k, inverse);
             fe a = sh[i0], c = mul(w, sh[i1]);
             sh[i0] = add(a, c);
             sh[i1] = sub(a, c);
@@ -578,7 +581,7 @@ __global__ __launch_bounds__(TPB) void k_ntt_lds(ntt_in in, fe *out, int64_t out
 // stages [s0, s0+G) in registers; thread = (batch, base with bits [s0,s0+G) clear)
 template <int G>
 __global__ __launch_bounds__(TPB) void k_ntt_regs(fe *data, int64_t stride, int64_t nb, int log_n, int s0, int inverse,
-                                                 fe scale, int do_scale, const fe *tw, int64_t SS, int log_SS)
+                                                 fe scale, int do_scale, launch::ntt_tabs T)
 {
     int64_t per = (int64_t)1 << (log_n - G);
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -598,7 +601,7 @@ __global__ __launch_bounds__(TPB) void k_ntt_regs(fe *data, int64_t stride, int6
             if (m & (1 << t)) continue;
             int m1 = m | (1 << t);
             int64_t e = lo + ((int64_t)(m & ((1 << t) - 1)) << s0);
-            fe w = twid(tw, e << (log_SS - s - 1), SS, inverse);
+            fe w = stage_tw(T, s, e, inverse);
             fe a = x[m], c = mul(w, x[m1]);
             x[m] = add(a, c);
             x[m1] = sub(a, c);
@@ -613,6 +616,32 @@ __global__ void k_twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2)
 {
     int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (e < n) tw[e] = mul(T1[e & 1023], T2[e >> 10]);
+}
+// stage-major twiddles and coset twist tables, read from tw[e] = w^e (e < SS)
+__global__ void k_ntt_tables(fe *W, fe *Wi, int maxlog, fe *TA, fe *TB, int R, int th, int tb, const fe *tw, int64_t SS)
+{
+    int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t nw = (int64_t)1 << maxlog, na = (int64_t)R << th, nbb = (int64_t)R << tb;
+    if (id < nw) {
+        if (id == 0) { W[0] = fone(); Wi[0] = fone(); return; }
+        int s = 63 - __clzll((unsigned long long)id);
+        int64_t k = id - ((int64_t)1 << s);
+        int64_t e = k * (SS >> (s + 1));
+        W[id] = tw[e];
+        Wi[id] = tw[(SS - e) & (SS - 1)];
+        return;
+    }
+    id -= nw;
+    if (id < na) {
+        int64_t s = id >> th, i = id & (((int64_t)1 << th) - 1);
+        TA[id] = tw[(s * i) & (SS - 1)];
+        return;
+    }
+    id -= na;
+    if (id < nbb) {
+        int64_t s = id >> tb, i = id & (((int64_t)1 << tb) - 1);
+        TB[id] = tw[((s * i) << th) & (SS - 1)];
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -788,14 +817,13 @@ __global__ void k_fe_mul(const fe *a, const fe *b, fe *o, int64_t n)
 int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
 
 void ntt_run(const ntt_in &in, fe *out, int64_t out_stride, int64_t nb, int log_n, int inverse, fe scale, int do_scale,
-             const fe *tw, int64_t SS, hipStream_t st)
+             const launch::ntt_tabs &T, hipStream_t st)
 {
-    int log_SS = ilog2(SS);
     int log_m = log_n < NTT_LM ? log_n : NTT_LM;
     int rest = log_n - log_m;
     dim3 grid((unsigned)((int64_t)1 << (log_n - log_m)), (unsigned)nb);
     hipLaunchKernelGGL(k_ntt_lds, grid, dim3(TPB), 0, st, in, out, out_stride, log_n, log_m, inverse, scale,
-                       rest == 0 ? do_scale : 0, tw, SS, log_SS);
+                       rest == 0 ? do_scale : 0, T);
     int s0 = log_m;
     while (rest > 0) {
         int g = rest > 4 ? 4 : rest;
@@ -804,10 +832,10 @@ void ntt_run(const ntt_in &in, fe *out, int64_t out_stride, int64_t nb, int log_
         int ds = rest == 0 ? do_scale : 0;
         dim3 gr(nblk(threads)), bk(TPB);
         switch (g) {
-        case 1: hipLaunchKernelGGL(k_ntt_regs<1>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, tw, SS, log_SS); break;
-        case 2: hipLaunchKernelGGL(k_ntt_regs<2>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, tw, SS, log_SS); break;
-        case 3: hipLaunchKernelGGL(k_ntt_regs<3>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, tw, SS, log_SS); break;
-        default: hipLaunchKernelGGL(k_ntt_regs<4>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, tw, SS, log_SS); break;
+        case 1: hipLaunchKernelGGL(k_ntt_regs<1>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, T); break;
+        case 2: hipLaunchKernelGGL(k_ntt_regs<2>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, T); break;
+        case 3: hipLaunchKernelGGL(k_ntt_regs<3>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, T); break;
+        default: hipLaunchKernelGGL(k_ntt_regs<4>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, T); break;
         }
         s0 += g;
     }
@@ -1055,25 +1083,32 @@ void general_multi(const layer_dev *D, int n, int ng_max, hipStream_t s)
     hipLaunchKernelGGL(k_general, dim3(blocks, n), dim3(TPB), 0, s, D);
     hipLaunchKernelGGL(k_reduce, dim3(1, n), dim3(1024), 0, s, D, (int)blocks);
 }
+void ntt_tables(fe *W, fe *Wi, int maxlog, fe *TA, fe *TB, int log_R, int th, int tb, const fe *tw, int64_t SS,
+                hipStream_t s)
+{
+    int R = TA ? 1 << log_R : 0;
+    int64_t n = ((int64_t)1 << maxlog) + ((int64_t)R << th) + ((int64_t)R << tb);
+    hipLaunchKernelGGL(k_ntt_tables, dim3(nblk(n)), dim3(TPB), 0, s, W, Wi, maxlog, TA, TB, R, th, tb, tw, SS);
+}
 void twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2, hipStream_t s)
 {
     hipLaunchKernelGGL(k_twiddles, dim3(nblk(n)), dim3(TPB), 0, s, tw, n, T1, T2);
 }
-void intt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, fe inv_n, const fe *tw, int64_t SS, hipStream_t s)
+void intt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, fe inv_n, const ntt_tabs &T, hipStream_t s)
 {
     ntt_in ni;
     ni.kind = in.kind; ni.src = in.src; ni.stride = in.src_stride;
     ni.l = in.l; ni.q = in.q; ni.cw_S = in.cw_S; ni.cw_stride = in.cw_stride; ni.step = in.step; ni.log_R = in.log_R;
     ni.lens = nullptr; ni.log_Rl = 0; ni.hmode = 0;
-    ntt_run(ni, out, out_stride, nb, log_n, 1, inv_n, 1, tw, SS, s);
+    ntt_run(ni, out, out_stride, nb, log_n, 1, inv_n, 1, T, s);
 }
-void lde(const fe *coef, int64_t coef_stride, const int *lens, int nb, fe *out, int log_C, int log_R, const fe *tw,
-         int64_t SS, hipStream_t s, int hmode)
+void lde(const fe *coef, int64_t coef_stride, const int *lens, int nb, fe *out, int log_C, int log_R, const ntt_tabs &T,
+         hipStream_t s, int hmode)
 {
     ntt_in ni;
     memset(&ni, 0, sizeof ni);
     ni.kind = 2; ni.src = coef; ni.stride = coef_stride; ni.lens = lens; ni.log_Rl = log_R; ni.hmode = hmode;
-    ntt_run(ni, out, (int64_t)1 << log_C, (int64_t)nb << log_R, log_C, 0, fone(), 0, tw, SS, s);
+    ntt_run(ni, out, (int64_t)1 << log_C, (int64_t)nb << log_R, log_C, 0, fone(), 0, T, s);
 }
 void virtual_oracle(const fe *l, const fe *q, const fe *h, const fe *c0, int nslice, int64_t SS, int log_R, int64_t real,
                     int64_t m, const fe *tw, fe *vo, hipStream_t s)
@@ -1094,14 +1129,14 @@ void merkle(uint8_t *tree, int64_t n_leaf, hipStream_t s)
     }
     if (n >= 1) hipLaunchKernelGGL(k_merkle_tail, dim3(1), dim3(1024), 0, s, tree, n);
 }
-void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int inverse, fe scale, int do_scale, const fe *tw,
-         int64_t SS, hipStream_t s)
+void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int inverse, fe scale, int do_scale,
+         const ntt_tabs &T, hipStream_t s)
 {
     ntt_in ni;
     memset(&ni, 0, sizeof ni);
     ni.kind = in.kind; ni.src = in.src; ni.stride = in.src_stride;
     ni.l = in.l; ni.q = in.q; ni.cw_S = in.cw_S; ni.cw_stride = in.cw_stride; ni.step = in.step; ni.log_R = in.log_R;
-    ntt_run(ni, out, out_stride, nb, log_n, inverse, scale, do_scale, tw, SS, s);
+    ntt_run(ni, out, out_stride, nb, log_n, inverse, scale, do_scale, T, s);
 }
 void sha3_batch(const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s)
 {

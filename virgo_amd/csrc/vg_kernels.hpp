@@ -115,6 +115,18 @@ void general_multi(const layer_dev *D, int n, int ng_max, hipStream_t s);
 
 // tw[e] = w^e, e < n, from T1[j] = w^j (j < 1024) and T2[j] = w^(1024 j)
 void twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2, hipStream_t s);
+// Twiddles of the NTTs: stage-major W[2^s + k] = w_{2^(s+1)}^k and Wi (the
+// inverse powers) for s < maxlog, so each stage reads a contiguous run; the
+// LDE's coset twist w^(s j) = TA[s][j mod 2^th] TB[s][j >> th] for rows
+// s < 2^log_R (TA, TB null when there is no LDE).  All read from the table
+// tw[e] = w^e (e < SS, w of order SS).
+struct ntt_tabs {
+    const fe *W, *Wi;
+    const fe *TA, *TB;
+    int th, tb;
+};
+void ntt_tables(fe *W, fe *Wi, int maxlog, fe *TA, fe *TB, int log_R, int th, int tb, const fe *tw, int64_t SS,
+                hipStream_t s);
 
 // Batched inverse NTT: out[b][k] = N^-1 sum_j in(b, j) w_N^{-jk}, natural order.
 //   kind 0: in(b, j) = src[b*src_stride + j]
@@ -124,12 +136,12 @@ struct intt_src {
     const fe *src; int64_t src_stride;
     const fe *l, *q; int64_t cw_S, cw_stride; int64_t step; int log_R;
 };
-void intt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, fe inv_n, const fe *tw, int64_t SS, hipStream_t s);
+void intt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, fe inv_n, const ntt_tabs &T, hipStream_t s);
 // Coset LDE: for each slice b and row s < R, out[b][s][t] = sum_{j<len_b} c_b[j] w_SS^{(tR+s) j}.
 // coef[b] has lens[b] (<= C) entries at coef + b*coef_stride.
 // hmode: coefficients of slice b start at coef + b*coef_stride + lens[b] (upper half of lq)
 void lde(const fe *coef, int64_t coef_stride, const int *lens, int nb, fe *out, int log_C, int log_R,
-         const fe *tw, int64_t SS, hipStream_t s, int hmode = 0);
+         const ntt_tabs &T, hipStream_t s, int hmode = 0);
 // virtual oracle of commit_public_array (poly_commit.h:246-263, 305-322)
 void virtual_oracle(const fe *l, const fe *q, const fe *h, const fe *c0, int nslice, int64_t SS, int log_R,
                     int64_t real, int64_t m, const fe *tw, fe *vo, hipStream_t s);
@@ -139,8 +151,8 @@ void slice_sums(const fe *lq_coef, int64_t stride, int nslice, int64_t real, int
 // heap Merkle tree over n_leaf leaves already at tree[n_leaf..2n_leaf) (merkle_tree.cpp:46-56)
 void merkle(uint8_t *tree, int64_t n_leaf, hipStream_t s);
 // generic batched NTT (inverse: w^-1), natural order in and out
-void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int inverse, fe scale, int do_scale, const fe *tw,
-         int64_t SS, hipStream_t s);
+void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int inverse, fe scale, int do_scale,
+         const ntt_tabs &T, hipStream_t s);
 void sha3_batch(const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s);
 // fills leaves [n_leaf, pad) of a tree with pad leaves with H(0^64)
 void merkle_pad(uint8_t *tree, int64_t pad, int64_t n_leaf, hipStream_t s);

@@ -35,19 +35,25 @@ struct dbuf {
 int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
 bool pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
 
-// tw[e] = w^e for e < n on the device
-int make_twiddles(const fe &w, int64_t n, dbuf &tw)
+// tw[e] = w^e for e < n on the device, and the stage-major tables of an
+// NTT of size n built from them
+int make_twiddles(const fe &w, int64_t n, dbuf &tw, dbuf &W, launch::ntt_tabs &T)
 {
     std::vector<fe> t(1024 + (n >> 10) + 1);
     fe x = fone();
     for (int j = 0; j < 1024; ++j) { t[j] = x; x = mul(x, w); }
     fe y = fone();
     for (int64_t j = 0; j < (n >> 10) + 1; ++j) { t[1024 + j] = y; y = mul(y, x); }
-    dbuf T;
-    OPCHK(hipMalloc(&T.p, sizeof(fe) * t.size()));
-    OPCHK(hipMemcpy(T.p, t.data(), sizeof(fe) * t.size(), hipMemcpyHostToDevice));
+    dbuf T12;
+    OPCHK(hipMalloc(&T12.p, sizeof(fe) * t.size()));
+    OPCHK(hipMemcpy(T12.p, t.data(), sizeof(fe) * t.size(), hipMemcpyHostToDevice));
     OPCHK(hipMalloc(&tw.p, sizeof(fe) * n));
-    launch::twiddles((fe *)tw.p, n, (fe *)T.p, (fe *)T.p + 1024, 0);
+    launch::twiddles((fe *)tw.p, n, (fe *)T12.p, (fe *)T12.p + 1024, 0);
+    int lg = ilog2(n);
+    OPCHK(hipMalloc(&W.p, sizeof(fe) * 2 * n));
+    fe *w0 = (fe *)W.p, *w1 = w0 + n;
+    launch::ntt_tables(w0, w1, lg, nullptr, nullptr, 0, 0, 0, (const fe *)tw.p, n, 0);
+    T = {w0, w1, nullptr, nullptr, 0, 0};
     OPCHK(hipDeviceSynchronize());
     return 0;
 }
@@ -59,8 +65,9 @@ int vg_fft(const vg_fe *coef, int coef_len, int order, const vg_fe *rou, vg_fe *
 {
     if (!pow2(coef_len) || !pow2(order) || coef_len > order) { op_err = "vg_fft: sizes must be powers of two, coef_len <= order"; return -1; }
     fe w; memcpy(&w, rou, sizeof w);
-    dbuf tw, buf;
-    int rc = make_twiddles(w, order, tw);
+    dbuf tw, buf, W;
+    launch::ntt_tabs T;
+    int rc = make_twiddles(w, order, tw, W, T);
     if (rc) return rc;
     OPCHK(hipMalloc(&buf.p, sizeof(fe) * order));
     OPCHK(hipMemset(buf.p, 0, sizeof(fe) * order));
@@ -68,7 +75,7 @@ int vg_fft(const vg_fe *coef, int coef_len, int order, const vg_fe *rou, vg_fe *
     dbuf out;
     OPCHK(hipMalloc(&out.p, sizeof(fe) * order));
     launch::ntt({0, (const fe *)buf.p, order, nullptr, nullptr, 0, 0, 0, 0}, (fe *)out.p, order, 1, ilog2(order), 0, fone(), 0,
-                (const fe *)tw.p, order, 0);
+                T, 0);
     OPCHK(hipGetLastError());
     OPCHK(hipMemcpy(result, out.p, sizeof(fe) * order, hipMemcpyDeviceToHost));
     return 0;
@@ -84,15 +91,16 @@ int vg_ifft(const vg_fe *evals, int coef_len, int order, const vg_fe *rou, vg_fe
     fe w; memcpy(&w, rou, sizeof w);
     fe nw = fone();
     for (int64_t i = 0; i < step; ++i) nw = mul(nw, w);
-    dbuf tw, in, out;
-    int rc = make_twiddles(nw, coef_len, tw);
+    dbuf tw, in, out, W;
+    launch::ntt_tabs T;
+    int rc = make_twiddles(nw, coef_len, tw, W, T);
     if (rc) return rc;
     OPCHK(hipMalloc(&in.p, sizeof(fe) * coef_len));
     OPCHK(hipMalloc(&out.p, sizeof(fe) * coef_len));
     OPCHK(hipMemcpy(in.p, sub.data(), sizeof(fe) * coef_len, hipMemcpyHostToDevice));
     fe inv_n = finv(mk((uint64_t)coef_len, 0));
     launch::ntt({0, (const fe *)in.p, coef_len, nullptr, nullptr, 0, 0, 0, 0}, (fe *)out.p, coef_len, 1, ilog2(coef_len), 1,
-                inv_n, 1, (const fe *)tw.p, coef_len, 0);
+                inv_n, 1, T, 0);
     OPCHK(hipGetLastError());
     OPCHK(hipMemcpy(result, out.p, sizeof(fe) * coef_len, hipMemcpyDeviceToHost));
     return 0;

@@ -472,6 +472,8 @@ struct vg_ctx {
     // commitment
     int n = 0, log_SS = 0, log_C = 0, log_R = 0;
     int64_t SS = 0, real = 0, m = 0, C_ = 0, LQ = 0;
+    fe *ntW = nullptr, *ntWi = nullptr, *ntTA = nullptr, *ntTB = nullptr;   // NTT stage and coset tables
+    launch::ntt_tabs NT = {};
     fe *tw = nullptr, *T12 = nullptr, *coefP = nullptr, *coefQ = nullptr, *l = nullptr, *q = nullptr, *h = nullptr,
        *vo = nullptr, *qv = nullptr, *eqs = nullptr, *lqc = nullptr, *c0 = nullptr, *all_sum = nullptr, *pri = nullptr,
        *pubm = nullptr, *cw = nullptr, *gout = nullptr;
@@ -498,6 +500,7 @@ struct vg_ctx {
     hipEvent_t ev[10] = {};     // stage boundaries
     hipEvent_t ev_leaf[2] = {}; // around the leaf-chain kernel
     std::chrono::steady_clock::time_point t_start;
+    double host_ms[3] = {};     // prepare (challenges, masks, gather table), enqueue, assemble
     bool pending = false;
     bool serial = false;        // VIRGO_SERIAL=1: commitment after the sumcheck on one stream
     fg_ctx *fg = nullptr;       // fft_gkr of the commitment (lg = n - 6), on its own stream
@@ -569,6 +572,15 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     int64_t SS = c->SS;
     c->T12 = carve<fe>(base, off, 1024 + (SS >> 10) + 1);
     c->tw = carve<fe>(base, off, SS);
+    {
+        int maxlog = c->log_C + 1, th = (c->log_C + 1) / 2, tb = c->log_C - th;
+        int64_t R = (int64_t)1 << c->log_R;
+        c->ntW = carve<fe>(base, off, (int64_t)1 << maxlog);
+        c->ntWi = carve<fe>(base, off, (int64_t)1 << maxlog);
+        c->ntTA = carve<fe>(base, off, R << th);
+        c->ntTB = carve<fe>(base, off, R << tb);
+        c->NT = {c->ntW, c->ntWi, c->ntTA, c->ntTB, th, tb};
+    }
     c->coefP = carve<fe>(base, off, 65 * c->C_);
     c->coefQ = carve<fe>(base, off, 65 * c->C_);
     c->l = carve<fe>(base, off, 65 * SS);
@@ -785,23 +797,23 @@ int enqueue(vg_ctx *c)
     fe *pri_d = c->chal + (c->n_chal - 2 * m), *pub_d = c->chal + (c->n_chal - m);
     fe inv_real = finv(mk((uint64_t)real, 0)), inv_m = finv(mk((uint64_t)m, 0));
     fe inv_2real = finv(mk((uint64_t)(2 * real), 0)), inv_2m = finv(mk((uint64_t)(2 * m), 0));
-    launch::intt({0, c->cv[0], real, nullptr, nullptr, 0, 0, 0, 0}, c->coefP, Cc, 64, lr, inv_real, c->tw, SS, st);
-    launch::intt({0, pri_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefP + 64 * Cc, Cc, 1, lm, inv_m, c->tw, SS, st);
-    launch::lde(c->coefP, Cc, c->lens, 65, c->l, c->log_C, c->log_R, c->tw, SS, st);
+    launch::intt({0, c->cv[0], real, nullptr, nullptr, 0, 0, 0, 0}, c->coefP, Cc, 64, lr, inv_real, c->NT, st);
+    launch::intt({0, pri_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefP + 64 * Cc, Cc, 1, lm, inv_m, c->NT, st);
+    launch::lde(c->coefP, Cc, c->lens, 65, c->l, c->log_C, c->log_R, c->NT, st);
     HIPCHK(hipEventRecord(c->ev[3], st));
     // ---- commit_public_array (poly_commit.h:125-351) ----
     {
         fe *q = c->ht + (int64_t)D * 8 * c->ht_stride;
         launch::eq_expand(c->qv, q, q + ((int64_t)1 << (n >> 1)), n >> 1, (int64_t)1 << n, st);
     }
-    launch::intt({0, c->qv, real, nullptr, nullptr, 0, 0, 0, 0}, c->coefQ, Cc, 64, lr, inv_real, c->tw, SS, st);
-    launch::intt({0, pub_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefQ + 64 * Cc, Cc, 1, lm, inv_m, c->tw, SS, st);
-    launch::lde(c->coefQ, Cc, c->lens, 65, c->q, c->log_C, c->log_R, c->tw, SS, st);
-    launch::intt({1, nullptr, 0, c->l, c->q, SS, SS, SS / (2 * real), c->log_R}, c->lqc, c->LQ, 64, lr + 1, inv_2real, c->tw, SS, st);
+    launch::intt({0, c->qv, real, nullptr, nullptr, 0, 0, 0, 0}, c->coefQ, Cc, 64, lr, inv_real, c->NT, st);
+    launch::intt({0, pub_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefQ + 64 * Cc, Cc, 1, lm, inv_m, c->NT, st);
+    launch::lde(c->coefQ, Cc, c->lens, 65, c->q, c->log_C, c->log_R, c->NT, st);
+    launch::intt({1, nullptr, 0, c->l, c->q, SS, SS, SS / (2 * real), c->log_R}, c->lqc, c->LQ, 64, lr + 1, inv_2real, c->NT, st);
     launch::intt({1, nullptr, 0, c->l + 64 * SS, c->q + 64 * SS, SS, SS, SS / (2 * m), c->log_R}, c->lqc + 64 * c->LQ, c->LQ, 1,
-                 lm + 1, inv_2m, c->tw, SS, st);
+                 lm + 1, inv_2m, c->NT, st);
     launch::slice_sums(c->lqc, c->LQ, 65, real, m, c->c0, c->msg + c->msg_allsum, st);
-    launch::lde(c->lqc, c->LQ, c->lens, 65, c->h, c->log_C, c->log_R, c->tw, SS, st, 1);
+    launch::lde(c->lqc, c->LQ, c->lens, 65, c->h, c->log_C, c->log_R, c->NT, st, 1);
     HIPCHK(hipEventRecord(c->ev[4], st));
     // ---- commit_phase folds (vpd_verifier.cpp:46-78, fri.cpp:321-390).  The
     // challenges are known in advance, so every fold runs before any hashing.
@@ -1141,6 +1153,7 @@ int ctx_init(vg_ctx *c)
         for (int64_t j = 0; j < (SS >> 10) + 1; ++j) { T12[1024 + j] = y; y = mul(y, w10); }
         HIPCHK(hipMemcpy(c->T12, T12.data(), sizeof(fe) * T12.size(), hipMemcpyHostToDevice));
         launch::twiddles(c->tw, SS, c->T12, c->T12 + 1024, 0);
+        launch::ntt_tables(c->ntW, c->ntWi, c->log_C + 1, c->ntTA, c->ntTB, c->log_R, c->NT.th, c->NT.tb, c->tw, SS, 0);
     }
     HIPCHK(hipDeviceSynchronize());
     return 0;
@@ -1222,16 +1235,22 @@ int vg_prove_async(vg_ctx *c)
 {
     if (!c) FAILV(-1, "null context");
     if (c->pending) FAILV(-1, "a proof is already in flight on this context");
+    auto ms_since = [](std::chrono::steady_clock::time_point t0) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
     c->t_start = std::chrono::steady_clock::now();
     int rc = prepare(c);
     if (rc) return rc;
     rc = build_gather(c);
     if (rc) return rc;
+    c->host_ms[0] = ms_since(c->t_start);
+    auto t1 = std::chrono::steady_clock::now();
     // fft_gkr first: its small latency-bound launches overlap the evaluation
     rc = fg_enqueue(c->fg);
     if (rc) return rc;
     rc = enqueue(c);
     if (rc) return rc;
+    c->host_ms[1] = ms_since(t1);
     c->pending = true;
     return 0;
 }
@@ -1242,11 +1261,13 @@ int vg_finish(vg_ctx *c, uint8_t **transcript, size_t *len, double *stage_ms)
     c->pending = false;
     HIPCHK(hipStreamSynchronize(c->st));
     HIPCHK(hipGetLastError());
+    auto t2 = std::chrono::steady_clock::now();
     tbuf T;
     assemble(c, T);
     double fg_ms = 0;
     int rc = fg_finish(c->fg, T.b, nullptr, nullptr, &fg_ms);
     if (rc) return rc;
+    c->host_ms[2] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t2).count();
     uint8_t *p = (uint8_t *)malloc(T.b.size());
     if (!p) FAILV(-5, "out of host memory");
     memcpy(p, T.b.data(), T.b.size());
@@ -1262,6 +1283,7 @@ int vg_finish(vg_ctx *c, uint8_t **transcript, size_t *len, double *stage_ms)
             stage_ms[k] = ms;
         }
         stage_ms[9] = fg_ms;
+        for (int k = 0; k < 3; ++k) stage_ms[10 + k] = c->host_ms[k];
     }
     return 0;
 }

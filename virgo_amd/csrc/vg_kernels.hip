@@ -545,40 +545,135 @@ __device__ __forceinline__ fe ntt_fetch(const ntt_in &in, int64_t b, int64_t j, 
     fe x = mul(T.TA[(s << T.th) + (j & ((1 << T.th) - 1))], T.TB[(s << T.tb) + (j >> T.th)]);
     return mul(c, x);
 }
-// stage s twiddle w_{2^(s+1)}^k (inverse: w^-k)
-__device__ __forceinline__ fe stage_tw(const launch::ntt_tabs &T, int s, int64_t k, int inverse)
+// x w_4 (forward w_4 = -i) or x w_4^-1 (= i)
+__device__ __forceinline__ fe mul_w4(const fe &x, int inverse)
 {
-    return (inverse ? T.Wi : T.W)[((int64_t)1 << s) + k];
+    return inverse ? mk(negp(x.img), x.real) : mk(x.img, negp(x.real));
+}
+// x 2^30 mod p for canonical x: a rotation within 61 bits
+__device__ __forceinline__ uint64_t rot30(uint64_t x) { return ((x << 30) & P) | (x >> 31); }
+// x w_8 (forward w_8 = 2^30 (1 - i)) or x w_8^-1 (= 2^30 (1 + i))
+__device__ __forceinline__ fe mul_w8(const fe &x, int inverse)
+{
+    uint64_t s = addp(x.real, x.img);
+    uint64_t d = inverse ? subp(x.real, x.img) : subp(x.img, x.real);
+    return inverse ? mk(rot30(d), rot30(s)) : mk(rot30(s), rot30(d));
+}
+__device__ __forceinline__ void bfly(fe &a, fe &b)
+{
+    fe t = b;
+    b = sub(a, t);
+    a = add(a, t);
+}
+// radix-2^G DIT on v[0..2^G) whose inputs are already premultiplied: stage
+// t pairs (m, m + 2^t) with twiddle w_{2^(t+1)}^(m mod 2^t) (1, w_4, w_8)
+template <int G>
+__device__ __forceinline__ void radix_core(fe *v, int inverse)
+{
+    if (G >= 1) {
+#pragma unroll
+        for (int m = 0; m < (1 << G); m += 2) bfly(v[m], v[m + 1]);
+    }
+    if (G >= 2) {
+#pragma unroll
+        for (int m = 0; m < (1 << G); m += 4) {
+            bfly(v[m], v[m + 2]);
+            v[m + 3] = mul_w4(v[m + 3], inverse);
+            bfly(v[m + 1], v[m + 3]);
+        }
+    }
+    if (G >= 3) {
+        bfly(v[0], v[4]);
+        v[5] = mul_w8(v[5], inverse);
+        bfly(v[1], v[5]);
+        v[6] = mul_w4(v[6], inverse);
+        bfly(v[2], v[6]);
+        v[7] = mul_w4(mul_w8(v[7], inverse), inverse);
+        bfly(v[3], v[7]);
+    }
+}
+// premultiply v[m] by w_{2^(s0+G)}^(lo brev_G(m)), stages [s0, s0+G) of a
+// DIT NTT then reduce to radix_core (w^(e + 2^S) = -w^e, S = s0+G-1)
+template <int G>
+__device__ __forceinline__ void radix_premul(fe *v, int s0, int64_t lo, const launch::ntt_tabs &T, int inverse)
+{
+    if (s0 == 0) return;
+    int S = s0 + G - 1;
+    const fe *W = (inverse ? T.Wi : T.W) + ((int64_t)1 << S);
+    int64_t half = (int64_t)1 << S;
+#pragma unroll
+    for (int m = 1; m < (1 << G); ++m) {
+        int br = 0;
+#pragma unroll
+        for (int i = 0; i < G; ++i) br |= ((m >> i) & 1) << (G - 1 - i);
+        int64_t e = lo * br;
+        fe w = W[e & (half - 1)];
+        fe x = mul(v[m], w);
+        v[m] = (e & half) ? neg(x) : x;
+    }
+}
+// One block: positions [q 2^log_m, (q+1) 2^log_m) of batch row b, read in
+// bit-reversed order, stages [0, log_m) as radix-8/4/2 passes (the first
+// reading global memory, the last writing it, the rest in LDS).
+constexpr int NTT_PAD = 4;          // one pad element per 16 breaks power-of-two LDS strides
+__device__ __forceinline__ int lpad(int i) { return i + (i >> NTT_PAD); }
+template <int G>
+__device__ __forceinline__ void ntt_pass(fe *sh, int M, int s0, int first, int last, const ntt_in &in, int64_t b, int64_t q,
+                                         int log_m, int log_n, fe *out, int64_t out_stride, fe scale, int do_scale,
+                                         const launch::ntt_tabs &T, int inverse)
+{
+    int ng = M >> G;
+    for (int g = threadIdx.x; g < ng; g += blockDim.x) {
+        int lo = g & ((1 << s0) - 1), hi = g >> s0;
+        int base = (hi << (s0 + G)) | lo;
+        fe v[1 << G];
+#pragma unroll
+        for (int m = 0; m < (1 << G); ++m) {
+            int i = base + (m << s0);
+            if (first) {
+                int64_t pos = (q << log_m) + i;
+                int64_t j = log_n ? (int64_t)(__brevll((unsigned long long)pos) >> (64 - log_n)) : 0;
+                v[m] = ntt_fetch(in, b, j, T);
+            } else {
+                v[m] = sh[lpad(i)];
+            }
+        }
+        radix_premul<G>(v, s0, lo, T, inverse);
+        radix_core<G>(v, inverse);
+#pragma unroll
+        for (int m = 0; m < (1 << G); ++m) {
+            int i = base + (m << s0);
+            if (last) out[b * out_stride + (q << log_m) + i] = do_scale ? mul(v[m], scale) : v[m];
+            else sh[lpad(i)] = v[m];
+        }
+    }
 }
 __global__ __launch_bounds__(TPB) void k_ntt_lds(ntt_in in, fe *out, int64_t out_stride, int log_n, int log_m, int inverse,
                                                 fe scale, int do_scale, launch::ntt_tabs T)
 {
-    __shared__ fe sh[1 << NTT_LM];
+    __shared__ fe sh[(1 << NTT_LM) + (1 << (NTT_LM - NTT_PAD))];
     int64_t b = blockIdx.y;
     int64_t q = blockIdx.x;
     int M = 1 << log_m;
-    for (int i = threadIdx.x; i < M; i += blockDim.x) {
-        int64_t pos = (q << log_m) + i;
-        int64_t j = log_n ? (int64_t)(__brevll((unsigned long long)pos) >> (64 - log_n)) : 0;
-        sh[i] = ntt_fetch(in, b, j, T);
-    }
-    __syncthreads();
-    for (int s = 0; s < log_m; ++s) {
-        int half = 1 << s;
-        for (int t = threadIdx.x; t < M / 2; t += blockDim.x) {
-            int k = t & (half - 1);
-            int i0 = ((t >> s) << (s + 1)) | k, i1 = i0 + half;
-            fe w = stage_tw(T, s, k, inverse);
-            fe a = sh[i0], c = mul(w, sh[i1]);
-            sh[i0] = add(a, c);
-            sh[i1] = sub(a, c);
+    if (log_m == 0) {                                  // a transform of size 1
+        if (threadIdx.x == 0) {
+            fe x = ntt_fetch(in, b, 0, T);
+            out[b * out_stride + q] = do_scale ? mul(x, scale) : x;
         }
+        return;
+    }
+    int s0 = 0;
+    while (s0 < log_m) {
+        int G = log_m - s0 >= 3 ? 3 : log_m - s0;
+        int first = s0 == 0, last = s0 + G == log_m;
+        if (G == 3) ntt_pass<3>(sh, M, s0, first, last, in, b, q, log_m, log_n, out, out_stride, scale, do_scale, T, inverse);
+        else if (G == 2) ntt_pass<2>(sh, M, s0, first, last, in, b, q, log_m, log_n, out, out_stride, scale, do_scale, T, inverse);
+        else ntt_pass<1>(sh, M, s0, first, last, in, b, q, log_m, log_n, out, out_stride, scale, do_scale, T, inverse);
+        s0 += G;
         __syncthreads();
     }
-    fe *o = out + b * out_stride + (q << log_m);
-    for (int i = threadIdx.x; i < M; i += blockDim.x) o[i] = do_scale ? mul(sh[i], scale) : sh[i];
 }
-// stages [s0, s0+G) in registers; thread = (batch, base with bits [s0,s0+G) clear)
+// stages [s0, s0+G) of rows in global memory; thread = (row, group)
 template <int G>
 __global__ __launch_bounds__(TPB) void k_ntt_regs(fe *data, int64_t stride, int64_t nb, int log_n, int s0, int inverse,
                                                  fe scale, int do_scale, launch::ntt_tabs T)
@@ -590,25 +685,13 @@ __global__ __launch_bounds__(TPB) void k_ntt_regs(fe *data, int64_t stride, int6
     int64_t lo = r & (((int64_t)1 << s0) - 1), hi = r >> s0;
     int64_t base = (hi << (s0 + G)) | lo;
     fe *d = data + b * stride;
-    fe x[1 << G];
+    fe v[1 << G];
 #pragma unroll
-    for (int m = 0; m < (1 << G); ++m) x[m] = d[base + ((int64_t)m << s0)];
+    for (int m = 0; m < (1 << G); ++m) v[m] = d[base + ((int64_t)m << s0)];
+    radix_premul<G>(v, s0, lo, T, inverse);
+    radix_core<G>(v, inverse);
 #pragma unroll
-    for (int t = 0; t < G; ++t) {
-        int s = s0 + t;
-#pragma unroll
-        for (int m = 0; m < (1 << G); ++m) {
-            if (m & (1 << t)) continue;
-            int m1 = m | (1 << t);
-            int64_t e = lo + ((int64_t)(m & ((1 << t) - 1)) << s0);
-            fe w = stage_tw(T, s, e, inverse);
-            fe a = x[m], c = mul(w, x[m1]);
-            x[m] = add(a, c);
-            x[m1] = sub(a, c);
-        }
-    }
-#pragma unroll
-    for (int m = 0; m < (1 << G); ++m) d[base + ((int64_t)m << s0)] = do_scale ? mul(x[m], scale) : x[m];
+    for (int m = 0; m < (1 << G); ++m) d[base + ((int64_t)m << s0)] = do_scale ? mul(v[m], scale) : v[m];
 }
 
 // tw[e] = T1[e & 1023] * T2[e >> 10]
@@ -826,7 +909,7 @@ void ntt_run(const ntt_in &in, fe *out, int64_t out_stride, int64_t nb, int log_
                        rest == 0 ? do_scale : 0, T);
     int s0 = log_m;
     while (rest > 0) {
-        int g = rest > 4 ? 4 : rest;
+        int g = rest > 3 ? 3 : rest;
         rest -= g;
         int64_t threads = nb * ((int64_t)1 << (log_n - g));
         int ds = rest == 0 ? do_scale : 0;
@@ -834,8 +917,7 @@ void ntt_run(const ntt_in &in, fe *out, int64_t out_stride, int64_t nb, int log_
         switch (g) {
         case 1: hipLaunchKernelGGL(k_ntt_regs<1>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, T); break;
         case 2: hipLaunchKernelGGL(k_ntt_regs<2>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, T); break;
-        case 3: hipLaunchKernelGGL(k_ntt_regs<3>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, T); break;
-        default: hipLaunchKernelGGL(k_ntt_regs<4>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, T); break;
+        default: hipLaunchKernelGGL(k_ntt_regs<3>, gr, bk, 0, st, out, out_stride, nb, log_n, s0, inverse, scale, ds, T); break;
         }
         s0 += g;
     }

@@ -46,9 +46,17 @@ void vg_free(void *p);
 /* Builds the layered circuit exactly as zk_verifier::read_circuit does
  * (input padding to 2^15 or 2^17, relay layer 1, bit lengths), uploads gates
  * and inputs, and allocates every device buffer of one proof.
- * stream: a hipStream_t, or NULL for a stream owned by the context. */
+ * stream: NULL -- the context owns its streams and overlaps the commitment,
+ * the sumcheck layers and fft_gkr (lowest latency for one proof); or a
+ * hipStream_t -- the whole proof runs in order on that stream, so many
+ * contexts on different streams overlap on the device (batched proving). */
 int vg_ctx_create(vg_ctx **ctx, const vg_circuit_desc *c, void *stream);
 int vg_ctx_destroy(vg_ctx *ctx);
+/* Creates / destroys a non-blocking HIP stream for vg_ctx_create (no
+ * counterpart in the reference: a convenience for callers without a HIP
+ * runtime binding of their own). */
+int vg_stream_create(void **stream);
+int vg_stream_destroy(void *stream);
 /* Bytes of device memory held by the context. */
 int64_t vg_ctx_device_bytes(const vg_ctx *ctx);
 
@@ -74,6 +82,15 @@ int vg_prove(vg_ctx *ctx, uint8_t **transcript, size_t *len, double *stage_ms);
  * several contexts on different streams overlap on one device. */
 int vg_prove_async(vg_ctx *ctx);
 int vg_finish(vg_ctx *ctx, uint8_t **transcript, size_t *len, double *stage_ms);
+
+/* Batched proving (no single counterpart in the reference, which proves one
+ * circuit per process): proves ctxs[0..n) with `workers` host threads,
+ * thread k taking i = k, k + workers, ...; transcripts[i] / lens[i] receive
+ * each proof (free with vg_free), byte-identical to a single vg_prove of the
+ * same context.  Contexts created on distinct caller streams overlap on the
+ * device; give the contexts of one worker the same stream.  Returns the
+ * first error (its message in vg_last_error); proofs not made are NULL. */
+int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcripts, size_t *lens);
 
 /* Output layer of the last proof (2^bits elements). */
 int vg_output(vg_ctx *ctx, vg_fe *out, size_t cap, int *bits);

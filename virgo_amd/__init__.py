@@ -23,7 +23,7 @@ FE = np.dtype([("img", "<u8"), ("real", "<u8")])     # prime_field.h:93 layout
 
 EXPORTS = ["vg_init", "vg_last_error", "vg_free", "vg_ctx_create", "vg_ctx_destroy", "vg_ctx_device_bytes",
            "vg_prove", "vg_prove_async", "vg_finish", "vg_output", "vg_fft", "vg_ifft", "vg_sha3_64",
-           "vg_merkle_tree", "vg_fe_mul", "vg_fft_gkr"]
+           "vg_merkle_tree", "vg_fe_mul", "vg_fft_gkr", "vg_stream_create", "vg_stream_destroy", "vg_prove_batch"]
 
 
 class VirgoError(RuntimeError):
@@ -62,6 +62,9 @@ def lib():
         L.vg_sha3_64.argtypes = [vp, vp, ctypes.c_size_t]
         L.vg_merkle_tree.argtypes = [vp, ctypes.c_int, vp]
         L.vg_fe_mul.argtypes = [vp, vp, vp, ctypes.c_size_t]
+        L.vg_prove_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.vg_stream_create.argtypes = [ctypes.POINTER(vp)]
+        L.vg_stream_destroy.argtypes = [vp]
         L.vg_fft_gkr.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t),
                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                  ctypes.POINTER(ctypes.c_double)]
@@ -97,13 +100,32 @@ STAGES = ["evaluate", "gkr", "lde_l", "lde_qh", "fri_fold", "hash", "queries", "
           "fft_gkr", "host_prepare", "host_enqueue", "host_assemble"]
 
 
+class Stream:
+    """A HIP stream owned by the library (for Prover(circ, stream=...))."""
+
+    def __init__(self):
+        init()
+        self.handle = ctypes.c_void_p()
+        _check(lib().vg_stream_create(ctypes.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            lib().vg_stream_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+
 class Prover:
     """One proof context: the circuit resident in HBM plus every buffer of a
     proof.  prove() runs the whole Virgo prover on the device and returns the
     transcript bytes (the prover messages in the reference verifier's order)."""
 
     def __init__(self, circ, stream=None):
+        """stream: None (the context owns streams and overlaps its stages) or
+        a Stream (the whole proof runs in order on it; contexts on different
+        streams overlap -- batched proving)."""
         init()
+        if isinstance(stream, Stream):
+            stream = stream.handle
         self._keep = tuple(np.ascontiguousarray(circ[k], dtype=t) for k, t in
                            (("n", np.int64), ("ty", np.int32), ("u", np.int64), ("v", np.int64)))
         n, ty, u, v = self._keep
@@ -159,6 +181,25 @@ def prove(circ):
         return p.prove()
     finally:
         p.close()
+
+
+def prove_batch(provers, workers):
+    """Proves every Prover's circuit with `workers` host threads in the
+    library (vg_prove_batch); returns the transcripts in order."""
+    n = len(provers)
+    ctxs = (ctypes.c_void_p * n)(*[p._ctx.value for p in provers])
+    outs = (ctypes.c_void_p * n)()
+    lens = (ctypes.c_size_t * n)()
+    rc = lib().vg_prove_batch(ctxs, n, workers, outs, lens)
+    res = []
+    for k in range(n):
+        if outs[k]:
+            res.append(ctypes.string_at(outs[k], lens[k]))
+            lib().vg_free(outs[k])
+        else:
+            res.append(None)
+    _check(rc)
+    return res
 
 
 def fft_gkr(lg):

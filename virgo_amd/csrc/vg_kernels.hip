@@ -777,30 +777,6 @@ __device__ __forceinline__ int64_t leaf_of(int64_t tid, int64_t nleaf, int log_R
     int64_t s = tid / cl, t = tid - s * cl;
     return (t << log_R) + s;
 }
-__global__ __launch_bounds__(TPB) void k_merkle_level(uint8_t *tree, int64_t start, int64_t n)
-{
-    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    k += start;
-    const uint64_t *c = (const uint64_t *)(tree + 2 * k * 32);
-    uint64_t m[8], h[4];
-    for (int i = 0; i < 8; ++i) m[i] = c[i];
-    sha3_64(m, h);
-    store_digest(tree, k, h);
-}
-__global__ __launch_bounds__(1024) void k_merkle_tail(uint8_t *tree, int64_t n)
-{
-    for (; n >= 1; n >>= 1) {
-        for (int64_t k = n + threadIdx.x; k < 2 * n; k += blockDim.x) {
-            const uint64_t *c = (const uint64_t *)(tree + 2 * k * 32);
-            uint64_t m[8], h[4];
-            for (int i = 0; i < 8; ++i) m[i] = c[i];
-            sha3_64(m, h);
-            store_digest(tree, k, h);
-        }
-        __syncthreads();
-    }
-}
 // FRI fold of 65 slices (fri.cpp:321-340, 367-379): next[p] = ((a+b) + w_S^-p r (a-b)) / 2,
 // a = prev[p], b = prev[p + S/2]; one thread per (slice, position), row-contiguous.
 __global__ __launch_bounds__(TPB) void k_fri_fold(const fe *prev, int64_t S, int log_R, const fe *rp, const fe *tw, int64_t SS,
@@ -817,6 +793,40 @@ __global__ __launch_bounds__(TPB) void k_fri_fold(const fe *prev, int64_t S, int
     fe imr = mul(tw[(SS - p * (SS / S)) & (SS - 1)], *rp);
     const fe inv2 = mk(1ull << 60, 0);
     next[tid] = mul(inv2, add(add(a, b), mul(imr, sub(a, b))));
+}
+// FRI levels with at most FRI_TAIL positions per slice run in k_fri_tail
+constexpr int FRI_TAIL = 4096;
+struct fri_levels {
+    fe *next[24];               // codeword after each tail level
+    int n;                      // tail levels
+};
+// the remaining folds of slice blockIdx.x in LDS: prev holds S positions per
+// slice (S <= FRI_TAIL), level k halves it with challenge r[k]
+__global__ __launch_bounds__(256) void k_fri_tail(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw,
+                                                int64_t SS, fri_levels F)
+{
+    __shared__ fe v[FRI_TAIL];
+    int64_t j = blockIdx.x;
+    for (int64_t p = threadIdx.x; p < S; p += blockDim.x) v[p] = prev[j * S + cw_idx(p, S, log_R)];
+    __syncthreads();
+    const fe inv2 = mk(1ull << 60, 0);
+    for (int k = 0; k < F.n; ++k) {
+        int64_t Sn = S / 2;
+        fe rr = r[k];
+        fe nv[FRI_TAIL / 2 / 256];
+        int c = 0;
+        for (int64_t p = threadIdx.x; p < Sn; p += blockDim.x, ++c) {
+            fe a = v[p], b = v[p + Sn];
+            fe imr = mul(tw[(SS - p * (SS / S)) & (SS - 1)], rr);
+            nv[c] = mul(inv2, add(add(a, b), mul(imr, sub(a, b))));
+            F.next[k][j * Sn + cw_idx(p, Sn, log_R)] = nv[c];
+        }
+        __syncthreads();
+        c = 0;
+        for (int64_t p = threadIdx.x; p < Sn; p += blockDim.x, ++c) v[p] = nv[c];
+        __syncthreads();
+        S = Sn;
+    }
 }
 struct leaf_segs {
     int n;
@@ -849,7 +859,10 @@ struct tree_set {
     uint8_t *tree[24];
     int64_t nleaf[24];
 };
-// level h (h >= 1) of every tree that has it: nodes [nleaf>>h, 2*(nleaf>>h))
+// Merkle levels with at most MK_TAIL nodes run in k_merkle_tails
+constexpr int MK_TAIL = 1024;
+// level h (h >= 1) of every tree whose level h has more than MK_TAIL nodes:
+// nodes [nleaf>>h, 2*(nleaf>>h))
 __global__ __launch_bounds__(TPB) void k_merkle_multi(tree_set T, int h, int64_t total)
 {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -857,6 +870,7 @@ __global__ __launch_bounds__(TPB) void k_merkle_multi(tree_set T, int h, int64_t
     int sg = 0;
     for (; sg < T.n; ++sg) {
         int64_t c = T.nleaf[sg] >> h;
+        if (c <= MK_TAIL) continue;
         if (tid < c) break;
         tid -= c;
     }
@@ -867,6 +881,43 @@ __global__ __launch_bounds__(TPB) void k_merkle_multi(tree_set T, int h, int64_t
     for (int q = 0; q < 8; ++q) m[q] = c[q];
     sha3_64(m, hh);
     store_digest(T.tree[sg], k, hh);
+}
+// the remaining levels of tree blockIdx.x (every level with <= MK_TAIL
+// nodes) in one block: the first from the tree in memory, the rest from LDS
+__global__ __launch_bounds__(256) void k_merkle_tails(tree_set T)   // 256 threads
+{
+    __shared__ uint64_t dg[MK_TAIL][4];
+    uint8_t *tree = T.tree[blockIdx.x];
+    int64_t nl = T.nleaf[blockIdx.x];
+    int h = 1;
+    while ((nl >> h) > MK_TAIL) ++h;
+    int cnt = (int)(nl >> h);
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        int64_t k = cnt + i;
+        const uint64_t *c = (const uint64_t *)(tree + 2 * k * 32);
+        uint64_t m[8], hh[4];
+        for (int q = 0; q < 8; ++q) m[q] = c[q];
+        sha3_64(m, hh);
+        store_digest(tree, k, hh);
+        for (int q = 0; q < 4; ++q) dg[i][q] = hh[q];
+    }
+    __syncthreads();
+    while (cnt > 1) {
+        cnt >>= 1;                                    // <= MK_TAIL / 2 nodes: at most 2 per thread
+        uint64_t hh[MK_TAIL / 512][4];
+        int c = 0;
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x, ++c) {
+            uint64_t m[8];
+            for (int q = 0; q < 4; ++q) { m[q] = dg[2 * i][q]; m[4 + q] = dg[2 * i + 1][q]; }
+            sha3_64(m, hh[c]);
+            store_digest(tree, cnt + i, hh[c]);
+        }
+        __syncthreads();
+        c = 0;
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x, ++c)
+            for (int q = 0; q < 4; ++q) dg[i][q] = hh[c][q];
+        __syncthreads();
+    }
 }
 __global__ void k_gather16(const uint64_t *src, int64_t n, fe *out)
 {
@@ -1202,15 +1253,6 @@ void slice_sums(const fe *lq_coef, int64_t stride, int nslice, int64_t real, int
 {
     hipLaunchKernelGGL(k_slice_sums, dim3(1), dim3(128), 0, s, lq_coef, stride, nslice, real, m, c0, all_sum);
 }
-void merkle(uint8_t *tree, int64_t n_leaf, hipStream_t s)
-{
-    int64_t n = n_leaf / 2;
-    while (n > 1024) {
-        hipLaunchKernelGGL(k_merkle_level, dim3(nblk(n)), dim3(TPB), 0, s, tree, n, n);
-        n /= 2;
-    }
-    if (n >= 1) hipLaunchKernelGGL(k_merkle_tail, dim3(1), dim3(1024), 0, s, tree, n);
-}
 void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int inverse, fe scale, int do_scale,
          const ntt_tabs &T, hipStream_t s)
 {
@@ -1237,6 +1279,23 @@ void fri_fold(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, i
 {
     hipLaunchKernelGGL(k_fri_fold, dim3(nblk(65 * (S / 2))), dim3(TPB), 0, s, prev, S, log_R, r, tw, SS, next);
 }
+void fri_folds(const fe *vo, int64_t SS, int nlev, int log_R, const fe *r, const fe *tw, fe *const *next, hipStream_t s)
+{
+    const fe *prev = vo;
+    int64_t S = SS;
+    int k = 0;
+    for (; k < nlev && S > FRI_TAIL; ++k, S /= 2) {
+        fri_fold(prev, S, log_R, r + k, tw, SS, next[k], s);
+        prev = next[k];
+    }
+    if (k < nlev) {
+        fri_levels F;
+        memset(&F, 0, sizeof F);
+        F.n = nlev - k;
+        for (int q = 0; q < F.n; ++q) F.next[q] = next[k + q];
+        hipLaunchKernelGGL(k_fri_tail, dim3(65), dim3(256), 0, s, prev, S, log_R, r + k, tw, SS, F);
+    }
+}
 void leaf_hash_multi(int nseg, const fe *const *cw, const int64_t *S, uint8_t *const *tree, int log_R, hipStream_t s)
 {
     leaf_segs L;
@@ -1254,13 +1313,15 @@ void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStre
     tree_set T;
     memset(&T, 0, sizeof T);
     T.n = ntree;
-    int64_t mx = 0;
-    for (int k = 0; k < ntree; ++k) { T.tree[k] = tree[k]; T.nleaf[k] = nleaf[k]; if (nleaf[k] > mx) mx = nleaf[k]; }
-    for (int h = 1; ((int64_t)1 << h) <= mx; ++h) {
+    for (int k = 0; k < ntree; ++k) { T.tree[k] = tree[k]; T.nleaf[k] = nleaf[k]; }
+    for (int h = 1;; ++h) {
         int64_t total = 0;
-        for (int k = 0; k < ntree; ++k) total += nleaf[k] >> h;
+        for (int k = 0; k < ntree; ++k)
+            if ((nleaf[k] >> h) > MK_TAIL) total += nleaf[k] >> h;
+        if (!total) break;
         hipLaunchKernelGGL(k_merkle_multi, dim3(nblk(total)), dim3(TPB), 0, s, T, h, total);
     }
+    hipLaunchKernelGGL(k_merkle_tails, dim3(ntree), dim3(256), 0, s, T);
 }
 void gather16(const uint64_t *src, int64_t n, fe *out, hipStream_t s)
 {

@@ -148,8 +148,6 @@ void virtual_oracle(const fe *l, const fe *q, const fe *h, const fe *c0, int nsl
 // all_sum[i] = (lq_coef[0] + lq_coef[len_i]) * len_i; c0[i] = lq_coef[0] + lq_coef[len_i]
 void slice_sums(const fe *lq_coef, int64_t stride, int nslice, int64_t real, int64_t m, fe *c0, fe *all_sum, hipStream_t s);
 
-// heap Merkle tree over n_leaf leaves already at tree[n_leaf..2n_leaf) (merkle_tree.cpp:46-56)
-void merkle(uint8_t *tree, int64_t n_leaf, hipStream_t s);
 // generic batched NTT (inverse: w^-1), natural order in and out
 void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int inverse, fe scale, int do_scale,
          const ntt_tabs &T, hipStream_t s);
@@ -159,6 +157,9 @@ void merkle_pad(uint8_t *tree, int64_t pad, int64_t n_leaf, hipStream_t s);
 void fe_mul(const fe *a, const fe *b, fe *o, int64_t n, hipStream_t s);
 // FRI fold only (no hashing): next (S/2 per slice) from prev (S per slice)
 void fri_fold(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, int64_t SS, fe *next, hipStream_t s);
+// every FRI fold of a commitment: nlev levels from vo (SS per slice), level k
+// with challenge r[k] into next[k]; the small levels in one launch
+void fri_folds(const fe *vo, int64_t SS, int nlev, int log_R, const fe *r, const fe *tw, fe *const *next, hipStream_t s);
 // leaf chains of nseg codewords in one launch (codeword k: S[k] positions per
 // slice, leaves S[k]/2 written to tree[k] + S[k]/2 nodes)
 void leaf_hash_multi(int nseg, const fe *const *cw, const int64_t *S, uint8_t *const *tree, int log_R, hipStream_t s);

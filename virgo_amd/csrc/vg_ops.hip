@@ -129,7 +129,9 @@ int vg_merkle_tree(const uint8_t *leaves, int n_leaf, uint8_t *tree)
     OPCHK(hipMemset(t.p, 0, 64 * pad));
     OPCHK(hipMemcpy((uint8_t *)t.p + 32 * pad, leaves, 32 * (size_t)n_leaf, hipMemcpyHostToDevice));
     launch::merkle_pad((uint8_t *)t.p, pad, n_leaf, 0);
-    launch::merkle((uint8_t *)t.p, pad, 0);
+    uint8_t *tp = (uint8_t *)t.p;
+    int64_t npad = pad;
+    launch::merkle_multi(1, &tp, &npad, 0);
     OPCHK(hipGetLastError());
     OPCHK(hipMemcpy(tree, t.p, 64 * pad, hipMemcpyDeviceToHost));
     return 0;

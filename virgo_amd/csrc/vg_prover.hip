@@ -9,6 +9,7 @@
 // scalar work on challenges only; it is computed beside the device work and
 // added to the device round sums when the transcript is assembled.
 #include <chrono>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -502,7 +503,8 @@ struct vg_ctx {
     std::chrono::steady_clock::time_point t_start;
     double host_ms[3] = {};     // prepare (challenges, masks, gather table), enqueue, assemble
     bool pending = false;
-    bool serial = false;        // VIRGO_SERIAL=1: commitment after the sumcheck on one stream
+    bool serial = false;        // the whole proof in order on st (caller's stream or VIRGO_SERIAL=1)
+    bool own_aux = false;       // st2 and gst[] created by the context
     fg_ctx *fg = nullptr;       // fft_gkr of the commitment (lg = n - 6), on its own stream
 };
 
@@ -762,13 +764,14 @@ int enqueue(vg_ctx *c)
     // The commitment of the input layer needs only the input values, the
     // host-known masks and challenges, never a sumcheck result: it runs on the
     // second stream beside the latency-bound sumcheck rounds.
-    HIPCHK(hipStreamWaitEvent(st2, c->ev[1], 0));
+    if (!c->serial) HIPCHK(hipStreamWaitEvent(st2, c->ev[1], 0));
     // ---- V_res (zk_prover.cpp:30-52) ----
     launch::mle_eval(c->cv[D - 1], C.bl[D - 1], c->chal + c->chal_r0, c->mle_t[0], c->mle_t[1], c->msg + c->msg_vres, st);
     // ---- GKR layers ----
     // Every layer's sumcheck depends only on the evaluated circuit and on
     // challenges known in advance, so layers run concurrently on NGS streams.
-    for (int w = 0; w < vg_ctx::NGS; ++w) HIPCHK(hipStreamWaitEvent(c->gst[w], c->ev[1], 0));
+    if (!c->serial)
+        for (int w = 0; w < vg_ctx::NGS; ++w) HIPCHK(hipStreamWaitEvent(c->gst[w], c->ev[1], 0));
     int slot = 0;
     for (size_t bi = 0; bi < c->batches.size(); ++bi) {
         const vg_ctx::batch &b = c->batches[bi];
@@ -782,10 +785,11 @@ int enqueue(vg_ctx *c)
         slot += n * launch::sumcheck_multi(I + n, n, b.bl, gs);
         launch::general_multi(Dl, n, b.ng_max, gs);
     }
-    for (int w = 0; w < vg_ctx::NGS; ++w) {
-        HIPCHK(hipEventRecord(c->gev[w], c->gst[w]));
-        HIPCHK(hipStreamWaitEvent(st, c->gev[w], 0));
-    }
+    if (!c->serial)
+        for (int w = 0; w < vg_ctx::NGS; ++w) {
+            HIPCHK(hipEventRecord(c->gev[w], c->gst[w]));
+            HIPCHK(hipStreamWaitEvent(st, c->gev[w], 0));
+        }
     if (slot != c->n_slots) FAILV(-30, "fold slot plan mismatch");
     launch::reduce_slots(c->slots, c->n_slots, c->slot_nb, c->slot_dst, c->msg, st);
     HIPCHK(hipEventRecord(c->ev[2], st));
@@ -818,13 +822,10 @@ int enqueue(vg_ctx *c)
     // ---- commit_phase folds (vpd_verifier.cpp:46-78, fri.cpp:321-390).  The
     // challenges are known in advance, so every fold runs before any hashing.
     launch::virtual_oracle(c->l, c->q, c->h, c->c0, 65, SS, c->log_R, real, m, c->tw, c->vo, st);
-    const fe *prev = c->vo;
-    int64_t S = SS;
-    for (int k = 0; k < n - 6; ++k) {
-        fe *next = c->cw + c->cw_off[k];
-        launch::fri_fold(prev, S, c->log_R, c->chal + c->chal_fri + k, c->tw, SS, next, st);
-        prev = next;
-        S /= 2;
+    {
+        fe *next[24];
+        for (int k = 0; k < n - 6; ++k) next[k] = c->cw + c->cw_off[k];
+        launch::fri_folds(c->vo, SS, n - 6, c->log_R, c->chal + c->chal_fri, c->tw, next, st);
     }
     HIPCHK(hipEventRecord(c->ev[5], st));
     // ---- every SHA3 leaf chain (fri.cpp:97-126, 397-420) in one launch, then
@@ -848,7 +849,7 @@ int enqueue(vg_ctx *c)
     }
     HIPCHK(hipEventRecord(c->ev[6], st));
     st = c->st;
-    HIPCHK(hipStreamWaitEvent(st, c->ev[6], 0));
+    if (!c->serial) HIPCHK(hipStreamWaitEvent(st, c->ev[6], 0));
     HIPCHK(hipEventRecord(c->ev[8], st));
     // ---- query answers (fri.cpp:148-289) ----
     launch::gather16(c->gsrc, c->n_gather, c->gout, st);
@@ -1171,6 +1172,19 @@ int vg_init(int device)
     HIPCHK(hipSetDevice(device));
     return 0;
 }
+int vg_stream_create(void **stream)
+{
+    if (!stream) FAILV(-1, "null argument");
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *stream = st;
+    return 0;
+}
+int vg_stream_destroy(void *stream)
+{
+    if (stream) HIPCHK(hipStreamDestroy((hipStream_t)stream));
+    return 0;
+}
 const char *vg_last_error(void) { return g_err.c_str(); }
 void vg_free(void *p) { free(p); }
 
@@ -1187,14 +1201,25 @@ int vg_ctx_create(vg_ctx **out, const vg_circuit_desc *d, void *stream)
         c->own_stream = true;
     }
     {
-        hipError_t e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
-        for (int w = 0; w < vg_ctx::NGS && e == hipSuccess; ++w) e = hipStreamCreateWithFlags(&c->gst[w], hipStreamNonBlocking);
-        for (int w = 0; w < vg_ctx::NGS && e == hipSuccess; ++w) e = hipEventCreateWithFlags(&c->gev[w], hipEventDisableTiming);
-        if (e != hipSuccess) { vg_ctx_destroy(c); FAILV(-20, "hipStreamCreate: %s", hipGetErrorString(e)); }
+        // A caller's stream (or VIRGO_SERIAL=1) runs the whole proof in
+        // order on one stream: several contexts then overlap on the device
+        // without sharing hardware queues.  Otherwise the context owns one
+        // stream for the commitment and four for the sumcheck layers.
+        const char *e = getenv("VIRGO_SERIAL");
+        c->serial = stream != nullptr || (e && e[0] == '1');
     }
     {
-        const char *e = getenv("VIRGO_SERIAL");
-        c->serial = e && e[0] == '1';
+        hipError_t e = hipSuccess;
+        if (c->serial) {
+            c->st2 = c->st;
+            for (int w = 0; w < vg_ctx::NGS; ++w) c->gst[w] = c->st;
+        } else {
+            e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
+            for (int w = 0; w < vg_ctx::NGS && e == hipSuccess; ++w) e = hipStreamCreateWithFlags(&c->gst[w], hipStreamNonBlocking);
+            c->own_aux = true;
+        }
+        for (int w = 0; w < vg_ctx::NGS && e == hipSuccess; ++w) e = hipEventCreateWithFlags(&c->gev[w], hipEventDisableTiming);
+        if (e != hipSuccess) { vg_ctx_destroy(c); FAILV(-20, "hipStreamCreate: %s", hipGetErrorString(e)); }
     }
     rc = ctx_init(c);
     if (rc) { vg_ctx_destroy(c); return rc; }
@@ -1208,9 +1233,11 @@ int vg_ctx_destroy(vg_ctx *c)
 {
     if (!c) return 0;
     if (c->st) (void)hipStreamSynchronize(c->st);
-    if (c->st2) (void)hipStreamSynchronize(c->st2);
-    for (int w = 0; w < vg_ctx::NGS; ++w)
-        if (c->gst[w]) (void)hipStreamSynchronize(c->gst[w]);
+    if (c->own_aux) {
+        if (c->st2) (void)hipStreamSynchronize(c->st2);
+        for (int w = 0; w < vg_ctx::NGS; ++w)
+            if (c->gst[w]) (void)hipStreamSynchronize(c->gst[w]);
+    }
     fg_destroy(c->fg);
     if (c->arena) (void)hipFree(c->arena);
     if (c->h_chal) (void)hipHostFree(c->h_chal);
@@ -1220,11 +1247,13 @@ int vg_ctx_destroy(vg_ctx *c)
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto &e : c->ev_leaf) if (e) (void)hipEventDestroy(e);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
-    if (c->st2) (void)hipStreamDestroy(c->st2);
-    for (int w = 0; w < vg_ctx::NGS; ++w) {
-        if (c->gst[w]) (void)hipStreamDestroy(c->gst[w]);
-        if (c->gev[w]) (void)hipEventDestroy(c->gev[w]);
+    if (c->own_aux) {
+        if (c->st2) (void)hipStreamDestroy(c->st2);
+        for (int w = 0; w < vg_ctx::NGS; ++w)
+            if (c->gst[w]) (void)hipStreamDestroy(c->gst[w]);
     }
+    for (int w = 0; w < vg_ctx::NGS; ++w)
+        if (c->gev[w]) (void)hipEventDestroy(c->gev[w]);
     delete c;
     return 0;
 }
@@ -1293,6 +1322,28 @@ int vg_prove(vg_ctx *c, uint8_t **transcript, size_t *len, double *stage_ms)
     int rc = vg_prove_async(c);
     if (rc) return rc;
     return vg_finish(c, transcript, len, stage_ms);
+}
+
+int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcripts, size_t *lens)
+{
+    if (n < 0 || (n > 0 && (!ctxs || !transcripts || !lens))) FAILV(-1, "vg_prove_batch: bad arguments");
+    for (int i = 0; i < n; ++i) { transcripts[i] = nullptr; lens[i] = 0; }
+    if (workers < 1) workers = 1;
+    if (workers > n) workers = n;
+    std::vector<int> rc(workers, 0);
+    std::vector<std::string> msg(workers);
+    std::vector<std::thread> th;
+    for (int k = 0; k < workers; ++k)
+        th.emplace_back([&, k]() {
+            for (int i = k; i < n && rc[k] == 0; i += workers) {
+                rc[k] = vg_prove(ctxs[i], &transcripts[i], &lens[i], nullptr);
+                if (rc[k]) msg[k] = g_err;                   // the worker's thread-local message
+            }
+        });
+    for (auto &t : th) t.join();
+    for (int k = 0; k < workers; ++k)
+        if (rc[k]) { g_err = msg[k]; return rc[k]; }
+    return 0;
 }
 
 int vg_output(vg_ctx *c, vg_fe *out, size_t cap, int *bits)

@@ -106,3 +106,21 @@ def read_reference_files(circuit_path):
         ty.append(arr[:, 0]); u.append(arr[:, 2]); v.append(arr[:, 3])
     return {"n": np.array(ns, np.int64), "ty": np.concatenate(ty).astype(np.int32),
             "u": np.concatenate(u), "v": np.concatenate(v)}
+
+
+def batch_jobs(count=256, seed=7):
+    """The batched workload: `count` independent proofs of uneven size, half
+    n x n matmul circuits with n log-uniform in {8, ..., 256}, half random
+    layered circuits with log2 width uniform in [10, 18] and depth uniform in
+    [4, 20] (add/mult 50/50, uniform wiring, inputs in [0, 2^31)).  Returns
+    a list of (name, circuit)."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(count):
+        if rng.integers(0, 2) == 0:
+            n = 1 << int(rng.integers(3, 9))
+            jobs.append(("matmul%d_s%d" % (n, k), matmul_circuit(n, seed=1000 + k)))
+        else:
+            lw, d = int(rng.integers(10, 19)), int(rng.integers(4, 21))
+            jobs.append(("random_w%d_d%d_s%d" % (lw, d, k), random_circuit(lw, d, seed=1000 + k)))
+    return jobs

@@ -181,7 +181,8 @@ static void fft(const fe *coef, int coef_len, int order, fe rou, fe *result)
 static void ifft(const fe *evals, int coef_len, int order, fe rou, fe *dst)
 {                                                /* RS_polynomial.cpp:169-240 */
     if (coef_len > order) coef_len = order;
-    fe *sub_eval = (fe *)malloc(sizeof(fe) * coef_len);
+    if (coef_len < 1) return;
+    fe *sub_eval = (fe *)calloc((size_t)coef_len, sizeof(fe));
     for (int i = 0; i < coef_len; ++i) sub_eval[i] = evals[(size_t)i * (order / coef_len)];
     fe new_rou = ONE();
     for (int i = 0; i < order / coef_len; ++i) new_rou = mul(new_rou, rou);

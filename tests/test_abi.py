@@ -33,3 +33,13 @@ def test_library_exports_every_declared_symbol():
 def test_python_export_list_matches_header():
     import virgo_amd
     assert sorted(virgo_amd.EXPORTS) == declared()
+
+
+def test_cpp_mirror_header_compiles():
+    """include/virgo_prover.hpp (the zk_prover mirror) compiles standalone."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = "#include \"virgo_prover.hpp\"\nint main() { return 0; }\n"
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I" + os.path.join(root, "include"),
+                        "-x", "c++", "-"], input=src.encode(), capture_output=True)
+    assert r.returncode == 0, r.stderr.decode()

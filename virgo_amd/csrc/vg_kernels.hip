@@ -290,18 +290,16 @@ __device__ __forceinline__ void fold_terms(const lpoly &V, const lpoly &aV, cons
 //   a += d_am d_V,  c += am0 V0 + aV0,  p(1) += am1 V1 + aV1,  b = p(1) - a - c
 // (three products; the reference's (a, b, c) follow exactly).
 //
-// A launch runs b rounds on chunks of 2^b consecutive values: each of the
-// 256 threads folds 2^(b-8) consecutive values through the first b - 8
-// rounds in registers, then 256 values run the last 8 rounds in LDS.  Round
-// sums of the register rounds accumulate per thread, those of the LDS rounds
-// in per-thread LDS slots, over every chunk the block processes; they are
-// reduced once when the block ends.  A chunk's result is its last value at
-// r[j0+b-1], so every launch reads plain value tables.  With a single chunk
-// (the tail) the final pair gives end[] instead.
+// Streaming launches (k_fold_reg) run up to 3 rounds with no communication:
+// each lane folds 2^b consecutive values of every table down to one value,
+// keeping its round sums in registers; the block reduces them once at the
+// end into per-block partials.  When at most 64 values remain, one wave per
+// instance (k_fold_tail) runs the last rounds in LDS and writes the round
+// sums and the final pair directly.
 // ---------------------------------------------------------------------
-constexpr int FB_B = 11;              // rounds of a multi-block launch (chunks of 2048 values)
+constexpr int FB_B = 3;               // rounds per streaming launch (8 values per lane)
+constexpr int FB_TAIL = 6;            // rounds of the tail (<= 64 values)
 constexpr int FB_MAXB = 1024;         // block partials per slot round
-constexpr int FB_LDS = 8;             // rounds in LDS (256 values)
 struct ftab { const fe *V, *aV, *am; };
 struct fsum { fe a, c, p1; };
 __device__ __forceinline__ void fsum_add(fsum &s, const fe &V0, const fe &V1, const fe &aV0, const fe &aV1,
@@ -312,127 +310,96 @@ __device__ __forceinline__ void fsum_add(fsum &s, const fe &V0, const fe &V1, co
     s.p1 = add(s.p1, add(mul(am1, V1), aV1));
 }
 __device__ __forceinline__ fe lerp(const fe &x0, const fe &x1, const fe &r) { return add(x0, mul(sub(x1, x0), r)); }
-
-__global__ __launch_bounds__(256) void k_fold_multi(const launch::fold_inst *I, int cur, int j0, int b, int64_t nchunks,
-                                                   int slot_round)
+__device__ __forceinline__ ftab fold_input(const launch::fold_inst &t, int j0, int cur)
+{
+    if (j0 == 0) return {t.Vb, t.aVb, t.amb};
+    return {t.V[cur ^ 1], t.aV[cur ^ 1], t.am[cur ^ 1]};
+}
+// rounds j0 .. j0+B-1, lane g folding values [g 2^B, (g+1) 2^B) to out[g]
+template <int B>
+__global__ __launch_bounds__(256) void k_fold_reg(const launch::fold_inst *I, int cur, int j0, int64_t nout, int slot_round)
 {
     const launch::fold_inst &t = I[blockIdx.y];
-    __shared__ fe sV[256], saV[256], sam[256];
-    __shared__ fsum lsum[255];
-    int tid = threadIdx.x;
-    int Lr = b > FB_LDS ? b - FB_LDS : 0;            // register rounds (<= 3)
-    int nl = 1 << (b - Lr);                           // values entering the LDS rounds
-    ftab X;
-    if (j0 == 0) X = {t.Vb, t.aVb, t.amb};
-    else X = {t.V[cur ^ 1], t.aV[cur ^ 1], t.am[cur ^ 1]};
+    ftab X = fold_input(t, j0, cur);
     const fe *r = t.r;
-    for (int k = tid; k < 255; k += blockDim.x) lsum[k] = {fzero(), fzero(), fzero()};
-    fsum acc[3];
+    fe rq[B];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) acc[q] = {fzero(), fzero(), fzero()};
-    __shared__ lpoly last[2];                         // final pair's V, am (single-chunk launches)
+    for (int q = 0; q < B; ++q) rq[q] = r[j0 + q];
+    fsum acc[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) acc[q] = {fzero(), fzero(), fzero()};
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nout; g += (int64_t)gridDim.x * blockDim.x) {
+        int64_t o = g << B;
+        fe v[3][1 << (B - 1)];
+#pragma unroll
+        for (int p2 = 0; p2 < (1 << (B - 1)); ++p2) {
+            fe V0 = X.V[o + 2 * p2], V1 = X.V[o + 2 * p2 + 1];
+            fe aV0 = X.aV[o + 2 * p2], aV1 = X.aV[o + 2 * p2 + 1];
+            fe am0 = X.am[o + 2 * p2], am1 = X.am[o + 2 * p2 + 1];
+            fsum_add(acc[0], V0, V1, aV0, aV1, am0, am1);
+            v[0][p2] = lerp(V0, V1, rq[0]); v[1][p2] = lerp(aV0, aV1, rq[0]); v[2][p2] = lerp(am0, am1, rq[0]);
+        }
+#pragma unroll
+        for (int q = 1; q < B; ++q) {
+#pragma unroll
+            for (int p2 = 0; p2 < (1 << (B - 1 - q)); ++p2) {
+                fsum_add(acc[q], v[0][2 * p2], v[0][2 * p2 + 1], v[1][2 * p2], v[1][2 * p2 + 1], v[2][2 * p2], v[2][2 * p2 + 1]);
+                v[0][p2] = lerp(v[0][2 * p2], v[0][2 * p2 + 1], rq[q]);
+                v[1][p2] = lerp(v[1][2 * p2], v[1][2 * p2 + 1], rq[q]);
+                v[2][p2] = lerp(v[2][2 * p2], v[2][2 * p2 + 1], rq[q]);
+            }
+        }
+        t.V[cur][g] = v[0][0]; t.aV[cur][g] = v[1][0]; t.am[cur][g] = v[2][0];
+    }
+    // per-block partials (a, b, c) with b = p(1) - a - c
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+        fe v3[3] = {acc[q].a, acc[q].c, acc[q].p1};
+        block_sum<3>(v3);
+        if (threadIdx.x == 0) {
+            fe o3[3] = {v3[0], sub(sub(v3[2], v3[0]), v3[1]), v3[1]};
+            for (int c = 0; c < 3; ++c) t.slots[((int64_t)(slot_round + q) * 3 + c) * FB_MAXB + blockIdx.x] = o3[c];
+        }
+    }
+}
+// the last b <= 6 rounds of each instance, one wave each: round sums to
+// sums[3j..], the final pair's V.a, V.b, am.a, am.b and v = V(r[L-1]) + vconst to end[]
+__global__ __launch_bounds__(64) void k_fold_tail(const launch::fold_inst *I, int cur, int j0, int b)
+{
+    const launch::fold_inst &t = I[blockIdx.y];
+    ftab X = fold_input(t, j0, cur);
+    __shared__ fe sV[64], saV[64], sam[64];
+    __shared__ fsum ls[32];
+    int lane = threadIdx.x, n = 1 << b;
+    if (lane < n) { sV[lane] = X.V[lane]; saV[lane] = X.aV[lane]; sam[lane] = X.am[lane]; }
     __syncthreads();
-    for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-        int64_t base = ch << b;
-        if (Lr > 0) {
-            // register rounds: 2^Lr consecutive values per thread
-            fe v[3][4];
-            int E = 1 << Lr;
-            int64_t o = base + (int64_t)tid * E;
-            fe r0 = r[j0];
-#pragma unroll
-            for (int p2 = 0; p2 < 4; ++p2) {
-                if (2 * p2 < E) {
-                    fe V0 = X.V[o + 2 * p2], V1 = X.V[o + 2 * p2 + 1];
-                    fe aV0 = X.aV[o + 2 * p2], aV1 = X.aV[o + 2 * p2 + 1];
-                    fe am0 = X.am[o + 2 * p2], am1 = X.am[o + 2 * p2 + 1];
-                    fsum_add(acc[0], V0, V1, aV0, aV1, am0, am1);
-                    v[0][p2] = lerp(V0, V1, r0); v[1][p2] = lerp(aV0, aV1, r0); v[2][p2] = lerp(am0, am1, r0);
-                }
+    for (int q = 0; q < b; ++q) {
+        int no = n >> 1;
+        fe nv0, nv1, nv2;
+        if (lane < no) {
+            fe V0 = sV[2 * lane], V1 = sV[2 * lane + 1], aV0 = saV[2 * lane], aV1 = saV[2 * lane + 1];
+            fe am0 = sam[2 * lane], am1 = sam[2 * lane + 1];
+            fsum f = {fzero(), fzero(), fzero()};
+            fsum_add(f, V0, V1, aV0, aV1, am0, am1);
+            ls[lane] = f;
+            fe rr = t.r[j0 + q];
+            nv0 = lerp(V0, V1, rr); nv1 = lerp(aV0, aV1, rr); nv2 = lerp(am0, am1, rr);
+            if (no == 1) {
+                t.end[0] = sub(V1, V0); t.end[1] = V0; t.end[2] = sub(am1, am0); t.end[3] = am0;
+                fe vc = t.vconst ? *t.vconst : fzero();
+                t.end[4] = add(nv0, vc);
             }
-#pragma unroll
-            for (int q = 1; q < 3; ++q) {
-                if (q < Lr) {
-                    fe rq = r[j0 + q];
-#pragma unroll
-                    for (int p2 = 0; p2 < 2; ++p2) {
-                        if (2 * p2 < (E >> q)) {
-                            fsum_add(acc[q], v[0][2 * p2], v[0][2 * p2 + 1], v[1][2 * p2], v[1][2 * p2 + 1], v[2][2 * p2],
-                                     v[2][2 * p2 + 1]);
-                            v[0][p2] = lerp(v[0][2 * p2], v[0][2 * p2 + 1], rq);
-                            v[1][p2] = lerp(v[1][2 * p2], v[1][2 * p2 + 1], rq);
-                            v[2][p2] = lerp(v[2][2 * p2], v[2][2 * p2 + 1], rq);
-                        }
-                    }
-                }
-            }
-            sV[tid] = v[0][0]; saV[tid] = v[1][0]; sam[tid] = v[2][0];
-        } else if (tid < nl) {
-            sV[tid] = X.V[base + tid]; saV[tid] = X.aV[base + tid]; sam[tid] = X.am[base + tid];
         }
         __syncthreads();
-        // LDS rounds Lr .. b-1
-        int n = nl, slot = 0;
-        for (int q = Lr; q < b; ++q) {
-            int no = n >> 1;
-            bool act = tid < no;
-            fe nv0, nv1, nv2;
-            if (act) {
-                fe V0 = sV[2 * tid], V1 = sV[2 * tid + 1], aV0 = saV[2 * tid], aV1 = saV[2 * tid + 1];
-                fe am0 = sam[2 * tid], am1 = sam[2 * tid + 1];
-                fsum ls = lsum[slot + tid];
-                fsum_add(ls, V0, V1, aV0, aV1, am0, am1);
-                lsum[slot + tid] = ls;
-                fe rq = r[j0 + q];
-                nv0 = lerp(V0, V1, rq); nv1 = lerp(aV0, aV1, rq); nv2 = lerp(am0, am1, rq);
-                if (no == 1) { last[0] = {sub(V1, V0), V0}; last[1] = {sub(am1, am0), am0}; }
-            }
-            __syncthreads();
-            if (act) { sV[tid] = nv0; saV[tid] = nv1; sam[tid] = nv2; }
-            __syncthreads();
-            slot += no;
-            n = no;
+        if (lane == 0) {
+            fsum sm = {fzero(), fzero(), fzero()};
+            for (int k = 0; k < no; ++k) { sm.a = add(sm.a, ls[k].a); sm.c = add(sm.c, ls[k].c); sm.p1 = add(sm.p1, ls[k].p1); }
+            fe *o = t.sums + 3 * (j0 + q);
+            o[0] = sm.a; o[1] = sub(sub(sm.p1, sm.a), sm.c); o[2] = sm.c;
         }
-        if (tid == 0 && nchunks > 1) { t.V[cur][ch] = sV[0]; t.aV[cur][ch] = saV[0]; t.am[cur][ch] = sam[0]; }
+        if (lane < no) { sV[lane] = nv0; saV[lane] = nv1; sam[lane] = nv2; }
         __syncthreads();
-    }
-    // round sums of this block: (a, b, c) with b = p(1) - a - c
-    auto emit = [&](int q, const fsum &sm) {
-        fe v3[3] = {sm.a, sub(sub(sm.p1, sm.a), sm.c), sm.c};
-        for (int c = 0; c < 3; ++c) {
-            if (nchunks > 1) t.slots[((int64_t)(slot_round + q) * 3 + c) * FB_MAXB + blockIdx.x] = v3[c];
-            else t.sums[3 * (j0 + q) + c] = v3[c];
-        }
-    };
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        if (q < Lr) {
-            fe v3[3] = {acc[q].a, acc[q].c, acc[q].p1};
-            block_sum<3>(v3);
-            if (tid == 0) emit(q, {v3[0], v3[1], v3[2]});
-        }
-    }
-    {
-        int n = nl, slot = 0;
-        for (int q = Lr; q < b; ++q) {
-            int no = n >> 1;
-            if (tid == q - Lr) {
-                fsum sm = {fzero(), fzero(), fzero()};
-                for (int k = 0; k < no; ++k) {
-                    const fsum &x = lsum[slot + k];
-                    sm.a = add(sm.a, x.a); sm.c = add(sm.c, x.c); sm.p1 = add(sm.p1, x.p1);
-                }
-                emit(q, sm);
-            }
-            slot += no;
-            n = no;
-        }
-    }
-    if (nchunks == 1 && tid == 0) {
-        lpoly lv = last[0], la = last[1];
-        t.end[0] = lv.a; t.end[1] = lv.b; t.end[2] = la.a; t.end[3] = la.b;
-        fe vc = t.vconst ? *t.vconst : fzero();
-        t.end[4] = add(add(mul(lv.a, r[j0 + b - 1]), lv.b), vc);
+        n = no;
     }
 }
 
@@ -739,22 +706,36 @@ __global__ void k_slice_sums(const fe *lq, int64_t stride, int nslice, int64_t r
     c0[i] = c;
     all_sum[i] = mul_base(c, (uint64_t)len);
 }
-__global__ void k_vo(const fe *l, const fe *q, const fe *h, const fe *c0, int nslice, int64_t SS, int log_R, int64_t real,
-                     int64_t m, const fe *tw, fe *vo)
+// virtual oracle at natural position p of slice `slice` (poly_commit.h:246-263, 305-322):
+// ((l q - (x^e - 1) h - c0) / x) e at x = w_SS^p, e = slice length
+__device__ __forceinline__ fe vo_at(const fe *l, const fe *q, const fe *h, const fe *c0, int slice, int nslice, int64_t p,
+                                    int64_t SS, int log_R, int64_t real, int64_t m, const fe *tw)
 {
-    int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (id >= (int64_t)nslice * SS) return;
-    int slice = (int)(id / SS);
-    int64_t st = id - slice * SS;
-    int64_t cols = SS >> log_R;
-    int64_t s = st / cols, t = st - s * cols;
-    int64_t p = (t << log_R) + s;
+    int64_t id = (int64_t)slice * SS + cw_idx(p, SS, log_R);
     int64_t e = (slice == nslice - 1) ? m : real;
     fe xe = tw[(p * e) & (SS - 1)], xinv = tw[(SS - p) & (SS - 1)];
     fe g = sub(mul(l[id], q[id]), mul(sub(xe, fone()), h[id]));
-    vo[id] = mul_base(mul(sub(g, c0[slice]), xinv), (uint64_t)e);
+    return mul_base(mul(sub(g, c0[slice]), xinv), (uint64_t)e);
 }
-
+// the virtual oracle fused into the first FRI fold: next (SS/2 per slice,
+// codeword layout) from vo at p and p + SS/2, which is never stored
+__global__ __launch_bounds__(TPB) void k_vo_fold(const fe *l, const fe *q, const fe *h, const fe *c0, int nslice, int64_t SS,
+                                                int log_R, int64_t real, int64_t m, const fe *tw, const fe *rp, fe *next)
+{
+    int64_t Sn = SS / 2;
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)nslice * Sn) return;
+    int j = (int)(tid / Sn);
+    int64_t st = tid - (int64_t)j * Sn;
+    int64_t cols = Sn >> log_R;
+    int64_t srow = st / cols, t = st - srow * cols;
+    int64_t p = (t << log_R) + srow;
+    fe a = vo_at(l, q, h, c0, j, nslice, p, SS, log_R, real, m, tw);
+    fe b = vo_at(l, q, h, c0, j, nslice, p + Sn, SS, log_R, real, m, tw);
+    fe imr = mul(tw[(SS - p) & (SS - 1)], *rp);
+    const fe inv2 = mk(1ull << 60, 0);
+    next[tid] = mul(inv2, add(add(a, b), mul(imr, sub(a, b))));
+}
 // ---------------------------------------------------------------------
 // hashing: leaf chains and Merkle levels
 // ---------------------------------------------------------------------
@@ -1162,24 +1143,23 @@ void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipS
 }
 int fold_plan(int L, int *js, int *ks)
 {
-    // launches of FB_B rounds while more than FB_B rounds remain; the last
-    // <= FB_B rounds run in the single-block tail
+    // streaming launches of <= FB_B rounds until at most FB_TAIL rounds remain
     int n = 0, j = 0;
-    while (L - j > FB_B) {
-        js[n] = j; ks[n] = FB_B; ++n;
-        j += FB_B;
+    while (L - j > FB_TAIL) {
+        int b = L - j - FB_TAIL < FB_B ? L - j - FB_TAIL : FB_B;
+        js[n] = j; ks[n] = b; ++n;
+        j += b;
     }
     return n;
 }
 int fold_blocks(int L, int j)
 {
-    // a few chunks per block, so the per-block reductions amortize
     int js[32], ks[32];
     int n = fold_plan(L, js, ks);
     for (int k = 0; k < n; ++k)
         if (js[k] == j) {
-            int64_t ch = (int64_t)1 << (L - j - ks[k]);
-            int64_t g = (ch + 3) / 4;
+            int64_t lanes = (int64_t)1 << (L - j - ks[k]);
+            int64_t g = (lanes + 255) / 256;
             return (int)(g > FB_MAXB ? FB_MAXB : g);
         }
     return 0;
@@ -1191,14 +1171,16 @@ int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s)
     int cur = 0, rounds = 0;
     for (int k = 0; k < nl; ++k) {
         int j = js[k], b = ks[k];
-        int64_t nch = (int64_t)1 << (L - j - b);
-        unsigned g = (unsigned)fold_blocks(L, j);
-        hipLaunchKernelGGL(k_fold_multi, dim3(g, ninst), dim3(256), 0, s, I, cur, j, b, nch, rounds);
+        int64_t nout = (int64_t)1 << (L - j - b);
+        dim3 g((unsigned)fold_blocks(L, j), ninst);
+        if (b == 3) hipLaunchKernelGGL(k_fold_reg<3>, g, dim3(256), 0, s, I, cur, j, nout, rounds);
+        else if (b == 2) hipLaunchKernelGGL(k_fold_reg<2>, g, dim3(256), 0, s, I, cur, j, nout, rounds);
+        else hipLaunchKernelGGL(k_fold_reg<1>, g, dim3(256), 0, s, I, cur, j, nout, rounds);
         cur ^= 1;
         rounds += b;
     }
     int j = nl ? js[nl - 1] + ks[nl - 1] : 0;
-    hipLaunchKernelGGL(k_fold_multi, dim3(1, ninst), dim3(256), 0, s, I, cur, j, L - j, (int64_t)1, rounds);
+    hipLaunchKernelGGL(k_fold_tail, dim3(1, ninst), dim3(64), 0, s, I, cur, j, L - j);
     return rounds;
 }
 void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s)
@@ -1243,12 +1225,6 @@ void lde(const fe *coef, int64_t coef_stride, const int *lens, int nb, fe *out, 
     ni.kind = 2; ni.src = coef; ni.stride = coef_stride; ni.lens = lens; ni.log_Rl = log_R; ni.hmode = hmode;
     ntt_run(ni, out, (int64_t)1 << log_C, (int64_t)nb << log_R, log_C, 0, fone(), 0, T, s);
 }
-void virtual_oracle(const fe *l, const fe *q, const fe *h, const fe *c0, int nslice, int64_t SS, int log_R, int64_t real,
-                    int64_t m, const fe *tw, fe *vo, hipStream_t s)
-{
-    int64_t n = (int64_t)nslice * SS;
-    hipLaunchKernelGGL(k_vo, dim3(nblk(n)), dim3(TPB), 0, s, l, q, h, c0, nslice, SS, log_R, real, m, tw, vo);
-}
 void slice_sums(const fe *lq_coef, int64_t stride, int nslice, int64_t real, int64_t m, fe *c0, fe *all_sum, hipStream_t s)
 {
     hipLaunchKernelGGL(k_slice_sums, dim3(1), dim3(128), 0, s, lq_coef, stride, nslice, real, m, c0, all_sum);
@@ -1279,10 +1255,19 @@ void fri_fold(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, i
 {
     hipLaunchKernelGGL(k_fri_fold, dim3(nblk(65 * (S / 2))), dim3(TPB), 0, s, prev, S, log_R, r, tw, SS, next);
 }
-void fri_folds(const fe *vo, int64_t SS, int nlev, int log_R, const fe *r, const fe *tw, fe *const *next, hipStream_t s)
+void fri_folds_vo(const fe *l, const fe *q, const fe *h, const fe *c0, int64_t real, int64_t m, int64_t SS, int nlev,
+                  int log_R, const fe *r, const fe *tw, fe *const *next, hipStream_t s)
+{
+    // level 0 straight from l, q, h; then as fri_folds
+    hipLaunchKernelGGL(k_vo_fold, dim3(nblk(65 * (SS / 2))), dim3(TPB), 0, s, l, q, h, c0, 65, SS, log_R, real, m, tw, r,
+                       next[0]);
+    if (nlev > 1) fri_folds(next[0], SS / 2, SS, nlev - 1, log_R, r + 1, tw, next + 1, s);
+}
+void fri_folds(const fe *vo, int64_t S0, int64_t SS, int nlev, int log_R, const fe *r, const fe *tw, fe *const *next,
+               hipStream_t s)
 {
     const fe *prev = vo;
-    int64_t S = SS;
+    int64_t S = S0;
     int k = 0;
     for (; k < nlev && S > FRI_TAIL; ++k, S /= 2) {
         fri_fold(prev, S, log_R, r + k, tw, SS, next[k], s);

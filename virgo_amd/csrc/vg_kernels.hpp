@@ -73,20 +73,19 @@ constexpr int MAX_PARTS = 1024;
 void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipStream_t s);
 // Sumcheck phases (zk_prover.cpp:600-633, 992-1025) of a batch of ninst
 // instances with flat tables of 2^L entries each, blockIdx.y = instance:
-// launches of 11 rounds on chunks of 2048 values (8 values per thread in
-// registers for the first 3 rounds, the last 8 in LDS), then the last <= 11
-// rounds in one block per instance.  Round j's pair polynomials are
-// evaluated at r[j].  Each multi-block launch writes its block partials of
-// round q to slots[(q*3 + c) * FW_BLOCKS + block] (q counted over the
-// instance's launches); the caller reduces every slot of the proof once with
-// reduce_slots.  Tail round sums go straight to sums[3j..].  After the last
-// round: end[0..3] = the final pair's V.a, V.b, am.a, am.b and
-// end[4] = V.a * r[L-1] + V.b + *vconst (= v_u or v_v).  Returns the slot
-// rounds per instance.
+// streaming launches of up to 3 rounds (8 consecutive values per lane, no
+// communication), then the last <= 6 rounds in one wave per instance.
+// Round j's pair polynomials are evaluated at r[j].  Each streaming launch
+// writes its block partials of round q to slots[(q*3 + c) * FW_BLOCKS +
+// block] (q counted over the instance's launches); the caller reduces every
+// slot of the proof once with reduce_slots.  Tail round sums go straight to
+// sums[3j..].  After the last round: end[0..3] = the final pair's V.a, V.b,
+// am.a, am.b and end[4] = V.a * r[L-1] + V.b + *vconst (= v_u or v_v).
+// Returns the slot rounds per instance.
 constexpr int FW_BLOCKS = 1024;
 // values of each table a phase on 2^L entries hands from one launch to the
-// next (the chunk results of its first launch, chunks of 2^11)
-inline int64_t fold_ws_len(int L) { return L > 21 ? ((int64_t)1 << (L - 11)) : 1024; }
+// next (the output of its first streaming launch)
+inline int64_t fold_ws_len(int L) { return L > 9 ? ((int64_t)1 << (L - 3)) : 64; }
 int fold_plan(int L, int *js, int *ks);
 int fold_blocks(int L, int j);
 struct fold_inst {
@@ -142,9 +141,6 @@ void intt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, fe inv_n,
 // hmode: coefficients of slice b start at coef + b*coef_stride + lens[b] (upper half of lq)
 void lde(const fe *coef, int64_t coef_stride, const int *lens, int nb, fe *out, int log_C, int log_R,
          const ntt_tabs &T, hipStream_t s, int hmode = 0);
-// virtual oracle of commit_public_array (poly_commit.h:246-263, 305-322)
-void virtual_oracle(const fe *l, const fe *q, const fe *h, const fe *c0, int nslice, int64_t SS, int log_R,
-                    int64_t real, int64_t m, const fe *tw, fe *vo, hipStream_t s);
 // all_sum[i] = (lq_coef[0] + lq_coef[len_i]) * len_i; c0[i] = lq_coef[0] + lq_coef[len_i]
 void slice_sums(const fe *lq_coef, int64_t stride, int nslice, int64_t real, int64_t m, fe *c0, fe *all_sum, hipStream_t s);
 
@@ -157,9 +153,14 @@ void merkle_pad(uint8_t *tree, int64_t pad, int64_t n_leaf, hipStream_t s);
 void fe_mul(const fe *a, const fe *b, fe *o, int64_t n, hipStream_t s);
 // FRI fold only (no hashing): next (S/2 per slice) from prev (S per slice)
 void fri_fold(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, int64_t SS, fe *next, hipStream_t s);
-// every FRI fold of a commitment: nlev levels from vo (SS per slice), level k
-// with challenge r[k] into next[k]; the small levels in one launch
-void fri_folds(const fe *vo, int64_t SS, int nlev, int log_R, const fe *r, const fe *tw, fe *const *next, hipStream_t s);
+// FRI folds: nlev levels from a codeword of S0 positions per slice (twiddles
+// w_SS), level k with challenge r[k] into next[k]; the small levels in one launch
+void fri_folds(const fe *cw, int64_t S0, int64_t SS, int nlev, int log_R, const fe *r, const fe *tw, fe *const *next,
+               hipStream_t s);
+// the same from the virtual oracle of l, q, h (poly_commit.h:246-263,
+// 305-322), computed inside the first fold and never stored
+void fri_folds_vo(const fe *l, const fe *q, const fe *h, const fe *c0, int64_t real, int64_t m, int64_t SS, int nlev,
+                  int log_R, const fe *r, const fe *tw, fe *const *next, hipStream_t s);
 // leaf chains of nseg codewords in one launch (codeword k: S[k] positions per
 // slice, leaves S[k]/2 written to tree[k] + S[k]/2 nodes)
 void leaf_hash_multi(int nseg, const fe *const *cw, const int64_t *S, uint8_t *const *tree, int log_R, hipStream_t s);

@@ -476,7 +476,7 @@ struct vg_ctx {
     fe *ntW = nullptr, *ntWi = nullptr, *ntTA = nullptr, *ntTB = nullptr;   // NTT stage and coset tables
     launch::ntt_tabs NT = {};
     fe *tw = nullptr, *T12 = nullptr, *coefP = nullptr, *coefQ = nullptr, *l = nullptr, *q = nullptr, *h = nullptr,
-       *vo = nullptr, *qv = nullptr, *eqs = nullptr, *lqc = nullptr, *c0 = nullptr, *all_sum = nullptr, *pri = nullptr,
+       *qv = nullptr, *eqs = nullptr, *lqc = nullptr, *c0 = nullptr, *all_sum = nullptr, *pri = nullptr,
        *pubm = nullptr, *cw = nullptr, *gout = nullptr;
     int *lens = nullptr;
     uint8_t *treeL = nullptr, *treeH = nullptr, *ctree = nullptr;
@@ -588,7 +588,6 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     c->l = carve<fe>(base, off, 65 * SS);
     c->q = carve<fe>(base, off, 65 * SS);
     c->h = carve<fe>(base, off, 65 * SS);
-    c->vo = carve<fe>(base, off, 65 * SS);
     c->qv = carve<fe>(base, off, (int64_t)1 << c->n);
     c->eqs = carve<fe>(base, off, 2 * ((int64_t)1 << (c->n / 2 + 1)));
     c->lqc = carve<fe>(base, off, 65 * c->LQ);
@@ -821,11 +820,10 @@ int enqueue(vg_ctx *c)
     HIPCHK(hipEventRecord(c->ev[4], st));
     // ---- commit_phase folds (vpd_verifier.cpp:46-78, fri.cpp:321-390).  The
     // challenges are known in advance, so every fold runs before any hashing.
-    launch::virtual_oracle(c->l, c->q, c->h, c->c0, 65, SS, c->log_R, real, m, c->tw, c->vo, st);
     {
         fe *next[24];
         for (int k = 0; k < n - 6; ++k) next[k] = c->cw + c->cw_off[k];
-        launch::fri_folds(c->vo, SS, n - 6, c->log_R, c->chal + c->chal_fri, c->tw, next, st);
+        launch::fri_folds_vo(c->l, c->q, c->h, c->c0, real, m, SS, n - 6, c->log_R, c->chal + c->chal_fri, c->tw, next, st);
     }
     HIPCHK(hipEventRecord(c->ev[5], st));
     // ---- every SHA3 leaf chain (fri.cpp:97-126, 397-420) in one launch, then

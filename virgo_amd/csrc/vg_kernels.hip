@@ -157,41 +157,72 @@ __device__ void reduce_prev(const fe *prev, int nprev, fe *dst)
 
 // ---------------------------------------------------------------------
 // Table builds by gather over CSR transposes (owner computes, no atomics).
-// Phase 1 (zk_prover.cpp:420-587): every table entry t sums the gates that
-// feed it through u (a sum gate feeds every t in its range).  Phase 2
-// (:845-972): entry t sums the gates whose v is t; the gates that feed entry
-// 0 unconditionally (sum, exp-sum, NOT, relay) are reduced separately.
+// Phase 1 (zk_prover.cpp:375-592): table entry t sums the contributions of
+// the gates that reach it through u (a sum gate reaches every t of its
+// range; its record carries t in place of v).  Phase 2 (:789-983): entry t
+// sums the gates whose v is t; the gates that reach entry 0 unconditionally
+// (sum, exp-sum, NOT, relay) are reduced separately into part[MAX_PARTS].
+// A record's contribution (am, aV) depends on the record alone; add and
+// mult, the common types, take one branch-free path.
 // ---------------------------------------------------------------------
-// Table builds of one sumcheck phase for a batch of layers (blockIdx.y =
-// layer), by gather over the CSR transposes built once per circuit: entry t
-// of the tables sums the contributions of the gates that reach it through u
-// (phase 1, zk_prover.cpp:375-592) or v (phase 2, :789-983).
+// types whose phase-1 contribution reads V(v)
+__device__ __forceinline__ bool p1_reads_v(int ty) { return ty <= 1 || ty == 7 || ty == 8 || ty == 9 || ty == 13; }
+// pv = V(R.v) for the types that read it (p1_reads_v), anything otherwise
+__device__ __forceinline__ void p1_rec(const launch::gate_rec &R, const bg_t &bg, const fe &pv, fe &am, fe &aV)
+{
+    int ty = R.ty;
+    if (ty <= 1) {
+        fe tmp = beta_g(bg, R.g), x = mul(pv, tmp);
+        am = ty == 0 ? tmp : x;
+        aV = ty == 0 ? x : fzero();
+        return;
+    }
+    am = fzero(); aV = fzero();
+    if (ty == 4) { am = beta_g(bg, R.u); return; }
+    fe tmp = beta_g(bg, R.g);
+    switch (ty) {
+    case 5: case 10: am = tmp; break;
+    case 12: am = mul_base(tmp, 1ull << ((R.v - R.u) % 61)); break;   // R.v = t; 2^61 == 1
+    case 6: am = neg(tmp); aV = tmp; break;
+    case 7: aV = neg(mul(pv, tmp)); am = tmp; break;
+    case 8: { fe tv = mul(tmp, pv); aV = tv; am = sub(tmp, dbl(tv)); break; }
+    case 13: { fe tv = mul(tmp, pv); am = sub(tmp, tv); break; }
+    case 9: { fe tv = mul(tmp, pv); aV = tv; am = neg(tv); break; }
+    default: break;
+    }
+}
+__device__ __forceinline__ void p2_rec(const launch::gate_rec &R, const bg_t &bg, const bu_t &bu, const fe &vu, fe &am,
+                                       fe &aV)
+{
+    int ty = R.ty;
+    fe tgu = mul(beta_g(bg, R.g), beta_u(bu, R.u)), x = mul(tgu, vu);
+    if (ty <= 1) {
+        am = ty == 0 ? tgu : x;
+        aV = ty == 0 ? x : fzero();
+        return;
+    }
+    am = fzero(); aV = fzero();
+    switch (ty) {
+    case 7: am = neg(tgu); aV = x; break;
+    case 8: am = sub(tgu, dbl(x)); aV = x; break;
+    case 13: am = neg(x); aV = x; break;
+    case 9: am = sub(tgu, x); break;
+    default: break;
+    }
+}
+// one entry per thread (tables of at most 64 entries: the tail fold reads them)
 __global__ __launch_bounds__(TPB) void k_p1_gather(const launch::layer_dev *D)
 {
     const launch::layer_dev &d = D[blockIdx.y];
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.nt) return;
     bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
-    const fe *pv = d.pv;
     fe am = fzero(), aV = fzero();
     for (uint32_t e = d.off1[t]; e < d.off1[t + 1]; ++e) {
         launch::gate_rec R = d.ent1[e];
-        int g = R.g, ty = R.ty, u = R.u, v = R.v;
-        if (ty == 4) { am = add(am, beta_g(bg, u)); continue; }
-        fe tmp = beta_g(bg, g);
-        switch (ty) {
-        case 0: aV = add(aV, mul(pv[v], tmp)); am = add(am, tmp); break;
-        case 1: am = add(am, mul(pv[v], tmp)); break;
-        case 5: am = add(am, tmp); break;
-        case 12: am = add(am, mul_base(tmp, 1ull << ((t - u) % 61))); break;   // 2^61 == 1
-        case 6: am = sub(am, tmp); aV = add(aV, tmp); break;
-        case 7: aV = sub(aV, mul(pv[v], tmp)); am = add(am, tmp); break;
-        case 8: { fe tv = mul(tmp, pv[v]); aV = add(aV, tv); am = sub(add(am, tmp), dbl(tv)); break; }
-        case 13: { fe tv = mul(tmp, pv[v]); am = add(sub(am, tv), tmp); break; }
-        case 9: { fe tv = mul(tmp, pv[v]); aV = add(aV, tv); am = sub(am, tv); break; }
-        case 10: am = add(am, tmp); break;
-        default: break;
-        }
+        fe m, a;
+        p1_rec(R, bg, d.pv[p1_reads_v(R.ty) ? R.v : 0], m, a);
+        am = add(am, m); aV = add(aV, a);
     }
     d.amb[t] = am;
     d.aVb[t] = aV;
@@ -204,20 +235,11 @@ __global__ __launch_bounds__(TPB) void k_p2_gather(const launch::layer_dev *D)
     bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
     bu_t bu = {d.bu.f, d.bu.s, d.bu.fh};
     fe vu = *d.vu;
-    fe am = fzero(), aV = fzero();
+    fe am = fzero(), aV = (t == 0 && d.nz > 0) ? d.part[launch::MAX_PARTS] : fzero();
     for (uint32_t e = d.off2[t]; e < d.off2[t + 1]; ++e) {
-        launch::gate_rec R = d.ent2[e];
-        int g = R.g, ty = R.ty, u = R.u;
-        fe tgu = mul(beta_g(bg, g), beta_u(bu, u));
-        switch (ty) {
-        case 1: am = add(am, mul(tgu, vu)); break;
-        case 0: am = add(am, tgu); aV = add(aV, mul(tgu, vu)); break;
-        case 7: am = sub(am, tgu); aV = add(aV, mul(tgu, vu)); break;
-        case 8: { fe tv = mul(tgu, vu); am = sub(add(am, tgu), dbl(tv)); aV = add(aV, tv); break; }
-        case 13: { fe tv = mul(tgu, vu); am = sub(am, tv); aV = add(aV, tv); break; }
-        case 9: am = add(am, sub(tgu, mul(vu, tgu))); break;
-        default: break;
-        }
+        fe m, a;
+        p2_rec(d.ent2[e], bg, bu, vu, m, a);
+        am = add(am, m); aV = add(aV, a);
     }
     d.amb[t] = am;
     d.aVb[t] = aV;
@@ -245,13 +267,13 @@ __global__ __launch_bounds__(TPB) void k_p2_zero(const launch::layer_dev *D)
     block_sum<1>(&acc);
     if (threadIdx.x == 0) d.part[blockIdx.x] = acc;
 }
-__global__ void k_add_parts(const launch::layer_dev *D, int n)
+__global__ void k_zero_sum(const launch::layer_dev *D, int n)
 {
     const launch::layer_dev &d = D[blockIdx.y];
     fe acc = fzero();
     for (int k = threadIdx.x; k < n; k += blockDim.x) acc = add(acc, d.part[k]);
     block_sum<1>(&acc);
-    if (threadIdx.x == 0) d.aVb[0] = add(d.aVb[0], acc);
+    if (threadIdx.x == 0) d.part[launch::MAX_PARTS] = acc;
 }
 
 // ---------------------------------------------------------------------
@@ -297,9 +319,9 @@ __device__ __forceinline__ void fold_terms(const lpoly &V, const lpoly &aV, cons
 // instance (k_fold_tail) runs the last rounds in LDS and writes the round
 // sums and the final pair directly.
 // ---------------------------------------------------------------------
-constexpr int FB_B = 3;               // rounds per streaming launch (8 values per lane)
-constexpr int FB_TAIL = 6;            // rounds of the tail (<= 64 values)
-constexpr int FB_MAXB = 1024;         // block partials per slot round
+using launch::FB_B;
+using launch::FB_TAIL;
+constexpr int FB_MAXB = launch::FW_BLOCKS;   // block partials per slot round
 struct ftab { const fe *V, *aV, *am; };
 struct fsum { fe a, c, p1; };
 __device__ __forceinline__ void fsum_add(fsum &s, const fe &V0, const fe &V1, const fe &aV0, const fe &aV1,
@@ -315,43 +337,39 @@ __device__ __forceinline__ ftab fold_input(const launch::fold_inst &t, int j0, i
     if (j0 == 0) return {t.Vb, t.aVb, t.amb};
     return {t.V[cur ^ 1], t.aV[cur ^ 1], t.am[cur ^ 1]};
 }
-// rounds j0 .. j0+B-1, lane g folding values [g 2^B, (g+1) 2^B) to out[g]
-template <int B>
-__global__ __launch_bounds__(256) void k_fold_reg(const launch::fold_inst *I, int cur, int j0, int64_t nout, int slot_round)
+// B rounds on the 2^B consecutive values one lane holds of each table;
+// ld(k, V, aV, am) supplies value k of the three tables (called in order,
+// one pair at a time, so at most half of them are live at once).  Round q's
+// sums go into acc[q]; the folded values into oV, oaV, oam.
+template <int B, class Load>
+__device__ __forceinline__ void fold_lane(Load ld, const fe *rq, fsum *acc, fe &oV, fe &oaV, fe &oam)
 {
-    const launch::fold_inst &t = I[blockIdx.y];
-    ftab X = fold_input(t, j0, cur);
-    const fe *r = t.r;
-    fe rq[B];
+    constexpr int H = 1 << (B - 1);
+    fe V[H], aV[H], am[H];
 #pragma unroll
-    for (int q = 0; q < B; ++q) rq[q] = r[j0 + q];
-    fsum acc[B];
-#pragma unroll
-    for (int q = 0; q < B; ++q) acc[q] = {fzero(), fzero(), fzero()};
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nout; g += (int64_t)gridDim.x * blockDim.x) {
-        int64_t o = g << B;
-        fe v[3][1 << (B - 1)];
-#pragma unroll
-        for (int p2 = 0; p2 < (1 << (B - 1)); ++p2) {
-            fe V0 = X.V[o + 2 * p2], V1 = X.V[o + 2 * p2 + 1];
-            fe aV0 = X.aV[o + 2 * p2], aV1 = X.aV[o + 2 * p2 + 1];
-            fe am0 = X.am[o + 2 * p2], am1 = X.am[o + 2 * p2 + 1];
-            fsum_add(acc[0], V0, V1, aV0, aV1, am0, am1);
-            v[0][p2] = lerp(V0, V1, rq[0]); v[1][p2] = lerp(aV0, aV1, rq[0]); v[2][p2] = lerp(am0, am1, rq[0]);
-        }
-#pragma unroll
-        for (int q = 1; q < B; ++q) {
-#pragma unroll
-            for (int p2 = 0; p2 < (1 << (B - 1 - q)); ++p2) {
-                fsum_add(acc[q], v[0][2 * p2], v[0][2 * p2 + 1], v[1][2 * p2], v[1][2 * p2 + 1], v[2][2 * p2], v[2][2 * p2 + 1]);
-                v[0][p2] = lerp(v[0][2 * p2], v[0][2 * p2 + 1], rq[q]);
-                v[1][p2] = lerp(v[1][2 * p2], v[1][2 * p2 + 1], rq[q]);
-                v[2][p2] = lerp(v[2][2 * p2], v[2][2 * p2 + 1], rq[q]);
-            }
-        }
-        t.V[cur][g] = v[0][0]; t.aV[cur][g] = v[1][0]; t.am[cur][g] = v[2][0];
+    for (int p2 = 0; p2 < H; ++p2) {
+        fe V0, V1, aV0, aV1, am0, am1;
+        ld(2 * p2, V0, aV0, am0);
+        ld(2 * p2 + 1, V1, aV1, am1);
+        fsum_add(acc[0], V0, V1, aV0, aV1, am0, am1);
+        V[p2] = lerp(V0, V1, rq[0]); aV[p2] = lerp(aV0, aV1, rq[0]); am[p2] = lerp(am0, am1, rq[0]);
     }
-    // per-block partials (a, b, c) with b = p(1) - a - c
+#pragma unroll
+    for (int q = 1; q < B; ++q) {
+#pragma unroll
+        for (int p2 = 0; p2 < (H >> q); ++p2) {
+            fsum_add(acc[q], V[2 * p2], V[2 * p2 + 1], aV[2 * p2], aV[2 * p2 + 1], am[2 * p2], am[2 * p2 + 1]);
+            V[p2] = lerp(V[2 * p2], V[2 * p2 + 1], rq[q]);
+            aV[p2] = lerp(aV[2 * p2], aV[2 * p2 + 1], rq[q]);
+            am[p2] = lerp(am[2 * p2], am[2 * p2 + 1], rq[q]);
+        }
+    }
+    oV = V[0]; oaV = aV[0]; oam = am[0];
+}
+// per-block partials (a, b, c) of B rounds, b = p(1) - a - c
+template <int B>
+__device__ __forceinline__ void fold_partials(fsum *acc, const launch::fold_inst &t, int slot_round)
+{
 #pragma unroll
     for (int q = 0; q < B; ++q) {
         fe v3[3] = {acc[q].a, acc[q].c, acc[q].p1};
@@ -361,6 +379,129 @@ __global__ __launch_bounds__(256) void k_fold_reg(const launch::fold_inst *I, in
             for (int c = 0; c < 3; ++c) t.slots[((int64_t)(slot_round + q) * 3 + c) * FB_MAXB + blockIdx.x] = o3[c];
         }
     }
+}
+// rounds j0 .. j0+B-1, lane g folding values [g 2^B, (g+1) 2^B) to out[g]
+template <int B>
+__global__ __launch_bounds__(256) void k_fold_reg(const launch::fold_inst *I, int cur, int j0, int64_t nout, int slot_round)
+{
+    const launch::fold_inst &t = I[blockIdx.y];
+    ftab X = fold_input(t, j0, cur);
+    fe rq[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) rq[q] = t.r[j0 + q];
+    fsum acc[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) acc[q] = {fzero(), fzero(), fzero()};
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nout; g += (int64_t)gridDim.x * blockDim.x) {
+        int64_t o = g << B;
+        fe oV, oaV, oam;
+        fold_lane<B>([&](int k, fe &V, fe &aV, fe &am) { V = X.V[o + k]; aV = X.aV[o + k]; am = X.am[o + k]; }, rq, acc,
+                     oV, oaV, oam);
+        t.V[cur][g] = oV; t.aV[cur][g] = oaV; t.am[cur][g] = oam;
+    }
+    fold_partials<B>(acc, t, slot_round);
+}
+// Table build of one phase fused with its first B fold rounds: the full
+// tables are never stored.  A block takes tiles of 256 2^B consecutive
+// entries (thread: 2^B of them); the tile's records, contiguous in the CSR
+// order, are evaluated in chunks of GF_CH, one record per thread (uniform
+// work whatever the per-entry counts), into LDS; each thread then sums the
+// chunk's records of its own entries and, after the last chunk, folds its
+// entries with the V values (the previous layer) as k_fold_reg does.
+constexpr int GF_CH = 1152;                  // records per chunk (a tile holds ~1024 at one per entry)
+constexpr int GF_NR = (GF_CH + 255) / 256;   // records per thread and chunk
+template <int PH, int B>
+__global__ __launch_bounds__(256) void k_gather_fold(const launch::layer_dev *D, const launch::fold_inst *I)
+{
+    constexpr int E = 1 << B;
+    const launch::layer_dev &d = D[blockIdx.y];
+    const launch::fold_inst &t = I[blockIdx.y];
+    static_assert(256 << B <= 2 * GF_CH, "a tile's V values must fit the chunk buffers");
+    __shared__ fe cam[2 * GF_CH];
+    __shared__ fe tot[3 * B];             // the block's round sums (a, c, p(1)) over its tiles
+    fe *caV = cam + GF_CH;
+    const uint32_t *off = PH == 0 ? d.off1 : d.off2;
+    const launch::gate_rec *ent = PH == 0 ? d.ent1 : d.ent2;
+    bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
+    bu_t bu = {d.bu.f, d.bu.s, d.bu.fh};
+    fe vu = PH ? *d.vu : fzero();
+    if (threadIdx.x < 3 * B) tot[threadIdx.x] = fzero();
+    const int64_t nt = d.nt, tile = (int64_t)blockDim.x * E;
+    for (int64_t T0 = blockIdx.x * tile; T0 < nt; T0 += (int64_t)gridDim.x * tile) {
+        int64_t tl = T0 + (int64_t)threadIdx.x * E;
+        bool live = tl < nt;
+        uint32_t E0 = off[T0], E1 = off[T0 + tile < nt ? T0 + tile : nt];
+        fe am[E], aV[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) { am[k] = fzero(); aV[k] = fzero(); }
+        for (uint32_t S = E0; S < E1; S += GF_CH) {
+            uint32_t n = E1 - S < (uint32_t)GF_CH ? E1 - S : (uint32_t)GF_CH;
+            __syncthreads();
+            // all loads of the thread's records first, then their contributions
+            launch::gate_rec R[GF_NR];
+            fe pvv[PH == 0 ? GF_NR : 1];
+#pragma unroll
+            for (int i = 0; i < GF_NR; ++i) {
+                uint32_t q = threadIdx.x + 256 * i;
+                R[i] = q < n ? ent[S + q] : launch::gate_rec{0, 0, 0, -1};
+            }
+            if (PH == 0) {
+#pragma unroll
+                for (int i = 0; i < GF_NR; ++i) pvv[i] = d.pv[R[i].ty >= 0 && p1_reads_v(R[i].ty) ? R[i].v : 0];
+            }
+#pragma unroll
+            for (int i = 0; i < GF_NR; ++i) {
+                uint32_t q = threadIdx.x + 256 * i;
+                if (q < n) {
+                    fe m, a;
+                    if (PH == 0) p1_rec(R[i], bg, pvv[i], m, a);
+                    else p2_rec(R[i], bg, bu, vu, m, a);
+                    cam[q] = m; caV[q] = a;
+                }
+            }
+            __syncthreads();
+            if (live) {
+                uint32_t ok = off[tl];
+#pragma unroll
+                for (int k = 0; k < E; ++k) {
+                    uint32_t ok1 = off[tl + k + 1];
+                    uint32_t lo = ok > S ? ok : S, hi = ok1 < S + n ? ok1 : S + n;
+#pragma unroll 1
+                    for (uint32_t j = lo; j < hi; ++j) { am[k] = add(am[k], cam[j - S]); aV[k] = add(aV[k], caV[j - S]); }
+                    ok = ok1;
+                }
+            }
+        }
+        // the tile's V values, staged through LDS by coalesced loads
+        __syncthreads();
+        fe *sv = cam;
+        for (int k = threadIdx.x; k < tile && T0 + k < nt; k += blockDim.x) sv[k] = d.pv[T0 + k];
+        __syncthreads();
+        fsum acc[B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) acc[q] = {fzero(), fzero(), fzero()};
+        if (live) {
+            if (PH == 1 && tl == 0 && d.nz > 0) aV[0] = add(aV[0], d.part[launch::MAX_PARTS]);
+            fe oV, oaV, oam;
+            const fe *myv = sv + threadIdx.x * E;
+            fold_lane<B>([&](int k, fe &V, fe &a, fe &m) { V = myv[k]; a = aV[k]; m = am[k]; }, t.r, acc, oV, oaV,
+                         oam);
+            int64_t g = tl >> B;
+            t.V[0][g] = oV; t.aV[0][g] = oaV; t.am[0][g] = oam;
+        }
+        fe v[3 * B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) { v[3 * q] = acc[q].a; v[3 * q + 1] = acc[q].c; v[3 * q + 2] = acc[q].p1; }
+        block_sum<3 * B>(v);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 3 * B; ++k) tot[k] = add(tot[k], v[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 0; q < B; ++q) {
+            fe o3[3] = {tot[3 * q], sub(sub(tot[3 * q + 2], tot[3 * q]), tot[3 * q + 1]), tot[3 * q + 1]};
+            for (int c = 0; c < 3; ++c) t.slots[((int64_t)q * 3 + c) * FB_MAXB + blockIdx.x] = o3[c];
+        }
 }
 // the last b <= 6 rounds of each instance, one wave each: round sums to
 // sums[3j..], the final pair's V.a, V.b, am.a, am.b and v = V(r[L-1]) + vconst to end[]
@@ -1126,6 +1267,15 @@ void eq_expand(fe *out, const fe *lo, const fe *hi, int fh, int64_t n, hipStream
 {
     hipLaunchKernelGGL(k_eq_expand, dim3(nblk(n)), dim3(TPB), 0, st, out, lo, hi, fh, n);
 }
+// phase-2 gates that reach entry 0 unconditionally -> part[MAX_PARTS]
+static void zero_list(const layer_dev *D, int n, int nz_max, hipStream_t s)
+{
+    if (nz_max <= 0) return;
+    unsigned blocks = nblk(nz_max);
+    if (blocks > MAX_PARTS) blocks = MAX_PARTS;
+    hipLaunchKernelGGL(k_p2_zero, dim3(blocks, n), dim3(TPB), 0, s, D);
+    hipLaunchKernelGGL(k_zero_sum, dim3(1, n), dim3(1024), 0, s, D, (int)blocks);
+}
 void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipStream_t s)
 {
     dim3 grid(nblk(nt), n);
@@ -1133,29 +1283,43 @@ void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipS
         hipLaunchKernelGGL(k_p1_gather, grid, dim3(TPB), 0, s, D);
         return;
     }
+    zero_list(D, n, nz_max, s);
     hipLaunchKernelGGL(k_p2_gather, grid, dim3(TPB), 0, s, D);
-    if (nz_max > 0) {
-        unsigned blocks = nblk(nz_max);
-        if (blocks > MAX_PARTS) blocks = MAX_PARTS;
-        hipLaunchKernelGGL(k_p2_zero, dim3(blocks, n), dim3(TPB), 0, s, D);
-        hipLaunchKernelGGL(k_add_parts, dim3(1, n), dim3(1024), 0, s, D, (int)blocks);
-    }
 }
-int fold_plan(int L, int *js, int *ks)
+int gather_fold(int phase, const layer_dev *D, const fold_inst *I, int n, int L, int nz_max, hipStream_t s)
 {
-    // streaming launches of <= FB_B rounds until at most FB_TAIL rounds remain
+    int js[32], ks[32];
+    if (fold_plan(L, js, ks, GF_B) == 0) return 0;
+    if (phase == 1) zero_list(D, n, nz_max, s);
+    int b = ks[0];
+    dim3 g((unsigned)fold_blocks(L, 0, GF_B), n);
+    static_assert(GF_B == 2, "gather_fold instantiates B = 1, 2");
+    if (phase == 0) {
+        if (b == 2) hipLaunchKernelGGL((k_gather_fold<0, 2>), g, dim3(256), 0, s, D, I);
+        else hipLaunchKernelGGL((k_gather_fold<0, 1>), g, dim3(256), 0, s, D, I);
+    } else {
+        if (b == 2) hipLaunchKernelGGL((k_gather_fold<1, 2>), g, dim3(256), 0, s, D, I);
+        else hipLaunchKernelGGL((k_gather_fold<1, 1>), g, dim3(256), 0, s, D, I);
+    }
+    return b;
+}
+int fold_plan(int L, int *js, int *ks, int first)
+{
+    // streaming launches of <= FB_B rounds (the first <= first) until at
+    // most FB_TAIL rounds remain
     int n = 0, j = 0;
     while (L - j > FB_TAIL) {
-        int b = L - j - FB_TAIL < FB_B ? L - j - FB_TAIL : FB_B;
+        int cap = n == 0 ? first : FB_B;
+        int b = L - j - FB_TAIL < cap ? L - j - FB_TAIL : cap;
         js[n] = j; ks[n] = b; ++n;
         j += b;
     }
     return n;
 }
-int fold_blocks(int L, int j)
+int fold_blocks(int L, int j, int first)
 {
     int js[32], ks[32];
-    int n = fold_plan(L, js, ks);
+    int n = fold_plan(L, js, ks, first);
     for (int k = 0; k < n; ++k)
         if (js[k] == j) {
             int64_t lanes = (int64_t)1 << (L - j - ks[k]);
@@ -1164,18 +1328,21 @@ int fold_blocks(int L, int j)
         }
     return 0;
 }
-int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s)
+int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s, int fused)
 {
     int js[32], ks[32];
-    int nl = fold_plan(L, js, ks);
+    int first = fused ? GF_B : FB_B;
+    int nl = fold_plan(L, js, ks, first);
     int cur = 0, rounds = 0;
     for (int k = 0; k < nl; ++k) {
         int j = js[k], b = ks[k];
-        int64_t nout = (int64_t)1 << (L - j - b);
-        dim3 g((unsigned)fold_blocks(L, j), ninst);
-        if (b == 3) hipLaunchKernelGGL(k_fold_reg<3>, g, dim3(256), 0, s, I, cur, j, nout, rounds);
-        else if (b == 2) hipLaunchKernelGGL(k_fold_reg<2>, g, dim3(256), 0, s, I, cur, j, nout, rounds);
-        else hipLaunchKernelGGL(k_fold_reg<1>, g, dim3(256), 0, s, I, cur, j, nout, rounds);
+        if (k > 0 || !fused) {
+            int64_t nout = (int64_t)1 << (L - j - b);
+            dim3 g((unsigned)fold_blocks(L, j, first), ninst);
+            if (b == 3) hipLaunchKernelGGL(k_fold_reg<3>, g, dim3(256), 0, s, I, cur, j, nout, rounds);
+            else if (b == 2) hipLaunchKernelGGL(k_fold_reg<2>, g, dim3(256), 0, s, I, cur, j, nout, rounds);
+            else hipLaunchKernelGGL(k_fold_reg<1>, g, dim3(256), 0, s, I, cur, j, nout, rounds);
+        }
         cur ^= 1;
         rounds += b;
     }

@@ -67,27 +67,43 @@ struct layer_dev {
     fe *part;                             // MAX_PARTS block partials
     fe *gv;                               // general value (null for layer 1)
 };
-constexpr int MAX_PARTS = 1024;
+constexpr int MAX_PARTS = 1024;        // block partials; part[MAX_PARTS] holds the zero-list sum
 // phase 0: tables by u (zk_prover.cpp:375-592); phase 1: by v (:789-983),
-// including the gates that add into addV[0] (nz_max: largest z2 list)
+// including the gates that add into addV[0] (nz_max: largest z2 list), into
+// the flat tables amb, aVb (used when the phase has no streaming launch)
 void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipStream_t s);
+// the same fused with the phase's first streaming fold launch, of at most
+// GF_B rounds (the flat tables are never written): returns its rounds, 0 when
+// the phase has no streaming launch (then layer_tables and the plain fold
+// apply)
+struct fold_inst;
+int gather_fold(int phase, const layer_dev *D, const fold_inst *I, int n, int L, int nz_max, hipStream_t s);
 // Sumcheck phases (zk_prover.cpp:600-633, 992-1025) of a batch of ninst
 // instances with flat tables of 2^L entries each, blockIdx.y = instance:
-// streaming launches of up to 3 rounds (8 consecutive values per lane, no
-// communication), then the last <= 6 rounds in one wave per instance.
-// Round j's pair polynomials are evaluated at r[j].  Each streaming launch
-// writes its block partials of round q to slots[(q*3 + c) * FW_BLOCKS +
-// block] (q counted over the instance's launches); the caller reduces every
-// slot of the proof once with reduce_slots.  Tail round sums go straight to
-// sums[3j..].  After the last round: end[0..3] = the final pair's V.a, V.b,
-// am.a, am.b and end[4] = V.a * r[L-1] + V.b + *vconst (= v_u or v_v).
-// Returns the slot rounds per instance.
+// streaming launches of up to FB_B rounds (2^FB_B consecutive values per
+// lane, no communication), then the last <= FB_TAIL rounds in one wave per
+// instance.  Round j's pair polynomials are evaluated at r[j].  Each
+// streaming launch writes its block partials of round q to slots[(q*3 + c) *
+// FW_BLOCKS + block] (q counted over the instance's launches); the caller
+// reduces every slot of the proof once with reduce_slots.  Tail round sums go
+// straight to sums[3j..].  After the last round: end[0..3] = the final pair's
+// V.a, V.b, am.a, am.b and end[4] = V.a * r[L-1] + V.b + *vconst (= v_u or
+// v_v).  Returns the slot rounds per instance.
+constexpr int FB_B = 3;                // rounds per streaming launch
+constexpr int FB_TAIL = 6;             // rounds of the tail (<= 64 values)
+constexpr int GF_B = 2;                // rounds of a fused table build + fold (gather_fold)
 constexpr int FW_BLOCKS = 1024;
-// values of each table a phase on 2^L entries hands from one launch to the
-// next (the output of its first streaming launch)
-inline int64_t fold_ws_len(int L) { return L > 9 ? ((int64_t)1 << (L - 3)) : 64; }
-int fold_plan(int L, int *js, int *ks);
-int fold_blocks(int L, int j);
+// launch plan of a phase on 2^L entries whose first launch folds at most
+// `first` rounds: launch k starts at round js[k] and folds ks[k] rounds
+int fold_plan(int L, int *js, int *ks, int first = FB_B);
+int fold_blocks(int L, int j, int first = FB_B);
+// values of each table a phase hands from one launch to the next (the
+// output of its first streaming launch)
+inline int64_t fold_ws_len(int L, int first = FB_B)
+{
+    int b = L - FB_TAIL < first ? L - FB_TAIL : first;
+    return L > FB_TAIL ? ((int64_t)1 << (L - b)) : 64;
+}
 struct fold_inst {
     const fe *Vb, *aVb, *amb;       // flat tables
     fe *V[2], *aV[2], *am[2];       // chunk results, fold_ws_len(L) values each
@@ -96,7 +112,9 @@ struct fold_inst {
     fe *sums, *end;                 // tail round sums, end values
     const fe *vconst;               // added to end[4]; null = 0
 };
-int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s);
+// fused: the phase's tables were built by gather_fold, which also ran its
+// first launch (plan with first = GF_B)
+int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s, int fused = 0);
 // msg[dst[R]..+3] = sum over the nb[R] block partials of slot round R
 void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s);
 struct half_desc {            // one eq half table: out[j] = f0 * prod_{i<nbits} eq(bit_i(j), r[i])

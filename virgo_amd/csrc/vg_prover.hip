@@ -541,11 +541,15 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     c->amb_l.assign(D, nullptr); c->aVb_l.assign(D, nullptr); c->part_l.assign(D, nullptr);
     c->ws_l.assign(6 * D, nullptr);
     for (int i = 1; i < D; ++i) {
-        int64_t T = (int64_t)1 << C.bl[i - 1];
-        c->amb_l[i] = carve<fe>(base, off, T);
-        c->aVb_l[i] = carve<fe>(base, off, T);
-        c->part_l[i] = carve<fe>(base, off, launch::MAX_PARTS);
-        for (int k = 0; k < 6; ++k) c->ws_l[6 * i + k] = carve<fe>(base, off, launch::fold_ws_len(C.bl[i - 1]));
+        // flat tables only where no streaming fold launch builds them on the fly
+        int js[32], ks[32];
+        if (launch::fold_plan(C.bl[i - 1], js, ks, launch::GF_B) == 0) {
+            int64_t T = (int64_t)1 << C.bl[i - 1];
+            c->amb_l[i] = carve<fe>(base, off, T);
+            c->aVb_l[i] = carve<fe>(base, off, T);
+        }
+        c->part_l[i] = carve<fe>(base, off, launch::MAX_PARTS + 1);
+        for (int k = 0; k < 6; ++k) c->ws_l[6 * i + k] = carve<fe>(base, off, launch::fold_ws_len(C.bl[i - 1], launch::GF_B));
     }
     c->d_layers = carve<launch::layer_dev>(base, off, D);
     c->d_inst = carve<launch::fold_inst>(base, off, 2 * D);
@@ -778,10 +782,13 @@ int enqueue(vg_ctx *c)
         int n = (int)b.layers.size();
         const launch::layer_dev *Dl = c->d_layers + b.d0;
         const launch::fold_inst *I = c->d_inst + 2 * b.d0;
-        launch::layer_tables(0, Dl, n, 1 << b.bl, 0, gs);
-        slot += n * launch::sumcheck_multi(I, n, b.bl, gs);
-        launch::layer_tables(1, Dl, n, 1 << b.bl, b.nz_max, gs);
-        slot += n * launch::sumcheck_multi(I + n, n, b.bl, gs);
+        for (int ph = 0; ph < 2; ++ph) {
+            const launch::fold_inst *Ip = I + ph * n;
+            int nz = ph ? b.nz_max : 0;
+            int f = launch::gather_fold(ph, Dl, Ip, n, b.bl, nz, gs);
+            if (!f) launch::layer_tables(ph, Dl, n, 1 << b.bl, nz, gs);
+            slot += n * launch::sumcheck_multi(Ip, n, b.bl, gs, f > 0);
+        }
         launch::general_multi(Dl, n, b.ng_max, gs);
     }
     if (!c->serial)
@@ -907,8 +914,9 @@ void assemble(vg_ctx *c, tbuf &T)
 // CSR transposes of one gate layer (the structure of zk_prover.cpp:420-587
 // and :845-972, computed once per circuit): phase 1 lists, for every entry u
 // of the previous layer, the gates that add into it through u (a sum gate
-// into every u of its range); phase 2 lists by v the gates that add into
-// entry v, and separately the gates that always add into entry 0.
+// into every u of its range, its record carrying that u in place of v);
+// phase 2 lists by v the gates that add into entry v, and separately the
+// gates that always add into entry 0.
 void build_csr(vg_ctx *c, int i)
 {
     const host_circuit &C = c->C;
@@ -943,7 +951,12 @@ void build_csr(vg_ctx *c, int i)
         int64_t lo, hi;
         p1_range(g, lo, hi);
         launch::gate_rec rec = {(int32_t)g, u[g], v[g], ty[g]};
-        for (int64_t t = lo; t < hi; ++t) e1[c1[t]++] = rec;
+        bool range = ty[g] == 5 || ty[g] == 12;        // the record names its entry in v
+        for (int64_t t = lo; t < hi; ++t) {
+            launch::gate_rec r1 = rec;
+            if (range) r1.v = (int32_t)t;
+            e1[c1[t]++] = r1;
+        }
         if (p2_by_v(g)) e2[c2[v[g]]++] = rec;
         if (p2_zero(g)) z.push_back((int32_t)g);
     }
@@ -1013,7 +1026,8 @@ int ctx_init(vg_ctx *c)
     c->slot0.assign(2 * C.depth, 0);
     for (auto &b : c->batches) {
         int js[32], ks[32];
-        int nl = launch::fold_plan(b.bl, js, ks);
+        // the main prover's phases build their tables with gather_fold
+        int nl = launch::fold_plan(b.bl, js, ks, launch::GF_B);
         for (int ph = 0; ph < 2; ++ph)
             for (size_t q = 0; q < b.layers.size(); ++q) {
                 int i = b.layers[q];
@@ -1021,7 +1035,7 @@ int ctx_init(vg_ctx *c)
                 int64_t sums = c->msg_layer[i] + (ph == 0 ? 0 : 3 * b.bl + 5);
                 for (int k = 0; k < nl; ++k)
                     for (int t = 0; t < ks[k]; ++t) {
-                        c->h_slot_nb.push_back(launch::fold_blocks(b.bl, js[k]));
+                        c->h_slot_nb.push_back(launch::fold_blocks(b.bl, js[k], launch::GF_B));
                         c->h_slot_dst.push_back(sums + 3 * (js[k] + t));
                     }
             }

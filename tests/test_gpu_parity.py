@@ -67,6 +67,8 @@ CASES = {
     "random_w10_d4": lambda: W.random_circuit(10, 4, seed=3),
     "random_w12_d3": lambda: W.random_circuit(12, 3, seed=4),
     "random_w17_d5": lambda: W.random_circuit(17, 5, seed=5),
+    # phase 1 spread evenly, phase 2's records crowded on 64 entries
+    "random_w13_d3_skew": lambda: W.random_circuit(13, 3, seed=6, v_bits=6),
 }
 
 

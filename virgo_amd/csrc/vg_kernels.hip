@@ -1328,15 +1328,14 @@ int fold_blocks(int L, int j, int first)
         }
     return 0;
 }
-int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s, int fused)
+int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s, int first, int skip_first)
 {
     int js[32], ks[32];
-    int first = fused ? GF_B : FB_B;
     int nl = fold_plan(L, js, ks, first);
     int cur = 0, rounds = 0;
     for (int k = 0; k < nl; ++k) {
         int j = js[k], b = ks[k];
-        if (k > 0 || !fused) {
+        if (k > 0 || !skip_first) {
             int64_t nout = (int64_t)1 << (L - j - b);
             dim3 g((unsigned)fold_blocks(L, j, first), ninst);
             if (b == 3) hipLaunchKernelGGL(k_fold_reg<3>, g, dim3(256), 0, s, I, cur, j, nout, rounds);

@@ -99,6 +99,12 @@ int fold_plan(int L, int *js, int *ks, int first = FB_B);
 int fold_blocks(int L, int j, int first = FB_B);
 // values of each table a phase hands from one launch to the next (the
 // output of its first streaming launch)
+// rounds and table entries per block of gather_fold on 2^L entries (0: none)
+inline int gf_rounds(int L) { return L > FB_TAIL ? (L - FB_TAIL < GF_B ? L - FB_TAIL : GF_B) : 0; }
+inline int64_t gf_tile(int L) { return (int64_t)256 << gf_rounds(L); }
+// gather_fold pays off when no block's entries hold more than this many
+// records per entry on average (a block walks its records sequentially)
+constexpr int GF_MAX_FANIN = 4;
 inline int64_t fold_ws_len(int L, int first = FB_B)
 {
     int b = L - FB_TAIL < first ? L - FB_TAIL : first;
@@ -112,9 +118,9 @@ struct fold_inst {
     fe *sums, *end;                 // tail round sums, end values
     const fe *vconst;               // added to end[4]; null = 0
 };
-// fused: the phase's tables were built by gather_fold, which also ran its
-// first launch (plan with first = GF_B)
-int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s, int fused = 0);
+// plan with first-launch cap `first`; skip_first: the first launch already
+// ran (gather_fold)
+int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s, int first = FB_B, int skip_first = 0);
 // msg[dst[R]..+3] = sum over the nb[R] block partials of slot round R
 void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s);
 struct half_desc {            // one eq half table: out[j] = f0 * prod_{i<nbits} eq(bit_i(j), r[i])

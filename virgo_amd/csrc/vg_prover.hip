@@ -440,6 +440,7 @@ struct vg_ctx {
         std::vector<int> layers;
         int nz_max = 0, ng_max = 0;
         int d0 = 0;                       // first entry in d_layers / d_inst
+        bool fused[2] = {true, true};     // phase tables built by gather_fold
     };
     std::vector<batch> batches;
     launch::layer_dev *d_layers = nullptr;    // in batch order
@@ -541,9 +542,11 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     c->amb_l.assign(D, nullptr); c->aVb_l.assign(D, nullptr); c->part_l.assign(D, nullptr);
     c->ws_l.assign(6 * D, nullptr);
     for (int i = 1; i < D; ++i) {
-        // flat tables only where no streaming fold launch builds them on the fly
-        int js[32], ks[32];
-        if (launch::fold_plan(C.bl[i - 1], js, ks, launch::GF_B) == 0) {
+        // flat tables only where gather_fold does not build them on the fly
+        bool flat = launch::gf_rounds(C.bl[i - 1]) == 0;
+        for (auto &b : c->batches)
+            if (b.bl == C.bl[i - 1] && (!b.fused[0] || !b.fused[1])) flat = true;
+        if (flat) {
             int64_t T = (int64_t)1 << C.bl[i - 1];
             c->amb_l[i] = carve<fe>(base, off, T);
             c->aVb_l[i] = carve<fe>(base, off, T);
@@ -785,9 +788,9 @@ int enqueue(vg_ctx *c)
         for (int ph = 0; ph < 2; ++ph) {
             const launch::fold_inst *Ip = I + ph * n;
             int nz = ph ? b.nz_max : 0;
-            int f = launch::gather_fold(ph, Dl, Ip, n, b.bl, nz, gs);
+            int f = b.fused[ph] ? launch::gather_fold(ph, Dl, Ip, n, b.bl, nz, gs) : 0;
             if (!f) launch::layer_tables(ph, Dl, n, 1 << b.bl, nz, gs);
-            slot += n * launch::sumcheck_multi(Ip, n, b.bl, gs, f > 0);
+            slot += n * launch::sumcheck_multi(Ip, n, b.bl, gs, launch::GF_B, f > 0);
         }
         launch::general_multi(Dl, n, b.ng_max, gs);
     }
@@ -1045,6 +1048,18 @@ int ctx_init(vg_ctx *c)
     c->h_off1.assign(C.depth, {}); c->h_off2.assign(C.depth, {});
     c->h_ent1.assign(C.depth, {}); c->h_ent2.assign(C.depth, {}); c->h_z2.assign(C.depth, {});
     for (int i = 1; i < C.depth; ++i) build_csr(c, i);
+    // gather_fold walks a block's records in sequence: phases whose records
+    // crowd onto few entries (high fan-out wiring) build flat tables with
+    // one thread per entry instead
+    for (auto &b : c->batches) {
+        int64_t nt = (int64_t)1 << b.bl, tile = launch::gf_tile(b.bl);
+        for (int ph = 0; ph < 2; ++ph)
+            for (int i : b.layers) {
+                const std::vector<uint32_t> &o = ph == 0 ? c->h_off1[i] : c->h_off2[i];
+                for (int64_t t0 = 0; t0 < nt && b.fused[ph]; t0 += tile)
+                    if (o[t0 + tile < nt ? t0 + tile : nt] - o[t0] > launch::GF_MAX_FANIN * tile) b.fused[ph] = false;
+            }
+    }
     int64_t bytes = layout(c, nullptr);
     HIPCHK(hipMalloc(&c->arena, bytes));
     c->arena_bytes = bytes;

@@ -51,10 +51,11 @@ def matmul_circuit(mat, seed=1):
     return _pack([inp, mult, summ])
 
 
-def random_circuit(log_width, depth, seed=1, log_inputs=None):
+def random_circuit(log_width, depth, seed=1, log_inputs=None, v_bits=None):
     """Random layered circuit: `depth` compute layers of 2^log_width gates,
     each gate add or mult with probability 1/2, u and v uniform over the
-    previous layer; 2^log_inputs inputs uniform in [0, 2^31)."""
+    previous layer; 2^log_inputs inputs uniform in [0, 2^31).  v_bits: v
+    uniform over the first 2^v_bits entries only (skewed fan-out)."""
     if log_inputs is None:
         log_inputs = log_width
     W, NI = 1 << log_width, 1 << log_inputs
@@ -65,7 +66,8 @@ def random_circuit(log_width, depth, seed=1, log_inputs=None):
         r = splitmix64(seed, 3 * W, 2 + layer)
         ty = (r[:W] & np.uint64(1)).astype(np.int32)
         u = (r[W:2 * W] % np.uint64(prev)).astype(np.int64)
-        v = (r[2 * W:] % np.uint64(prev)).astype(np.int64)
+        vr = prev if v_bits is None else min(prev, 1 << v_bits)
+        v = (r[2 * W:] % np.uint64(vr)).astype(np.int64)
         layers.append((ty, u, v))
         prev = W
     return _pack(layers)

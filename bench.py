@@ -167,11 +167,25 @@ def main():
                      "kernel": "k_leaf_multi (SHA3-256 leaf chains, %d chains x 65 Keccak-f)" % chains,
                      "kernel_ms": round(leaf_ms, 3)},
     }
+    # the device verifier on the same transcript (the sumcheck round checks
+    # on the host, predicates, public array, query evaluations and paths on
+    # the device)
+    p.verify(tr)
+    vt = []
+    for _ in range(3):
+        ok, why, vms = p.verify(tr)
+        vt.append(vms)
+    line["verify"] = {"accepted": bool(ok), "device_verifier_ms": round(min(vt), 2)}
     p.close()
     if not args.no_batch:
         line["batched"] = batched(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args)
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        ok_o, _ = O.verify(circ, tr)
+        line["verify"]["oracle_verifier_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        line["verify"]["oracle_accepted"] = bool(ok_o)
     print(json.dumps(line))
 
 

@@ -95,6 +95,18 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
 /* Output layer of the last proof (2^bits elements). */
 int vg_output(vg_ctx *ctx, vg_fe *out, size_t cap, int *bits);
 
+/* Verifies a transcript of this context's circuit, as vg_prove writes it,
+ * with the checks of zk_verifier::verify (src/linear_gkr/zk_verifier.cpp:
+ * 910-1253), poly_commit_verifier::verify_poly_commitment
+ * (src/VPD/vpd_verifier.cpp:80-335) and the fft_gkr program.  The round
+ * checks run on the host; the wiring predicates (zk_verifier.cpp:219-606),
+ * public_array_prepare (:884-908), the first-round evaluations at the query
+ * points (vpd_verifier.cpp:215-226) and every authentication path
+ * (vpd_verifier.cpp:8-41) run on the device.  Returns 0 when the check ran:
+ * *accepted = 1, or 0 with the failed check in vg_last_error().  ms: wall
+ * time.  No proof may be in flight on ctx. */
+int vg_verify(vg_ctx *ctx, const uint8_t *transcript, size_t len, int *accepted, double *ms);
+
 /* ---- single-operation entry points on host arrays ---- */
 
 /* fast_fourier_transform (RS_polynomial.cpp:27-167): result[k] =

@@ -528,7 +528,16 @@ static void tr_get(trd *r, void *x, size_t k)
     memcpy(x, r->p + r->off, k);
     r->off += k;
 }
-static fe tr_fe(trd *r) { fe x; tr_get(r, &x, 16); return x; }
+/* a field element of the transcript; a non-canonical one (a component
+ * >= p) marks the transcript malformed: every operation assumes canonical
+ * inputs, as the reference's in-process messages always are */
+static fe tr_fe(trd *r)
+{
+    fe x;
+    tr_get(r, &x, 16);
+    if (x.real >= P61 || x.img >= P61) r->bad = 1;
+    return x;
+}
 static dig tr_dig(trd *r) { dig d; tr_get(r, &d, 32); return d; }
 
 /* polynomial helpers (polynomial.cpp) */
@@ -1822,7 +1831,7 @@ int vgo_verify(const vgo_circuit_desc *dsc, const uint8_t *tr, size_t len)
     }
     free(fin);
     fin = NULL;
-    if (R.bad || R.off > R.n) REJ("transcript truncated");
+    if (R.bad || R.off > R.n) REJ("transcript truncated or not canonical");
     if (!vgo_fft_gkr_verify(n - 6, R.p + R.off, R.n - R.off)) REJ("fft_gkr section rejected");
     R.off = R.n;
 vdone:

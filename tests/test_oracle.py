@@ -245,6 +245,23 @@ def test_verify_rejects_mutation(small_proof, where):
     assert not ok
 
 
+def test_verify_rejects_non_canonical(small_proof):
+    """A field element with a component >= p is malformed, also where it is
+    congruent to the honest value (V_res + p in its real part)."""
+    c, tr = small_proof
+    depth = int.from_bytes(tr[4:8], "little")
+    hdr = 12 + 4 * depth                             # magic, depth, input bits, bit lengths
+    real = int.from_bytes(tr[hdr + 8:hdr + 16], "little")
+    b = bytearray(tr)
+    b[hdr + 8:hdr + 16] = (real + P).to_bytes(8, "little")
+    ok, msg = O.verify(c, bytes(b))
+    assert not ok and "canonical" in msg
+    # the last element (in the fft_gkr section) with its top bit set
+    b = bytearray(tr)
+    b[-1] ^= 0x80
+    assert not O.verify(c, bytes(b))[0]
+
+
 def test_verify_rejects_wrong_circuit(small_proof):
     c, tr = small_proof
     c2 = {k: v.copy() for k, v in c.items()}

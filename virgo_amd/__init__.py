@@ -23,7 +23,8 @@ FE = np.dtype([("img", "<u8"), ("real", "<u8")])     # prime_field.h:93 layout
 
 EXPORTS = ["vg_init", "vg_last_error", "vg_free", "vg_ctx_create", "vg_ctx_destroy", "vg_ctx_device_bytes",
            "vg_prove", "vg_prove_async", "vg_finish", "vg_output", "vg_fft", "vg_ifft", "vg_sha3_64",
-           "vg_merkle_tree", "vg_fe_mul", "vg_fft_gkr", "vg_stream_create", "vg_stream_destroy", "vg_prove_batch"]
+           "vg_merkle_tree", "vg_fe_mul", "vg_fft_gkr", "vg_stream_create", "vg_stream_destroy", "vg_prove_batch",
+           "vg_verify"]
 
 
 class VirgoError(RuntimeError):
@@ -57,6 +58,8 @@ def lib():
         L.vg_prove_async.argtypes = [vp]
         L.vg_finish.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t), vp]
         L.vg_output.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]
+        L.vg_verify.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_double)]
         L.vg_fft.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.vg_ifft.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.vg_sha3_64.argtypes = [vp, vp, ctypes.c_size_t]
@@ -155,6 +158,15 @@ class Prover:
         data = ctypes.string_at(out, ln.value)
         lib().vg_free(out)
         return data
+
+    def verify(self, transcript):
+        """Checks a transcript of this context's circuit with the reference
+        verifier's checks (hot loops on the device).  Returns (accepted,
+        reason, ms): reason names the failed check when not accepted."""
+        ok = ctypes.c_int()
+        ms = ctypes.c_double()
+        _check(lib().vg_verify(self._ctx, bytes(transcript), len(transcript), ctypes.byref(ok), ctypes.byref(ms)))
+        return bool(ok.value), ("" if ok.value else lib().vg_last_error().decode()), ms.value
 
     def output(self):
         cap = 1 << 26

@@ -226,10 +226,10 @@ launch::beta_g bg_of(const fg_ctx *c, const fg_layer &l)
 }
 
 // the program's inline checks (fft_circuit_GKR.cpp:327-352, :456-484,
-// :677-731 and the round sums) on the prover's messages
-bool check(const fg_ctx *c)
+// :677-731 and the round sums) on the message block M
+bool check(const fg_ctx *c, const fe *M)
 {
-    const fe *h = c->h_chal, *M = c->h_msg;
+    const fe *h = c->h_chal;
     int lg = c->lg;
     fe one = fone();
     auto q_eval = [](const fe *q, const fe &x) { return add(mul(add(mul(q[0], x), q[1]), x), q[2]); };
@@ -473,7 +473,7 @@ int fg_finish(fg_ctx *c, std::vector<uint8_t> &out, int *accepted, int *proof_si
         }
     }
     out.insert(out.end(), t.b.begin(), t.b.end());
-    if (accepted) *accepted = check(c) ? 1 : 0;
+    if (accepted) *accepted = check(c, c->h_msg) ? 1 : 0;
     if (proof_size) {                                    // sizeof(quadratic_poly) per round (:342 and like)
         int q = (lg + 6) + lg + 2 * lg * lg;
         for (int i = 1; i <= lg; ++i) q += i;            // extension_gkr :745-754
@@ -484,6 +484,42 @@ int fg_finish(fg_ctx *c, std::vector<uint8_t> &out, int *accepted, int *proof_si
         FG_CHK(hipEventElapsedTime(&ms, c->e0, c->e1));
         *p_ms = ms;
     }
+    return 0;
+}
+
+int fg_verify(fg_ctx *c, const uint8_t *sec, size_t len, int *accepted)
+{
+    *accepted = 0;
+    if (c->pending) FG_FAIL(-3, "fft_gkr: a proof is in flight");
+    draw(c);
+    std::vector<fe> M(c->msg_len, fzero());
+    size_t off = 0;
+    bool bad = false;
+    auto get = [&](void *x, size_t k) {
+        if (off + k > len) { bad = true; memset(x, 0, k); off = len; return; }
+        memcpy(x, sec + off, k);
+        off += k;
+    };
+    auto f = [&]() {                              // canonical elements only
+        fe x;
+        get(&x, 16);
+        if (x.real >= P || x.img >= P) bad = true;
+        return x;
+    };
+    char magic[4];
+    uint32_t lg = 0;
+    get(magic, 4);
+    get(&lg, 4);
+    if (bad || memcmp(magic, "VGF1", 4) || lg != (uint32_t)c->lg) return 0;
+    M[0] = f();
+    for (auto &l : c->L)
+        for (int ph = 0; ph < (l.msg2 >= 0 ? 2 : 1); ++ph) {       // fg_finish's order
+            int64_t m = ph == 0 ? l.msg1 : l.msg2;
+            for (int i = 0; i < 3 * l.bits; ++i) M[m + i] = f();
+            M[m + 3 * l.bits + 4] = f();
+        }
+    if (bad || off != len) return 0;
+    *accepted = check(c, M.data()) ? 1 : 0;
     return 0;
 }
 

@@ -627,6 +627,88 @@ __global__ void k_reduce(const launch::layer_dev *D, int n)
 }
 
 // ---------------------------------------------------------------------
+// verifier: wiring predicates per gate type (zk_verifier.cpp:219-606).
+// For layer d (blockIdx.y) and each type t, sum over its gates g of type t
+// of beta_g(g) beta_u(u) beta_v(v); a sum gate (5) contributes beta_v(0)
+// times beta_u over its range [u, v), an exp-sum gate (12) beta_v(0) 2^k
+// beta_u(u + k) over [u, v]; type 10 sums beta_g beta_u(u) and is scaled by
+// beta_v(0) at the end.  Layer 1 (gv null) has no predicates.  Each thread
+// keeps its per-type sums in its own LDS column; block partials go to
+// part[(layer * blocks + block) * 14 + type].
+// ---------------------------------------------------------------------
+constexpr int PRED_T = 14;
+__global__ __launch_bounds__(TPB) void k_predicates(const launch::layer_dev *D, fe *part)
+{
+    const launch::layer_dev &d = D[blockIdx.y];
+    __shared__ fe acc[PRED_T][TPB];
+    for (int t = 0; t < PRED_T; ++t) acc[t][threadIdx.x] = fzero();
+    if (d.gv) {
+        bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
+        bu_t bu = {d.bu.f, d.bu.s, d.bu.fh}, bv = {d.bv.f, d.bv.s, d.bv.fh};
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += gridDim.x * blockDim.x) {
+            int ty = d.ty[i], u = d.u[i], v = d.v[i];
+            fe c;
+            switch (ty) {
+            case 0: case 1: case 6: case 7: case 8: case 9: case 13:
+                c = mul(mul(beta_g(bg, i), beta_u(bu, u)), beta_u(bv, v));
+                break;
+            case 5: case 12: {
+                fe tg = beta_g(bg, i), b0 = beta_u(bv, 0);
+                c = fzero();
+                for (int j = u; ty == 5 ? j < v : j <= v; ++j) {
+                    c = add(c, mul(mul(tg, b0), beta_u(bu, j)));
+                    if (ty == 12) b0 = dbl(b0);
+                }
+                break;
+            }
+            case 10: c = mul(beta_g(bg, i), beta_u(bu, u)); break;
+            default: continue;
+            }
+            acc[ty][threadIdx.x] = add(acc[ty][threadIdx.x], c);
+        }
+    }
+    __syncthreads();
+    for (int t = 0; t < PRED_T; ++t) {
+        fe x = acc[t][threadIdx.x];
+        block_sum<1>(&x);
+        if (threadIdx.x == 0) {
+            if (t == 10 && d.gv) x = mul(x, beta_u({d.bv.f, d.bv.s, d.bv.fh}, 0));
+            part[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * PRED_T + t] = x;
+        }
+    }
+}
+__global__ void k_predicates_sum(const fe *part, int blocks, fe *out)
+{
+    int l = blockIdx.x, t = threadIdx.x;
+    if (t >= PRED_T) return;
+    fe s = fzero();
+    for (int b = 0; b < blocks; ++b) s = add(s, part[((int64_t)l * blocks + b) * PRED_T + t]);
+    out[l * PRED_T + t] = s;
+}
+// out[p] = sum_k coef[k] x_p^k (one block per point)
+__global__ __launch_bounds__(TPB) void k_poly_eval(const fe *coef, int64_t n, const fe *pts, fe *out)
+{
+    fe x = pts[blockIdx.x];
+    fe xt = fone(), base = x;
+    for (int e = threadIdx.x; e; e >>= 1) {               // x^threadIdx.x
+        if (e & 1) xt = mul(xt, base);
+        base = mul(base, base);
+    }
+    fe step = fone();
+    base = x;
+    for (int e = blockDim.x; e; e >>= 1) {
+        if (e & 1) step = mul(step, base);
+        base = mul(base, base);
+    }
+    fe acc = fzero();
+    for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
+        acc = add(acc, mul(coef[k], xt));
+        xt = mul(xt, step);
+    }
+    block_sum<1>(&acc);
+    if (threadIdx.x == 0) out[blockIdx.x] = acc;
+}
+// ---------------------------------------------------------------------
 // NTT.  DIT radix-2 with bit-reversed loads and natural-order output:
 // stages [0, log_m) in LDS on blocks of 2^log_m, later stages in registers
 // (radix 2^g passes).  Twiddles w_N^e = tw[e * SS/N] from one table of
@@ -976,6 +1058,37 @@ __global__ __launch_bounds__(TPB) void k_leaf_multi(leaf_segs L)
     for (int j = 0; j <= 64; ++j) chain_step(h, cw[j * S + k0], cw[j * S + k1]);
     store_digest(L.tree[sg], half + i, h);
 }
+// one query path (vpd_verifier.cpp:8-41): the leaf chain over the 65 value
+// pairs must give the last path entry, and hashing up the path (sibling
+// order by the position's bits) must give the root
+__global__ void k_verify_paths(const launch::path_job *J, int njobs, const uint64_t *dg, const fe *vals, int *ok)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= njobs) return;
+    launch::path_job j = J[k];
+    const uint64_t *path = dg + 4 * j.path;
+    uint64_t vh[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 65; ++i) chain_step(vh, vals[j.vals + 2 * i], vals[j.vals + 2 * i + 1]);
+    bool good = true;
+    const uint64_t *last = path + 4 * (j.len - 1);
+    for (int q = 0; q < 4; ++q) good &= vh[q] == last[q];
+    uint64_t cur[4] = {last[0], last[1], last[2], last[3]};
+    int64_t pw = j.pow;
+    for (int i = 0; i < j.len - 1; ++i) {
+        const uint64_t *sib = path + 4 * i;
+        uint64_t m[8];
+        for (int q = 0; q < 4; ++q) {
+            m[q] = (pw & 1) ? sib[q] : cur[q];
+            m[4 + q] = (pw & 1) ? cur[q] : sib[q];
+        }
+        pw >>= 1;
+        sha3_64(m, cur);
+    }
+    const uint64_t *root = dg + 4 * j.root;
+    for (int q = 0; q < 4; ++q) good &= cur[q] == root[q];
+    ok[k] = good ? 1 : 0;
+}
+
 struct tree_set {
     int n;
     uint8_t *tree[24];
@@ -1479,6 +1592,20 @@ void gather16(const uint64_t *src, int64_t n, fe *out, hipStream_t s)
     if (n) hipLaunchKernelGGL(k_gather16, dim3(nblk(n)), dim3(TPB), 0, s, src, n, out);
 }
 
+
+void predicates(const layer_dev *D, int n, int ng_max, fe *part, int blocks, fe *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_predicates, dim3(blocks, n), dim3(TPB), 0, s, D, part);
+    hipLaunchKernelGGL(k_predicates_sum, dim3(n), dim3(64), 0, s, part, blocks, out);
+}
+void poly_eval(const fe *coef, int64_t n, const fe *pts, int npts, fe *out, hipStream_t s)
+{
+    if (npts) hipLaunchKernelGGL(k_poly_eval, dim3(npts), dim3(TPB), 0, s, coef, n, pts, out);
+}
+void verify_paths(const path_job *J, int njobs, const uint64_t *dg, const fe *vals, int *ok, hipStream_t s)
+{
+    if (njobs) hipLaunchKernelGGL(k_verify_paths, dim3(nblk(njobs, 64)), dim3(64), 0, s, J, njobs, dg, vals, ok);
+}
 
 // ---- fft_gkr ----
 void fg_eq(fe *out, const fe *r, int lg, hipStream_t s)

@@ -193,6 +193,21 @@ void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStre
 // out[k] = 16 bytes at address src[k]
 void gather16(const uint64_t *src, int64_t n, fe *out, hipStream_t s);
 
+// ---- verifier (zk_verifier.cpp:219-606, 884-908; vpd_verifier.cpp:8-41, 80-335) ----
+// out[layer * 14 + type] = the layer's wiring predicate of that gate type
+// (zero for the layer without a general value); part: n * blocks * 14
+void predicates(const layer_dev *D, int n, int ng_max, fe *part, int blocks, fe *out, hipStream_t s);
+// out[p] = sum_{k < n} coef[k] pts[p]^k
+void poly_eval(const fe *coef, int64_t n, const fe *pts, int npts, fe *out, hipStream_t s);
+struct path_job {              // one authentication path of a query answer
+    int64_t pow;               // position whose bits order each level's pair
+    int64_t root, path;        // digest indices: root, first path entry
+    int64_t vals;              // first of the 65 value pairs
+    int len;                   // path entries (the last is the leaf digest)
+};
+// ok[k] = 1 when job k's leaf chain and path reach its root
+void verify_paths(const path_job *J, int njobs, const uint64_t *dg, const fe *vals, int *ok, hipStream_t s);
+
 // ---- fft_gkr (fft_circuit_GKR.cpp:22-743), N = 2^lg ----
 struct fg_pw_t {              // power tables of 65 points: 64 evaluation points, then inv_rou
     const fe *plo, *phi;      // x_i^t (t < 2^lo) and x_i^(s 2^lo) (s < 2^hi), row i

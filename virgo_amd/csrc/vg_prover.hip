@@ -1187,6 +1187,354 @@ int ctx_init(vg_ctx *c)
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Verifier (zk_verifier::verify, zk_verifier.cpp:910-1253, with
+// poly_commit_verifier::verify_poly_commitment, vpd_verifier.cpp:80-335, and
+// the fft_gkr program's checks) reading the transcript assemble() writes.
+// The round checks are a few field operations each and run on the host; the
+// hot loops run on the device: the wiring predicates of every layer
+// (zk_verifier.cpp:219-606) in one launch, public_array_prepare (:884-908)
+// as an eq table and an iNTT, the first-round evaluations of the public
+// polynomial at the query points (vpd_verifier.cpp:215-226) and every query
+// path's leaf chain and Merkle climb (vpd_verifier.cpp:8-41).  Slice j of
+// the public array is Q[k] q_r[j] (public_array_prepare), so its value at x
+// is q_r[j] Q(x): one evaluation per point serves all 64 slices, with the
+// exact values of the reference's per-slice loop.
+// ---------------------------------------------------------------------------
+struct treader {
+    const uint8_t *p;
+    size_t n, off;
+    bool bad;
+    void get(void *x, size_t k)
+    {
+        if (off + k > n) { bad = true; memset(x, 0, k); off = n; return; }
+        memcpy(x, p + off, k);
+        off += k;
+    }
+    // a non-canonical element (a component >= p) makes the transcript
+    // malformed: the field operations assume canonical inputs, as the
+    // reference's in-process messages always are
+    fe f()
+    {
+        fe x;
+        get(&x, 16);
+        if (x.real >= P || x.img >= P) bad = true;
+        return x;
+    }
+    uint32_t u32() { uint32_t x; get(&x, 4); return x; }
+    qpoly q3() { qpoly q; q.a = f(); q.b = f(); q.c = f(); return q; }
+};
+// quintuple_poly in member order, evaluated as polynomial.cpp:19-22
+fe q5_eval(const fe *q, const fe &x)
+{
+    fe r = q[0];
+    for (int k = 1; k < 6; ++k) r = add(mul(r, x), q[k]);
+    return r;
+}
+// relay term of layer 1 (zk_verifier.cpp:1015-1044)
+fe direct_relay(const host_circuit &C, int i, const std::vector<fe> &rg, const std::vector<fe> &ru)
+{
+    if (i != 1) return fzero();
+    fe r = fone(), two = from_u64(2);
+    for (int k = 0; k < C.bl[1]; ++k) r = mul(r, add(sub(sub(fone(), rg[k]), ru[k]), mul(mul(two, rg[k]), ru[k])));
+    return r;
+}
+// the layer's predicates combined with v_u, v_v (zk_verifier.cpp:1149)
+fe general_value(const fe *pv, const fe &vu, const fe &vv)
+{
+    fe one = fone(), uv = mul(vu, vv);
+    fe g = add(mul(pv[0], add(vu, vv)), mul(pv[1], uv));
+    g = add(g, mul(pv[6], sub(one, vu)));
+    g = add(g, mul(pv[7], sub(vu, vv)));
+    g = add(g, mul(pv[8], sub(add(vu, vv), dbl(uv))));
+    g = add(g, mul(pv[9], sub(vv, uv)));
+    g = add(g, mul(add(add(pv[5], pv[10]), pv[12]), vu));
+    g = add(g, mul(pv[13], mul(vu, sub(one, vv))));
+    return g;
+}
+struct vscratch {                 // device buffers of one verification
+    std::vector<void *> p;
+    ~vscratch() { for (void *x : p) (void)hipFree(x); }
+    template <class T>
+    T *get(size_t count)
+    {
+        void *x = nullptr;
+        if (hipMalloc(&x, sizeof(T) * (count ? count : 1)) != hipSuccess) return nullptr;
+        p.push_back(x);
+        return (T *)x;
+    }
+};
+#define VREJ(...) do { char b_[512]; snprintf(b_, sizeof b_, __VA_ARGS__); g_err = b_; *accepted = 0; return 0; } while (0)
+#define VNULL(x) do { if (!(x)) FAILV(-21, "verifier: device allocation failed"); } while (0)
+
+int verify(vg_ctx *c, const uint8_t *tr, size_t len, int *accepted)
+{
+    *accepted = 0;
+    const host_circuit &C = c->C;
+    int D = C.depth, n = c->n, nf = n - 6;
+    hipStream_t st = c->st;
+    int rc = prepare(c);                          // the challenge schedule (public randomness)
+    if (rc) return rc;
+    const schedule &S = c->S;
+    vscratch V;
+    // ---- wiring predicates of every layer, on the device ----
+    int nl = D - 1, ngm = 1;
+    for (auto &b : c->batches) ngm = b.ng_max > ngm ? b.ng_max : ngm;
+    int pblocks = (ngm + 255) / 256;
+    if (pblocks > 256) pblocks = 256;
+    fe *ppart = V.get<fe>((size_t)nl * pblocks * 14), *pvd = V.get<fe>((size_t)nl * 14);
+    VNULL(ppart); VNULL(pvd);
+    HIPCHK(hipMemcpyAsync(c->chal, c->h_chal, sizeof(fe) * c->n_chal, hipMemcpyHostToDevice, st));
+    launch::half_multi(c->hdesc, c->n_hdesc, c->ht_total, st);
+    launch::predicates(c->d_layers, nl, ngm, ppart, pblocks, pvd, st);
+    std::vector<fe> pvh((size_t)nl * 14);
+    HIPCHK(hipMemcpyAsync(pvh.data(), pvd, sizeof(fe) * pvh.size(), hipMemcpyDeviceToHost, st));
+    std::vector<const fe *> pv_of(D, nullptr);
+    for (auto &b : c->batches)
+        for (size_t q = 0; q < b.layers.size(); ++q) pv_of[b.layers[q]] = pvh.data() + (size_t)(b.d0 + q) * 14;
+    // ---- header ----
+    treader R = {tr, len, 0, false};
+    char magic[4];
+    R.get(magic, 4);
+    uint32_t hd = R.u32(), hn = R.u32();
+    if (R.bad || memcmp(magic, "VGT1", 4) || hd != (uint32_t)D || hn != (uint32_t)C.bl[0]) VREJ("bad transcript header");
+    for (int i = 0; i < D; ++i)
+        if (R.u32() != (uint32_t)C.bl[i]) VREJ("bit length mismatch at layer %d", i);
+    HIPCHK(hipStreamSynchronize(st));
+    // ---- GKR layers ----
+    fe one = fone(), alpha = one, beta = fzero();
+    std::vector<fe> r0 = S.r0, r1 = S.r1;
+    fe claim = R.f();                             // alpha * V_res
+    fe mask_sum = fzero(), preu1_prev = fzero(), prev1_prev = fzero();
+    std::vector<fe> pub;
+    // one sumcheck phase: bl rounds, the last a quintuple; `shift` is added
+    // to every round's p(0) + p(1) and to the next claim
+    auto phase = [&](const std::vector<fe> &rr, int bl, const fe &shift, int i, int ph) -> bool {
+        for (int j = 0; j < bl; ++j) {
+            fe p0, p1, pr;
+            if (j == bl - 1) {
+                fe q[6];
+                for (auto &x : q) x = R.f();
+                p0 = q5_eval(q, fzero()); p1 = q5_eval(q, one); pr = q5_eval(q, rr[j]);
+            } else {
+                qpoly q = R.q3();
+                p0 = q_eval(q, fzero()); p1 = q_eval(q, one); pr = q_eval(q, rr[j]);
+            }
+            if (R.bad) { g_err = "transcript truncated or not canonical"; return false; }
+            if (!feq(add(add(p0, p1), shift), claim)) {
+                char b_[128];
+                snprintf(b_, sizeof b_, "phase-%d check failed, layer %d bit %d", ph, i, j);
+                g_err = b_;
+                return false;
+            }
+            claim = add(pr, shift);
+        }
+        return true;
+    };
+    for (int i = D - 1; i >= 1; --i) {
+        const layer_sched &l = S.L[i];
+        int bl = l.bl;
+        claim = add(claim, R.f());                // maskpoly_sumc
+        std::vector<fe> ru = l.r_u, rv(bl, fzero());
+        if (i != 1) rv = l.r_v;
+        fe drv = add(mul(alpha, direct_relay(C, i, r0, ru)), mul(beta, direct_relay(C, i, r1, ru)));
+        fe rcv = i == 1 ? fzero() : l.r_c;
+        push_pub_masks(pub, l, i, preu1_prev, prev1_prev);
+        if (!phase(ru, bl, fzero(), i, 1)) { *accepted = 0; return 0; }
+        fe vu = R.f();
+        fe dv = mul(drv, vu);
+        if (!phase(rv, bl, dv, i, 2)) { *accepted = 0; return 0; }
+        fe vv = R.f();
+        fe gv = general_value(pv_of[i], vu, vv);
+        qpoly fp = R.q3();
+        if (R.bad) VREJ("transcript truncated or not canonical");
+        if (!feq(add(add(q_eval(fp, fzero()), q_eval(fp, one)), dv), claim)) VREJ("final round check failed, layer %d", i);
+        claim = add(q_eval(fp, rcv), dv);
+        fe mpv = R.f(), rg1 = R.f(), rg2 = R.f();
+        mask_sum = add(mask_sum, add(add(mul(rg1, l.random_combine), mul(mul(l.linear_coeff, rg2), l.random_combine)),
+                                     mul(l.linear_combine, mpv)));
+        fe Iuv = R.f(), pZu = R.f(), pZv = R.f();
+        fe rhs = add(add(add(add(mul(rcv, gv), mul(mul(mul(alpha, Iuv), pZu), rg1)), mul(mul(mul(beta, Iuv), pZv), rg2)), mpv), dv);
+        if (R.bad) VREJ("transcript truncated or not canonical");
+        if (!feq(claim, rhs)) VREJ("semi-final check failed, layer %d", i);
+        alpha = l.alpha; beta = l.beta;
+        claim = i != 1 ? add(mul(alpha, vu), mul(beta, vv)) : vu;
+        r0 = ru; r1 = rv;
+        preu1_prev = ru[bl - 1];
+        prev1_prev = rv[bl - 1];
+    }
+    fe Zu = R.f(), preu1 = R.f();
+    qpoly sumRc = R.q3();
+    pub.push_back(Zu);
+    pub.push_back(mul(Zu, preu1));
+    fe zs = mul(Zu, q_eval(sumRc, preu1));
+    fe target = add(sub(claim, zs), add(mask_sum, zs));
+    // ---- polynomial commitment (vpd_verifier.cpp:80-335) ----
+    int64_t SS = c->SS, cs = c->real, gap = SS / (int64_t)pub.size();
+    for (int j = 0; j < 64; ++j)
+        if ((1LL << j) <= gap && (1LL << (j + 1)) > gap) { gap = 1LL << j; break; }
+    int64_t m = SS / gap;
+    if (m != c->m) FAILV(-30, "verifier: public mask length %lld differs from the context's %lld", (long long)m, (long long)c->m);
+    pub.resize(m, fzero());
+    uint8_t rootL[32], rootH[32];
+    R.get(rootL, 32);
+    R.get(rootH, 32);
+    fe all_sum[65], tot = fzero();
+    for (auto &x : all_sum) { x = R.f(); tot = add(tot, x); }
+    if (R.bad) VREJ("transcript truncated or not canonical");
+    // the prover's own assertion (poly_commit.h:193): the slices add up to the target
+    if (!feq(tot, target)) VREJ("sum of all_sum != inner-product target");
+    std::vector<uint8_t> froot((size_t)32 * nf);
+    R.get(froot.data(), froot.size());
+    std::vector<fe> fin(2048 + 32);
+    for (auto &x : fin) x = R.f();
+    // final codewords constant (vpd_verifier.cpp:315-332)
+    for (int i = 0; i < 64; ++i)
+        for (int j = 1; j < 16; ++j)
+            if (!feq(fin[(j << 7) | (i << 1)], fin[i << 1])) VREJ("final rs code not constant");
+    for (int j = 1; j < 32; ++j)
+        if (!feq(fin[2048 + j], fin[2048 + j - 1])) VREJ("final mask rs code not constant");
+    // query answers: every value pair and digest, and one path job per
+    // authentication path; digests 0, 1 = roots L, H, 2 + k = FRI root k
+    std::vector<fe> vals;
+    std::vector<uint8_t> dg(32 * 2 + froot.size());
+    memcpy(dg.data(), rootL, 32);
+    memcpy(dg.data() + 32, rootH, 32);
+    memcpy(dg.data() + 64, froot.data(), froot.size());
+    std::vector<launch::path_job> jobs;
+    std::vector<int64_t> vpos((size_t)33 * nf), alpos(33), ahpos(33);
+    auto take_vals = [&]() { int64_t o = (int64_t)vals.size(); for (int k = 0; k < 130; ++k) vals.push_back(R.f()); return o; };
+    auto take_path = [&](int plen) {
+        int64_t o = (int64_t)(dg.size() / 32);
+        size_t at = dg.size();
+        dg.resize(at + (size_t)32 * plen);
+        R.get(dg.data() + at, (size_t)32 * plen);
+        return o;
+    };
+    std::vector<fe> pts(66);
+    fe wS = root_of_unity(c->log_SS);
+    for (int rep = 0; rep < 33; ++rep) {
+        int64_t pw = S.pow[rep];
+        for (int i = 0; i < nf; ++i) {
+            int64_t mod = (int64_t)1 << (n - 1 - i);
+            if (i > 0) pw %= mod;
+            int64_t s0p = pw / 2, s1p = (pw + mod) / 2;
+            if (i == 0) {
+                int64_t mn = s0p < s1p ? s0p : s1p;
+                alpos[rep] = take_vals();
+                int64_t pl = take_path(n - 1);
+                ahpos[rep] = take_vals();
+                int64_t ph = take_path(n - 1);
+                jobs.push_back({mn, 0, pl, alpos[rep], n - 1});
+                jobs.push_back({mn, 1, ph, ahpos[rep], n - 1});
+                pts[2 * rep] = fpow(wS, (unsigned __int128)s0p);
+                pts[2 * rep + 1] = fpow(wS, (unsigned __int128)s1p);
+            }
+            vpos[(size_t)rep * nf + i] = take_vals();
+            int plen = n - 2 - i;
+            int64_t pp = take_path(plen);
+            jobs.push_back({pw / 2, 2 + i, pp, vpos[(size_t)rep * nf + i], plen});
+        }
+    }
+    if (R.bad) VREJ("transcript truncated or not canonical");
+    // ---- device: public polynomial Q, mask polynomial, their values at the
+    // query points, every path ----
+    int lcs = ilog2(cs), lm = ilog2(m);
+    fe *qv = V.get<fe>(cs), *qc = V.get<fe>(cs), *pubd = V.get<fe>(m), *pubc = V.get<fe>(m), *ptsd = V.get<fe>(66),
+       *evq = V.get<fe>(66), *evm = V.get<fe>(66), *vd = V.get<fe>(vals.size());
+    uint64_t *dgd = V.get<uint64_t>(dg.size() / 8);
+    launch::path_job *jd = V.get<launch::path_job>(jobs.size());
+    int *okd = V.get<int>(jobs.size());
+    launch::half_desc *hq = V.get<launch::half_desc>(1);
+    VNULL(qv); VNULL(qc); VNULL(pubd); VNULL(pubc); VNULL(ptsd); VNULL(evq); VNULL(evm); VNULL(vd); VNULL(dgd); VNULL(jd);
+    VNULL(okd); VNULL(hq);
+    // q_eval_real of the input point (zk_verifier.cpp:884-908): eq over its
+    // low n-6 bits, then the 64 slice weights over the high 6
+    launch::half_desc hdq = {qv, nullptr, c->chal + c->chal_ru[1], lcs, 0};
+    HIPCHK(hipMemcpyAsync(hq, &hdq, sizeof hdq, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(pubd, pub.data(), sizeof(fe) * m, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ptsd, pts.data(), sizeof(fe) * 66, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(vd, vals.data(), sizeof(fe) * vals.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dgd, dg.data(), dg.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(jd, jobs.data(), sizeof(launch::path_job) * jobs.size(), hipMemcpyHostToDevice, st));
+    launch::half_multi(hq, 1, cs, st);
+    launch::intt({0, qv, cs, nullptr, nullptr, 0, 0, 0, 0}, qc, cs, 1, lcs, finv(from_u64((uint64_t)cs)), c->NT, st);
+    launch::intt({0, pubd, m, nullptr, nullptr, 0, 0, 0, 0}, pubc, m, 1, lm, finv(from_u64((uint64_t)m)), c->NT, st);
+    launch::poly_eval(qc, cs, ptsd, 66, evq, st);
+    launch::poly_eval(pubc, m, ptsd, 66, evm, st);
+    launch::verify_paths(jd, (int)jobs.size(), dgd, vd, okd, st);
+    std::vector<fe> hq_ev(66), hm_ev(66);
+    std::vector<int> ok(jobs.size());
+    HIPCHK(hipMemcpyAsync(hq_ev.data(), evq, sizeof(fe) * 66, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hm_ev.data(), evm, sizeof(fe) * 66, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ok.data(), okd, sizeof(int) * ok.size(), hipMemcpyDeviceToHost, st));
+    // meanwhile: the slice weights q_r over the top 6 bits of the input point
+    fe qr[65];
+    {
+        const std::vector<fe> &rin = S.L[1].r_u;
+        for (int j = 0; j < 64; ++j) {
+            fe w = one;
+            for (int b = 0; b < 6; ++b) w = mul(w, ((j >> b) & 1) ? rin[n - 6 + b] : sub(one, rin[n - 6 + b]));
+            qr[j] = w;
+        }
+        qr[64] = one;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    for (size_t k = 0; k < jobs.size(); ++k)
+        if (!ok[k]) VREJ("merkle path %zu failed (query %zu)", k, k / (size_t)(nf + 2));
+    // ---- FRI consistency (vpd_verifier.cpp:160-313) ----
+    fe inv2 = finv(from_u64(2)), sg = from_u64((uint64_t)(SS / gap)), s32 = from_u64((uint64_t)(SS >> 5));
+    for (int rep = 0; rep < 33; ++rep) {
+        int64_t pw = S.pow[rep];
+        fe rou = root_of_unity(n - 1);
+        for (int i = 0; i < nf; ++i) {
+            int64_t mod = (int64_t)1 << (n - 1 - i);
+            if (i > 0) { rou = mul(rou, rou); pw %= mod; }
+            int64_t s0p = pw / 2, s1p = (pw + mod) / 2;
+            fe inv_mu = fpow(rou, (unsigned __int128)(mod - pw / 2));      // rou has order mod
+            fe rmu = mul(mul(inv2, S.fri_r[i]), inv_mu);
+            const fe *v = vals.data() + vpos[(size_t)rep * nf + i];
+            if (i == 0) {
+                const fe *al = vals.data() + alpos[rep], *ah = vals.data() + ahpos[rep];
+                fe x0 = pts[2 * rep], x1 = pts[2 * rep + 1];
+                fe ix0 = fpow(wS, (unsigned __int128)(SS - s0p)), ix1 = fpow(wS, (unsigned __int128)(SS - s1p));
+                fe mr0 = fpow(x0, (unsigned __int128)(SS / gap)), mr1 = fpow(x1, (unsigned __int128)(SS / gap));
+                fe ro0 = fpow(x0, (unsigned __int128)(SS >> 5)), ro1 = fpow(x1, (unsigned __int128)(SS >> 5));
+                for (int j = 0; j < 65; ++j) {
+                    bool msk = j == 64;
+                    fe t0 = msk ? hm_ev[2 * rep] : mul(qr[j], hq_ev[2 * rep]);
+                    fe t1 = msk ? hm_ev[2 * rep + 1] : mul(qr[j], hq_ev[2 * rep + 1]);
+                    fe z0 = msk ? mr0 : ro0, z1 = msk ? mr1 : ro1, sc = msk ? sg : s32;
+                    fe a = sub(mul(al[2 * j], t0), mul(sub(z0, one), ah[2 * j]));
+                    a = mul(sub(mul(a, sc), all_sum[j]), ix0);
+                    fe b = sub(mul(al[2 * j + 1], t1), mul(sub(z1, one), ah[2 * j + 1]));
+                    b = mul(sub(mul(b, sc), all_sum[j]), ix1);
+                    if (s0p > s1p) std::swap(a, b);
+                    fe pval = add(mul(add(a, b), inv2), mul(sub(a, b), rmu));
+                    if (!feq(pval, v[2 * j]) && !feq(pval, v[2 * j + 1]))
+                        VREJ("FRI first-round consistency failed, query %d slice %d", rep, j);
+                }
+            } else {
+                const fe *pv = vals.data() + vpos[(size_t)rep * nf + i - 1];
+                for (int j = 0; j < 65; ++j) {
+                    fe a = pv[2 * j], b = pv[2 * j + 1];
+                    fe h0 = mul(add(a, b), inv2);
+                    fe p0 = add(h0, mul(sub(a, b), rmu)), p1 = add(h0, mul(sub(b, a), rmu));
+                    if (!feq(p0, v[2 * j]) && !feq(p0, v[2 * j + 1]) && !feq(p1, v[2 * j]) && !feq(p1, v[2 * j + 1]))
+                        VREJ("FRI consistency failed, query %d round %d", rep, i);
+                }
+            }
+        }
+    }
+    // ---- the fft_gkr section ----
+    int fok = 0;
+    rc = fg_verify(c->fg, tr + R.off, len - R.off, &fok);
+    if (rc) return rc;
+    if (!fok) VREJ("fft_gkr section rejected");
+    *accepted = 1;
+    return 0;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1371,6 +1719,18 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
     for (int k = 0; k < workers; ++k)
         if (rc[k]) { g_err = msg[k]; return rc[k]; }
     return 0;
+}
+
+int vg_verify(vg_ctx *c, const uint8_t *transcript, size_t len, int *accepted, double *ms)
+{
+    if (!c || !transcript || !accepted) FAILV(-1, "vg_verify: null argument");
+    if (c->pending) FAILV(-1, "vg_verify: a proof is in flight on this context");
+    auto t0 = std::chrono::steady_clock::now();
+    for (hipStream_t s : {c->st2, c->gst[0], c->gst[1], c->gst[2], c->gst[3]})
+        if (s && s != c->st) HIPCHK(hipStreamSynchronize(s));
+    int rc = verify(c, transcript, len, accepted);
+    if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
 }
 
 int vg_output(vg_ctx *c, vg_fe *out, size_t cap, int *bits)

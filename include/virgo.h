@@ -47,9 +47,11 @@ void vg_free(void *p);
  * (input padding to 2^15 or 2^17, relay layer 1, bit lengths), uploads gates
  * and inputs, and allocates every device buffer of one proof.
  * stream: NULL -- the context owns its streams and overlaps the commitment,
- * the sumcheck layers and fft_gkr (lowest latency for one proof); or a
- * hipStream_t -- the whole proof runs in order on that stream, so many
- * contexts on different streams overlap on the device (batched proving). */
+ * the sumcheck layers and fft_gkr where that lowers one proof's latency
+ * (inputs below 2^20; larger proofs run in order on the context's stream,
+ * VIRGO_SERIAL=1 / =0 forces either); or a hipStream_t -- the whole proof
+ * runs in order on that stream, so many contexts on different streams
+ * overlap on the device (batched proving). */
 int vg_ctx_create(vg_ctx **ctx, const vg_circuit_desc *c, void *stream);
 int vg_ctx_destroy(vg_ctx *ctx);
 /* Creates / destroys a non-blocking HIP stream for vg_ctx_create (no

@@ -455,6 +455,8 @@ int fg_enqueue(fg_ctx *c)
     return 0;
 }
 
+hipEvent_t fg_done_event(const fg_ctx *c) { return c->e1; }
+
 int fg_finish(fg_ctx *c, std::vector<uint8_t> &out, int *accepted, int *proof_size, double *p_ms)
 {
     if (!c->pending) FG_FAIL(-3, "fft_gkr: no proof in flight");

@@ -22,6 +22,8 @@ int fg_enqueue(fg_ctx *c);
 // Waits for the proof, appends the "VGF1" transcript section to `out` and
 // replays the program's inline checks (its verdict).  p_ms: device time.
 int fg_finish(fg_ctx *c, std::vector<uint8_t> &out, int *accepted, int *proof_size, double *p_ms);
+// Recorded on the proof's stream after its last device work (fg_enqueue).
+hipEvent_t fg_done_event(const fg_ctx *c);
 // Verifies a "VGF1" section (as fg_finish writes it) with the program's
 // checks: *accepted = 1 or 0.  No proof may be in flight on c.
 int fg_verify(fg_ctx *c, const uint8_t *sec, size_t len, int *accepted);

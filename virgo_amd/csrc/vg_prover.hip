@@ -504,7 +504,7 @@ struct vg_ctx {
     std::chrono::steady_clock::time_point t_start;
     double host_ms[3] = {};     // prepare (challenges, masks, gather table), enqueue, assemble
     bool pending = false;
-    bool serial = false;        // the whole proof in order on st (caller's stream or VIRGO_SERIAL=1)
+    bool serial = false;        // the whole proof in order on st (caller's stream, large input or VIRGO_SERIAL=1)
     bool own_aux = false;       // st2 and gst[] created by the context
     fg_ctx *fg = nullptr;       // fft_gkr of the commitment (lg = n - 6), on its own stream
 };
@@ -850,6 +850,14 @@ int enqueue(vg_ctx *c)
             cws[ns] = c->cw + c->cw_off[k]; Ss[ns] = Sk; trees[ns] = c->ctree + c->ctree_off[k]; ++ns;
         }
         for (int k = 0; k < ns; ++k) nl[k] = Ss[k] / 2;
+        // the leaf chains keep every SIMD busy on their own: beside them the
+        // latency-bound sumcheck and fft_gkr launches only slow both down,
+        // so they start once those are done (the LDEs and folds before them
+        // do overlap the sumcheck)
+        if (!c->serial) {
+            for (int w = 0; w < vg_ctx::NGS; ++w) HIPCHK(hipStreamWaitEvent(st, c->gev[w], 0));
+            if (c->fg) HIPCHK(hipStreamWaitEvent(st, fg_done_event(c->fg), 0));
+        }
         HIPCHK(hipEventRecord(c->ev_leaf[0], st));
         launch::leaf_hash_multi(ns, cws, Ss, trees, c->log_R, st);
         HIPCHK(hipEventRecord(c->ev_leaf[1], st));
@@ -1576,12 +1584,17 @@ int vg_ctx_create(vg_ctx **out, const vg_circuit_desc *d, void *stream)
         c->own_stream = true;
     }
     {
-        // A caller's stream (or VIRGO_SERIAL=1) runs the whole proof in
-        // order on one stream: several contexts then overlap on the device
-        // without sharing hardware queues.  Otherwise the context owns one
-        // stream for the commitment and four for the sumcheck layers.
+        // A caller's stream runs the whole proof in order on one stream:
+        // several contexts then overlap on the device without sharing
+        // hardware queues.  Otherwise the context owns one stream for the
+        // commitment, four for the sumcheck layers and one for fft_gkr, so
+        // that the latency-bound chains of a small proof overlap; from 2^20
+        // inputs on, every stage fills the device by itself and overlapping
+        // them only adds contention (measured: 13.1 vs 12.9 ms at 2^20 gates
+        // x 16 layers, 4.1 vs 4.3 ms at 2^18 x 16), so such proofs run in
+        // order.  VIRGO_SERIAL=1 / =0 forces one choice.
         const char *e = getenv("VIRGO_SERIAL");
-        c->serial = stream != nullptr || (e && e[0] == '1');
+        c->serial = stream != nullptr || (e && e[0] == '1') || (!(e && e[0] == '0') && c->C.bl[0] >= 20);
     }
     {
         hipError_t e = hipSuccess;

@@ -753,20 +753,29 @@ int build_gather(vg_ctx *c)
     return 0;
 }
 
+// ---- evaluate (zk_prover.cpp:54-157): needs no challenge, so it is
+// enqueued before the host draws them ----
+int enqueue_eval(vg_ctx *c)
+{
+    hipStream_t st = c->st;
+    host_circuit &C = c->C;
+    HIPCHK(hipEventRecord(c->ev[0], st));
+    launch::eval_input(c->d_u[0], c->cv[0], 1 << C.bl[0], st);
+    for (int i = 1; i < C.depth; ++i)
+        launch::eval_layer({c->d_ty[i], c->d_u[i], c->d_v[i], C.bl[i]}, c->cv[i - 1], c->cv[i], st);
+    HIPCHK(hipEventRecord(c->ev[1], st));
+    return 0;
+}
+
 int enqueue(vg_ctx *c)
 {
     hipStream_t st = c->st, st2 = c->st2;
     host_circuit &C = c->C;
     int D = C.depth;
-    HIPCHK(hipEventRecord(c->ev[0], st));
     HIPCHK(hipMemcpyAsync(c->chal, c->h_chal, sizeof(fe) * c->n_chal, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(c->gsrc, c->h_gsrc, sizeof(uint64_t) * c->n_gather, hipMemcpyHostToDevice, st));
     // every eq half table of the proof, from the uploaded challenges
     launch::half_multi(c->hdesc, c->n_hdesc, c->ht_total, st);
-    // ---- evaluate (zk_prover.cpp:54-157) ----
-    launch::eval_input(c->d_u[0], c->cv[0], 1 << C.bl[0], st);
-    for (int i = 1; i < D; ++i) launch::eval_layer({c->d_ty[i], c->d_u[i], c->d_v[i], C.bl[i]}, c->cv[i - 1], c->cv[i], st);
-    HIPCHK(hipEventRecord(c->ev[1], st));
     // The commitment of the input layer needs only the input values, the
     // host-known masks and challenges, never a sumcheck result: it runs on the
     // second stream beside the latency-bound sumcheck rounds.
@@ -1656,13 +1665,18 @@ int vg_prove_async(vg_ctx *c)
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
     c->t_start = std::chrono::steady_clock::now();
-    int rc = prepare(c);
+    // the evaluation runs while the host draws the challenges
+    int rc = enqueue_eval(c);
+    if (rc) return rc;
+    auto tp = std::chrono::steady_clock::now();
+    rc = prepare(c);
     if (rc) return rc;
     rc = build_gather(c);
     if (rc) return rc;
-    c->host_ms[0] = ms_since(c->t_start);
+    c->host_ms[0] = ms_since(tp);
     auto t1 = std::chrono::steady_clock::now();
-    // fft_gkr first: its small latency-bound launches overlap the evaluation
+    // fft_gkr (its own stream unless the proof runs in order): its small
+    // latency-bound launches overlap the main proof
     rc = fg_enqueue(c->fg);
     if (rc) return rc;
     rc = enqueue(c);

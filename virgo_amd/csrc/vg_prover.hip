@@ -499,7 +499,7 @@ struct vg_ctx {
     std::vector<int64_t> msg_layer;                   // offset of each layer's block in msg
     int64_t msg_vres = 0, msg_allsum = 0;
     fe post[5];
-    hipEvent_t ev[10] = {};     // stage boundaries
+    hipEvent_t ev[10] = {};     // stage boundaries; ev[9]: inputs of the side streams ready
     hipEvent_t ev_leaf[2] = {}; // around the leaf-chain kernel
     std::chrono::steady_clock::time_point t_start;
     double host_ms[3] = {};     // prepare (challenges, masks, gather table), enqueue, assemble
@@ -776,17 +776,19 @@ int enqueue(vg_ctx *c)
     HIPCHK(hipMemcpyAsync(c->gsrc, c->h_gsrc, sizeof(uint64_t) * c->n_gather, hipMemcpyHostToDevice, st));
     // every eq half table of the proof, from the uploaded challenges
     launch::half_multi(c->hdesc, c->n_hdesc, c->ht_total, st);
+    // evaluated circuit + challenges + half tables: what the other streams need
+    HIPCHK(hipEventRecord(c->ev[9], st));
     // The commitment of the input layer needs only the input values, the
     // host-known masks and challenges, never a sumcheck result: it runs on the
     // second stream beside the latency-bound sumcheck rounds.
-    if (!c->serial) HIPCHK(hipStreamWaitEvent(st2, c->ev[1], 0));
+    if (!c->serial) HIPCHK(hipStreamWaitEvent(st2, c->ev[9], 0));
     // ---- V_res (zk_prover.cpp:30-52) ----
     launch::mle_eval(c->cv[D - 1], C.bl[D - 1], c->chal + c->chal_r0, c->mle_t[0], c->mle_t[1], c->msg + c->msg_vres, st);
     // ---- GKR layers ----
     // Every layer's sumcheck depends only on the evaluated circuit and on
     // challenges known in advance, so layers run concurrently on NGS streams.
     if (!c->serial)
-        for (int w = 0; w < vg_ctx::NGS; ++w) HIPCHK(hipStreamWaitEvent(c->gst[w], c->ev[1], 0));
+        for (int w = 0; w < vg_ctx::NGS; ++w) HIPCHK(hipStreamWaitEvent(c->gst[w], c->ev[9], 0));
     int slot = 0;
     for (size_t bi = 0; bi < c->batches.size(); ++bi) {
         const vg_ctx::batch &b = c->batches[bi];

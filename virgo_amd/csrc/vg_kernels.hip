@@ -1043,20 +1043,37 @@ struct leaf_segs {
 // Merkle leaves of every commitment at once: a leaf i of a codeword with S
 // positions per slice chains SHA3 over the 64 slice pairs (i, i+S/2) and then
 // the mask pair (fri.cpp:97-126, 397-420); the digest goes to tree[S/2 + i].
+// leaf t (in s-major thread order) of a codeword with S positions per slice
+__device__ __forceinline__ void leaf_chain(const fe *cw, int64_t S, uint8_t *tree, int log_R, int64_t t)
+{
+    int64_t half = S / 2;
+    int64_t i = leaf_of(t, half, log_R);
+    int64_t k0 = cw_idx(i, S, log_R), k1 = cw_idx(i + half, S, log_R);
+    uint64_t h[4] = {0, 0, 0, 0};
+    for (int j = 0; j <= 64; ++j) chain_step(h, cw[j * S + k0], cw[j * S + k1]);
+    store_digest(tree, half + i, h);
+}
 __global__ __launch_bounds__(TPB) void k_leaf_multi(leaf_segs L)
 {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (tid >= L.start[L.n]) return;
     int sg = 0;
     while (tid >= L.start[sg + 1]) ++sg;
-    tid -= L.start[sg];
-    int64_t S = L.S[sg], half = S / 2;
-    const fe *cw = L.cw[sg];
-    int64_t i = leaf_of(tid, half, L.log_R);
-    int64_t k0 = cw_idx(i, S, L.log_R), k1 = cw_idx(i + half, S, L.log_R);
-    uint64_t h[4] = {0, 0, 0, 0};
-    for (int j = 0; j <= 64; ++j) chain_step(h, cw[j * S + k0], cw[j * S + k1]);
-    store_digest(L.tree[sg], half + i, h);
+    leaf_chain(L.cw[sg], L.S[sg], L.tree[sg], L.log_R, tid - L.start[sg]);
+}
+// the same over a segment table in device memory (the codewords of many
+// proofs in one launch): start[] = prefix sums of leaf counts
+__global__ __launch_bounds__(TPB) void k_leaf_table(const launch::leaf_seg *G, const int64_t *start, int nseg)
+{
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= start[nseg]) return;
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {                         // last segment with start <= tid
+        int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= tid) lo = mid; else hi = mid - 1;
+    }
+    const launch::leaf_seg &g = G[lo];
+    leaf_chain(g.cw, g.S, g.tree, g.log_R, tid - start[lo]);
 }
 // one query path (vpd_verifier.cpp:8-41): the leaf chain over the 65 value
 // pairs must give the last path entry, and hashing up the path (sibling
@@ -1571,6 +1588,10 @@ void leaf_hash_multi(int nseg, const fe *const *cw, const int64_t *S, uint8_t *c
         L.start[k + 1] = L.start[k] + S[k] / 2;
     }
     hipLaunchKernelGGL(k_leaf_multi, dim3(nblk(L.start[nseg])), dim3(TPB), 0, s, L);
+}
+void leaf_hash_table(const leaf_seg *G, const int64_t *start, int nseg, int64_t total, hipStream_t s)
+{
+    if (total) hipLaunchKernelGGL(k_leaf_table, dim3(nblk(total)), dim3(TPB), 0, s, G, start, nseg);
 }
 void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStream_t s)
 {

@@ -188,6 +188,16 @@ void fri_folds_vo(const fe *l, const fe *q, const fe *h, const fe *c0, int64_t r
 // leaf chains of nseg codewords in one launch (codeword k: S[k] positions per
 // slice, leaves S[k]/2 written to tree[k] + S[k]/2 nodes)
 void leaf_hash_multi(int nseg, const fe *const *cw, const int64_t *S, uint8_t *const *tree, int log_R, hipStream_t s);
+// leaf chains of many codewords (of many proofs) in one launch, from a
+// segment table in device memory; start[k] = sum of S/2 over segments < k,
+// start[nseg] = total
+struct leaf_seg {
+    const fe *cw;
+    int64_t S;
+    uint8_t *tree;
+    int log_R;
+};
+void leaf_hash_table(const leaf_seg *G, const int64_t *start, int nseg, int64_t total, hipStream_t s);
 // every Merkle tree of the set, level by level across trees
 void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStream_t s);
 // out[k] = 16 bytes at address src[k]

@@ -9,6 +9,7 @@
 // scalar work on challenges only; it is computed beside the device work and
 // added to the device round sums when the transcript is assembled.
 #include <chrono>
+#include <functional>
 #include <thread>
 #include <cstdio>
 #include <cstdlib>
@@ -501,7 +502,7 @@ struct vg_ctx {
     fe post[5];
     hipEvent_t ev[10] = {};     // stage boundaries; ev[9]: inputs of the side streams ready
     hipEvent_t ev_leaf[2] = {}; // around the leaf-chain kernel
-    std::chrono::steady_clock::time_point t_start;
+    std::chrono::steady_clock::time_point t_start, t_enq;
     double host_ms[3] = {};     // prepare (challenges, masks, gather table), enqueue, assemble
     bool pending = false;
     bool serial = false;        // the whole proof in order on st (caller's stream, large input or VIRGO_SERIAL=1)
@@ -767,7 +768,8 @@ int enqueue_eval(vg_ctx *c)
     return 0;
 }
 
-int enqueue(vg_ctx *c)
+// everything of a proof up to its leaf chains (the FRI folds end on ev[5])
+int enqueue_pre(vg_ctx *c)
 {
     hipStream_t st = c->st, st2 = c->st2;
     host_circuit &C = c->C;
@@ -847,20 +849,37 @@ int enqueue(vg_ctx *c)
         launch::fri_folds_vo(c->l, c->q, c->h, c->c0, real, m, SS, n - 6, c->log_R, c->chal + c->chal_fri, c->tw, next, st);
     }
     HIPCHK(hipEventRecord(c->ev[5], st));
-    // ---- every SHA3 leaf chain (fri.cpp:97-126, 397-420) in one launch, then
-    // every Merkle tree (merkle_tree.cpp:46-56) level by level ----
-    {
-        const fe *cws[24];
-        int64_t Ss[24], nl[24];
-        uint8_t *trees[24];
-        int ns = 0;
-        cws[ns] = c->l; Ss[ns] = SS; trees[ns] = c->treeL; ++ns;
-        cws[ns] = c->h; Ss[ns] = SS; trees[ns] = c->treeH; ++ns;
-        int64_t Sk = SS / 2;
-        for (int k = 0; k < n - 6; ++k, Sk /= 2) {
-            cws[ns] = c->cw + c->cw_off[k]; Ss[ns] = Sk; trees[ns] = c->ctree + c->ctree_off[k]; ++ns;
-        }
-        for (int k = 0; k < ns; ++k) nl[k] = Ss[k] / 2;
+    return 0;
+}
+
+// the committed codewords, in leaf-launch order: L, H, then every FRI level
+int commit_segments(const vg_ctx *c, launch::leaf_seg *g)
+{
+    int ns = 0;
+    g[ns++] = {c->l, c->SS, c->treeL, c->log_R};
+    g[ns++] = {c->h, c->SS, c->treeH, c->log_R};
+    int64_t Sk = c->SS / 2;
+    for (int k = 0; k < c->n - 6; ++k, Sk /= 2) g[ns++] = {c->cw + c->cw_off[k], Sk, c->ctree + c->ctree_off[k], c->log_R};
+    return ns;
+}
+
+// ---- every SHA3 leaf chain (fri.cpp:97-126, 397-420) in one launch, then
+// every Merkle tree (merkle_tree.cpp:46-56) level by level, and the query
+// answers.  leaf_done: the leaf chains ran elsewhere (a batch's merged
+// launch) and completion of that event means they are done.
+int enqueue_post(vg_ctx *c, hipEvent_t leaf_done)
+{
+    hipStream_t st = c->serial ? c->st : c->st2;
+    launch::leaf_seg g[24];
+    int ns = commit_segments(c, g);
+    const fe *cws[24];
+    int64_t Ss[24], nl[24];
+    uint8_t *trees[24];
+    for (int k = 0; k < ns; ++k) { cws[k] = g[k].cw; Ss[k] = g[k].S; trees[k] = g[k].tree; nl[k] = g[k].S / 2; }
+    if (leaf_done) {
+        HIPCHK(hipStreamWaitEvent(st, leaf_done, 0));
+        HIPCHK(hipEventRecord(c->ev_leaf[0], st));
+    } else {
         // the leaf chains keep every SIMD busy on their own: beside them the
         // latency-bound sumcheck and fft_gkr launches only slow both down,
         // so they start once those are done (the LDEs and folds before them
@@ -871,9 +890,9 @@ int enqueue(vg_ctx *c)
         }
         HIPCHK(hipEventRecord(c->ev_leaf[0], st));
         launch::leaf_hash_multi(ns, cws, Ss, trees, c->log_R, st);
-        HIPCHK(hipEventRecord(c->ev_leaf[1], st));
-        launch::merkle_multi(ns, trees, nl, st);
     }
+    HIPCHK(hipEventRecord(c->ev_leaf[1], st));
+    launch::merkle_multi(ns, trees, nl, st);
     HIPCHK(hipEventRecord(c->ev[6], st));
     st = c->st;
     if (!c->serial) HIPCHK(hipStreamWaitEvent(st, c->ev[6], 0));
@@ -1204,6 +1223,33 @@ int ctx_init(vg_ctx *c)
     }
     HIPCHK(hipDeviceSynchronize());
     return 0;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+// host work and device work of a proof up to its leaf chains
+int prove_start(vg_ctx *c)
+{
+    if (!c) FAILV(-1, "null context");
+    if (c->pending) FAILV(-1, "a proof is already in flight on this context");
+    c->t_start = std::chrono::steady_clock::now();
+    // the evaluation runs while the host draws the challenges
+    int rc = enqueue_eval(c);
+    if (rc) return rc;
+    auto tp = std::chrono::steady_clock::now();
+    rc = prepare(c);
+    if (rc) return rc;
+    rc = build_gather(c);
+    if (rc) return rc;
+    c->host_ms[0] = ms_since(tp);
+    c->t_enq = std::chrono::steady_clock::now();
+    // fft_gkr (its own stream unless the proof runs in order): its small
+    // latency-bound launches overlap the main proof
+    rc = fg_enqueue(c->fg);
+    if (rc) return rc;
+    return enqueue_pre(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1666,29 +1712,11 @@ int64_t vg_ctx_device_bytes(const vg_ctx *c) { return c ? c->arena_bytes + fg_de
 
 int vg_prove_async(vg_ctx *c)
 {
-    if (!c) FAILV(-1, "null context");
-    if (c->pending) FAILV(-1, "a proof is already in flight on this context");
-    auto ms_since = [](std::chrono::steady_clock::time_point t0) {
-        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    };
-    c->t_start = std::chrono::steady_clock::now();
-    // the evaluation runs while the host draws the challenges
-    int rc = enqueue_eval(c);
+    int rc = prove_start(c);
     if (rc) return rc;
-    auto tp = std::chrono::steady_clock::now();
-    rc = prepare(c);
+    rc = enqueue_post(c, nullptr);
     if (rc) return rc;
-    rc = build_gather(c);
-    if (rc) return rc;
-    c->host_ms[0] = ms_since(tp);
-    auto t1 = std::chrono::steady_clock::now();
-    // fft_gkr (its own stream unless the proof runs in order): its small
-    // latency-bound launches overlap the main proof
-    rc = fg_enqueue(c->fg);
-    if (rc) return rc;
-    rc = enqueue(c);
-    if (rc) return rc;
-    c->host_ms[1] = ms_since(t1);
+    c->host_ms[1] = ms_since(c->t_enq);
     c->pending = true;
     return 0;
 }
@@ -1737,22 +1765,81 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
 {
     if (n < 0 || (n > 0 && (!ctxs || !transcripts || !lens))) FAILV(-1, "vg_prove_batch: bad arguments");
     for (int i = 0; i < n; ++i) { transcripts[i] = nullptr; lens[i] = 0; }
+    if (n == 0) return 0;
     if (workers < 1) workers = 1;
     if (workers > n) workers = n;
-    std::vector<int> rc(workers, 0);
-    std::vector<std::string> msg(workers);
-    std::vector<std::thread> th;
-    for (int k = 0; k < workers; ++k)
-        th.emplace_back([&, k]() {
-            for (int i = k; i < n && rc[k] == 0; i += workers) {
-                rc[k] = vg_prove(ctxs[i], &transcripts[i], &lens[i], nullptr);
-                if (rc[k]) msg[k] = g_err;                   // the worker's thread-local message
-            }
-        });
-    for (auto &t : th) t.join();
-    for (int k = 0; k < workers; ++k)
-        if (rc[k]) { g_err = msg[k]; return rc[k]; }
-    return 0;
+    // fn(i) for every context, worker k taking i = k, k + workers, ...;
+    // returns the first error (its thread-local message moved here)
+    auto run = [&](const std::function<int(int)> &fn) {
+        std::vector<int> rc(workers, 0);
+        std::vector<std::string> msg(workers);
+        std::vector<std::thread> th;
+        for (int k = 0; k < workers; ++k)
+            th.emplace_back([&, k]() {
+                for (int i = k; i < n && rc[k] == 0; i += workers) {
+                    rc[k] = fn(i);
+                    if (rc[k]) msg[k] = g_err;
+                }
+            });
+        for (auto &t : th) t.join();
+        for (int k = 0; k < workers; ++k)
+            if (rc[k]) { g_err = msg[k]; return rc[k]; }
+        return 0;
+    };
+    // 1. every proof up to its leaf chains
+    std::vector<char> started(n, 0);
+    int rc = run([&](int i) { int r = prove_start(ctxs[i]); started[i] = r == 0; return r; });
+    // 2. the leaf chains of every proof in one launch: 60 % of a mixed
+    // batch's VALU work, and one full-width launch keeps every SIMD busy
+    // where many small ones (one per proof) do not
+    std::vector<launch::leaf_seg> G;
+    std::vector<int64_t> start(1, 0);
+    hipStream_t ls = nullptr;
+    hipEvent_t done = nullptr;
+    launch::leaf_seg *dG = nullptr;
+    int64_t *dstart = nullptr;
+    bool merged = false;
+    if (rc == 0) {
+        for (int i = 0; i < n; ++i) {
+            launch::leaf_seg g[24];
+            int ns = commit_segments(ctxs[i], g);
+            for (int k = 0; k < ns; ++k) { G.push_back(g[k]); start.push_back(start.back() + g[k].S / 2); }
+        }
+        bool ok = hipStreamCreateWithFlags(&ls, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&done, hipEventDisableTiming) == hipSuccess &&
+                  hipMalloc(&dG, sizeof(launch::leaf_seg) * G.size()) == hipSuccess &&
+                  hipMalloc(&dstart, sizeof(int64_t) * start.size()) == hipSuccess;
+        for (int i = 0; ok && i < n; ++i) ok = hipStreamWaitEvent(ls, ctxs[i]->ev[5], 0) == hipSuccess;
+        ok = ok && hipMemcpyAsync(dG, G.data(), sizeof(launch::leaf_seg) * G.size(), hipMemcpyHostToDevice, ls) == hipSuccess &&
+             hipMemcpyAsync(dstart, start.data(), sizeof(int64_t) * start.size(), hipMemcpyHostToDevice, ls) == hipSuccess;
+        if (ok) {
+            launch::leaf_hash_table(dG, dstart, (int)G.size(), start.back(), ls);
+            ok = hipEventRecord(done, ls) == hipSuccess && hipGetLastError() == hipSuccess;
+        }
+        merged = ok;
+        if (!ok) { g_err = "vg_prove_batch: merged leaf launch failed"; rc = -20; }
+    }
+    // 3. the rest of every started proof (its own leaf launch when the merged
+    // one did not happen), then the transcripts
+    int rc2 = run([&](int i) {
+        if (!started[i]) return 0;
+        int r = enqueue_post(ctxs[i], merged ? done : nullptr);
+        ctxs[i]->pending = true;
+        return r;
+    });
+    int rc3 = run([&](int i) {
+        if (!started[i] || !ctxs[i]->pending) return 0;
+        return vg_finish(ctxs[i], &transcripts[i], &lens[i], nullptr);
+    });
+    if (ls) (void)hipStreamSynchronize(ls);
+    if (dG) (void)hipFree(dG);
+    if (dstart) (void)hipFree(dstart);
+    if (done) (void)hipEventDestroy(done);
+    if (ls) (void)hipStreamDestroy(ls);
+    int first = rc ? rc : rc2 ? rc2 : rc3;
+    if (first)
+        for (int i = 0; i < n; ++i) { free(transcripts[i]); transcripts[i] = nullptr; lens[i] = 0; }
+    return first;
 }
 
 int vg_verify(vg_ctx *c, const uint8_t *transcript, size_t len, int *accepted, double *ms)

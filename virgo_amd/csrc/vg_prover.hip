@@ -446,6 +446,8 @@ struct vg_ctx {
     std::vector<batch> batches;
     launch::layer_dev *d_layers = nullptr;    // in batch order
     launch::fold_inst *d_inst = nullptr;      // batch order: phase 1 of its layers, then phase 2
+    std::vector<launch::layer_dev> h_layers;  // host copies (merged launches of many proofs)
+    std::vector<launch::fold_inst> h_inst;
     std::vector<int> slot0;                   // first slot round of (layer entry, phase)
     hipStream_t gst[NGS] = {};
     hipEvent_t gev[NGS] = {};
@@ -768,8 +770,10 @@ int enqueue_eval(vg_ctx *c)
     return 0;
 }
 
-// everything of a proof up to its leaf chains (the FRI folds end on ev[5])
-int enqueue_pre(vg_ctx *c)
+// everything of a proof up to its leaf chains (the FRI folds end on ev[5]);
+// gkr = false leaves the sumcheck layers and their slot reduction to a
+// batch's merged launches (batch_gkr) and enqueue_post
+int enqueue_pre(vg_ctx *c, bool gkr = true)
 {
     hipStream_t st = c->st, st2 = c->st2;
     host_circuit &C = c->C;
@@ -789,10 +793,10 @@ int enqueue_pre(vg_ctx *c)
     // ---- GKR layers ----
     // Every layer's sumcheck depends only on the evaluated circuit and on
     // challenges known in advance, so layers run concurrently on NGS streams.
-    if (!c->serial)
+    if (!c->serial && gkr)
         for (int w = 0; w < vg_ctx::NGS; ++w) HIPCHK(hipStreamWaitEvent(c->gst[w], c->ev[9], 0));
     int slot = 0;
-    for (size_t bi = 0; bi < c->batches.size(); ++bi) {
+    for (size_t bi = 0; gkr && bi < c->batches.size(); ++bi) {
         const vg_ctx::batch &b = c->batches[bi];
         hipStream_t gs = c->gst[bi % vg_ctx::NGS];
         int n = (int)b.layers.size();
@@ -807,14 +811,16 @@ int enqueue_pre(vg_ctx *c)
         }
         launch::general_multi(Dl, n, b.ng_max, gs);
     }
-    if (!c->serial)
-        for (int w = 0; w < vg_ctx::NGS; ++w) {
-            HIPCHK(hipEventRecord(c->gev[w], c->gst[w]));
-            HIPCHK(hipStreamWaitEvent(st, c->gev[w], 0));
-        }
-    if (slot != c->n_slots) FAILV(-30, "fold slot plan mismatch");
-    launch::reduce_slots(c->slots, c->n_slots, c->slot_nb, c->slot_dst, c->msg, st);
-    HIPCHK(hipEventRecord(c->ev[2], st));
+    if (gkr) {
+        if (!c->serial)
+            for (int w = 0; w < vg_ctx::NGS; ++w) {
+                HIPCHK(hipEventRecord(c->gev[w], c->gst[w]));
+                HIPCHK(hipStreamWaitEvent(st, c->gev[w], 0));
+            }
+        if (slot != c->n_slots) FAILV(-30, "fold slot plan mismatch");
+        launch::reduce_slots(c->slots, c->n_slots, c->slot_nb, c->slot_dst, c->msg, st);
+        HIPCHK(hipEventRecord(c->ev[2], st));
+    }
     if (!c->serial) st = st2;
     // ---- commit_private_array (poly_commit.h:42-124) ----
     int n = c->n;
@@ -852,6 +858,81 @@ int enqueue_pre(vg_ctx *c)
     return 0;
 }
 
+// The sumcheck layers of many proofs in merged launches on stream gs: the
+// layer batches of equal table width and table-build path of every context
+// form one instance list (each instance keeps its own tables, slots and
+// message block), so a launch covers them all.  Waits for every context's
+// challenges (ev[9]); records gkr_done.  *buf: device descriptor arrays,
+// freed by the caller after the work completes; host_keep must outlive the
+// upload.
+int batch_gkr(vg_ctx *const *ctxs, int n, hipStream_t gs, hipEvent_t gkr_done, void **buf, std::vector<uint8_t> &host_keep)
+{
+    struct group {
+        int bl;
+        bool f0, f1;
+        int nz_max = 0, ng_max = 0;
+        std::vector<launch::layer_dev> L;
+        std::vector<launch::fold_inst> I[2];
+    };
+    std::vector<group> G;
+    for (int i = 0; i < n; ++i) {
+        vg_ctx *c = ctxs[i];
+        HIPCHK(hipStreamWaitEvent(gs, c->ev[9], 0));
+        for (auto &b : c->batches) {
+            group *g = nullptr;
+            for (auto &x : G)
+                if (x.bl == b.bl && x.f0 == b.fused[0] && x.f1 == b.fused[1]) g = &x;
+            if (!g) {
+                G.push_back(group());
+                g = &G.back();
+                g->bl = b.bl; g->f0 = b.fused[0]; g->f1 = b.fused[1];
+            }
+            int m = (int)b.layers.size();
+            g->nz_max = b.nz_max > g->nz_max ? b.nz_max : g->nz_max;
+            g->ng_max = b.ng_max > g->ng_max ? b.ng_max : g->ng_max;
+            for (int q = 0; q < m; ++q) {
+                g->L.push_back(c->h_layers[b.d0 + q]);
+                g->I[0].push_back(c->h_inst[2 * b.d0 + q]);
+                g->I[1].push_back(c->h_inst[2 * b.d0 + m + q]);
+            }
+        }
+    }
+    size_t bytes = 0;
+    for (auto &g : G) bytes += sizeof(launch::layer_dev) * g.L.size() + 2 * sizeof(launch::fold_inst) * g.L.size();
+    host_keep.assign(bytes, 0);
+    std::vector<size_t> offL(G.size()), offI(G.size());
+    size_t o = 0;
+    for (size_t k = 0; k < G.size(); ++k) {
+        offL[k] = o;
+        memcpy(host_keep.data() + o, G[k].L.data(), sizeof(launch::layer_dev) * G[k].L.size());
+        o += sizeof(launch::layer_dev) * G[k].L.size();
+        offI[k] = o;
+        for (int ph = 0; ph < 2; ++ph) {
+            memcpy(host_keep.data() + o, G[k].I[ph].data(), sizeof(launch::fold_inst) * G[k].I[ph].size());
+            o += sizeof(launch::fold_inst) * G[k].I[ph].size();
+        }
+    }
+    HIPCHK(hipMalloc(buf, bytes ? bytes : 1));
+    uint8_t *d = (uint8_t *)*buf;
+    HIPCHK(hipMemcpyAsync(d, host_keep.data(), bytes, hipMemcpyHostToDevice, gs));
+    for (size_t k = 0; k < G.size(); ++k) {
+        const group &g = G[k];
+        int m = (int)g.L.size();
+        const launch::layer_dev *Dl = (const launch::layer_dev *)(d + offL[k]);
+        for (int ph = 0; ph < 2; ++ph) {
+            const launch::fold_inst *Ip = (const launch::fold_inst *)(d + offI[k]) + ph * m;
+            int nz = ph ? g.nz_max : 0;
+            int f = (ph ? g.f1 : g.f0) ? launch::gather_fold(ph, Dl, Ip, m, g.bl, nz, gs) : 0;
+            if (!f) launch::layer_tables(ph, Dl, m, 1 << g.bl, nz, gs);
+            launch::sumcheck_multi(Ip, m, g.bl, gs, launch::GF_B, f > 0);
+        }
+        launch::general_multi(Dl, m, g.ng_max, gs);
+    }
+    HIPCHK(hipEventRecord(gkr_done, gs));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // the committed codewords, in leaf-launch order: L, H, then every FRI level
 int commit_segments(const vg_ctx *c, launch::leaf_seg *g)
 {
@@ -867,8 +948,14 @@ int commit_segments(const vg_ctx *c, launch::leaf_seg *g)
 // every Merkle tree (merkle_tree.cpp:46-56) level by level, and the query
 // answers.  leaf_done: the leaf chains ran elsewhere (a batch's merged
 // launch) and completion of that event means they are done.
-int enqueue_post(vg_ctx *c, hipEvent_t leaf_done)
+int enqueue_post(vg_ctx *c, hipEvent_t leaf_done, hipEvent_t gkr_done = nullptr)
 {
+    if (gkr_done) {
+        // the sumcheck layers ran in a batch's merged launches
+        HIPCHK(hipStreamWaitEvent(c->st, gkr_done, 0));
+        launch::reduce_slots(c->slots, c->n_slots, c->slot_nb, c->slot_dst, c->msg, c->st);
+        HIPCHK(hipEventRecord(c->ev[2], c->st));
+    }
     hipStream_t st = c->serial ? c->st : c->st2;
     launch::leaf_seg g[24];
     int ns = commit_segments(c, g);
@@ -1205,6 +1292,8 @@ int ctx_init(vg_ctx *c)
             }
         HIPCHK(hipMemcpy(c->d_layers, LD.data(), sizeof(launch::layer_dev) * LD.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->d_inst, FI.data(), sizeof(launch::fold_inst) * FI.size(), hipMemcpyHostToDevice));
+        c->h_layers = LD;
+        c->h_inst = FI;
     }
     std::vector<int> lens(65, (int)c->real);
     lens[64] = (int)c->m;
@@ -1229,8 +1318,9 @@ double ms_since(std::chrono::steady_clock::time_point t0)
 {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
-// host work and device work of a proof up to its leaf chains
-int prove_start(vg_ctx *c)
+// host work and device work of a proof up to its leaf chains (gkr: as
+// enqueue_pre)
+int prove_start(vg_ctx *c, bool gkr = true)
 {
     if (!c) FAILV(-1, "null context");
     if (c->pending) FAILV(-1, "a proof is already in flight on this context");
@@ -1249,7 +1339,7 @@ int prove_start(vg_ctx *c)
     // latency-bound launches overlap the main proof
     rc = fg_enqueue(c->fg);
     if (rc) return rc;
-    return enqueue_pre(c);
+    return enqueue_pre(c, gkr);
 }
 
 // ---------------------------------------------------------------------------
@@ -1786,44 +1876,51 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
             if (rc[k]) { g_err = msg[k]; return rc[k]; }
         return 0;
     };
-    // 1. every proof up to its leaf chains
+    // 1. every proof up to its leaf chains, but for its sumcheck layers
     std::vector<char> started(n, 0);
-    int rc = run([&](int i) { int r = prove_start(ctxs[i]); started[i] = r == 0; return r; });
-    // 2. the leaf chains of every proof in one launch: 60 % of a mixed
-    // batch's VALU work, and one full-width launch keeps every SIMD busy
-    // where many small ones (one per proof) do not
-    std::vector<launch::leaf_seg> G;
-    std::vector<int64_t> start(1, 0);
-    hipStream_t ls = nullptr;
-    hipEvent_t done = nullptr;
+    int rc = run([&](int i) { int r = prove_start(ctxs[i], false); started[i] = r == 0; return r; });
+    std::vector<vg_ctx *> S;
+    for (int i = 0; i < n; ++i)
+        if (started[i]) S.push_back(ctxs[i]);
+    // 2. the sumcheck layers of every proof in merged launches (one per
+    // step and table width across all proofs), and the leaf chains of every
+    // proof in one launch (60 % of a mixed batch's VALU work): full-width
+    // launches keep every SIMD busy where many small ones (one per proof)
+    // do not.  Both run beside each other on two streams.
+    hipStream_t ls = nullptr, gs = nullptr;
+    hipEvent_t done = nullptr, gdone = nullptr;
     launch::leaf_seg *dG = nullptr;
     int64_t *dstart = nullptr;
-    bool merged = false;
-    if (rc == 0) {
-        for (int i = 0; i < n; ++i) {
-            launch::leaf_seg g[24];
-            int ns = commit_segments(ctxs[i], g);
-            for (int k = 0; k < ns; ++k) { G.push_back(g[k]); start.push_back(start.back() + g[k].S / 2); }
-        }
-        bool ok = hipStreamCreateWithFlags(&ls, hipStreamNonBlocking) == hipSuccess &&
-                  hipEventCreateWithFlags(&done, hipEventDisableTiming) == hipSuccess &&
-                  hipMalloc(&dG, sizeof(launch::leaf_seg) * G.size()) == hipSuccess &&
-                  hipMalloc(&dstart, sizeof(int64_t) * start.size()) == hipSuccess;
-        for (int i = 0; ok && i < n; ++i) ok = hipStreamWaitEvent(ls, ctxs[i]->ev[5], 0) == hipSuccess;
-        ok = ok && hipMemcpyAsync(dG, G.data(), sizeof(launch::leaf_seg) * G.size(), hipMemcpyHostToDevice, ls) == hipSuccess &&
-             hipMemcpyAsync(dstart, start.data(), sizeof(int64_t) * start.size(), hipMemcpyHostToDevice, ls) == hipSuccess;
-        if (ok) {
-            launch::leaf_hash_table(dG, dstart, (int)G.size(), start.back(), ls);
-            ok = hipEventRecord(done, ls) == hipSuccess && hipGetLastError() == hipSuccess;
-        }
-        merged = ok;
-        if (!ok) { g_err = "vg_prove_batch: merged leaf launch failed"; rc = -20; }
+    void *gbuf = nullptr;
+    std::vector<uint8_t> gkeep;
+    std::vector<launch::leaf_seg> G;
+    std::vector<int64_t> start(1, 0);
+    for (vg_ctx *c : S) {
+        launch::leaf_seg g[24];
+        int ns = commit_segments(c, g);
+        for (int k = 0; k < ns; ++k) { G.push_back(g[k]); start.push_back(start.back() + g[k].S / 2); }
     }
-    // 3. the rest of every started proof (its own leaf launch when the merged
-    // one did not happen), then the transcripts
+    bool ok = hipStreamCreateWithFlags(&ls, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&gs, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&done, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&gdone, hipEventDisableTiming) == hipSuccess &&
+              hipMalloc(&dG, sizeof(launch::leaf_seg) * (G.size() + 1)) == hipSuccess &&
+              hipMalloc(&dstart, sizeof(int64_t) * start.size()) == hipSuccess;
+    std::string gerr;
+    if (ok && batch_gkr(S.data(), (int)S.size(), gs, gdone, &gbuf, gkeep) != 0) { ok = false; gerr = g_err; }
+    for (size_t i = 0; ok && i < S.size(); ++i) ok = hipStreamWaitEvent(ls, S[i]->ev[5], 0) == hipSuccess;
+    ok = ok && hipMemcpyAsync(dG, G.data(), sizeof(launch::leaf_seg) * G.size(), hipMemcpyHostToDevice, ls) == hipSuccess &&
+         hipMemcpyAsync(dstart, start.data(), sizeof(int64_t) * start.size(), hipMemcpyHostToDevice, ls) == hipSuccess;
+    if (ok) {
+        launch::leaf_hash_table(dG, dstart, (int)G.size(), start.back(), ls);
+        ok = hipEventRecord(done, ls) == hipSuccess && hipGetLastError() == hipSuccess;
+    }
+    if (!ok && rc == 0) { g_err = gerr.empty() ? std::string("vg_prove_batch: merged launches failed") : gerr; rc = -20; }
+    // 3. the rest of every started proof, then the transcripts (a failed
+    // merge still drains every proof in flight; its transcripts are dropped)
     int rc2 = run([&](int i) {
         if (!started[i]) return 0;
-        int r = enqueue_post(ctxs[i], merged ? done : nullptr);
+        int r = ok ? enqueue_post(ctxs[i], done, gdone) : 0;
         ctxs[i]->pending = true;
         return r;
     });
@@ -1832,10 +1929,14 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
         return vg_finish(ctxs[i], &transcripts[i], &lens[i], nullptr);
     });
     if (ls) (void)hipStreamSynchronize(ls);
+    if (gs) (void)hipStreamSynchronize(gs);
     if (dG) (void)hipFree(dG);
     if (dstart) (void)hipFree(dstart);
+    if (gbuf) (void)hipFree(gbuf);
     if (done) (void)hipEventDestroy(done);
+    if (gdone) (void)hipEventDestroy(gdone);
     if (ls) (void)hipStreamDestroy(ls);
+    if (gs) (void)hipStreamDestroy(gs);
     int first = rc ? rc : rc2 ? rc2 : rc3;
     if (first)
         for (int i = 0; i < n; ++i) { free(transcripts[i]); transcripts[i] = nullptr; lens[i] = 0; }

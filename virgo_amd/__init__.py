@@ -12,10 +12,6 @@ import numpy as np
 
 from . import workloads  # noqa: F401
 
-# A proof runs on six streams (evaluation/queries, commitment, four sumcheck
-# layer streams); give the process that many hardware queues before the HIP
-# runtime starts (HIP's default is 4).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvirgo_hip.so")

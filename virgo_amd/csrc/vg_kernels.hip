@@ -1112,7 +1112,7 @@ struct tree_set {
     int64_t nleaf[24];
 };
 // Merkle levels with at most MK_TAIL nodes run in k_merkle_tails
-constexpr int MK_TAIL = 1024;
+constexpr int MK_TAIL = launch::MERKLE_TAIL;
 // level h (h >= 1) of every tree whose level h has more than MK_TAIL nodes:
 // nodes [nleaf>>h, 2*(nleaf>>h))
 __global__ __launch_bounds__(TPB) void k_merkle_multi(tree_set T, int h, int64_t total)
@@ -1134,13 +1134,31 @@ __global__ __launch_bounds__(TPB) void k_merkle_multi(tree_set T, int h, int64_t
     sha3_64(m, hh);
     store_digest(T.tree[sg], k, hh);
 }
-// the remaining levels of tree blockIdx.x (every level with <= MK_TAIL
-// nodes) in one block: the first from the tree in memory, the rest from LDS
-__global__ __launch_bounds__(256) void k_merkle_tails(tree_set T)   // 256 threads
+// level h of many trees (a batch of proofs): item k covers the nodes
+// [first, 2 first) of its tree, start = prefix sum of the node counts
+__global__ __launch_bounds__(TPB) void k_merkle_level_table(const launch::mk_item *L, int n, int64_t total)
+{
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= total) return;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (L[mid].start <= tid) lo = mid; else hi = mid - 1;
+    }
+    int64_t k = L[lo].first + (tid - L[lo].start);
+    uint8_t *tree = L[lo].tree;
+    const uint64_t *c = (const uint64_t *)(tree + 2 * k * 32);
+    uint64_t m[8], hh[4];
+    for (int q = 0; q < 8; ++q) m[q] = c[q];
+    sha3_64(m, hh);
+    store_digest(tree, k, hh);
+}
+// the remaining levels of one tree (every level with <= MK_TAIL nodes) in
+// one block of 256 threads: the first from the tree in memory, the rest
+// from LDS
+__device__ __forceinline__ void merkle_tail(uint8_t *tree, int64_t nl)
 {
     __shared__ uint64_t dg[MK_TAIL][4];
-    uint8_t *tree = T.tree[blockIdx.x];
-    int64_t nl = T.nleaf[blockIdx.x];
     int h = 1;
     while ((nl >> h) > MK_TAIL) ++h;
     int cnt = (int)(nl >> h);
@@ -1170,6 +1188,14 @@ __global__ __launch_bounds__(256) void k_merkle_tails(tree_set T)   // 256 threa
             for (int q = 0; q < 4; ++q) dg[i][q] = hh[c][q];
         __syncthreads();
     }
+}
+__global__ __launch_bounds__(256) void k_merkle_tails(tree_set T)
+{
+    merkle_tail(T.tree[blockIdx.x], T.nleaf[blockIdx.x]);
+}
+__global__ __launch_bounds__(256) void k_merkle_tails_table(const launch::leaf_seg *G)
+{
+    merkle_tail(G[blockIdx.x].tree, G[blockIdx.x].S / 2);
 }
 __global__ void k_gather16(const uint64_t *src, int64_t n, fe *out)
 {
@@ -1592,6 +1618,14 @@ void leaf_hash_multi(int nseg, const fe *const *cw, const int64_t *S, uint8_t *c
 void leaf_hash_table(const leaf_seg *G, const int64_t *start, int nseg, int64_t total, hipStream_t s)
 {
     if (total) hipLaunchKernelGGL(k_leaf_table, dim3(nblk(total)), dim3(TPB), 0, s, G, start, nseg);
+}
+void merkle_level_table(const mk_item *items, int n, int64_t total, hipStream_t s)
+{
+    if (total) hipLaunchKernelGGL(k_merkle_level_table, dim3(nblk(total)), dim3(TPB), 0, s, items, n, total);
+}
+void merkle_tails_table(const leaf_seg *G, int ntree, hipStream_t s)
+{
+    if (ntree) hipLaunchKernelGGL(k_merkle_tails_table, dim3(ntree), dim3(256), 0, s, G);
 }
 void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStream_t s)
 {

@@ -198,6 +198,17 @@ struct leaf_seg {
     int log_R;
 };
 void leaf_hash_table(const leaf_seg *G, const int64_t *start, int nseg, int64_t total, hipStream_t s);
+// Merkle trees of many proofs: levels with more than MERKLE_TAIL nodes level
+// by level from item tables (item: a tree's nodes [first, 2 first), start =
+// prefix sum of the node counts of the level), the rest one block per tree
+// G[k] (nleaf = S / 2)
+constexpr int MERKLE_TAIL = 1024;
+struct mk_item {
+    uint8_t *tree;
+    int64_t first, start;
+};
+void merkle_level_table(const mk_item *items, int n, int64_t total, hipStream_t s);
+void merkle_tails_table(const leaf_seg *G, int ntree, hipStream_t s);
 // every Merkle tree of the set, level by level across trees
 void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStream_t s);
 // out[k] = 16 bytes at address src[k]

@@ -946,8 +946,8 @@ int commit_segments(const vg_ctx *c, launch::leaf_seg *g)
 
 // ---- every SHA3 leaf chain (fri.cpp:97-126, 397-420) in one launch, then
 // every Merkle tree (merkle_tree.cpp:46-56) level by level, and the query
-// answers.  leaf_done: the leaf chains ran elsewhere (a batch's merged
-// launch) and completion of that event means they are done.
+// answers.  leaf_done: the leaf chains and Merkle trees ran elsewhere (a
+// batch's merged launches) and completion of that event means they are done.
 int enqueue_post(vg_ctx *c, hipEvent_t leaf_done, hipEvent_t gkr_done = nullptr)
 {
     if (gkr_done) {
@@ -979,7 +979,7 @@ int enqueue_post(vg_ctx *c, hipEvent_t leaf_done, hipEvent_t gkr_done = nullptr)
         launch::leaf_hash_multi(ns, cws, Ss, trees, c->log_R, st);
     }
     HIPCHK(hipEventRecord(c->ev_leaf[1], st));
-    launch::merkle_multi(ns, trees, nl, st);
+    if (!leaf_done) launch::merkle_multi(ns, trees, nl, st);
     HIPCHK(hipEventRecord(c->ev[6], st));
     st = c->st;
     if (!c->serial) HIPCHK(hipStreamWaitEvent(st, c->ev[6], 0));
@@ -1911,8 +1911,33 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
     for (size_t i = 0; ok && i < S.size(); ++i) ok = hipStreamWaitEvent(ls, S[i]->ev[5], 0) == hipSuccess;
     ok = ok && hipMemcpyAsync(dG, G.data(), sizeof(launch::leaf_seg) * G.size(), hipMemcpyHostToDevice, ls) == hipSuccess &&
          hipMemcpyAsync(dstart, start.data(), sizeof(int64_t) * start.size(), hipMemcpyHostToDevice, ls) == hipSuccess;
+    // every Merkle tree of every proof: the large levels from item tables,
+    // then the tails one block per tree
+    std::vector<launch::mk_item> items;
+    std::vector<size_t> lvl_off;
+    std::vector<int64_t> lvl_tot;
+    for (int h = 1;; ++h) {
+        int64_t tot = 0;
+        size_t o = items.size();
+        for (auto &g : G) {
+            int64_t cnt = (g.S / 2) >> h;
+            if (cnt > launch::MERKLE_TAIL) { items.push_back({g.tree, cnt, tot}); tot += cnt; }
+        }
+        if (!tot) break;
+        lvl_off.push_back(o);
+        lvl_tot.push_back(tot);
+    }
+    launch::mk_item *dI = nullptr;
+    ok = ok && hipMalloc(&dI, sizeof(launch::mk_item) * (items.size() + 1)) == hipSuccess;
+    ok = ok && (items.empty() ||
+                hipMemcpyAsync(dI, items.data(), sizeof(launch::mk_item) * items.size(), hipMemcpyHostToDevice, ls) == hipSuccess);
     if (ok) {
         launch::leaf_hash_table(dG, dstart, (int)G.size(), start.back(), ls);
+        for (size_t l = 0; l < lvl_off.size(); ++l) {
+            size_t end = l + 1 < lvl_off.size() ? lvl_off[l + 1] : items.size();
+            launch::merkle_level_table(dI + lvl_off[l], (int)(end - lvl_off[l]), lvl_tot[l], ls);
+        }
+        launch::merkle_tails_table(dG, (int)G.size(), ls);
         ok = hipEventRecord(done, ls) == hipSuccess && hipGetLastError() == hipSuccess;
     }
     if (!ok && rc == 0) { g_err = gerr.empty() ? std::string("vg_prove_batch: merged launches failed") : gerr; rc = -20; }
@@ -1933,6 +1958,7 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
     if (dG) (void)hipFree(dG);
     if (dstart) (void)hipFree(dstart);
     if (gbuf) (void)hipFree(gbuf);
+    if (dI) (void)hipFree(dI);
     if (done) (void)hipEventDestroy(done);
     if (gdone) (void)hipEventDestroy(gdone);
     if (ls) (void)hipStreamDestroy(ls);

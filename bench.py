@@ -42,6 +42,11 @@ def parse():
     ap.add_argument("--batch-workers", type=int, default=16)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=96)
+    ap.add_argument("--serve-rate", type=float, default=60.0, help="offered requests per second (Poisson)")
+    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-budget-gb", type=int, default=96)
+    ap.add_argument("--no-serve", action="store_true")
     return ap.parse_args()
 
 
@@ -116,6 +121,54 @@ def batched(args):
             "setup_s": round(setup, 1)}
 
 
+def serving(args):
+    """Continuous batching under a Poisson stream of requests
+    (workloads.serve_jobs: log2 width U[10,20], depth U[8,24]) at a fixed
+    offered rate, through the library's proof server: each request is
+    admitted as a slot and its lease of the HBM budget free up, with every
+    admitted proof's state resident on the device until its transcript is on
+    the host.  Latency = submit to transcript, measured in the library.  Each
+    transcript is compared with the same circuit proved alone and checked by
+    the device verifier."""
+    import numpy as np
+    import virgo_amd as V
+    from virgo_amd import workloads as W
+    jobs = W.serve_jobs(args.serve_requests)
+    srv = V.Server(device_budget=args.serve_budget_gb << 30, slots=args.serve_slots)
+    for _, c in jobs[:4]:                                # warm-up
+        srv.wait(srv.submit(c))
+    gaps = np.random.default_rng(13).exponential(1.0 / args.serve_rate, len(jobs))
+    t0 = time.perf_counter()
+    arrive = t0 + np.cumsum(gaps)
+    ids, t_sub = [], []
+    for k, (_, c) in enumerate(jobs):
+        delay = arrive[k] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        t_sub.append(time.perf_counter())
+        ids.append(srv.submit(c))
+    res = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    lat = np.array([t["latency"] for _, t in res])
+    t_end = max(ts + t["latency"] * 1e-3 for ts, (_, t) in zip(t_sub, res))
+    same, accepted = True, True
+    for (_, c), (tr, _) in zip(jobs, res):
+        p = V.Prover(c)
+        same &= p.prove() == tr
+        accepted &= p.verify(tr)[0]
+        p.close()
+    mean = lambda k: round(float(np.mean([t[k] for _, t in res])), 2)  # noqa: E731
+    return {"workload": "%d requests: random layered log2 width U[10,20] x depth U[8,24], seed 11; Poisson "
+                        "arrivals, seed 13" % len(jobs),
+            "offered_per_s": args.serve_rate, "proofs_per_s": round(len(jobs) / (t_end - t0), 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "mean_ms": round(float(lat.mean()), 2), "mean_queued_ms": mean("queued"), "mean_plan_ms": mean("plan"),
+            "mean_prove_ms": mean("prove"), "slots": args.serve_slots, "budget_gb": args.serve_budget_gb,
+            "peak_admitted": st["peak_admitted"], "peak_device_gb": round(st["device_peak"] / 1e9, 1),
+            "bit_exact_vs_alone": bool(same), "all_accepted": bool(accepted)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -183,6 +236,8 @@ def main():
     p.close()
     if not args.no_batch:
         line["batched"] = batched(args)
+    if not args.no_serve:
+        line["serving"] = serving(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args)
         if not args.no_verify:

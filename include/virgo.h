@@ -94,6 +94,41 @@ int vg_finish(vg_ctx *ctx, uint8_t **transcript, size_t *len, double *stage_ms);
  * first error (its message in vg_last_error); proofs not made are NULL. */
 int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcripts, size_t *lens);
 
+/* ---- proof server: continuous batching (no counterpart in the reference,
+ * which proves one circuit per process, main_zk.cpp:11-25) ----
+ * Requests of any size are submitted at any time and admitted while others
+ * run.  The server reserves device_budget bytes of HBM and pinned_budget
+ * bytes of page-locked host memory (<= 0: 1 GiB) once; every admitted proof
+ * takes its whole working set from them as one lease, keeps it resident on
+ * the device until its transcript is on the host, and gives it back, so no
+ * device memory is allocated or freed while proofs run.  `slots` worker
+ * threads (1..64), each on its own stream, take requests in arrival order;
+ * requests waiting for memory are admitted in arrival order as soon as their
+ * lease fits.  Every transcript is byte-identical to vg_prove of the same
+ * circuit alone.  A request larger than the budgets fails (its wait returns
+ * the error); the others are unaffected. */
+typedef struct vg_server vg_server;
+typedef struct {
+    int64_t submitted, completed, failed;   /* requests so far (failed: among completed) */
+    int64_t queued, running;                /* waiting for a slot; in a slot (planning, waiting for memory or proving) */
+    int64_t device_in_use, device_peak;     /* bytes of the device budget leased now / at most */
+    int64_t peak_admitted;                  /* most proofs holding a lease at once */
+    int64_t device_budget;
+} vg_server_stats_t;
+int vg_server_create(vg_server **server, int64_t device_budget, int64_t pinned_budget, int slots);
+/* Queues a proof of circuit c; *id names it.  The arrays c points to must
+ * stay valid until vg_server_wait returns for this id. */
+int vg_server_submit(vg_server *server, const vg_circuit_desc *c, int64_t *id);
+/* Waits for request id and hands over its transcript (free with vg_free).
+ * times_ms (may be NULL) receives 5 values: latency (submit to transcript),
+ * time queued for a slot, host planning + waiting for memory, upload + proof,
+ * and the device bytes of its lease.  Each id is collected once. */
+int vg_server_wait(vg_server *server, int64_t id, uint8_t **transcript, size_t *len, double *times_ms);
+int vg_server_stats(vg_server *server, vg_server_stats_t *out);
+/* Finishes every queued request, then releases the server (results not
+ * collected are dropped). */
+int vg_server_destroy(vg_server *server);
+
 /* Output layer of the last proof (2^bits elements). */
 int vg_output(vg_ctx *ctx, vg_fe *out, size_t cap, int *bits);
 

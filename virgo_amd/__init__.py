@@ -20,7 +20,8 @@ FE = np.dtype([("img", "<u8"), ("real", "<u8")])     # prime_field.h:93 layout
 EXPORTS = ["vg_init", "vg_last_error", "vg_free", "vg_ctx_create", "vg_ctx_destroy", "vg_ctx_device_bytes",
            "vg_prove", "vg_prove_async", "vg_finish", "vg_output", "vg_fft", "vg_ifft", "vg_sha3_64",
            "vg_merkle_tree", "vg_fe_mul", "vg_fft_gkr", "vg_stream_create", "vg_stream_destroy", "vg_prove_batch",
-           "vg_verify"]
+           "vg_verify", "vg_server_create", "vg_server_submit", "vg_server_wait", "vg_server_stats",
+           "vg_server_destroy"]
 
 
 class VirgoError(RuntimeError):
@@ -64,6 +65,11 @@ def lib():
         L.vg_prove_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.vg_stream_create.argtypes = [ctypes.POINTER(vp)]
         L.vg_stream_destroy.argtypes = [vp]
+        L.vg_server_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
+        L.vg_server_submit.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64)]
+        L.vg_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t), vp]
+        L.vg_server_stats.argtypes = [vp, vp]
+        L.vg_server_destroy.argtypes = [vp]
         L.vg_fft_gkr.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t),
                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                  ctypes.POINTER(ctypes.c_double)]
@@ -175,6 +181,67 @@ class Prover:
         if self._ctx:
             lib().vg_ctx_destroy(self._ctx)
             self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "failed", "queued", "running",
+                                               "device_in_use", "device_peak", "peak_admitted", "device_budget")]
+
+
+class Server:
+    """Continuous-batching proof server (vg_server_*): requests of any size
+    are submitted at any time; each is admitted when a slot and its lease of
+    the device budget are free, and proves beside the others on the device.
+    submit() returns an id; wait(id) returns (transcript, times) with times
+    in ms: latency (submit to transcript), queued (for a slot), plan (host
+    planning + waiting for memory), prove (upload + device proof), and the
+    device bytes of the request's lease."""
+
+    TIMES = ("latency", "queued", "plan", "prove", "device_bytes")
+
+    def __init__(self, device_budget, slots=16, pinned_budget=0):
+        init()
+        self._s = ctypes.c_void_p()
+        self._keep = {}
+        _check(lib().vg_server_create(ctypes.byref(self._s), int(device_budget), int(pinned_budget), int(slots)))
+
+    def submit(self, circ):
+        keep = tuple(np.ascontiguousarray(circ[k], dtype=t) for k, t in
+                     (("n", np.int64), ("ty", np.int32), ("u", np.int64), ("v", np.int64)))
+        n, ty, u, v = keep
+        d = _Desc(len(n), _p(n), _p(ty), _p(u), _p(v))
+        jid = ctypes.c_int64()
+        _check(lib().vg_server_submit(self._s, ctypes.byref(d), ctypes.byref(jid)))
+        self._keep[jid.value] = keep        # the arrays stay alive until wait()
+        return jid.value
+
+    def wait(self, jid):
+        out = ctypes.c_void_p()
+        ln = ctypes.c_size_t()
+        t = np.zeros(5, dtype=np.float64)
+        rc = lib().vg_server_wait(self._s, jid, ctypes.byref(out), ctypes.byref(ln), _p(t))
+        self._keep.pop(jid, None)
+        _check(rc)
+        data = ctypes.string_at(out, ln.value)
+        lib().vg_free(out)
+        return data, dict(zip(self.TIMES, t.tolist()))
+
+    def stats(self):
+        st = _ServerStats()
+        _check(lib().vg_server_stats(self._s, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self._s:
+            lib().vg_server_destroy(self._s)
+            self._s = ctypes.c_void_p()
+            self._keep.clear()
 
     def __del__(self):
         try:

@@ -45,6 +45,7 @@ struct fg_ctx {
     int64_t N = 0;
     hipStream_t st = nullptr;
     bool own_stream = false;
+    mem_source *mem = nullptr;       // where arena, h_chal and h_msg came from
     uint8_t *arena = nullptr;
     int64_t bytes = 0;
     // challenges: [0] one, [1] zero, r[lg], pts[64] + inv_rou, r0, r1, layers
@@ -327,7 +328,7 @@ bool check(const fg_ctx *c, const fe *M)
 
 }  // namespace
 
-int fg_create(fg_ctx **out, int lg, hipStream_t stream)
+int fg_plan(fg_ctx **out, int lg, hipStream_t stream, int64_t *dev_bytes, int64_t *pin_bytes)
 {
     *out = nullptr;
     if (lg < 1 || lg > 20) FG_FAIL(-1, "fft_gkr: lg = %d out of range [1, 20]", lg);
@@ -343,15 +344,23 @@ int fg_create(fg_ctx **out, int lg, hipStream_t stream)
         if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; FG_FAIL(-20, "stream"); }
         c->own_stream = true;
     }
-    int64_t bytes = layout(c, nullptr);
-    if (hipMalloc(&c->arena, bytes) != hipSuccess) { fg_destroy(c); FG_FAIL(-21, "fft_gkr: cannot allocate %lld bytes", (long long)bytes); }
-    c->bytes = bytes;
+    c->bytes = layout(c, nullptr);
+    if (dev_bytes) *dev_bytes = c->bytes;
+    if (pin_bytes) *pin_bytes = (int64_t)sizeof(fe) * (c->n_chal + c->msg_len) + 512;
+    *out = c;
+    return 0;
+}
+
+int fg_alloc(fg_ctx *c, mem_source *mem)
+{
+    int lg = c->lg;
+    c->mem = mem;
+    c->arena = (uint8_t *)mem->device(c->bytes);
+    if (!c->arena) FG_FAIL(-21, "fft_gkr: cannot allocate %lld bytes", (long long)c->bytes);
     layout(c, c->arena);
-    if (hipHostMalloc(&c->h_chal, sizeof(fe) * c->n_chal, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&c->h_msg, sizeof(fe) * c->msg_len, hipHostMallocDefault) != hipSuccess) {
-        fg_destroy(c);
-        FG_FAIL(-21, "fft_gkr: pinned host allocation failed");
-    }
+    c->h_chal = (fe *)mem->pinned(sizeof(fe) * c->n_chal);
+    c->h_msg = (fe *)mem->pinned(sizeof(fe) * c->msg_len);
+    if (!c->h_chal || !c->h_msg) FG_FAIL(-21, "fft_gkr: pinned host allocation failed");
     // resolve the half-table descriptors and upload the static tables
     std::vector<launch::half_desc> d = c->h_hdesc;
     for (auto &x : d) {
@@ -395,21 +404,29 @@ int fg_create(fg_ctx **out, int lg, hipStream_t stream)
     if (ok && !c->h_slot_nb.empty())
         ok = hipMemcpy(c->slot_nb, c->h_slot_nb.data(), sizeof(int) * c->h_slot_nb.size(), hipMemcpyHostToDevice) == hipSuccess &&
              hipMemcpy(c->slot_dst, c->h_slot_dst.data(), sizeof(int64_t) * c->h_slot_dst.size(), hipMemcpyHostToDevice) == hipSuccess;
-    if (!ok || hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
-        fg_destroy(c);
+    if (!ok || hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess)
         FG_FAIL(-20, "fft_gkr: setup copies failed");
-    }
-    *out = c;
     return 0;
+}
+
+int fg_create(fg_ctx **out, int lg, hipStream_t stream)
+{
+    int rc = fg_plan(out, lg, stream, nullptr, nullptr);
+    if (rc) return rc;
+    rc = fg_alloc(*out, hip_mem());
+    if (rc) { fg_destroy(*out); *out = nullptr; }
+    return rc;
 }
 
 void fg_destroy(fg_ctx *c)
 {
     if (!c) return;
     if (c->st) (void)hipStreamSynchronize(c->st);
-    if (c->arena) (void)hipFree(c->arena);
-    if (c->h_chal) (void)hipHostFree(c->h_chal);
-    if (c->h_msg) (void)hipHostFree(c->h_msg);
+    if (c->mem) {
+        if (c->arena) c->mem->device_free(c->arena);
+        if (c->h_chal) c->mem->pinned_free(c->h_chal);
+        if (c->h_msg) c->mem->pinned_free(c->h_msg);
+    }
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);

@@ -14,6 +14,12 @@ struct fg_ctx;
 // Allocates every device buffer of one fft_gkr proof with 2^lg coefficients;
 // work runs on `stream` (nullptr: a stream owned by the context).
 int fg_create(fg_ctx **out, int lg, hipStream_t stream);
+// The same in two steps: fg_plan lays the proof out and reports the device
+// and pinned host bytes it needs (nothing allocated); fg_alloc takes them
+// from `mem` and uploads the static tables.  fg_destroy after either.
+struct mem_source;
+int fg_plan(fg_ctx **out, int lg, hipStream_t stream, int64_t *dev_bytes, int64_t *pin_bytes);
+int fg_alloc(fg_ctx *c, mem_source *mem);
 void fg_destroy(fg_ctx *c);
 int64_t fg_device_bytes(const fg_ctx *c);
 // Draws the challenge stream and enqueues the whole proof (upload, circuit,

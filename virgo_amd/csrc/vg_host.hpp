@@ -44,6 +44,23 @@ struct glibc_rand {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Where a proof context takes its device arena and pinned host staging from:
+// hipMalloc / hipHostMalloc for a context of its own (hip_mem()), or a lease
+// carved from a region a proof server reserved once (no allocation, and no
+// hipFree / hipHostFree, which would wait for the whole device, while other
+// proofs run).  A context asks for its memory in a few large blocks at
+// creation and frees them at destruction.
+// ---------------------------------------------------------------------------
+struct mem_source {
+    virtual ~mem_source() {}
+    virtual void *device(int64_t bytes) = 0;     // nullptr: no memory
+    virtual void device_free(void *p) = 0;
+    virtual void *pinned(int64_t bytes) = 0;     // page-locked host memory
+    virtual void pinned_free(void *p) = 0;
+};
+mem_source *hip_mem();
+
 // transcript bytes: F = 16 bytes {img, real}, little-endian u32
 struct tbuf {
     std::vector<uint8_t> b;

@@ -20,6 +20,7 @@
 #include "vg_kernels.hpp"
 #include "vg_host.hpp"
 #include "vg_fftgkr.hpp"
+#include "vg_internal.hpp"
 #include "../../include/virgo.h"
 
 using namespace vg;
@@ -426,6 +427,9 @@ struct vg_ctx {
     bool own_stream = false;
     uint8_t *arena = nullptr;
     int64_t arena_bytes = 0;
+    int64_t pin_bytes = 0, gout_len = 0;     // pinned staging: chal, gsrc, gout, msg
+    int64_t fg_bytes = 0, fg_pin = 0;        // the fft_gkr context's
+    mem_source *mem = nullptr;               // where arena and staging come from
     // device views
     std::vector<uint8_t *> d_ty;
     std::vector<int32_t *> d_u, d_v;
@@ -1090,7 +1094,9 @@ void build_csr(vg_ctx *c, int i)
     }
 }
 
-int ctx_init(vg_ctx *c)
+// Host planning of a proof: sizes, challenge and message layout, layer
+// batches, CSR transposes and the arena layout (nothing allocated).
+int ctx_plan(vg_ctx *c)
 {
     host_circuit &C = c->C;
     int n = c->n = C.bl[0];
@@ -1185,14 +1191,27 @@ int ctx_init(vg_ctx *c)
                     if (o[t0 + tile < nt ? t0 + tile : nt] - o[t0] > launch::GF_MAX_FANIN * tile) b.fused[ph] = false;
             }
     }
-    int64_t bytes = layout(c, nullptr);
-    HIPCHK(hipMalloc(&c->arena, bytes));
-    c->arena_bytes = bytes;
+    c->arena_bytes = layout(c, nullptr);
+    c->gout_len = c->gather_cap > (1024 + (c->SS >> 10) + 1) ? c->gather_cap : (1024 + (c->SS >> 10) + 1);
+    c->pin_bytes = (int64_t)sizeof(fe) * (c->n_chal + c->gout_len + c->msg_len) + (int64_t)sizeof(uint64_t) * c->gather_cap + 1024;
+    return 0;
+}
+
+// Takes the planned arena and pinned staging from c->mem and uploads the
+// circuit and the static tables.  Work runs on the context's stream only
+// (other proofs may be running on the device).
+int ctx_alloc(vg_ctx *c)
+{
+    host_circuit &C = c->C;
+    int n = c->n;
+    c->arena = (uint8_t *)c->mem->device(c->arena_bytes);
+    if (!c->arena) FAILV(-21, "cannot allocate %lld device bytes", (long long)c->arena_bytes);
     layout(c, c->arena);
-    HIPCHK(hipHostMalloc(&c->h_chal, sizeof(fe) * c->n_chal, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc(&c->h_gsrc, sizeof(uint64_t) * c->gather_cap, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc(&c->h_gout, sizeof(fe) * (c->gather_cap > (1024 + (c->SS >> 10) + 1) ? c->gather_cap : (1024 + (c->SS >> 10) + 1)), hipHostMallocDefault));
-    HIPCHK(hipHostMalloc(&c->h_msg, sizeof(fe) * c->msg_len, hipHostMallocDefault));
+    c->h_chal = (fe *)c->mem->pinned(sizeof(fe) * c->n_chal);
+    c->h_gsrc = (uint64_t *)c->mem->pinned(sizeof(uint64_t) * c->gather_cap);
+    c->h_gout = (fe *)c->mem->pinned(sizeof(fe) * c->gout_len);
+    c->h_msg = (fe *)c->mem->pinned(sizeof(fe) * c->msg_len);
+    if (!c->h_chal || !c->h_gsrc || !c->h_gout || !c->h_msg) FAILV(-21, "pinned host allocation failed");
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
     for (auto &e : c->ev_leaf) HIPCHK(hipEventCreate(&e));
     // gates and inputs
@@ -1307,10 +1326,10 @@ int ctx_init(vg_ctx *c)
         fe w10 = x, y = fone();
         for (int64_t j = 0; j < (SS >> 10) + 1; ++j) { T12[1024 + j] = y; y = mul(y, w10); }
         HIPCHK(hipMemcpy(c->T12, T12.data(), sizeof(fe) * T12.size(), hipMemcpyHostToDevice));
-        launch::twiddles(c->tw, SS, c->T12, c->T12 + 1024, 0);
-        launch::ntt_tables(c->ntW, c->ntWi, c->log_C + 1, c->ntTA, c->ntTB, c->log_R, c->NT.th, c->NT.tb, c->tw, SS, 0);
+        launch::twiddles(c->tw, SS, c->T12, c->T12 + 1024, c->st);
+        launch::ntt_tables(c->ntW, c->ntWi, c->log_C + 1, c->ntTA, c->ntTB, c->log_R, c->NT.th, c->NT.tb, c->tw, SS, c->st);
     }
-    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipStreamSynchronize(c->st));
     return 0;
 }
 
@@ -1695,6 +1714,30 @@ int verify(vg_ctx *c, const uint8_t *tr, size_t len, int *accepted)
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+namespace vg {
+namespace {
+struct hip_source : mem_source {
+    void *device(int64_t bytes) override
+    {
+        void *p = nullptr;
+        return hipMalloc(&p, bytes ? bytes : 1) == hipSuccess ? p : nullptr;
+    }
+    void device_free(void *p) override { (void)hipFree(p); }
+    void *pinned(int64_t bytes) override
+    {
+        void *p = nullptr;
+        return hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) == hipSuccess ? p : nullptr;
+    }
+    void pinned_free(void *p) override { (void)hipHostFree(p); }
+};
+}  // namespace
+mem_source *hip_mem()
+{
+    static hip_source s;
+    return &s;
+}
+}  // namespace vg
+
 extern "C" {
 
 int vg_init(int device)
@@ -1718,9 +1761,15 @@ int vg_stream_destroy(void *stream)
 const char *vg_last_error(void) { return g_err.c_str(); }
 void vg_free(void *p) { free(p); }
 
-int vg_ctx_create(vg_ctx **out, const vg_circuit_desc *d, void *stream)
+}  // extern "C"
+
+// Context creation in two steps, so that a proof server can reserve the
+// memory before anything is allocated: ctx_create_planned builds the circuit
+// and plans the proof (host work only); ctx_materialize takes the device
+// arena and pinned staging from `mem` (nullptr: hipMalloc / hipHostMalloc)
+// and uploads.  Both destroy the context on failure.
+int ctx_create_planned(vg_ctx **out, const vg_circuit_desc *d, void *stream)
 {
-    if (!out || !d) FAILV(-1, "null argument");
     vg_ctx *c = new vg_ctx();
     int rc = build_circuit(d, c->C);
     if (rc) { delete c; return rc; }
@@ -1756,17 +1805,46 @@ int vg_ctx_create(vg_ctx **out, const vg_circuit_desc *d, void *stream)
         for (int w = 0; w < vg_ctx::NGS && e == hipSuccess; ++w) e = hipEventCreateWithFlags(&c->gev[w], hipEventDisableTiming);
         if (e != hipSuccess) { vg_ctx_destroy(c); FAILV(-20, "hipStreamCreate: %s", hipGetErrorString(e)); }
     }
-    rc = ctx_init(c);
+    rc = ctx_plan(c);
     if (rc) { vg_ctx_destroy(c); return rc; }
     // fft_gkr depends on nothing of the main proof: a context that owns its
     // streams gives it one of its own even when the main proof runs in
     // order (its latency-bound launches then hide beside the main proof's
     // large ones: 12.4 vs 13.0 ms at 2^20 gates x 16 layers); on a caller's
     // stream everything stays in order
-    rc = fg_create(&c->fg, c->n - 6, c->own_stream ? nullptr : c->st);
+    rc = fg_plan(&c->fg, c->n - 6, c->own_stream ? nullptr : c->st, &c->fg_bytes, &c->fg_pin);
     if (rc) { vg_ctx_destroy(c); return rc; }
     *out = c;
     return 0;
+}
+
+// device and pinned host bytes a planned context will take from its source
+void ctx_needs(const vg_ctx *c, int64_t *dev, int64_t *pin)
+{
+    *dev = ((c->arena_bytes + 255) & ~(int64_t)255) + ((c->fg_bytes + 255) & ~(int64_t)255);
+    *pin = c->pin_bytes + c->fg_pin;
+}
+
+int ctx_materialize(vg_ctx *c, mem_source *mem, vg_ctx **out)
+{
+    c->mem = mem ? mem : hip_mem();
+    int rc = ctx_alloc(c);
+    if (!rc) rc = fg_alloc(c->fg, c->mem);
+    if (rc) { vg_ctx_destroy(c); return rc; }
+    *out = c;
+    return 0;
+}
+
+extern "C" {
+
+int vg_ctx_create(vg_ctx **out, const vg_circuit_desc *d, void *stream)
+{
+    if (!out || !d) FAILV(-1, "null argument");
+    *out = nullptr;
+    vg_ctx *c = nullptr;
+    int rc = ctx_create_planned(&c, d, stream);
+    if (rc) return rc;
+    return ctx_materialize(c, nullptr, out);
 }
 
 int vg_ctx_destroy(vg_ctx *c)
@@ -1779,11 +1857,13 @@ int vg_ctx_destroy(vg_ctx *c)
             if (c->gst[w]) (void)hipStreamSynchronize(c->gst[w]);
     }
     fg_destroy(c->fg);
-    if (c->arena) (void)hipFree(c->arena);
-    if (c->h_chal) (void)hipHostFree(c->h_chal);
-    if (c->h_gsrc) (void)hipHostFree(c->h_gsrc);
-    if (c->h_gout) (void)hipHostFree(c->h_gout);
-    if (c->h_msg) (void)hipHostFree(c->h_msg);
+    if (c->mem) {
+        if (c->arena) c->mem->device_free(c->arena);
+        if (c->h_chal) c->mem->pinned_free(c->h_chal);
+        if (c->h_gsrc) c->mem->pinned_free(c->h_gsrc);
+        if (c->h_gout) c->mem->pinned_free(c->h_gout);
+        if (c->h_msg) c->mem->pinned_free(c->h_msg);
+    }
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto &e : c->ev_leaf) if (e) (void)hipEventDestroy(e);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);

@@ -18,10 +18,17 @@ def splitmix64(seed, n, stream=0):
     """n outputs of splitmix64 started at seed ^ (stream * golden^2)."""
     with np.errstate(over="ignore"):
         base = np.uint64(seed) ^ (np.uint64(stream) * np.uint64(0xD1B54A32D192ED03))
-        z = base + (np.arange(1, n + 1, dtype=np.uint64) * GAMMA)
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        return z ^ (z >> np.uint64(31))
+        z = np.arange(1, n + 1, dtype=np.uint64)
+        z *= GAMMA
+        z += base
+        t = np.empty_like(z)
+        for sh, mul in ((30, 0xBF58476D1CE4E5B9), (27, 0x94D049BB133111EB)):
+            np.right_shift(z, np.uint64(sh), out=t)
+            z ^= t
+            z *= np.uint64(mul)
+        np.right_shift(z, np.uint64(31), out=t)
+        z ^= t
+        return z
 
 
 def _pack(layers):
@@ -51,6 +58,13 @@ def matmul_circuit(mat, seed=1):
     return _pack([inp, mult, summ])
 
 
+def _mod(r, m):
+    """r mod m as int64 (a mask when m is a power of two: the same values)."""
+    if m & (m - 1) == 0:
+        return (r & np.uint64(m - 1)).astype(np.int64)
+    return (r % np.uint64(m)).astype(np.int64)
+
+
 def random_circuit(log_width, depth, seed=1, log_inputs=None, v_bits=None):
     """Random layered circuit: `depth` compute layers of 2^log_width gates,
     each gate add or mult with probability 1/2, u and v uniform over the
@@ -65,9 +79,9 @@ def random_circuit(log_width, depth, seed=1, log_inputs=None, v_bits=None):
     for layer in range(depth):
         r = splitmix64(seed, 3 * W, 2 + layer)
         ty = (r[:W] & np.uint64(1)).astype(np.int32)
-        u = (r[W:2 * W] % np.uint64(prev)).astype(np.int64)
         vr = prev if v_bits is None else min(prev, 1 << v_bits)
-        v = (r[2 * W:] % np.uint64(vr)).astype(np.int64)
+        u = _mod(r[W:2 * W], prev)
+        v = _mod(r[2 * W:], vr)
         layers.append((ty, u, v))
         prev = W
     return _pack(layers)
@@ -125,4 +139,17 @@ def batch_jobs(count=256, seed=7):
         else:
             lw, d = int(rng.integers(10, 19)), int(rng.integers(4, 21))
             jobs.append(("random_w%d_d%d_s%d" % (lw, d, k), random_circuit(lw, d, seed=1000 + k)))
+    return jobs
+
+
+def serve_jobs(count=96, seed=11):
+    """The serving workload: `count` proof requests, each a random layered
+    circuit with log2 width uniform in [10, 20] and depth uniform in [8, 24]
+    (add/mult 50/50, uniform wiring, inputs in [0, 2^31)).  Returns a list
+    of (name, circuit) in arrival order."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(count):
+        lw, d = int(rng.integers(10, 21)), int(rng.integers(8, 25))
+        jobs.append(("random_w%d_d%d_s%d" % (lw, d, k), random_circuit(lw, d, seed=5000 + k)))
     return jobs

@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=96)
-    ap.add_argument("--serve-rate", type=float, default=60.0, help="offered requests per second (Poisson)")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="offered requests per second (Poisson)")
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-budget-gb", type=int, default=96)
     ap.add_argument("--no-serve", action="store_true")

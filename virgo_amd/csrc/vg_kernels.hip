@@ -1379,6 +1379,95 @@ __global__ void k_fg_ifft_p2m(const launch::fg_layer_dev *D, const fe *plo, cons
     d.amb[v] = sub(mul(t1, x), mul(t2, x));
     d.aVb[v] = add(mul(t1, vu), mul(t2, vu));
 }
+
+// ---------------------------------------------------------------------
+// CSR transposes of one gate layer, built on the device when a context is
+// created (the structure of zk_prover.cpp:420-587 and :845-972): phase 1
+// lists, for every entry x of the previous layer, the gates that add into
+// it through u (a sum gate into every x of its range, its record carrying x
+// in place of v); phase 2 lists by v the gates that add into entry v.
+// Counts by atomics, offsets by a two-level scan, records placed by atomic
+// cursors: the order of records within one entry is arbitrary, which no
+// result depends on (the table builds only add field elements).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void csr_p1_range(int t, int32_t u, int32_t v, int64_t &lo, int64_t &hi)
+{
+    switch (t) {
+    case 0: case 1: case 4: case 6: case 7: case 8: case 9: case 10: case 13: lo = u; hi = (int64_t)u + 1; break;
+    case 5: lo = u; hi = v; break;
+    case 12: lo = u; hi = (int64_t)v + 1; break;
+    default: lo = hi = 0; break;
+    }
+}
+__device__ __forceinline__ bool csr_p2_by_v(int t) { return t == 0 || t == 1 || t == 7 || t == 8 || t == 9 || t == 13; }
+
+__global__ void k_csr_count(const uint8_t *ty, const int32_t *u, const int32_t *v, int64_t ng, uint32_t *c1, uint32_t *c2)
+{
+    int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    int t = ty[g];
+    int32_t uu = u[g], vv = v[g];
+    int64_t lo, hi;
+    csr_p1_range(t, uu, vv, lo, hi);
+    for (int64_t x = lo; x < hi; ++x) atomicAdd(&c1[x + 1], 1u);
+    if (csr_p2_by_v(t)) atomicAdd(&c2[vv + 1], 1u);
+}
+
+__global__ void k_csr_fill(const uint8_t *ty, const int32_t *u, const int32_t *v, int64_t ng, const uint32_t *o1,
+                           const uint32_t *o2, uint32_t *cur1, uint32_t *cur2, launch::gate_rec *e1, launch::gate_rec *e2)
+{
+    int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    int t = ty[g];
+    int32_t uu = u[g], vv = v[g];
+    launch::gate_rec rec = {(int32_t)g, uu, vv, t};
+    int64_t lo, hi;
+    csr_p1_range(t, uu, vv, lo, hi);
+    bool range = t == 5 || t == 12;
+    for (int64_t x = lo; x < hi; ++x) {
+        launch::gate_rec r1 = rec;
+        if (range) r1.v = (int32_t)x;
+        e1[o1[x] + atomicAdd(&cur1[x], 1u)] = r1;
+    }
+    if (csr_p2_by_v(t)) e2[o2[vv] + atomicAdd(&cur2[vv], 1u)] = rec;
+}
+
+// inclusive scan of 2048 consecutive values per block; tops[b] = block total
+constexpr int SCAN_T = 256, SCAN_E = 8;
+__global__ void __launch_bounds__(SCAN_T) k_scan_block(uint32_t *a, int64_t n, uint32_t *tops)
+{
+    __shared__ uint32_t sh[SCAN_T];
+    int64_t base = blockIdx.x * (int64_t)(SCAN_T * SCAN_E) + threadIdx.x * SCAN_E;
+    uint32_t x[SCAN_E], sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_E; ++k) {
+        sum += base + k < n ? a[base + k] : 0u;
+        x[k] = sum;
+    }
+    sh[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < SCAN_T; off <<= 1) {
+        uint32_t y = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += y;
+        __syncthreads();
+    }
+    uint32_t pre = threadIdx.x ? sh[threadIdx.x - 1] : 0u;
+#pragma unroll
+    for (int k = 0; k < SCAN_E; ++k)
+        if (base + k < n) a[base + k] = x[k] + pre;
+    if (tops && threadIdx.x == SCAN_T - 1) tops[blockIdx.x] = sh[SCAN_T - 1];
+}
+// adds the inclusive prefix of the previous blocks' totals
+__global__ void k_scan_add(uint32_t *a, int64_t n, const uint32_t *tops)
+{
+    if (blockIdx.x == 0) return;
+    uint32_t add = tops[blockIdx.x - 1];
+    int64_t base = blockIdx.x * (int64_t)(SCAN_T * SCAN_E);
+    for (int k = threadIdx.x; k < SCAN_T * SCAN_E; k += SCAN_T)
+        if (base + k < n) a[base + k] += add;
+}
+
 }  // namespace
 
 namespace launch {
@@ -1390,6 +1479,41 @@ void zero(void *p, size_t bytes, hipStream_t s)
     unsigned g = nblk(n);
     if (g > 4096) g = 4096;
     hipLaunchKernelGGL(k_zero, dim3(g), dim3(TPB), 0, s, (uint32_t *)p, n);
+}
+// ---- CSR transposes ----
+int64_t scan_scratch_len(int64_t n)
+{
+    const int64_t B = SCAN_T * SCAN_E;
+    int64_t tot = 0;
+    while (n > B) { n = (n + B - 1) / B; tot += n; }
+    return tot + 1;
+}
+// inclusive prefix sum of a[0..n) in place; scratch: scan_scratch_len(n)
+void scan_u32(uint32_t *a, int64_t n, uint32_t *scratch, hipStream_t s)
+{
+    const int64_t B = SCAN_T * SCAN_E;
+    int64_t nb = (n + B - 1) / B;
+    if (nb <= 1) {
+        hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(SCAN_T), 0, s, a, n, (uint32_t *)nullptr);
+        return;
+    }
+    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(SCAN_T), 0, s, a, n, scratch);
+    scan_u32(scratch, nb, scratch + nb, s);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SCAN_T), 0, s, a, n, (const uint32_t *)scratch);
+}
+void csr_build(gates_dev g, int64_t ng, int64_t nt, uint32_t *off1, uint32_t *off2, gate_rec *ent1, gate_rec *ent2,
+               uint32_t *cur, uint32_t *scratch, hipStream_t s)
+{
+    zero(off1, 4 * (size_t)(nt + 1), s);
+    zero(off2, 4 * (size_t)(nt + 1), s);
+    zero(cur, 8 * (size_t)nt, s);
+    if (ng <= 0) return;
+    unsigned nb = nblk(ng);
+    hipLaunchKernelGGL(k_csr_count, dim3(nb), dim3(TPB), 0, s, g.ty, g.u, g.v, ng, off1, off2);
+    scan_u32(off1, nt + 1, scratch, s);
+    scan_u32(off2, nt + 1, scratch, s);
+    hipLaunchKernelGGL(k_csr_fill, dim3(nb), dim3(TPB), 0, s, g.ty, g.u, g.v, ng, (const uint32_t *)off1,
+                       (const uint32_t *)off2, cur, cur + nt, ent1, ent2);
 }
 void eval_input(const int32_t *u, fe *out, int n, hipStream_t s)
 {

@@ -49,6 +49,14 @@ struct beta_u { const fe *f, *s; int fh; };
 struct __align__(16) gate_rec {
     int32_t g, u, v, ty;
 };
+// CSR transposes of one gate layer (ng gates over nt = 2^bl(i-1) table
+// entries): off1 / off2 (nt + 1 offsets each) and ent1 / ent2 by u and by v
+// (record counts as the host's plan counted them).  cur: 2 nt words,
+// scratch: scan_scratch_len(nt + 1) words.  Enqueued on s.
+int64_t scan_scratch_len(int64_t n);
+void scan_u32(uint32_t *a, int64_t n, uint32_t *scratch, hipStream_t s);
+void csr_build(gates_dev g, int64_t ng, int64_t nt, uint32_t *off1, uint32_t *off2, gate_rec *ent1, gate_rec *ent2,
+               uint32_t *cur, uint32_t *scratch, hipStream_t s);
 // One GKR layer of the main prover on the device (all pointers fixed per
 // context).  Batches of layers with equal table size run in one launch.
 struct layer_dev {

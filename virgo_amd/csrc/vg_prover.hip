@@ -8,6 +8,8 @@
 // (zk_prover.cpp:162-333 and the mask terms of every sumcheck round) is host
 // scalar work on challenges only; it is computed beside the device work and
 // added to the device round sums when the transcript is assembled.
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <functional>
 #include <thread>
@@ -44,6 +46,24 @@ struct host_circuit {
 
 int bitlen(int64_t x) { int c = 0; while (x) { ++c; x >>= 1; } return c; }
 
+// Runs fn(i) for i in [lo, hi) on up to 8 host threads when the work is
+// large (a proof's per-layer planning), else in order on this thread.
+void for_layers(int lo, int hi, int64_t work, const std::function<void(int)> &fn)
+{
+    int nth = (int)std::min<int64_t>({8, (int64_t)std::thread::hardware_concurrency(), (int64_t)(hi - lo)});
+    if (work < (1 << 20) || nth < 2) {
+        for (int i = lo; i < hi; ++i) fn(i);
+        return;
+    }
+    std::atomic<int> next(lo);
+    std::vector<std::thread> th;
+    for (int k = 0; k < nth; ++k)
+        th.emplace_back([&] {
+            for (int i = next++; i < hi; i = next++) fn(i);
+        });
+    for (auto &t : th) t.join();
+}
+
 int build_circuit(const vg_circuit_desc *d, host_circuit &C)
 {
     int D = d->d;
@@ -53,57 +73,75 @@ int build_circuit(const vg_circuit_desc *d, host_circuit &C)
     C.ty.assign(D + 1, {});
     C.u.assign(D + 1, {});
     C.v.assign(D + 1, {});
-    size_t off = 0;
+    // sizes of every file layer first, then the gates of all layers in
+    // parallel; the first error in file order is reported
+    std::vector<int64_t> n(D + 1), n_pad(D + 1), goff(D + 1);
+    int64_t off = 0;
     for (int i = 1; i <= D; ++i) {
-        int64_t n = d->n[i - 1];
+        n[i] = d->n[i - 1];
         int pad_req = D > 3 ? 17 : 15;
-        int64_t n_pad = (i == 1 && n < (1LL << pad_req)) ? (1LL << pad_req) : n;
-        if (n < 1 || (n != 1 && (n & (n - 1)))) FAILV(-2, "layer %d: gate count %lld is not a power of two", i, (long long)n);
-        if (i == 1 && n == 1) FAILV(-2, "single-input circuits are not supported");
-        if (n_pad > (1LL << 30)) FAILV(-2, "layer %d too wide", i);
-        int64_t alloc = n == 1 ? 2 : n_pad;
+        n_pad[i] = (i == 1 && n[i] < (1LL << pad_req)) ? (1LL << pad_req) : n[i];
+        if (n[i] < 1 || (n[i] != 1 && (n[i] & (n[i] - 1)))) FAILV(-2, "layer %d: gate count %lld is not a power of two", i, (long long)n[i]);
+        if (i == 1 && n[i] == 1) FAILV(-2, "single-input circuits are not supported");
+        if (n_pad[i] > (1LL << 30)) FAILV(-2, "layer %d too wide", i);
+        goff[i] = off;
+        off += n[i];
+        C.bl[i] = n_pad[i] == 1 ? 1 : bitlen(n_pad[i] - 1);
+    }
+    C.bl[0] = C.bl[1];
+    std::vector<std::string> err(D + 1);
+    std::vector<int> code(D + 1, 0);
+    auto layer = [&](int i) {
+        char b[256];
+        auto fail = [&](int rc) { code[i] = rc; err[i] = b; };
+        int64_t alloc = n[i] == 1 ? 2 : n_pad[i];
         int li = i == 1 ? 0 : i;
         C.ty[li].assign(alloc, 2); C.u[li].assign(alloc, 0); C.v[li].assign(alloc, 0);
         if (i == 1) { C.ty[1].assign(alloc, 4); C.u[1].assign(alloc, 0); C.v[1].assign(alloc, 0); }
-        int prev_bl = C.bl[i - 1];
-        for (int64_t j = 0; j < n; ++j, ++off) {
-            int t = d->ty[off];
-            int64_t uu = d->u[off], vv = d->v[off];
+        int prev_bl = i == 1 ? 0 : C.bl[i - 1];      // the input layer's gates are checked against 2^0
+        const int32_t *dty = d->ty + goff[i];
+        const int64_t *du = d->u + goff[i], *dv = d->v + goff[i];
+        uint8_t *ty = C.ty[i].data();
+        int32_t *u = C.u[i].data(), *v = C.v[i].data();
+        for (int64_t j = 0; j < n[i]; ++j) {
+            int t = dty[j];
+            int64_t uu = du[j], vv = dv[j];
             if (t != 3) {
                 if (t == 5) {
-                    if (!(uu >= 0 && uu < (1LL << prev_bl)) || !(vv > uu && vv <= (1LL << prev_bl)))
-                        FAILV(-3, "layer %d gate %lld: sum range out of bounds", i, (long long)j);
-                } else if (!(uu >= 0 && uu < (1LL << prev_bl)) || !(vv >= 0 && vv < (1LL << prev_bl)))
-                    FAILV(-3, "layer %d gate %lld: input index out of bounds", i, (long long)j);
+                    if (!(uu >= 0 && uu < (1LL << prev_bl)) || !(vv > uu && vv <= (1LL << prev_bl))) {
+                        snprintf(b, sizeof b, "layer %d gate %lld: sum range out of bounds", i, (long long)j);
+                        return fail(-3);
+                    }
+                } else if (!(uu >= 0 && uu < (1LL << prev_bl)) || !(vv >= 0 && vv < (1LL << prev_bl))) {
+                    snprintf(b, sizeof b, "layer %d gate %lld: input index out of bounds", i, (long long)j);
+                    return fail(-3);
+                }
             }
             if (t == 6 || t == 10) vv = 0;
-            if (t == 13 && uu != vv) FAILV(-3, "bit-test gate needs u == v");
-            if (t < 0 || t > 13 || t == 11) FAILV(-3, "unknown gate type %d", t);
+            if (t == 13 && uu != vv) { snprintf(b, sizeof b, "bit-test gate needs u == v"); return fail(-3); }
+            if (t < 0 || t > 13 || t == 11) { snprintf(b, sizeof b, "unknown gate type %d", t); return fail(-3); }
             if (i != 1) {
-                C.ty[i][j] = (uint8_t)t; C.u[i][j] = (int32_t)uu; C.v[i][j] = (int32_t)vv;
+                ty[j] = (uint8_t)t; u[j] = (int32_t)uu; v[j] = (int32_t)vv;
             } else {
-                if (t != 2 && t != 3) FAILV(-3, "input layer gates must be type 2 or 3");
+                if (t != 2 && t != 3) { snprintf(b, sizeof b, "input layer gates must be type 2 or 3"); return fail(-3); }
                 C.ty[1][j] = 4; C.u[1][j] = (int32_t)j; C.v[1][j] = 0;
                 C.ty[0][j] = (uint8_t)t; C.u[0][j] = (int32_t)uu; C.v[0][j] = (int32_t)vv;   // (int) truncation, zk_prover.cpp:60
             }
         }
-        if (i == 1) {
-            for (int64_t g = n; g < n_pad; ++g) { C.ty[1][g] = 4; C.u[1][g] = (int32_t)g; C.ty[0][g] = 3; C.u[0][g] = 0; }
-            n = n_pad;
-        }
-        if (n == 1) {
-            C.ty[i][1] = 2; C.u[i][1] = 0; C.v[i][1] = 0;
-            C.bl[i] = 1;
-        } else {
-            C.bl[i] = bitlen(n - 1);
-            if (i == 1) C.bl[0] = C.bl[1];
-        }
+        if (i == 1)
+            for (int64_t g = n[i]; g < n_pad[i]; ++g) { C.ty[1][g] = 4; C.u[1][g] = (int32_t)g; C.ty[0][g] = 3; C.u[0][g] = 0; }
+        if (n[i] == 1) { C.ty[i][1] = 2; C.u[i][1] = 0; C.v[i][1] = 0; }
         // a direct-relay gate indexes the layer's own beta table with u
         // (zk_prover.cpp:496-497); the reference reads past it otherwise
         for (size_t j = 0; j < C.ty[i].size(); ++j)
-            if (C.ty[i][j] == 4 && C.u[i][j] >= (1LL << C.bl[i]))
-                FAILV(-3, "layer %d: relay gate %zu reads u=%d beyond its own layer size", i, j, C.u[i][j]);
-    }
+            if (C.ty[i][j] == 4 && C.u[i][j] >= (1LL << C.bl[i])) {
+                snprintf(b, sizeof b, "layer %d: relay gate %zu reads u=%d beyond its own layer size", i, j, C.u[i][j]);
+                return fail(-3);
+            }
+    };
+    for_layers(1, D + 1, off, layer);
+    for (int i = 1; i <= D; ++i)
+        if (code[i]) FAILV(code[i], "%s", err[i].c_str());
     if (C.bl[0] < 6 + 1) FAILV(-2, "input layer too small for the commitment");
     return 0;
 }
@@ -456,9 +494,11 @@ struct vg_ctx {
     hipStream_t gst[NGS] = {};
     hipEvent_t gev[NGS] = {};
     // CSR transposes of every gate layer (circuit structure only, built once)
-    std::vector<std::vector<uint32_t>> h_off1, h_off2;
-    std::vector<std::vector<launch::gate_rec>> h_ent1, h_ent2;
+    // (built on the device at creation; the host plans sizes and the z2 lists)
+    std::vector<int64_t> n_ent1, n_ent2;      // records by u / by v per layer
+    std::vector<char> crowd1, crowd2;         // a gather_fold tile holds more than GF_MAX_FANIN records per entry
     std::vector<std::vector<int32_t>> h_z2;
+    uint32_t *csr_cur = nullptr, *csr_scan = nullptr;   // scratch of the device CSR build
     std::vector<uint32_t *> off1, off2;
     std::vector<launch::gate_rec *> ent1, ent2;
     std::vector<int32_t *> z2;
@@ -565,13 +605,18 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     c->d_inst = carve<launch::fold_inst>(base, off, 2 * D);
     c->off1.assign(D, nullptr); c->off2.assign(D, nullptr);
     c->ent1.assign(D, nullptr); c->ent2.assign(D, nullptr); c->z2.assign(D, nullptr);
+    int64_t max_nt = 1;
     for (int i = 1; i < D; ++i) {
-        c->off1[i] = carve<uint32_t>(base, off, (int64_t)c->h_off1[i].size());
-        c->off2[i] = carve<uint32_t>(base, off, (int64_t)c->h_off2[i].size());
-        c->ent1[i] = carve<launch::gate_rec>(base, off, (int64_t)c->h_ent1[i].size() + 1);
-        c->ent2[i] = carve<launch::gate_rec>(base, off, (int64_t)c->h_ent2[i].size() + 1);
+        int64_t nt = (int64_t)1 << C.bl[i - 1];
+        if (nt > max_nt) max_nt = nt;
+        c->off1[i] = carve<uint32_t>(base, off, nt + 1);
+        c->off2[i] = carve<uint32_t>(base, off, nt + 1);
+        c->ent1[i] = carve<launch::gate_rec>(base, off, c->n_ent1[i] + 1);
+        c->ent2[i] = carve<launch::gate_rec>(base, off, c->n_ent2[i] + 1);
         c->z2[i] = carve<int32_t>(base, off, (int64_t)c->h_z2[i].size() + 1);
     }
+    c->csr_cur = carve<uint32_t>(base, off, 2 * max_nt);
+    c->csr_scan = carve<uint32_t>(base, off, launch::scan_scratch_len(max_nt + 1));
     c->ht_stride = (int64_t)1 << ((maxbl >> 1) + 1);
     c->ht = carve<fe>(base, off, (int64_t)D * 8 * c->ht_stride + 2 * ((int64_t)1 << (C.bl[0] / 2 + 1)));
     c->hdesc = carve<launch::half_desc>(base, off, (int64_t)D * 8 + 2);
@@ -1043,54 +1088,53 @@ void assemble(vg_ctx *c, tbuf &T)
     T.put(go + k, (size_t)16 * (c->n_gather - k));
 }
 
-// CSR transposes of one gate layer (the structure of zk_prover.cpp:420-587
-// and :845-972, computed once per circuit): phase 1 lists, for every entry u
-// of the previous layer, the gates that add into it through u (a sum gate
-// into every u of its range, its record carrying that u in place of v);
-// phase 2 lists by v the gates that add into entry v, and separately the
-// gates that always add into entry 0.
-void build_csr(vg_ctx *c, int i)
+// Plan of the CSR transposes of one gate layer (the structure of
+// zk_prover.cpp:420-587 and :845-972; the transposes themselves are built on
+// the device, launch::csr_build): phase 1 lists, for every entry u of the
+// previous layer, the gates that add into it through u (a sum gate into
+// every u of its range); phase 2 lists by v the gates that add into entry v,
+// and separately the gates that always add into entry 0 (z2, kept here).
+// Counts the records of each phase and whether any gather_fold tile of the
+// layer's table holds more than GF_MAX_FANIN records per entry.
+void csr_plan(vg_ctx *c, int i)
 {
     const host_circuit &C = c->C;
     int64_t nt = (int64_t)1 << C.bl[i - 1];
     int64_t ng = (int64_t)C.ty[i].size();
-    std::vector<uint32_t> &o1 = c->h_off1[i], &o2 = c->h_off2[i];
-    std::vector<launch::gate_rec> &e1 = c->h_ent1[i], &e2 = c->h_ent2[i];
+    int64_t tile = launch::gf_tile(C.bl[i - 1]);
+    int64_t ntile = (nt + tile - 1) / tile;
+    std::vector<int64_t> t1(ntile, 0), t2(ntile, 0);
     std::vector<int32_t> &z = c->h_z2[i];
-    o1.assign(nt + 1, 0); o2.assign(nt + 1, 0);
+    z.clear();
     const uint8_t *ty = C.ty[i].data();
     const int32_t *u = C.u[i].data(), *v = C.v[i].data();
-    auto p1_range = [&](int64_t g, int64_t &lo, int64_t &hi) {
-        switch (ty[g]) {
-        case 0: case 1: case 4: case 6: case 7: case 8: case 9: case 10: case 13: lo = u[g]; hi = u[g] + 1; break;
+    int64_t e1 = 0, e2 = 0;
+    for (int64_t g = 0; g < ng; ++g) {
+        int t = ty[g];
+        int64_t lo = 0, hi = 0;
+        switch (t) {
+        case 0: case 1: case 4: case 6: case 7: case 8: case 9: case 10: case 13: lo = u[g]; hi = (int64_t)u[g] + 1; break;
         case 5: lo = u[g]; hi = v[g]; break;
         case 12: lo = u[g]; hi = (int64_t)v[g] + 1; break;
-        default: lo = hi = 0; break;
+        default: break;
         }
-    };
-    auto p2_by_v = [&](int64_t g) { int t = ty[g]; return t == 0 || t == 1 || t == 7 || t == 8 || t == 9 || t == 13; };
-    auto p2_zero = [&](int64_t g) { int t = ty[g]; return t == 5 || t == 12 || t == 6 || t == 10; };
-    for (int64_t g = 0; g < ng; ++g) {
-        int64_t lo, hi;
-        p1_range(g, lo, hi);
-        for (int64_t t = lo; t < hi; ++t) o1[t + 1]++;
-        if (p2_by_v(g)) o2[v[g] + 1]++;
+        if (hi == lo + 1) t1[lo / tile]++;
+        else
+            for (int64_t x = lo; x < hi;) {
+                int64_t k = x / tile, end = (k + 1) * tile < hi ? (k + 1) * tile : hi;
+                t1[k] += end - x;
+                x = end;
+            }
+        e1 += hi > lo ? hi - lo : 0;
+        if (t == 0 || t == 1 || t == 7 || t == 8 || t == 9 || t == 13) { t2[v[g] / tile]++; ++e2; }
+        if (t == 5 || t == 12 || t == 6 || t == 10) z.push_back((int32_t)g);
     }
-    for (int64_t t = 0; t < nt; ++t) { o1[t + 1] += o1[t]; o2[t + 1] += o2[t]; }
-    e1.assign(o1[nt], launch::gate_rec()); e2.assign(o2[nt], launch::gate_rec()); z.clear();
-    std::vector<uint32_t> c1(o1.begin(), o1.end() - 1), c2(o2.begin(), o2.end() - 1);
-    for (int64_t g = 0; g < ng; ++g) {
-        int64_t lo, hi;
-        p1_range(g, lo, hi);
-        launch::gate_rec rec = {(int32_t)g, u[g], v[g], ty[g]};
-        bool range = ty[g] == 5 || ty[g] == 12;        // the record names its entry in v
-        for (int64_t t = lo; t < hi; ++t) {
-            launch::gate_rec r1 = rec;
-            if (range) r1.v = (int32_t)t;
-            e1[c1[t]++] = r1;
-        }
-        if (p2_by_v(g)) e2[c2[v[g]]++] = rec;
-        if (p2_zero(g)) z.push_back((int32_t)g);
+    c->n_ent1[i] = e1;
+    c->n_ent2[i] = e2;
+    c->crowd1[i] = c->crowd2[i] = 0;
+    for (int64_t k = 0; k < ntile; ++k) {
+        if (t1[k] > launch::GF_MAX_FANIN * tile) c->crowd1[i] = 1;
+        if (t2[k] > launch::GF_MAX_FANIN * tile) c->crowd2[i] = 1;
     }
 }
 
@@ -1176,21 +1220,23 @@ int ctx_plan(vg_ctx *c)
     }
     c->n_slots = (int)c->h_slot_nb.size();
     c->gather_cap = gather_count(n);
-    c->h_off1.assign(C.depth, {}); c->h_off2.assign(C.depth, {});
-    c->h_ent1.assign(C.depth, {}); c->h_ent2.assign(C.depth, {}); c->h_z2.assign(C.depth, {});
-    for (int i = 1; i < C.depth; ++i) build_csr(c, i);
+    c->n_ent1.assign(C.depth, 0); c->n_ent2.assign(C.depth, 0);
+    c->crowd1.assign(C.depth, 0); c->crowd2.assign(C.depth, 0); c->h_z2.assign(C.depth, {});
+    {
+        int64_t gates = 0;
+        for (int i = 1; i < C.depth; ++i) gates += (int64_t)C.ty[i].size();
+        for_layers(1, C.depth, gates, [&](int i) { csr_plan(c, i); });
+    }
+    for (int i = 1; i < C.depth; ++i)
+        if (c->n_ent1[i] >= (1LL << 31) || c->n_ent2[i] >= (1LL << 31)) FAILV(-2, "layer %d: too many wiring records", i);
     // gather_fold walks a block's records in sequence: phases whose records
     // crowd onto few entries (high fan-out wiring) build flat tables with
     // one thread per entry instead
-    for (auto &b : c->batches) {
-        int64_t nt = (int64_t)1 << b.bl, tile = launch::gf_tile(b.bl);
-        for (int ph = 0; ph < 2; ++ph)
-            for (int i : b.layers) {
-                const std::vector<uint32_t> &o = ph == 0 ? c->h_off1[i] : c->h_off2[i];
-                for (int64_t t0 = 0; t0 < nt && b.fused[ph]; t0 += tile)
-                    if (o[t0 + tile < nt ? t0 + tile : nt] - o[t0] > launch::GF_MAX_FANIN * tile) b.fused[ph] = false;
-            }
-    }
+    for (auto &b : c->batches)
+        for (int i : b.layers) {
+            if (c->crowd1[i]) b.fused[0] = false;
+            if (c->crowd2[i]) b.fused[1] = false;
+        }
     c->arena_bytes = layout(c, nullptr);
     c->gout_len = c->gather_cap > (1024 + (c->SS >> 10) + 1) ? c->gather_cap : (1024 + (c->SS >> 10) + 1);
     c->pin_bytes = (int64_t)sizeof(fe) * (c->n_chal + c->gout_len + c->msg_len) + (int64_t)sizeof(uint64_t) * c->gather_cap + 1024;
@@ -1213,6 +1259,7 @@ int ctx_alloc(vg_ctx *c)
     c->h_msg = (fe *)c->mem->pinned(sizeof(fe) * c->msg_len);
     if (!c->h_chal || !c->h_gsrc || !c->h_gout || !c->h_msg) FAILV(-21, "pinned host allocation failed");
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+    for (auto &e : c->gev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto &e : c->ev_leaf) HIPCHK(hipEventCreate(&e));
     // gates and inputs
     for (int i = 0; i < C.depth; ++i) {
@@ -1228,13 +1275,10 @@ int ctx_alloc(vg_ctx *c)
         HIPCHK(hipMemcpy(c->slot_dst, c->h_slot_dst.data(), sizeof(int64_t) * c->n_slots, hipMemcpyHostToDevice));
     }
     for (int i = 1; i < C.depth; ++i) {
-        HIPCHK(hipMemcpy(c->off1[i], c->h_off1[i].data(), 4 * c->h_off1[i].size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->off2[i], c->h_off2[i].data(), 4 * c->h_off2[i].size(), hipMemcpyHostToDevice));
-        if (!c->h_ent1[i].empty())
-            HIPCHK(hipMemcpy(c->ent1[i], c->h_ent1[i].data(), sizeof(launch::gate_rec) * c->h_ent1[i].size(), hipMemcpyHostToDevice));
-        if (!c->h_ent2[i].empty())
-            HIPCHK(hipMemcpy(c->ent2[i], c->h_ent2[i].data(), sizeof(launch::gate_rec) * c->h_ent2[i].size(), hipMemcpyHostToDevice));
         if (!c->h_z2[i].empty()) HIPCHK(hipMemcpy(c->z2[i], c->h_z2[i].data(), 4 * c->h_z2[i].size(), hipMemcpyHostToDevice));
+        gates_dev g = {c->d_ty[i], c->d_u[i], c->d_v[i], C.bl[i]};
+        launch::csr_build(g, (int64_t)C.ty[i].size(), (int64_t)1 << C.bl[i - 1], c->off1[i], c->off2[i], c->ent1[i],
+                          c->ent2[i], c->csr_cur, c->csr_scan, c->st);
     }
     {
         // eq half tables of every layer, computed in one launch per proof
@@ -1802,7 +1846,6 @@ int ctx_create_planned(vg_ctx **out, const vg_circuit_desc *d, void *stream)
             for (int w = 0; w < vg_ctx::NGS && e == hipSuccess; ++w) e = hipStreamCreateWithFlags(&c->gst[w], hipStreamNonBlocking);
             c->own_aux = true;
         }
-        for (int w = 0; w < vg_ctx::NGS && e == hipSuccess; ++w) e = hipEventCreateWithFlags(&c->gev[w], hipEventDisableTiming);
         if (e != hipSuccess) { vg_ctx_destroy(c); FAILV(-20, "hipStreamCreate: %s", hipGetErrorString(e)); }
     }
     rc = ctx_plan(c);

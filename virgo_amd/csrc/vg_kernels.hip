@@ -210,39 +210,67 @@ __device__ __forceinline__ void p2_rec(const launch::gate_rec &R, const bg_t &bg
     default: break;
     }
 }
-// one entry per thread (tables of at most 64 entries: the tail fold reads them)
-__global__ __launch_bounds__(TPB) void k_p1_gather(const launch::layer_dev *D)
+// 2^kl lanes per entry (kl <= 6, the lanes of an entry inside one wave):
+// lane `sub` takes records sub, sub + 2^kl, ... of its entry and the lanes
+// add their sums with butterfly shuffles; crowded tables (many records per
+// entry, e.g. the inputs of a matrix product) get many lanes per entry
+template <int C>
+__device__ __forceinline__ void lanes_sum(fe *x, int kl)
 {
-    const launch::layer_dev &d = D[blockIdx.y];
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= d.nt) return;
-    bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
-    fe am = fzero(), aV = fzero();
-    for (uint32_t e = d.off1[t]; e < d.off1[t + 1]; ++e) {
-        launch::gate_rec R = d.ent1[e];
-        fe m, a;
-        p1_rec(R, bg, d.pv[p1_reads_v(R.ty) ? R.v : 0], m, a);
-        am = add(am, m); aV = add(aV, a);
-    }
-    d.amb[t] = am;
-    d.aVb[t] = aV;
+    for (int o = 1; o < (1 << kl); o <<= 1)
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            fe y;
+            y.real = __shfl_xor((unsigned long long)x[c].real, o);
+            y.img = __shfl_xor((unsigned long long)x[c].img, o);
+            x[c] = add(x[c], y);
+        }
 }
-__global__ __launch_bounds__(TPB) void k_p2_gather(const launch::layer_dev *D)
+__global__ __launch_bounds__(TPB) void k_p1_gather(const launch::layer_dev *D, int kl)
 {
     const launch::layer_dev &d = D[blockIdx.y];
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= d.nt) return;
+    int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t t = gt >> kl;
+    int sub = (int)(gt & ((1 << kl) - 1));
+    bool live = t < d.nt;
+    bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
+    fe acc[2] = {fzero(), fzero()};     // am, aV
+    if (live)
+        for (uint32_t e = d.off1[t] + sub; e < d.off1[t + 1]; e += 1u << kl) {
+            launch::gate_rec R = d.ent1[e];
+            fe m, a;
+            p1_rec(R, bg, d.pv[p1_reads_v(R.ty) ? R.v : 0], m, a);
+            acc[0] = add(acc[0], m); acc[1] = add(acc[1], a);
+        }
+    lanes_sum<2>(acc, kl);
+    if (live && sub == 0) {
+        d.amb[t] = acc[0];
+        d.aVb[t] = acc[1];
+    }
+}
+__global__ __launch_bounds__(TPB) void k_p2_gather(const launch::layer_dev *D, int kl)
+{
+    const launch::layer_dev &d = D[blockIdx.y];
+    int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t t = gt >> kl;
+    int sub = (int)(gt & ((1 << kl) - 1));
+    bool live = t < d.nt;
     bg_t bg = {d.bg.g0f, d.bg.g0s, d.bg.g1f, d.bg.g1s, d.bg.fh};
     bu_t bu = {d.bu.f, d.bu.s, d.bu.fh};
     fe vu = *d.vu;
-    fe am = fzero(), aV = (t == 0 && d.nz > 0) ? d.part[launch::MAX_PARTS] : fzero();
-    for (uint32_t e = d.off2[t]; e < d.off2[t + 1]; ++e) {
-        fe m, a;
-        p2_rec(d.ent2[e], bg, bu, vu, m, a);
-        am = add(am, m); aV = add(aV, a);
+    fe acc[2] = {fzero(), fzero()};     // am, aV
+    if (live && t == 0 && sub == 0 && d.nz > 0) acc[1] = d.part[launch::MAX_PARTS];
+    if (live)
+        for (uint32_t e = d.off2[t] + sub; e < d.off2[t + 1]; e += 1u << kl) {
+            fe m, a;
+            p2_rec(d.ent2[e], bg, bu, vu, m, a);
+            acc[0] = add(acc[0], m); acc[1] = add(acc[1], a);
+        }
+    lanes_sum<2>(acc, kl);
+    if (live && sub == 0) {
+        d.amb[t] = acc[0];
+        d.aVb[t] = acc[1];
     }
-    d.amb[t] = am;
-    d.aVb[t] = aV;
 }
 // gates adding into addV[0] in phase 2 (sum, exp-sum, NOT, relay); per-block partials
 __global__ __launch_bounds__(TPB) void k_p2_zero(const launch::layer_dev *D)
@@ -1556,15 +1584,17 @@ static void zero_list(const layer_dev *D, int n, int nz_max, hipStream_t s)
     hipLaunchKernelGGL(k_p2_zero, dim3(blocks, n), dim3(TPB), 0, s, D);
     hipLaunchKernelGGL(k_zero_sum, dim3(1, n), dim3(1024), 0, s, D, (int)blocks);
 }
-void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipStream_t s)
+void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipStream_t s, int kl)
 {
-    dim3 grid(nblk(nt), n);
+    if (kl < 0) kl = 0;
+    if (kl > 6) kl = 6;
+    dim3 grid(nblk((int64_t)nt << kl), n);
     if (phase == 0) {
-        hipLaunchKernelGGL(k_p1_gather, grid, dim3(TPB), 0, s, D);
+        hipLaunchKernelGGL(k_p1_gather, grid, dim3(TPB), 0, s, D, kl);
         return;
     }
     zero_list(D, n, nz_max, s);
-    hipLaunchKernelGGL(k_p2_gather, grid, dim3(TPB), 0, s, D);
+    hipLaunchKernelGGL(k_p2_gather, grid, dim3(TPB), 0, s, D, kl);
 }
 int gather_fold(int phase, const layer_dev *D, const fold_inst *I, int n, int L, int nz_max, hipStream_t s)
 {

@@ -78,8 +78,10 @@ struct layer_dev {
 constexpr int MAX_PARTS = 1024;        // block partials; part[MAX_PARTS] holds the zero-list sum
 // phase 0: tables by u (zk_prover.cpp:375-592); phase 1: by v (:789-983),
 // including the gates that add into addV[0] (nz_max: largest z2 list), into
-// the flat tables amb, aVb (used when the phase has no streaming launch)
-void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipStream_t s);
+// the flat tables amb, aVb (used when the phase has no streaming launch),
+// 2^kl lanes per table entry (0..6: more for tables with many records per
+// entry)
+void layer_tables(int phase, const layer_dev *D, int n, int nt, int nz_max, hipStream_t s, int kl = 0);
 // the same fused with the phase's first streaming fold launch, of at most
 // GF_B rounds (the flat tables are never written): returns its rounds, 0 when
 // the phase has no streaming launch (then layer_tables and the plain fold

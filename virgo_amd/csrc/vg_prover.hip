@@ -484,6 +484,7 @@ struct vg_ctx {
         int nz_max = 0, ng_max = 0;
         int d0 = 0;                       // first entry in d_layers / d_inst
         bool fused[2] = {true, true};     // phase tables built by gather_fold
+        int kl[2] = {0, 0};               // log2 lanes per entry of a flat table build
     };
     std::vector<batch> batches;
     launch::layer_dev *d_layers = nullptr;    // in batch order
@@ -497,6 +498,7 @@ struct vg_ctx {
     // (built on the device at creation; the host plans sizes and the z2 lists)
     std::vector<int64_t> n_ent1, n_ent2;      // records by u / by v per layer
     std::vector<char> crowd1, crowd2;         // a gather_fold tile holds more than GF_MAX_FANIN records per entry
+    std::vector<int> kl1, kl2;                // log2 lanes per entry for a flat build (densest tile's records per entry)
     std::vector<std::vector<int32_t>> h_z2;
     uint32_t *csr_cur = nullptr, *csr_scan = nullptr;   // scratch of the device CSR build
     std::vector<uint32_t *> off1, off2;
@@ -819,10 +821,11 @@ int enqueue_eval(vg_ctx *c)
     return 0;
 }
 
-// everything of a proof up to its leaf chains (the FRI folds end on ev[5]);
-// gkr = false leaves the sumcheck layers and their slot reduction to a
-// batch's merged launches (batch_gkr) and enqueue_post
-int enqueue_pre(vg_ctx *c, bool gkr = true)
+// challenges, half tables, V_res and the sumcheck layers of a proof (ev[9]:
+// what the other streams need); gkr = false leaves the sumcheck layers and
+// their slot reduction to a batch's merged launches (batch_gkr) and
+// enqueue_post
+int enqueue_gkr(vg_ctx *c, bool gkr = true)
 {
     hipStream_t st = c->st, st2 = c->st2;
     host_circuit &C = c->C;
@@ -855,7 +858,7 @@ int enqueue_pre(vg_ctx *c, bool gkr = true)
             const launch::fold_inst *Ip = I + ph * n;
             int nz = ph ? b.nz_max : 0;
             int f = b.fused[ph] ? launch::gather_fold(ph, Dl, Ip, n, b.bl, nz, gs) : 0;
-            if (!f) launch::layer_tables(ph, Dl, n, 1 << b.bl, nz, gs);
+            if (!f) launch::layer_tables(ph, Dl, n, 1 << b.bl, nz, gs, b.kl[ph]);
             slot += n * launch::sumcheck_multi(Ip, n, b.bl, gs, launch::GF_B, f > 0);
         }
         launch::general_multi(Dl, n, b.ng_max, gs);
@@ -870,7 +873,16 @@ int enqueue_pre(vg_ctx *c, bool gkr = true)
         launch::reduce_slots(c->slots, c->n_slots, c->slot_nb, c->slot_dst, c->msg, st);
         HIPCHK(hipEventRecord(c->ev[2], st));
     }
-    if (!c->serial) st = st2;
+    return 0;
+}
+
+// the commitment up to its leaf chains: LDEs, public array, FRI folds (ends
+// on ev[5]); on the second stream when the context overlaps its stages
+int enqueue_commit(vg_ctx *c)
+{
+    hipStream_t st = c->serial ? c->st : c->st2;
+    host_circuit &C = c->C;
+    int D = C.depth;
     // ---- commit_private_array (poly_commit.h:42-124) ----
     int n = c->n;
     int64_t SS = c->SS, real = c->real, m = c->m, Cc = c->C_;
@@ -914,19 +926,22 @@ int enqueue_pre(vg_ctx *c, bool gkr = true)
 // challenges (ev[9]); records gkr_done.  *buf: device descriptor arrays,
 // freed by the caller after the work completes; host_keep must outlive the
 // upload.
-int batch_gkr(vg_ctx *const *ctxs, int n, hipStream_t gs, hipEvent_t gkr_done, void **buf, std::vector<uint8_t> &host_keep)
+// host_buf: pinned copy of the launch tables (free with hipHostFree once gs
+// has drained), so that the upload is asynchronous
+int batch_gkr(vg_ctx *const *ctxs, int n, hipStream_t gs, hipEvent_t gkr_done, void **buf, void **host_buf,
+              bool wait = true)
 {
     struct group {
         int bl;
         bool f0, f1;
-        int nz_max = 0, ng_max = 0;
+        int nz_max = 0, ng_max = 0, kl[2] = {0, 0};
         std::vector<launch::layer_dev> L;
         std::vector<launch::fold_inst> I[2];
     };
     std::vector<group> G;
     for (int i = 0; i < n; ++i) {
         vg_ctx *c = ctxs[i];
-        HIPCHK(hipStreamWaitEvent(gs, c->ev[9], 0));
+        if (wait) HIPCHK(hipStreamWaitEvent(gs, c->ev[9], 0));
         for (auto &b : c->batches) {
             group *g = nullptr;
             for (auto &x : G)
@@ -939,6 +954,8 @@ int batch_gkr(vg_ctx *const *ctxs, int n, hipStream_t gs, hipEvent_t gkr_done, v
             int m = (int)b.layers.size();
             g->nz_max = b.nz_max > g->nz_max ? b.nz_max : g->nz_max;
             g->ng_max = b.ng_max > g->ng_max ? b.ng_max : g->ng_max;
+            g->kl[0] = std::max(g->kl[0], b.kl[0]);
+            g->kl[1] = std::max(g->kl[1], b.kl[1]);
             for (int q = 0; q < m; ++q) {
                 g->L.push_back(c->h_layers[b.d0 + q]);
                 g->I[0].push_back(c->h_inst[2 * b.d0 + q]);
@@ -948,22 +965,23 @@ int batch_gkr(vg_ctx *const *ctxs, int n, hipStream_t gs, hipEvent_t gkr_done, v
     }
     size_t bytes = 0;
     for (auto &g : G) bytes += sizeof(launch::layer_dev) * g.L.size() + 2 * sizeof(launch::fold_inst) * g.L.size();
-    host_keep.assign(bytes, 0);
+    HIPCHK(hipHostMalloc(host_buf, bytes ? bytes : 1, hipHostMallocDefault));
+    uint8_t *hk = (uint8_t *)*host_buf;
     std::vector<size_t> offL(G.size()), offI(G.size());
     size_t o = 0;
     for (size_t k = 0; k < G.size(); ++k) {
         offL[k] = o;
-        memcpy(host_keep.data() + o, G[k].L.data(), sizeof(launch::layer_dev) * G[k].L.size());
+        memcpy(hk + o, G[k].L.data(), sizeof(launch::layer_dev) * G[k].L.size());
         o += sizeof(launch::layer_dev) * G[k].L.size();
         offI[k] = o;
         for (int ph = 0; ph < 2; ++ph) {
-            memcpy(host_keep.data() + o, G[k].I[ph].data(), sizeof(launch::fold_inst) * G[k].I[ph].size());
+            memcpy(hk + o, G[k].I[ph].data(), sizeof(launch::fold_inst) * G[k].I[ph].size());
             o += sizeof(launch::fold_inst) * G[k].I[ph].size();
         }
     }
     HIPCHK(hipMalloc(buf, bytes ? bytes : 1));
     uint8_t *d = (uint8_t *)*buf;
-    HIPCHK(hipMemcpyAsync(d, host_keep.data(), bytes, hipMemcpyHostToDevice, gs));
+    HIPCHK(hipMemcpyAsync(d, hk, bytes, hipMemcpyHostToDevice, gs));
     for (size_t k = 0; k < G.size(); ++k) {
         const group &g = G[k];
         int m = (int)g.L.size();
@@ -972,7 +990,7 @@ int batch_gkr(vg_ctx *const *ctxs, int n, hipStream_t gs, hipEvent_t gkr_done, v
             const launch::fold_inst *Ip = (const launch::fold_inst *)(d + offI[k]) + ph * m;
             int nz = ph ? g.nz_max : 0;
             int f = (ph ? g.f1 : g.f0) ? launch::gather_fold(ph, Dl, Ip, m, g.bl, nz, gs) : 0;
-            if (!f) launch::layer_tables(ph, Dl, m, 1 << g.bl, nz, gs);
+            if (!f) launch::layer_tables(ph, Dl, m, 1 << g.bl, nz, gs, g.kl[ph]);
             launch::sumcheck_multi(Ip, m, g.bl, gs, launch::GF_B, f > 0);
         }
         launch::general_multi(Dl, m, g.ng_max, gs);
@@ -1132,10 +1150,23 @@ void csr_plan(vg_ctx *c, int i)
     c->n_ent1[i] = e1;
     c->n_ent2[i] = e2;
     c->crowd1[i] = c->crowd2[i] = 0;
+    int64_t m1 = 0, m2 = 0;
     for (int64_t k = 0; k < ntile; ++k) {
-        if (t1[k] > launch::GF_MAX_FANIN * tile) c->crowd1[i] = 1;
-        if (t2[k] > launch::GF_MAX_FANIN * tile) c->crowd2[i] = 1;
+        m1 = std::max(m1, t1[k]);
+        m2 = std::max(m2, t2[k]);
     }
+    c->crowd1[i] = m1 > launch::GF_MAX_FANIN * tile;
+    c->crowd2[i] = m2 > launch::GF_MAX_FANIN * tile;
+    // lanes per entry of a flat build: the densest tile's records per entry,
+    // rounded up to a power of two, at most a wave
+    auto lanes = [&](int64_t m) {
+        int64_t per = (m + std::min(tile, nt) - 1) / std::min(tile, nt);
+        int k = 0;
+        while (k < 6 && ((int64_t)1 << k) < per) ++k;
+        return k;
+    };
+    c->kl1[i] = lanes(m1);
+    c->kl2[i] = lanes(m2);
 }
 
 // Host planning of a proof: sizes, challenge and message layout, layer
@@ -1222,6 +1253,7 @@ int ctx_plan(vg_ctx *c)
     c->gather_cap = gather_count(n);
     c->n_ent1.assign(C.depth, 0); c->n_ent2.assign(C.depth, 0);
     c->crowd1.assign(C.depth, 0); c->crowd2.assign(C.depth, 0); c->h_z2.assign(C.depth, {});
+    c->kl1.assign(C.depth, 0); c->kl2.assign(C.depth, 0);
     {
         int64_t gates = 0;
         for (int i = 1; i < C.depth; ++i) gates += (int64_t)C.ty[i].size();
@@ -1236,6 +1268,8 @@ int ctx_plan(vg_ctx *c)
         for (int i : b.layers) {
             if (c->crowd1[i]) b.fused[0] = false;
             if (c->crowd2[i]) b.fused[1] = false;
+            b.kl[0] = std::max(b.kl[0], c->kl1[i]);
+            b.kl[1] = std::max(b.kl[1], c->kl2[i]);
         }
     c->arena_bytes = layout(c, nullptr);
     c->gout_len = c->gather_cap > (1024 + (c->SS >> 10) + 1) ? c->gather_cap : (1024 + (c->SS >> 10) + 1);
@@ -1381,9 +1415,9 @@ double ms_since(std::chrono::steady_clock::time_point t0)
 {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
-// host work and device work of a proof up to its leaf chains (gkr: as
-// enqueue_pre)
-int prove_start(vg_ctx *c, bool gkr = true)
+// host work and device work of a proof up to its sumcheck layers (gkr: as
+// enqueue_gkr)
+int prove_start_gkr(vg_ctx *c, bool gkr)
 {
     if (!c) FAILV(-1, "null context");
     if (c->pending) FAILV(-1, "a proof is already in flight on this context");
@@ -1398,11 +1432,23 @@ int prove_start(vg_ctx *c, bool gkr = true)
     if (rc) return rc;
     c->host_ms[0] = ms_since(tp);
     c->t_enq = std::chrono::steady_clock::now();
-    // fft_gkr (its own stream unless the proof runs in order): its small
-    // latency-bound launches overlap the main proof
-    rc = fg_enqueue(c->fg);
+    return enqueue_gkr(c, gkr);
+}
+// fft_gkr (its own stream unless the proof runs in order: its small
+// latency-bound launches overlap the main proof) and the commitment up to
+// the leaf chains
+int prove_start_commit(vg_ctx *c)
+{
+    int rc = fg_enqueue(c->fg);
     if (rc) return rc;
-    return enqueue_pre(c, gkr);
+    return enqueue_commit(c);
+}
+// everything of a proof up to its leaf chains
+int prove_start(vg_ctx *c, bool gkr = true)
+{
+    int rc = prove_start_gkr(c, gkr);
+    if (rc) return rc;
+    return prove_start_commit(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1981,112 +2027,172 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
     if (n == 0) return 0;
     if (workers < 1) workers = 1;
     if (workers > n) workers = n;
-    // fn(i) for every context, worker k taking i = k, k + workers, ...;
-    // returns the first error (its thread-local message moved here)
-    auto run = [&](const std::function<int(int)> &fn) {
-        std::vector<int> rc(workers, 0);
-        std::vector<std::string> msg(workers);
+    // fn(i) for every i in idx, worker k taking idx[k], idx[k + workers],
+    // ...; returns the first error (its thread-local message moved here)
+    auto run = [&](const std::vector<int> &idx, const std::function<int(int)> &fn) {
+        int m = (int)idx.size(), nw = workers < m ? workers : m;
+        std::vector<int> rc(nw, 0);
+        std::vector<std::string> msg(nw);
         std::vector<std::thread> th;
-        for (int k = 0; k < workers; ++k)
+        for (int k = 0; k < nw; ++k)
             th.emplace_back([&, k]() {
-                for (int i = k; i < n && rc[k] == 0; i += workers) {
-                    rc[k] = fn(i);
+                for (int j = k; j < m && rc[k] == 0; j += nw) {
+                    rc[k] = fn(idx[j]);
                     if (rc[k]) msg[k] = g_err;
                 }
             });
         for (auto &t : th) t.join();
-        for (int k = 0; k < workers; ++k)
+        for (int k = 0; k < nw; ++k)
             if (rc[k]) { g_err = msg[k]; return rc[k]; }
         return 0;
     };
-    // 1. every proof up to its leaf chains, but for its sumcheck layers
+    // The proofs run in groups, pipelined: (1a) every proof of a group up to
+    // its sumcheck layers (evaluation, challenges, half tables) and (1b) its
+    // commitment up to the leaf chains, on the proofs' streams; (2) the
+    // group's sumcheck layers in merged launches (one per step and table
+    // width across its proofs) once every (1a) is done, and its leaf chains
+    // and Merkle trees in merged launches (60 % of a mixed batch's VALU work)
+    // once every (1b) is done, on two streams of the batch; (3) the rest of
+    // each proof.  The dependencies of (2) and (3) are resolved on the host
+    // (helper threads wait for the events, then enqueue): a stream waiting on
+    // the device would stall the whole hardware queue it shares with other
+    // streams.  Group g+1's (1a) and (1b) are enqueued before group g's (3),
+    // so the proofs' streams run group g+1's latency-bound stages beside
+    // group g's full-width merged launches.
+    int ngroups = n >= 96 ? 4 : n >= 32 ? 2 : 1;
+    if (const char *e = getenv("VIRGO_BATCH_GROUPS")) ngroups = std::max(1, std::min(n, atoi(e)));
+    struct group {
+        std::vector<int> idx;
+        std::vector<vg_ctx *> S;
+        hipEvent_t done = nullptr, gdone = nullptr;
+        launch::leaf_seg *dG = nullptr;
+        int64_t *dstart = nullptr;
+        launch::mk_item *dI = nullptr;
+        void *gbuf = nullptr, *gpin = nullptr, *pin = nullptr;   // device / pinned launch tables
+        bool ok = false, gok = false;
+        std::string err;
+        std::thread tg, tl;                                        // helper threads of (2)
+    };
+    std::vector<group> grp(ngroups);
+    for (int g = 0; g < ngroups; ++g)
+        for (int i = (int)((int64_t)n * g / ngroups); i < (int)((int64_t)n * (g + 1) / ngroups); ++i) grp[g].idx.push_back(i);
     std::vector<char> started(n, 0);
-    int rc = run([&](int i) { int r = prove_start(ctxs[i], false); started[i] = r == 0; return r; });
-    std::vector<vg_ctx *> S;
-    for (int i = 0; i < n; ++i)
-        if (started[i]) S.push_back(ctxs[i]);
-    // 2. the sumcheck layers of every proof in merged launches (one per
-    // step and table width across all proofs), and the leaf chains of every
-    // proof in one launch (60 % of a mixed batch's VALU work): full-width
-    // launches keep every SIMD busy where many small ones (one per proof)
-    // do not.  Both run beside each other on two streams.
     hipStream_t ls = nullptr, gs = nullptr;
-    hipEvent_t done = nullptr, gdone = nullptr;
-    launch::leaf_seg *dG = nullptr;
-    int64_t *dstart = nullptr;
-    void *gbuf = nullptr;
-    std::vector<uint8_t> gkeep;
-    std::vector<launch::leaf_seg> G;
-    std::vector<int64_t> start(1, 0);
-    for (vg_ctx *c : S) {
-        launch::leaf_seg g[24];
-        int ns = commit_segments(c, g);
-        for (int k = 0; k < ns; ++k) { G.push_back(g[k]); start.push_back(start.back() + g[k].S / 2); }
-    }
-    bool ok = hipStreamCreateWithFlags(&ls, hipStreamNonBlocking) == hipSuccess &&
-              hipStreamCreateWithFlags(&gs, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&done, hipEventDisableTiming) == hipSuccess &&
-              hipEventCreateWithFlags(&gdone, hipEventDisableTiming) == hipSuccess &&
-              hipMalloc(&dG, sizeof(launch::leaf_seg) * (G.size() + 1)) == hipSuccess &&
-              hipMalloc(&dstart, sizeof(int64_t) * start.size()) == hipSuccess;
-    std::string gerr;
-    if (ok && batch_gkr(S.data(), (int)S.size(), gs, gdone, &gbuf, gkeep) != 0) { ok = false; gerr = g_err; }
-    for (size_t i = 0; ok && i < S.size(); ++i) ok = hipStreamWaitEvent(ls, S[i]->ev[5], 0) == hipSuccess;
-    ok = ok && hipMemcpyAsync(dG, G.data(), sizeof(launch::leaf_seg) * G.size(), hipMemcpyHostToDevice, ls) == hipSuccess &&
-         hipMemcpyAsync(dstart, start.data(), sizeof(int64_t) * start.size(), hipMemcpyHostToDevice, ls) == hipSuccess;
-    // every Merkle tree of every proof: the large levels from item tables,
-    // then the tails one block per tree
-    std::vector<launch::mk_item> items;
-    std::vector<size_t> lvl_off;
-    std::vector<int64_t> lvl_tot;
-    for (int h = 1;; ++h) {
-        int64_t tot = 0;
-        size_t o = items.size();
-        for (auto &g : G) {
-            int64_t cnt = (g.S / 2) >> h;
-            if (cnt > launch::MERKLE_TAIL) { items.push_back({g.tree, cnt, tot}); tot += cnt; }
+    bool streams_ok = hipStreamCreateWithFlags(&ls, hipStreamNonBlocking) == hipSuccess &&
+                      hipStreamCreateWithFlags(&gs, hipStreamNonBlocking) == hipSuccess;
+    int rc = 0;
+    auto merged_gkr = [&](group &G) {
+        bool ok = streams_ok && hipEventCreateWithFlags(&G.gdone, hipEventDisableTiming) == hipSuccess;
+        for (vg_ctx *c : G.S) ok = ok && hipEventSynchronize(c->ev[9]) == hipSuccess;
+        if (ok && batch_gkr(G.S.data(), (int)G.S.size(), gs, G.gdone, &G.gbuf, &G.gpin, false) != 0) { ok = false; G.err = g_err; }
+        G.gok = ok;
+    };
+    auto merged = [&](group &G) {
+        std::vector<launch::leaf_seg> segs;
+        std::vector<int64_t> start(1, 0);
+        for (vg_ctx *c : G.S) {
+            launch::leaf_seg sg[24];
+            int ns = commit_segments(c, sg);
+            for (int k = 0; k < ns; ++k) { segs.push_back(sg[k]); start.push_back(start.back() + sg[k].S / 2); }
         }
-        if (!tot) break;
-        lvl_off.push_back(o);
-        lvl_tot.push_back(tot);
-    }
-    launch::mk_item *dI = nullptr;
-    ok = ok && hipMalloc(&dI, sizeof(launch::mk_item) * (items.size() + 1)) == hipSuccess;
-    ok = ok && (items.empty() ||
-                hipMemcpyAsync(dI, items.data(), sizeof(launch::mk_item) * items.size(), hipMemcpyHostToDevice, ls) == hipSuccess);
-    if (ok) {
-        launch::leaf_hash_table(dG, dstart, (int)G.size(), start.back(), ls);
-        for (size_t l = 0; l < lvl_off.size(); ++l) {
-            size_t end = l + 1 < lvl_off.size() ? lvl_off[l + 1] : items.size();
-            launch::merkle_level_table(dI + lvl_off[l], (int)(end - lvl_off[l]), lvl_tot[l], ls);
+        bool ok = streams_ok && hipEventCreateWithFlags(&G.done, hipEventDisableTiming) == hipSuccess;
+        // every Merkle tree of the group: the large levels from item tables,
+        // then the tails one block per tree
+        std::vector<launch::mk_item> items;
+        std::vector<size_t> lvl_off;
+        std::vector<int64_t> lvl_tot;
+        for (int h = 1;; ++h) {
+            int64_t tot = 0;
+            size_t o = items.size();
+            for (auto &sg : segs) {
+                int64_t cnt = (sg.S / 2) >> h;
+                if (cnt > launch::MERKLE_TAIL) { items.push_back({sg.tree, cnt, tot}); tot += cnt; }
+            }
+            if (!tot) break;
+            lvl_off.push_back(o);
+            lvl_tot.push_back(tot);
         }
-        launch::merkle_tails_table(dG, (int)G.size(), ls);
-        ok = hipEventRecord(done, ls) == hipSuccess && hipGetLastError() == hipSuccess;
+        // segment, start and item tables: one pinned staging block, one
+        // device block
+        size_t bG = sizeof(launch::leaf_seg) * segs.size(), bS = sizeof(int64_t) * start.size(),
+               bI = sizeof(launch::mk_item) * items.size(), oS = (bG + 255) & ~(size_t)255,
+               oI = oS + ((bS + 255) & ~(size_t)255), tot = oI + bI + 256;
+        uint8_t *dblk = nullptr;
+        ok = ok && hipHostMalloc(&G.pin, tot, hipHostMallocDefault) == hipSuccess &&
+             hipMalloc((void **)&dblk, tot) == hipSuccess;
+        if (ok) {
+            uint8_t *h = (uint8_t *)G.pin;
+            memcpy(h, segs.data(), bG);
+            memcpy(h + oS, start.data(), bS);
+            if (bI) memcpy(h + oI, items.data(), bI);
+            G.dG = (launch::leaf_seg *)dblk;
+            G.dstart = (int64_t *)(dblk + oS);
+            G.dI = (launch::mk_item *)(dblk + oI);
+        }
+        for (vg_ctx *c : G.S) ok = ok && hipEventSynchronize(c->ev[5]) == hipSuccess;
+        ok = ok && hipMemcpyAsync(dblk, G.pin, tot, hipMemcpyHostToDevice, ls) == hipSuccess;
+        if (ok) {
+            launch::leaf_hash_table(G.dG, G.dstart, (int)segs.size(), start.back(), ls);
+            for (size_t l = 0; l < lvl_off.size(); ++l) {
+                size_t end = l + 1 < lvl_off.size() ? lvl_off[l + 1] : items.size();
+                launch::merkle_level_table(G.dI + lvl_off[l], (int)(end - lvl_off[l]), lvl_tot[l], ls);
+            }
+            launch::merkle_tails_table(G.dG, (int)segs.size(), ls);
+            ok = hipEventRecord(G.done, ls) == hipSuccess && hipGetLastError() == hipSuccess;
+        }
+        G.ok = ok;
+    };
+    // (3) of a group, once its merged launches have completed
+    auto post = [&](group &G) {
+        G.tg.join();
+        G.tl.join();
+        bool ok = G.ok && G.gok && hipEventSynchronize(G.gdone) == hipSuccess && hipEventSynchronize(G.done) == hipSuccess;
+        if (!ok && rc == 0) { g_err = G.err.empty() ? std::string("vg_prove_batch: merged launches failed") : G.err; rc = -20; }
+        int r = run(G.idx, [&](int i) {
+            if (!started[i]) return 0;
+            int q = ok ? enqueue_post(ctxs[i], G.done, G.gdone) : 0;
+            ctxs[i]->pending = true;
+            return q;
+        });
+        if (r && rc == 0) rc = r;
+    };
+    for (int g = 0; g < ngroups; ++g) {
+        group &G = grp[g];
+        // the evaluations, challenges and half tables of every proof of the
+        // group first, so that the merged sumcheck layers start early, beside
+        // the commitments enqueued after them on the same streams
+        int r = run(G.idx, [&](int i) { int q = prove_start_gkr(ctxs[i], false); started[i] = q == 0; return q; });
+        if (r && rc == 0) rc = r;
+        for (int i : G.idx)
+            if (started[i]) G.S.push_back(ctxs[i]);
+        G.tg = std::thread([&, g] { merged_gkr(grp[g]); });
+        r = run(G.idx, [&](int i) { return started[i] ? prove_start_commit(ctxs[i]) : 0; });
+        if (r && rc == 0) rc = r;
+        G.tl = std::thread([&, g] { merged(grp[g]); });
+        if (g > 0) post(grp[g - 1]);
     }
-    if (!ok && rc == 0) { g_err = gerr.empty() ? std::string("vg_prove_batch: merged launches failed") : gerr; rc = -20; }
-    // 3. the rest of every started proof, then the transcripts (a failed
-    // merge still drains every proof in flight; its transcripts are dropped)
-    int rc2 = run([&](int i) {
-        if (!started[i]) return 0;
-        int r = ok ? enqueue_post(ctxs[i], done, gdone) : 0;
-        ctxs[i]->pending = true;
-        return r;
-    });
-    int rc3 = run([&](int i) {
+    post(grp[ngroups - 1]);
+    // transcripts (a failed merge still drains every proof in flight; its
+    // transcripts are dropped)
+    std::vector<int> all(n);
+    for (int i = 0; i < n; ++i) all[i] = i;
+    int rc3 = run(all, [&](int i) {
         if (!started[i] || !ctxs[i]->pending) return 0;
         return vg_finish(ctxs[i], &transcripts[i], &lens[i], nullptr);
     });
     if (ls) (void)hipStreamSynchronize(ls);
     if (gs) (void)hipStreamSynchronize(gs);
-    if (dG) (void)hipFree(dG);
-    if (dstart) (void)hipFree(dstart);
-    if (gbuf) (void)hipFree(gbuf);
-    if (dI) (void)hipFree(dI);
-    if (done) (void)hipEventDestroy(done);
-    if (gdone) (void)hipEventDestroy(gdone);
+    for (auto &G : grp) {
+        if (G.dG) (void)hipFree(G.dG);                 // the block holding dstart and dI too
+        if (G.gbuf) (void)hipFree(G.gbuf);
+        if (G.gpin) (void)hipHostFree(G.gpin);
+        if (G.pin) (void)hipHostFree(G.pin);
+        if (G.done) (void)hipEventDestroy(G.done);
+        if (G.gdone) (void)hipEventDestroy(G.gdone);
+    }
     if (ls) (void)hipStreamDestroy(ls);
     if (gs) (void)hipStreamDestroy(gs);
-    int first = rc ? rc : rc2 ? rc2 : rc3;
+    int first = rc ? rc : rc3;
     if (first)
         for (int i = 0; i < n; ++i) { free(transcripts[i]); transcripts[i] = nullptr; lens[i] = 0; }
     return first;

@@ -85,8 +85,9 @@ def batched(args):
     fresh context, summed over the jobs.  Batched: every circuit resident in
     HBM at once, contexts created round-robin on `batch_workers` streams, and
     vg_prove_batch proving them from that many library threads (each proof in
-    order on its stream, the streams overlapping on the device).  Every
-    batched transcript must equal the plain one byte for byte."""
+    order on its stream, the streams overlapping on the device); the median
+    of three timed batches.  Every batched transcript must equal the plain
+    one byte for byte."""
     import virgo_amd as V
     from virgo_amd import workloads as W
     jobs = W.batch_jobs(args.batch_jobs)
@@ -104,9 +105,12 @@ def batched(args):
     setup = time.perf_counter() - t0
     dev_bytes = sum(p.device_bytes() for p in provers)
     V.prove_batch(provers, args.batch_workers)          # warm-up
-    t0 = time.perf_counter()
-    out = V.prove_batch(provers, args.batch_workers)
-    t_batch = time.perf_counter() - t0
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        out = V.prove_batch(provers, args.batch_workers)
+        runs.append(time.perf_counter() - t0)
+    t_batch = sorted(runs)[1]                           # median of three
     same = all(a == b for a, b in zip(plain, out))
     for p in provers:
         p.close()

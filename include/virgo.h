@@ -156,6 +156,12 @@ int vg_sha3_64(const uint8_t *in, uint8_t *out, size_t n);
 /* merkle_tree_prover::create_tree (merkle_tree.cpp:4-57): heap tree of
  * 2*pad digests, pad = next power of two >= n_leaf. */
 int vg_merkle_tree(const uint8_t *leaves, int n_leaf, uint8_t *tree);
+/* One FRI fold of a natural-order codeword of 2^log_n values over the
+ * 2^log_n-th roots of unity w = root of order 2^log_n, as
+ * fri::commit_phase_step folds each slice (fri.cpp:321-340): out[i] =
+ * ((a + b) + r w^-i (a - b)) / 2 with a = cw[i], b = cw[i + 2^(log_n-1)],
+ * i < 2^(log_n-1); the result is the codeword over the squared domain. */
+int vg_fri_fold(const vg_fe *cw, int log_n, const vg_fe *r, vg_fe *out);
 /* element-wise F_{p^2} product (prime_field.h:125-151) */
 int vg_fe_mul(const vg_fe *a, const vg_fe *b, vg_fe *out, size_t n);
 

@@ -49,6 +49,7 @@ def lib():
         L.vgo_lde_slice.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
         L.vgo_leaf_hashes.argtypes = [vp, vp, ctypes.c_int, vp]
         L.vgo_fri_fold.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.vgo_fri_fold1.argtypes = [vp, ctypes.c_int, vp, vp]
         L.vgo_eq_table.argtypes = [vp, ctypes.c_int, vp]
         L.vgo_fft_gkr.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
@@ -232,6 +233,14 @@ def fri_fold(slices, log_S, r):
     slices = np.ascontiguousarray(slices, dtype=FE)
     o = np.zeros(65 * ((1 << log_S) // 2), dtype=FE)
     lib().vgo_fri_fold(_p(slices), log_S, _p(np.ascontiguousarray(r, dtype=FE)), _p(o))
+    return o
+
+
+def fri_fold1(cw, r):
+    """One FRI fold of a single natural-order codeword (fri.cpp:321-340)."""
+    cw = np.ascontiguousarray(cw, dtype=FE)
+    o = np.zeros(len(cw) // 2, dtype=FE)
+    lib().vgo_fri_fold1(_p(cw), len(cw).bit_length() - 1, _p(np.ascontiguousarray(r, dtype=FE).reshape(1)), _p(o))
     return o
 
 

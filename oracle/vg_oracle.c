@@ -1898,6 +1898,18 @@ void vgo_fri_fold(const fe *sl, int log_S, const fe *r, fe *out)
         imu = mul(imu, iw);
     }
 }
+/* one slice of the fold above (fri.cpp:321-340): a natural-order codeword of
+ * 2^log_S values over the 2^log_S-th roots of unity */
+void vgo_fri_fold1(const fe *cw, int log_S, const fe *r, fe *out)
+{
+    int64_t S = (int64_t)1 << log_S, half = S / 2;
+    fe iw = finv(root_of_unity(log_S)), inv_2 = finv(fe_u(2)), imu = ONE();
+    for (int64_t i = 0; i < half; ++i) {
+        fe a = cw[i], b = cw[i + half];
+        out[i] = mul(inv_2, add(add(a, b), mul(mul(imu, *r), sub(a, b))));
+        imu = mul(imu, iw);
+    }
+}
 void vgo_eq_table(const fe *r, int len, fe *out) { eq_table(r, len, out); }
 int vgo_evaluate(const vgo_circuit_desc *dsc, fe *out, size_t cap, int *bits)
 {

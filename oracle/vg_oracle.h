@@ -73,6 +73,8 @@ void vgo_leaf_hashes(const vgo_fe *slices /*64 x S*/, const vgo_fe *mask /*S*/,
 void vgo_fri_fold(const vgo_fe *slices /*65 x S*/, int log_S, const vgo_fe *r,
                   vgo_fe *out /*65 x S/2*/);
 /* eq-table q[x] = prod_i (x_i ? r_i : 1-r_i)  (zk_verifier.cpp:838-847) */
+/* fri.cpp:321-340 for one natural-order codeword of 2^log_S values */
+void vgo_fri_fold1(const vgo_fe *cw, int log_S, const vgo_fe *r, vgo_fe *out);
 void vgo_eq_table(const vgo_fe *r, int len, vgo_fe *out);
 
 /* ---- whole protocol ---- */

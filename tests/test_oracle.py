@@ -169,6 +169,28 @@ def test_fri_fold_halves_degree():
         assert all(x == (0, 0) for x in to_int(coef[4:]))
 
 
+def test_fri_fold1_closed_form():
+    """One fold of the codeword of f = sum c_k x^k is the codeword (over the
+    squared domain) of sum (c_2k + r c_2k+1) y^k, checked with big-int
+    arithmetic on the coefficients; and it equals slice 0 of the 65-slice
+    fold of fri.cpp:321-340."""
+    rng = np.random.default_rng(12)
+    log_S, d = 7, 16
+    S = 1 << log_S
+    c = np.zeros(S, dtype=O.FE)
+    c[:d] = rand_fe(rng, d)
+    cw = O.fft(c, S, O.root_of_unity(log_S))
+    r = rand_fe(rng, 1)
+    got = O.fri_fold1(cw, r)
+    ci, ri = to_int(c), to_int(r)[0]
+    folded = [((a[0] + py_mul(ri, b)[0]) % P, (a[1] + py_mul(ri, b)[1]) % P) for a, b in zip(ci[0::2], ci[1::2])]
+    want = O.fft(from_pairs(folded), S // 2, O.root_of_unity(log_S - 1))
+    assert to_int(got) == to_int(want)
+    sl = np.zeros(65 * S, dtype=O.FE)
+    sl[:S] = cw
+    assert to_int(O.fri_fold(sl, log_S, r)[:S // 2]) == to_int(got)
+
+
 def test_eq_table_bigint():
     rng = np.random.default_rng(9)
     r = rand_fe(rng, 5)

@@ -19,7 +19,7 @@ FE = np.dtype([("img", "<u8"), ("real", "<u8")])     # prime_field.h:93 layout
 
 EXPORTS = ["vg_init", "vg_last_error", "vg_free", "vg_ctx_create", "vg_ctx_destroy", "vg_ctx_device_bytes",
            "vg_prove", "vg_prove_async", "vg_finish", "vg_output", "vg_fft", "vg_ifft", "vg_sha3_64",
-           "vg_merkle_tree", "vg_fe_mul", "vg_fft_gkr", "vg_stream_create", "vg_stream_destroy", "vg_prove_batch",
+           "vg_merkle_tree", "vg_fri_fold", "vg_fe_mul", "vg_fft_gkr", "vg_stream_create", "vg_stream_destroy", "vg_prove_batch",
            "vg_verify", "vg_server_create", "vg_server_submit", "vg_server_wait", "vg_server_stats",
            "vg_server_destroy"]
 
@@ -62,6 +62,7 @@ def lib():
         L.vg_sha3_64.argtypes = [vp, vp, ctypes.c_size_t]
         L.vg_merkle_tree.argtypes = [vp, ctypes.c_int, vp]
         L.vg_fe_mul.argtypes = [vp, vp, vp, ctypes.c_size_t]
+        L.vg_fri_fold.argtypes = [vp, ctypes.c_int, vp, vp]
         L.vg_prove_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.vg_stream_create.argtypes = [ctypes.POINTER(vp)]
         L.vg_stream_destroy.argtypes = [vp]
@@ -321,6 +322,18 @@ def sha3_64(blocks):
     blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 64)
     o = np.zeros((len(blocks), 32), dtype=np.uint8)
     _check(lib().vg_sha3_64(_p(blocks), _p(o), len(blocks)))
+    return o
+
+
+def fri_fold(cw, r):
+    """One FRI fold of a natural-order codeword (len a power of two) with
+    challenge r: the codeword over the squared domain (vg_fri_fold)."""
+    init()
+    cw = np.ascontiguousarray(cw, dtype=FE)
+    n = len(cw)
+    assert n >= 2 and n & (n - 1) == 0
+    o = np.zeros(n // 2, dtype=FE)
+    _check(lib().vg_fri_fold(_p(cw), n.bit_length() - 1, _p(np.ascontiguousarray(r, dtype=FE).reshape(1)), _p(o)))
     return o
 
 

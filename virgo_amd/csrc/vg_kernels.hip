@@ -913,6 +913,16 @@ __global__ __launch_bounds__(TPB) void k_ntt_regs(fe *data, int64_t stride, int6
 }
 
 // tw[e] = T1[e & 1023] * T2[e >> 10]
+// one FRI fold of a natural-order codeword over the 2n-th roots of unity
+// (fri.cpp:321-340 for one slice): out[i] = ((a + b) + r w^-i (a - b)) / 2
+// with a = in[i], b = in[i + n] = f(-w^i); iw[i] = w^-i
+__global__ void k_fri_fold_nat(const fe *in, int64_t n, const fe *r, const fe *iw, fe *out)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fe a = in[i], b = in[i + n];
+    out[i] = mul_base(add(add(a, b), mul(mul(*r, iw[i]), sub(a, b))), 1ull << 60);    // 2^-1 = 2^60
+}
 __global__ void k_twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2)
 {
     int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1680,6 +1690,10 @@ void ntt_tables(fe *W, fe *Wi, int maxlog, fe *TA, fe *TB, int log_R, int th, in
     int R = TA ? 1 << log_R : 0;
     int64_t n = ((int64_t)1 << maxlog) + ((int64_t)R << th) + ((int64_t)R << tb);
     hipLaunchKernelGGL(k_ntt_tables, dim3(nblk(n)), dim3(TPB), 0, s, W, Wi, maxlog, TA, TB, R, th, tb, tw, SS);
+}
+void fri_fold_nat(const fe *in, int64_t n, const fe *r, const fe *iw, fe *out, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fri_fold_nat, dim3(nblk(n)), dim3(TPB), 0, s, in, n, r, iw, out);
 }
 void twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2, hipStream_t s)
 {

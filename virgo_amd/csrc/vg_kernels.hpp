@@ -148,6 +148,9 @@ void general_multi(const layer_dev *D, int n, int ng_max, hipStream_t s);
 
 // tw[e] = w^e, e < n, from T1[j] = w^j (j < 1024) and T2[j] = w^(1024 j)
 void twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2, hipStream_t s);
+// one FRI fold of a natural-order codeword of 2n values (fri.cpp:321-340):
+// out[i] = ((in[i] + in[i+n]) + r iw[i] (in[i] - in[i+n])) / 2, iw[i] = w^-i
+void fri_fold_nat(const fe *in, int64_t n, const fe *r, const fe *iw, fe *out, hipStream_t s);
 // Twiddles of the NTTs: stage-major W[2^s + k] = w_{2^(s+1)}^k and Wi (the
 // inverse powers) for s < maxlog, so each stage reads a contiguous run; the
 // LDE's coset twist w^(s j) = TA[s][j mod 2^th] TB[s][j >> th] for rows

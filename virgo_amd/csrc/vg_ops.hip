@@ -1,8 +1,8 @@
 // Single-operation entry points of the C ABI on host arrays: the drop-in
 // replacements of fast_fourier_transform / inverse_fast_fourier_transform
 // (RS_polynomial.cpp:27-240), my_hhash (my_hhash.h:39-46),
-// merkle_tree_prover::create_tree (merkle_tree.cpp:4-57) and the field
-// product (prime_field.h:125-151).  Each call copies in, runs the HIP kernels
+// merkle_tree_prover::create_tree (merkle_tree.cpp:4-57), one FRI fold of a
+// codeword (fri.cpp:321-340) and the field product (prime_field.h:125-151).  Each call copies in, runs the HIP kernels
 // on the null stream and copies out.
 #include <cstdio>
 #include <cstring>
@@ -35,9 +35,8 @@ struct dbuf {
 int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
 bool pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
 
-// tw[e] = w^e for e < n on the device, and the stage-major tables of an
-// NTT of size n built from them
-int make_twiddles(const fe &w, int64_t n, dbuf &tw, dbuf &W, launch::ntt_tabs &T)
+// tw[e] = w^e for e < n on the device (from two small tables)
+int make_powers(const fe &w, int64_t n, dbuf &tw)
 {
     std::vector<fe> t(1024 + (n >> 10) + 1);
     fe x = fone();
@@ -47,8 +46,17 @@ int make_twiddles(const fe &w, int64_t n, dbuf &tw, dbuf &W, launch::ntt_tabs &T
     dbuf T12;
     OPCHK(hipMalloc(&T12.p, sizeof(fe) * t.size()));
     OPCHK(hipMemcpy(T12.p, t.data(), sizeof(fe) * t.size(), hipMemcpyHostToDevice));
-    OPCHK(hipMalloc(&tw.p, sizeof(fe) * n));
+    OPCHK(hipMalloc(&tw.p, sizeof(fe) * (n ? n : 1)));
     launch::twiddles((fe *)tw.p, n, (fe *)T12.p, (fe *)T12.p + 1024, 0);
+    OPCHK(hipDeviceSynchronize());
+    return 0;
+}
+// tw[e] = w^e for e < n on the device, and the stage-major tables of an
+// NTT of size n built from them
+int make_twiddles(const fe &w, int64_t n, dbuf &tw, dbuf &W, launch::ntt_tabs &T)
+{
+    int rc = make_powers(w, n, tw);
+    if (rc) return rc;
     int lg = ilog2(n);
     OPCHK(hipMalloc(&W.p, sizeof(fe) * 2 * n));
     fe *w0 = (fe *)W.p, *w1 = w0 + n;
@@ -134,6 +142,24 @@ int vg_merkle_tree(const uint8_t *leaves, int n_leaf, uint8_t *tree)
     launch::merkle_multi(1, &tp, &npad, 0);
     OPCHK(hipGetLastError());
     OPCHK(hipMemcpy(tree, t.p, 64 * pad, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vg_fri_fold(const vg_fe *cw, int log_n, const vg_fe *r, vg_fe *out)
+{
+    if (log_n < 1 || log_n > 30 || !cw || !r || !out) { op_err = "vg_fri_fold: bad arguments (1 <= log_n <= 30)"; return -1; }
+    int64_t n = (int64_t)1 << log_n, half = n / 2;
+    dbuf iw, in, o, rr;
+    int rc = make_powers(finv(root_of_unity(log_n)), half, iw);
+    if (rc) return rc;
+    OPCHK(hipMalloc(&in.p, sizeof(fe) * n));
+    OPCHK(hipMalloc(&o.p, sizeof(fe) * half));
+    OPCHK(hipMalloc(&rr.p, sizeof(fe)));
+    OPCHK(hipMemcpy(in.p, cw, sizeof(fe) * n, hipMemcpyHostToDevice));
+    OPCHK(hipMemcpy(rr.p, r, sizeof(fe), hipMemcpyHostToDevice));
+    launch::fri_fold_nat((const fe *)in.p, half, (const fe *)rr.p, (const fe *)iw.p, (fe *)o.p, 0);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(out, o.p, sizeof(fe) * half, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -554,7 +554,8 @@ struct vg_ctx {
     double host_ms[3] = {};     // prepare (challenges, masks, gather table), enqueue, assemble
     bool pending = false;
     bool serial = false;        // the whole proof in order on st (caller's stream, large input or VIRGO_SERIAL=1)
-    bool own_aux = false;       // st2 and gst[] created by the context
+    bool own_aux = false;       // st2 created by the context
+    int own_gst = 0;            // gst[0..own_gst) created by the context (else gst[] = st)
     fg_ctx *fg = nullptr;       // fft_gkr of the commitment (lg = n - 6), on its own stream
 };
 
@@ -844,7 +845,8 @@ int enqueue_gkr(vg_ctx *c, bool gkr = true)
     launch::mle_eval(c->cv[D - 1], C.bl[D - 1], c->chal + c->chal_r0, c->mle_t[0], c->mle_t[1], c->msg + c->msg_vres, st);
     // ---- GKR layers ----
     // Every layer's sumcheck depends only on the evaluated circuit and on
-    // challenges known in advance, so layers run concurrently on NGS streams.
+    // challenges known in advance, so layer batches may run on several
+    // streams (gst[], the main stream unless VIRGO_GKR_STREAMS asks for more).
     if (!c->serial && gkr)
         for (int w = 0; w < vg_ctx::NGS; ++w) HIPCHK(hipStreamWaitEvent(c->gst[w], c->ev[9], 0));
     int slot = 0;
@@ -1872,13 +1874,15 @@ int ctx_create_planned(vg_ctx **out, const vg_circuit_desc *d, void *stream)
     {
         // A caller's stream runs the whole proof in order on one stream:
         // several contexts then overlap on the device without sharing
-        // hardware queues.  Otherwise the context owns one stream for the
-        // commitment, four for the sumcheck layers and one for fft_gkr, so
-        // that the latency-bound chains of a small proof overlap; from 2^20
-        // inputs on, every stage fills the device by itself and overlapping
-        // them only adds contention (measured: 13.1 vs 12.9 ms at 2^20 gates
-        // x 16 layers, 4.1 vs 4.3 ms at 2^18 x 16), so such proofs run in
-        // order.  VIRGO_SERIAL=1 / =0 forces one choice.
+        // hardware queues.  Otherwise the context owns three streams (main:
+        // evaluation and sumcheck; commitment; fft_gkr), each on its own
+        // hardware queue, so that a small proof's latency-bound chains
+        // overlap (median ms, serial vs overlapped: 16x16 matmul 1.63 vs
+        // 1.36, 2^16 x 12 layers 2.66 vs 2.31, 2^18 x 16 3.81 vs 3.55); from
+        // 2^20 inputs on, the sumcheck and the extensions each keep the VALU
+        // busy and overlapping them gains nothing (2^20 x 16: 12.24 vs
+        // 12.50), so such proofs run in order, fft_gkr still on its own
+        // stream.  VIRGO_SERIAL=1 / =0 forces one choice.
         const char *e = getenv("VIRGO_SERIAL");
         c->serial = stream != nullptr || (e && e[0] == '1') || (!(e && e[0] == '0') && c->C.bl[0] >= 20);
     }
@@ -1889,8 +1893,19 @@ int ctx_create_planned(vg_ctx **out, const vg_circuit_desc *d, void *stream)
             for (int w = 0; w < vg_ctx::NGS; ++w) c->gst[w] = c->st;
         } else {
             e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
-            for (int w = 0; w < vg_ctx::NGS && e == hipSuccess; ++w) e = hipStreamCreateWithFlags(&c->gst[w], hipStreamNonBlocking);
             c->own_aux = true;
+            // The sumcheck layers run on the main stream, or on k streams of
+            // their own (VIRGO_GKR_STREAMS=k).  Every stream takes one of the
+            // process's hardware queues (4 by default) in turn, and kernels of
+            // streams that share a queue run one after another: the main
+            // stream, the commitment's and fft_gkr's then hold three distinct
+            // queues and truly overlap.
+            int k = 1;
+            if (const char *g = getenv("VIRGO_GKR_STREAMS")) k = std::max(1, std::min(vg_ctx::NGS, atoi(g)));
+            for (int w = 0; w < vg_ctx::NGS; ++w) c->gst[w] = c->st;
+            for (int w = 0; k > 1 && w < k && e == hipSuccess; ++w)
+                if ((e = hipStreamCreateWithFlags(&c->gst[w], hipStreamNonBlocking)) == hipSuccess) c->own_gst = w + 1;
+            for (int w = k; k > 1 && w < vg_ctx::NGS; ++w) c->gst[w] = c->gst[w % k];
         }
         if (e != hipSuccess) { vg_ctx_destroy(c); FAILV(-20, "hipStreamCreate: %s", hipGetErrorString(e)); }
     }
@@ -1956,11 +1971,8 @@ int vg_ctx_destroy(vg_ctx *c)
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto &e : c->ev_leaf) if (e) (void)hipEventDestroy(e);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
-    if (c->own_aux) {
-        if (c->st2) (void)hipStreamDestroy(c->st2);
-        for (int w = 0; w < vg_ctx::NGS; ++w)
-            if (c->gst[w]) (void)hipStreamDestroy(c->gst[w]);
-    }
+    if (c->own_aux && c->st2) (void)hipStreamDestroy(c->st2);
+    for (int w = 0; w < c->own_gst; ++w) (void)hipStreamDestroy(c->gst[w]);
     for (int w = 0; w < vg_ctx::NGS; ++w)
         if (c->gev[w]) (void)hipEventDestroy(c->gev[w]);
     delete c;

@@ -104,7 +104,8 @@ def batched(args):
     provers = [V.Prover(c, stream=streams[k % args.batch_workers]) for k, (_, c) in enumerate(jobs)]
     setup = time.perf_counter() - t0
     dev_bytes = sum(p.device_bytes() for p in provers)
-    V.prove_batch(provers, args.batch_workers)          # warm-up
+    for _ in range(2):                                  # warm-up
+        V.prove_batch(provers, args.batch_workers)
     runs = []
     for _ in range(3):
         t0 = time.perf_counter()

@@ -228,6 +228,9 @@ def main():
                      "kernel": "k_leaf_multi (SHA3-256 leaf chains, %d chains x 65 Keccak-f)" % chains,
                      "kernel_ms": round(leaf_ms, 3)},
     }
+    # the NTT autotuner's choice for this proof's transform sizes
+    line["config"]["ntt_tiles"] = {"2^%d" % k: V.ntt_plan(k)[0] for k in (n - 6, n - 5)}
+    line["config"]["ntt_tile_us"] = {"2^%d" % k: V.ntt_plan(k)[1] for k in (n - 6, n - 5)}
     # the device verifier on the same transcript (the sumcheck round checks
     # on the host, predicates, public array, query evaluations and paths on
     # the device)

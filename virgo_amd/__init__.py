@@ -21,7 +21,7 @@ EXPORTS = ["vg_init", "vg_last_error", "vg_free", "vg_ctx_create", "vg_ctx_destr
            "vg_prove", "vg_prove_async", "vg_finish", "vg_output", "vg_fft", "vg_ifft", "vg_sha3_64",
            "vg_merkle_tree", "vg_fri_fold", "vg_fe_mul", "vg_fft_gkr", "vg_stream_create", "vg_stream_destroy", "vg_prove_batch",
            "vg_verify", "vg_server_create", "vg_server_submit", "vg_server_wait", "vg_server_stats",
-           "vg_server_destroy"]
+           "vg_server_destroy", "vg_ntt_plan"]
 
 
 class VirgoError(RuntimeError):
@@ -63,6 +63,7 @@ def lib():
         L.vg_merkle_tree.argtypes = [vp, ctypes.c_int, vp]
         L.vg_fe_mul.argtypes = [vp, vp, vp, ctypes.c_size_t]
         L.vg_fri_fold.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.vg_ntt_plan.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), vp]
         L.vg_prove_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.vg_stream_create.argtypes = [ctypes.POINTER(vp)]
         L.vg_stream_destroy.argtypes = [vp]
@@ -323,6 +324,15 @@ def sha3_64(blocks):
     o = np.zeros((len(blocks), 32), dtype=np.uint8)
     _check(lib().vg_sha3_64(_p(blocks), _p(o), len(blocks)))
     return o
+
+
+def ntt_plan(log_n):
+    """(tile_log, {tile_log: us}) of the NTT autotuner for 2^log_n
+    transforms (vg_ntt_plan)."""
+    t = ctypes.c_int()
+    us = np.zeros(3, dtype=np.float32)
+    _check(lib().vg_ntt_plan(log_n, ctypes.byref(t), _p(us)))
+    return t.value, {10 + k: round(float(us[k]), 1) for k in range(3) if us[k] > 0}
 
 
 def fri_fold(cw, r):

@@ -4,7 +4,9 @@
 // every field operation returns the canonical representative.
 #include "vg_kernels.hpp"
 #include "vg_keccak.hpp"
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 namespace vg {
 namespace {
@@ -866,10 +868,12 @@ __device__ __forceinline__ void ntt_pass(fe *sh, int M, int s0, int first, int l
         }
     }
 }
+// LM: log2 of the LDS tile (the block's transform size when log_n >= LM)
+template <int LM>
 __global__ __launch_bounds__(TPB) void k_ntt_lds(ntt_in in, fe *out, int64_t out_stride, int log_n, int log_m, int inverse,
                                                 fe scale, int do_scale, launch::ntt_tabs T)
 {
-    __shared__ fe sh[(1 << NTT_LM) + (1 << (NTT_LM - NTT_PAD))];
+    __shared__ fe sh[(1 << LM) + (1 << (LM - NTT_PAD))];
     int64_t b = blockIdx.y;
     int64_t q = blockIdx.x;
     int M = 1 << log_m;
@@ -1266,14 +1270,41 @@ __global__ void k_fe_mul(const fe *a, const fe *b, fe *o, int64_t n)
 
 int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
 
-void ntt_run(const ntt_in &in, fe *out, int64_t out_stride, int64_t nb, int log_n, int inverse, fe scale, int do_scale,
-             const launch::ntt_tabs &T, hipStream_t st)
+// ---- NTT tile autotuner: the LDS tile (2^10, 2^11 or 2^12 elements) of
+// each transform size, timed once per process on the first context that
+// needs that size; any tile gives the same bits (exact arithmetic) ----
+constexpr int NTT_LM_MIN = 10, NTT_LM_MAX = 12;
+std::mutex g_tune_mu;
+int g_ntt_lm[64];                     // chosen tile per log_n (0: not tuned)
+float g_ntt_us[64][NTT_LM_MAX + 1];   // measured us per candidate (0: not run)
+int forced_lm()
 {
-    int log_m = log_n < NTT_LM ? log_n : NTT_LM;
+    static int v = [] {
+        const char *e = getenv("VIRGO_NTT_LM");
+        int x = e ? atoi(e) : 0;
+        return x >= NTT_LM_MIN && x <= NTT_LM_MAX ? x : 0;
+    }();
+    return v;
+}
+int ntt_lm_for(int log_n)
+{
+    if (int f = forced_lm()) return f;
+    int t = (log_n >= 0 && log_n < 64) ? __atomic_load_n(&g_ntt_lm[log_n], __ATOMIC_ACQUIRE) : 0;
+    return t ? t : NTT_LM;
+}
+
+void ntt_run_lm(const ntt_in &in, fe *out, int64_t out_stride, int64_t nb, int log_n, int inverse, fe scale, int do_scale,
+                const launch::ntt_tabs &T, hipStream_t st, int lm)
+{
+    int log_m = log_n < lm ? log_n : lm;
     int rest = log_n - log_m;
     dim3 grid((unsigned)((int64_t)1 << (log_n - log_m)), (unsigned)nb);
-    hipLaunchKernelGGL(k_ntt_lds, grid, dim3(TPB), 0, st, in, out, out_stride, log_n, log_m, inverse, scale,
-                       rest == 0 ? do_scale : 0, T);
+    int ds0 = rest == 0 ? do_scale : 0;
+    switch (lm) {
+    case 10: hipLaunchKernelGGL(k_ntt_lds<10>, grid, dim3(TPB), 0, st, in, out, out_stride, log_n, log_m, inverse, scale, ds0, T); break;
+    case 12: hipLaunchKernelGGL(k_ntt_lds<12>, grid, dim3(TPB), 0, st, in, out, out_stride, log_n, log_m, inverse, scale, ds0, T); break;
+    default: hipLaunchKernelGGL(k_ntt_lds<NTT_LM>, grid, dim3(TPB), 0, st, in, out, out_stride, log_n, log_m, inverse, scale, ds0, T); break;
+    }
     int s0 = log_m;
     while (rest > 0) {
         int g = rest > 3 ? 3 : rest;
@@ -1288,6 +1319,11 @@ void ntt_run(const ntt_in &in, fe *out, int64_t out_stride, int64_t nb, int log_
         }
         s0 += g;
     }
+}
+void ntt_run(const ntt_in &in, fe *out, int64_t out_stride, int64_t nb, int log_n, int inverse, fe scale, int do_scale,
+             const launch::ntt_tabs &T, hipStream_t st)
+{
+    ntt_run_lm(in, out, out_stride, nb, log_n, inverse, scale, do_scale, T, st, ntt_lm_for(log_n));
 }
 
 
@@ -1690,6 +1726,46 @@ void ntt_tables(fe *W, fe *Wi, int maxlog, fe *TA, fe *TB, int log_R, int th, in
     int R = TA ? 1 << log_R : 0;
     int64_t n = ((int64_t)1 << maxlog) + ((int64_t)R << th) + ((int64_t)R << tb);
     hipLaunchKernelGGL(k_ntt_tables, dim3(nblk(n)), dim3(TPB), 0, s, W, Wi, maxlog, TA, TB, R, th, tb, tw, SS);
+}
+int ntt_tune(int log_n, int64_t nb, const fe *in, fe *out, const ntt_tabs &T, hipStream_t s)
+{
+    if (log_n <= NTT_LM_MIN || log_n >= 64 || forced_lm()) return 0;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (g_ntt_lm[log_n]) return 0;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return -1;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return -1; }
+    ntt_in ni;
+    memset(&ni, 0, sizeof ni);
+    ni.kind = 0; ni.src = in; ni.stride = (int64_t)1 << log_n;
+    int best = NTT_LM;
+    float best_us = 0;
+    bool ok = true;
+    for (int lm = NTT_LM_MIN; lm <= NTT_LM_MAX && lm <= log_n && ok; ++lm) {
+        float us = 0;
+        for (int rep = 0; rep < 4 && ok; ++rep) {
+            ok = hipEventRecord(e0, s) == hipSuccess;
+            ntt_run_lm(ni, out, (int64_t)1 << log_n, nb, log_n, 0, fone(), 0, T, s, lm);
+            ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+            float ms = 0;
+            ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            if (rep > 0 && (us == 0 || ms * 1000 < us)) us = ms * 1000;   // best of 3 after a warm-up
+        }
+        g_ntt_us[log_n][lm] = us;
+        if (ok && (best_us == 0 || us < best_us)) { best_us = us; best = lm; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!ok) return -1;
+    __atomic_store_n(&g_ntt_lm[log_n], best, __ATOMIC_RELEASE);
+    return 0;
+}
+int ntt_plan(int log_n, float *us)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (log_n < 0 || log_n >= 64) return 0;
+    for (int lm = NTT_LM_MIN; lm <= NTT_LM_MAX; ++lm) us[lm - NTT_LM_MIN] = g_ntt_us[log_n][lm];
+    return ntt_lm_for(log_n);
 }
 void fri_fold_nat(const fe *in, int64_t n, const fe *r, const fe *iw, fe *out, hipStream_t s)
 {

@@ -1409,6 +1409,12 @@ int ctx_alloc(vg_ctx *c)
         launch::twiddles(c->tw, SS, c->T12, c->T12 + 1024, c->st);
         launch::ntt_tables(c->ntW, c->ntWi, c->log_C + 1, c->ntTA, c->ntTB, c->log_R, c->NT.th, c->NT.tb, c->tw, SS, c->st);
     }
+    // the LDS tile of this proof's transform sizes (the extensions: 65 x 2^R
+    // rows of 2^log_C; the l q products: 65 rows of 2^(log_C+1)), tuned once
+    // per process on the context's codeword buffers before they hold anything
+    if (launch::ntt_tune(c->log_C, (int64_t)65 << c->log_R, c->l, c->q, c->NT, c->st) ||
+        launch::ntt_tune(c->log_C + 1, 65, c->l, c->q, c->NT, c->st))
+        FAILV(-20, "NTT tile tuning failed");
     HIPCHK(hipStreamSynchronize(c->st));
     return 0;
 }
@@ -1976,6 +1982,15 @@ int vg_ctx_destroy(vg_ctx *c)
     for (int w = 0; w < vg_ctx::NGS; ++w)
         if (c->gev[w]) (void)hipEventDestroy(c->gev[w]);
     delete c;
+    return 0;
+}
+
+int vg_ntt_plan(int log_n, int *tile_log, float *us)
+{
+    if (!tile_log) FAILV(-1, "vg_ntt_plan: null argument");
+    float t[3] = {0, 0, 0};
+    *tile_log = launch::ntt_plan(log_n, t);
+    if (us) for (int k = 0; k < 3; ++k) us[k] = t[k];
     return 0;
 }
 

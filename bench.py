@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-budget-gb", type=int, default=96)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--part-only", choices=["batch", "serve"],
+                    help="run only the batched or the serving part and print its object (for profiling)")
     return ap.parse_args()
 
 
@@ -182,6 +184,9 @@ def main():
         sys.exit(2)
     import virgo_amd as V
     from virgo_amd import workloads as W
+    if args.part_only:
+        print(json.dumps({args.part_only: batched(args) if args.part_only == "batch" else serving(args)}))
+        return
     circ = W.random_circuit(args.log_width, args.depth, seed=1)
     p = V.Prover(circ)
     for _ in range(args.warmup):

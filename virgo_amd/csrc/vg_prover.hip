@@ -1040,10 +1040,9 @@ int enqueue_post(vg_ctx *c, hipEvent_t leaf_done, hipEvent_t gkr_done = nullptr)
         // latency-bound sumcheck and fft_gkr launches only slow both down,
         // so they start once those are done (the LDEs and folds before them
         // do overlap the sumcheck)
-        if (!c->serial) {
+        if (!c->serial)
             for (int w = 0; w < vg_ctx::NGS; ++w) HIPCHK(hipStreamWaitEvent(st, c->gev[w], 0));
-            if (c->fg) HIPCHK(hipStreamWaitEvent(st, fg_done_event(c->fg), 0));
-        }
+        if (c->fg && c->own_stream) HIPCHK(hipStreamWaitEvent(st, fg_done_event(c->fg), 0));   // fft_gkr on its own stream
         HIPCHK(hipEventRecord(c->ev_leaf[0], st));
         launch::leaf_hash_multi(ns, cws, Ss, trees, c->log_R, st);
     }

@@ -100,3 +100,23 @@ def test_server_repeat_and_collect_errors(alone):
     with pytest.raises(V.VirgoError, match="unknown or already collected"):
         srv.wait(ids[0])
     srv.close()
+
+
+def test_server_bad_request_fails_alone(alone):
+    """A request the circuit loader rejects fails on its own wait; the
+    requests around it are unaffected; a server needs a budget."""
+    circs, ref = alone
+    with pytest.raises(V.VirgoError, match="bad budget"):
+        V.Server(device_budget=0, slots=2)
+    srv = V.Server(device_budget=2 << 30, slots=2)
+    bad = dict(circs["random_w10_d4"])
+    bad["u"] = bad["u"].copy()
+    bad["u"][-1] = 1 << 20                      # past the previous layer
+    jb = srv.submit(bad)
+    jg = srv.submit(circs["matmul8"])
+    with pytest.raises(V.VirgoError, match="out of bounds"):
+        srv.wait(jb)
+    assert srv.wait(jg)[0] == ref["matmul8"]
+    st = srv.stats()
+    srv.close()
+    assert st["failed"] == 1 and st["completed"] == 2

@@ -35,43 +35,15 @@
 
 #include <hip/hip_runtime.h>
 #include "vg_internal.hpp"
+#include "vg_range.hpp"
 
 using clk = std::chrono::steady_clock;
 
 namespace {
 
-constexpr int64_t ALIGN = 256;
-int64_t up(int64_t n) { return (n + ALIGN - 1) & ~(ALIGN - 1); }
+using vg::range_alloc;
+int64_t up(int64_t n) { return vg::range_alloc::up(n); }
 double ms(clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); }
-
-// First-fit allocator over [0, size) with coalescing of freed ranges.
-struct range_alloc {
-    std::map<int64_t, int64_t> free_;      // offset -> length
-    void init(int64_t n) { free_.clear(); if (n > 0) free_[0] = n; }
-    int64_t take(int64_t n)
-    {
-        n = up(n);
-        for (auto it = free_.begin(); it != free_.end(); ++it)
-            if (it->second >= n) {
-                int64_t off = it->first, len = it->second;
-                free_.erase(it);
-                if (len > n) free_[off + n] = len - n;
-                return off;
-            }
-        return -1;
-    }
-    void give(int64_t off, int64_t n)
-    {
-        n = up(n);
-        auto it = free_.emplace(off, n).first;
-        auto nx = std::next(it);
-        if (nx != free_.end() && it->first + it->second == nx->first) { it->second += nx->second; free_.erase(nx); }
-        if (it != free_.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) { pv->second += it->second; free_.erase(it); }
-        }
-    }
-};
 
 // A proof's lease: bump allocation inside its reserved ranges; frees are
 // no-ops (the whole lease goes back at once).

@@ -585,6 +585,37 @@ __global__ void k_reduce_slots(const fe *part, const int *nb, const int64_t *dst
     block_sum<3>(acc);
     if (threadIdx.x == 0) { msg[dst[R]] = acc[0]; msg[dst[R] + 1] = acc[1]; msg[dst[R] + 2] = acc[2]; }
 }
+// last job j with start[j] <= x
+__device__ __forceinline__ int job_of(const int64_t *start, int n, int64_t x)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+// k_reduce_slots over the slots of many proofs: block = one slot of one job
+__global__ void k_reduce_slots_table(const launch::slot_job *J, const int64_t *start, int n)
+{
+    int j = job_of(start, n, blockIdx.x);
+    const launch::slot_job &q = J[j];
+    int R = (int)(blockIdx.x - start[j]);
+    fe acc[3] = {fzero(), fzero(), fzero()};
+    for (int b = threadIdx.x; b < q.nb[R]; b += blockDim.x)
+        for (int c = 0; c < 3; ++c) acc[c] = add(acc[c], q.part[((int64_t)R * 3 + c) * FB_MAXB + b]);
+    block_sum<3>(acc);
+    if (threadIdx.x == 0) { q.msg[q.dst[R]] = acc[0]; q.msg[q.dst[R] + 1] = acc[1]; q.msg[q.dst[R] + 2] = acc[2]; }
+}
+__global__ void k_copy_table(const launch::copy_job *J, const int64_t *start, int n, int64_t total)
+{
+    int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (x >= total) return;
+    int j = job_of(start, n, x);
+    const launch::copy_job &q = J[j];
+    int64_t k = x - start[j];
+    q.out[k] = q.addr ? *(const fe *)(uintptr_t)q.addr[k] : q.base[k];
+}
 // every eq half table of the proof in one launch: out[j] = f0 * prod_{i<nb} eq(bit_i(j), r[i])
 __global__ void k_half_multi(const launch::half_desc *d, int nd, int64_t total)
 {
@@ -1885,6 +1916,14 @@ void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStre
         hipLaunchKernelGGL(k_merkle_multi, dim3(nblk(total)), dim3(TPB), 0, s, T, h, total);
     }
     hipLaunchKernelGGL(k_merkle_tails, dim3(ntree), dim3(256), 0, s, T);
+}
+void reduce_slots_table(const slot_job *J, const int64_t *start, int n, int64_t total, hipStream_t s)
+{
+    if (total) hipLaunchKernelGGL(k_reduce_slots_table, dim3((unsigned)total), dim3(256), 0, s, J, start, n);
+}
+void copy_table(const copy_job *J, const int64_t *start, int n, int64_t total, hipStream_t s)
+{
+    if (total) hipLaunchKernelGGL(k_copy_table, dim3(nblk(total)), dim3(TPB), 0, s, J, start, n, total);
 }
 void gather16(const uint64_t *src, int64_t n, fe *out, hipStream_t s)
 {

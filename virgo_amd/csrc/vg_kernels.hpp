@@ -225,6 +225,23 @@ void leaf_hash_table(const leaf_seg *G, const int64_t *start, int nseg, int64_t 
 // prefix sum of the node counts of the level), the rest one block per tree
 // G[k] (nleaf = S / 2)
 constexpr int MERKLE_TAIL = 1024;
+// phase 3 of many proofs in two launches (batched proving): every proof's
+// slot reduction (as reduce_slots), then every proof's query answers and
+// message block copied straight into its page-locked host staging
+struct slot_job {
+    const fe *part;
+    const int *nb;
+    const int64_t *dst;
+    fe *msg;
+};
+struct copy_job {
+    const uint64_t *addr;     // gather: out[k] = *(fe *)addr[k]; null: out[k] = base[k]
+    const fe *base;
+    fe *out;                  // may be page-locked host memory
+};
+// start[j]: first slot (reduce) / element (copy) of job j; start[n] = total
+void reduce_slots_table(const slot_job *J, const int64_t *start, int n, int64_t total, hipStream_t s);
+void copy_table(const copy_job *J, const int64_t *start, int n, int64_t total, hipStream_t s);
 struct mk_item {
     uint8_t *tree;
     int64_t first, start;

@@ -924,12 +924,11 @@ int enqueue_commit(vg_ctx *c)
 // The sumcheck layers of many proofs in merged launches on stream gs: the
 // layer batches of equal table width and table-build path of every context
 // form one instance list (each instance keeps its own tables, slots and
-// message block), so a launch covers them all.  Waits for every context's
-// challenges (ev[9]); records gkr_done.  *buf: device descriptor arrays,
-// freed by the caller after the work completes; host_keep must outlive the
-// upload.
-// host_buf: pinned copy of the launch tables (free with hipHostFree once gs
-// has drained), so that the upload is asynchronous
+// message block), so a launch covers them all.  wait: gs waits for every
+// context's challenges (ev[9]) on the device (false: the caller has waited
+// on the host); records gkr_done.  *buf: device descriptor arrays, *host_buf:
+// their pinned copy for the asynchronous upload; the caller frees both once
+// gs has drained.
 int batch_gkr(vg_ctx *const *ctxs, int n, hipStream_t gs, hipEvent_t gkr_done, void **buf, void **host_buf,
               bool wait = true)
 {
@@ -1015,16 +1014,9 @@ int commit_segments(const vg_ctx *c, launch::leaf_seg *g)
 
 // ---- every SHA3 leaf chain (fri.cpp:97-126, 397-420) in one launch, then
 // every Merkle tree (merkle_tree.cpp:46-56) level by level, and the query
-// answers.  leaf_done: the leaf chains and Merkle trees ran elsewhere (a
-// batch's merged launches) and completion of that event means they are done.
-int enqueue_post(vg_ctx *c, hipEvent_t leaf_done, hipEvent_t gkr_done = nullptr)
+// answers (a batch does all of this in merged launches instead).
+int enqueue_post(vg_ctx *c)
 {
-    if (gkr_done) {
-        // the sumcheck layers ran in a batch's merged launches
-        HIPCHK(hipStreamWaitEvent(c->st, gkr_done, 0));
-        launch::reduce_slots(c->slots, c->n_slots, c->slot_nb, c->slot_dst, c->msg, c->st);
-        HIPCHK(hipEventRecord(c->ev[2], c->st));
-    }
     hipStream_t st = c->serial ? c->st : c->st2;
     launch::leaf_seg g[24];
     int ns = commit_segments(c, g);
@@ -1032,22 +1024,17 @@ int enqueue_post(vg_ctx *c, hipEvent_t leaf_done, hipEvent_t gkr_done = nullptr)
     int64_t Ss[24], nl[24];
     uint8_t *trees[24];
     for (int k = 0; k < ns; ++k) { cws[k] = g[k].cw; Ss[k] = g[k].S; trees[k] = g[k].tree; nl[k] = g[k].S / 2; }
-    if (leaf_done) {
-        HIPCHK(hipStreamWaitEvent(st, leaf_done, 0));
-        HIPCHK(hipEventRecord(c->ev_leaf[0], st));
-    } else {
-        // the leaf chains keep every SIMD busy on their own: beside them the
-        // latency-bound sumcheck and fft_gkr launches only slow both down,
-        // so they start once those are done (the LDEs and folds before them
-        // do overlap the sumcheck)
-        if (!c->serial)
-            for (int w = 0; w < vg_ctx::NGS; ++w) HIPCHK(hipStreamWaitEvent(st, c->gev[w], 0));
-        if (c->fg && c->own_stream) HIPCHK(hipStreamWaitEvent(st, fg_done_event(c->fg), 0));   // fft_gkr on its own stream
-        HIPCHK(hipEventRecord(c->ev_leaf[0], st));
-        launch::leaf_hash_multi(ns, cws, Ss, trees, c->log_R, st);
-    }
+    // the leaf chains keep every SIMD busy on their own: beside them the
+    // latency-bound sumcheck and fft_gkr launches only slow both down, so
+    // they start once those are done (the LDEs and folds before them do
+    // overlap the sumcheck)
+    if (!c->serial)
+        for (int w = 0; w < vg_ctx::NGS; ++w) HIPCHK(hipStreamWaitEvent(st, c->gev[w], 0));
+    if (c->fg && c->own_stream) HIPCHK(hipStreamWaitEvent(st, fg_done_event(c->fg), 0));   // fft_gkr on its own stream
+    HIPCHK(hipEventRecord(c->ev_leaf[0], st));
+    launch::leaf_hash_multi(ns, cws, Ss, trees, c->log_R, st);
     HIPCHK(hipEventRecord(c->ev_leaf[1], st));
-    if (!leaf_done) launch::merkle_multi(ns, trees, nl, st);
+    launch::merkle_multi(ns, trees, nl, st);
     HIPCHK(hipEventRecord(c->ev[6], st));
     st = c->st;
     if (!c->serial) HIPCHK(hipStreamWaitEvent(st, c->ev[6], 0));
@@ -1999,7 +1986,7 @@ int vg_prove_async(vg_ctx *c)
 {
     int rc = prove_start(c);
     if (rc) return rc;
-    rc = enqueue_post(c, nullptr);
+    rc = enqueue_post(c);
     if (rc) return rc;
     c->host_ms[1] = ms_since(c->t_enq);
     c->pending = true;
@@ -2095,6 +2082,7 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
         int64_t *dstart = nullptr;
         launch::mk_item *dI = nullptr;
         void *gbuf = nullptr, *gpin = nullptr, *pin = nullptr;   // device / pinned launch tables
+        void *pin3 = nullptr, *d3 = nullptr;                       // the same for (3)
         bool ok = false, gok = false;
         std::string err;
         std::thread tg, tl;                                        // helper threads of (2)
@@ -2168,19 +2156,46 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
         }
         G.ok = ok;
     };
-    // (3) of a group, once its merged launches have completed
+    // (3) of a group in two launches on the leaf stream, after its Merkle
+    // trees: every proof's slot reduction (once the merged sumcheck layers
+    // are done), then every proof's query answers and message block copied
+    // straight into its page-locked host staging
     auto post = [&](group &G) {
         G.tg.join();
         G.tl.join();
-        bool ok = G.ok && G.gok && hipEventSynchronize(G.gdone) == hipSuccess && hipEventSynchronize(G.done) == hipSuccess;
+        bool ok = G.ok && G.gok && hipEventSynchronize(G.gdone) == hipSuccess;
+        std::vector<launch::slot_job> sj;
+        std::vector<launch::copy_job> cj;
+        std::vector<int64_t> ss(1, 0), cs(1, 0);
+        for (vg_ctx *c : G.S) {
+            if (c->n_slots) { sj.push_back({c->slots, c->slot_nb, c->slot_dst, c->msg}); ss.push_back(ss.back() + c->n_slots); }
+            if (c->n_gather) { cj.push_back({c->gsrc, nullptr, c->h_gout}); cs.push_back(cs.back() + c->n_gather); }
+            cj.push_back({nullptr, c->msg, c->h_msg});
+            cs.push_back(cs.back() + c->msg_len);
+        }
+        size_t b1 = sizeof(launch::slot_job) * sj.size(), b2 = sizeof(int64_t) * ss.size(),
+               b3 = sizeof(launch::copy_job) * cj.size(), b4 = sizeof(int64_t) * cs.size();
+        size_t o2 = (b1 + 255) & ~(size_t)255, o3 = o2 + ((b2 + 255) & ~(size_t)255), o4 = o3 + ((b3 + 255) & ~(size_t)255),
+               tot = o4 + b4 + 256;
+        uint8_t *d = nullptr;
+        ok = ok && hipHostMalloc(&G.pin3, tot, hipHostMallocDefault) == hipSuccess && hipMalloc((void **)&d, tot) == hipSuccess;
+        if (ok) {
+            G.d3 = d;
+            uint8_t *h = (uint8_t *)G.pin3;
+            if (b1) memcpy(h, sj.data(), b1);
+            memcpy(h + o2, ss.data(), b2);
+            if (b3) memcpy(h + o3, cj.data(), b3);
+            memcpy(h + o4, cs.data(), b4);
+            ok = hipMemcpyAsync(d, h, tot, hipMemcpyHostToDevice, ls) == hipSuccess;
+        }
+        if (ok) {
+            launch::reduce_slots_table((const launch::slot_job *)d, (const int64_t *)(d + o2), (int)sj.size(), ss.back(), ls);
+            launch::copy_table((const launch::copy_job *)(d + o3), (const int64_t *)(d + o4), (int)cj.size(), cs.back(), ls);
+            ok = hipGetLastError() == hipSuccess;
+        }
         if (!ok && rc == 0) { g_err = G.err.empty() ? std::string("vg_prove_batch: merged launches failed") : G.err; rc = -20; }
-        int r = run(G.idx, [&](int i) {
-            if (!started[i]) return 0;
-            int q = ok ? enqueue_post(ctxs[i], G.done, G.gdone) : 0;
-            ctxs[i]->pending = true;
-            return q;
-        });
-        if (r && rc == 0) rc = r;
+        for (int i : G.idx)
+            if (started[i]) ctxs[i]->pending = true;
     };
     for (int g = 0; g < ngroups; ++g) {
         group &G = grp[g];
@@ -2198,6 +2213,8 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
         if (g > 0) post(grp[g - 1]);
     }
     post(grp[ngroups - 1]);
+    // every (3) done: the proofs' answers are in their host staging
+    if (ls && hipStreamSynchronize(ls) != hipSuccess && rc == 0) { g_err = "vg_prove_batch: leaf stream failed"; rc = -20; }
     // transcripts (a failed merge still drains every proof in flight; its
     // transcripts are dropped)
     std::vector<int> all(n);
@@ -2213,6 +2230,8 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
         if (G.gbuf) (void)hipFree(G.gbuf);
         if (G.gpin) (void)hipHostFree(G.gpin);
         if (G.pin) (void)hipHostFree(G.pin);
+        if (G.pin3) (void)hipHostFree(G.pin3);
+        if (G.d3) (void)hipFree(G.d3);
         if (G.done) (void)hipEventDestroy(G.done);
         if (G.gdone) (void)hipEventDestroy(G.gdone);
     }

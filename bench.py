@@ -88,8 +88,8 @@ def batched(args):
     HBM at once, contexts created round-robin on `batch_workers` streams, and
     vg_prove_batch proving them from that many library threads (each proof in
     order on its stream, the streams overlapping on the device); the median
-    of five timed batches after three warm-up batches.  Every batched transcript must equal the plain
-    one byte for byte."""
+    of five timed batches after three warm-up batches.  Every batched
+    transcript must equal the plain one byte for byte."""
     import virgo_amd as V
     from virgo_amd import workloads as W
     jobs = W.batch_jobs(args.batch_jobs)

@@ -14,13 +14,24 @@
  * compatibility and not re-read.  Errors throw std::runtime_error with the
  * library's message (the reference asserts).
  *
- * The commitment (poly_commit_prover, fri::) and fft_gkr results are exposed
- * as the parsed transcript sections: commitment_roots(), all_sum(),
- * fri_roots(), final_codeword(), query_section(), fft_gkr_section().
+ * The commitment side keeps the names of poly_commit::poly_commit_prover
+ * (poly_commit.h:42-351) and of the fri:: query functions (fri.cpp:148-289)
+ * as verify_poly_commitment calls them (vpd_verifier.cpp:80-335):
+ * commit_private_array() returns root L, commit_public_array() fills all_sum
+ * and returns root H, commit_phase() returns the FRI roots and the final
+ * codeword, and request_init_value_with_merkle() / request_step_commit()
+ * return the 65 value pairs and the authentication path of each query in the
+ * verifier's order (33 repetitions: L, H, then one step per FRI level).
+ * new_size receives the bytes of that answer (the reference's proof-size
+ * statistic also drops what earlier queries already sent; that accounting is
+ * not reproduced).  The raw sections stay available: commitment_roots(),
+ * all_sum(), fri_roots(), final_codeword(), query_section(),
+ * fft_gkr_section().
  */
 #ifndef VIRGO_PROVER_HPP
 #define VIRGO_PROVER_HPP
 
+#include <array>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -31,6 +42,15 @@
 
 namespace vg {
 
+using hhash_digest = std::array<uint8_t, 32>;        // __hhash_digest (my_hhash.h)
+// poly_commit.h:24-32: the FRI roots of commit_phase and its final codeword
+struct ldt_commitment {
+    std::vector<hhash_digest> commitment_hhash;
+    const vg_fe *final_rs_code = nullptr;             // 2048 values (+ 32 of the mask)
+    int mx_depth = 0;
+    int repeat_no = 33;
+};
+using query_answer = std::pair<std::vector<std::pair<vg_fe, vg_fe>>, std::vector<hhash_digest>>;
 struct quadratic_poly { vg_fe a, b, c; };            // polynomial.h: ax^2 + bx + c
 struct quintuple_poly { vg_fe a, b, c, d, e, f; };   // polynomial.h: degree-5 member order
 
@@ -127,6 +147,40 @@ public:
     std::pair<const uint8_t *, size_t> fft_gkr_section() const { return {fg_sec_, fg_len_}; }
     const std::vector<uint8_t> &transcript() const { return tr_; }
 
+    // poly_commit_prover::commit_private_array (poly_commit.h:42-124): root L
+    hhash_digest commit_private_array(const vg_fe *, int, const std::vector<vg_fe> &) const { return digest(root_l_); }
+    // poly_commit_prover::commit_public_array (poly_commit.h:125-351): writes
+    // all_sum[65] (64 slices + mask; may be null), returns root H
+    hhash_digest commit_public_array(const std::vector<vg_fe> &, const vg_fe *, int, const vg_fe &, vg_fe *all_sum) const
+    {
+        if (all_sum) std::memcpy(all_sum, all_sum_, 65 * sizeof(vg_fe));
+        return digest(root_h_);
+    }
+    // poly_commit_prover::commit_phase (vpd_verifier.cpp:46-78)
+    ldt_commitment commit_phase(int) const
+    {
+        ldt_commitment c;
+        for (int k = 0; k < n_ - 6; ++k) c.commitment_hhash.push_back(digest(fri_roots_ + 32 * k));
+        c.final_rs_code = final_cw_;
+        c.mx_depth = n_ - 6;
+        return c;
+    }
+    // fri::request_init_value_with_merkle (fri.cpp:148-201): oracle 0 = L, 1 = H
+    query_answer request_init_value_with_merkle(long long, long long, int &new_size, int oracle_indicator)
+    {
+        if (step_ != 0 || oracle_indicator != init_next_) fail("request_init_value_with_merkle out of the verifier's order");
+        init_next_ ^= 1;
+        if (init_next_ == 0) step_ = 1;
+        return answer(n_ - 1, new_size);
+    }
+    // fri::request_step_commit (fri.cpp:227-289)
+    query_answer request_step_commit(int lvl, long long, int &new_size)
+    {
+        if (step_ == 0 || lvl != step_ - 1) fail("request_step_commit out of the verifier's order");
+        step_ = lvl + 2 > n_ - 6 ? 0 : lvl + 2;
+        return answer(n_ - 2 - lvl, new_size);
+    }
+
 private:
     vg_ctx *ctx_ = nullptr;
     std::vector<uint8_t> tr_;
@@ -138,7 +192,33 @@ private:
     const uint8_t *root_l_ = nullptr, *root_h_ = nullptr, *fri_roots_ = nullptr, *query_sec_ = nullptr,
                   *fg_sec_ = nullptr;
     const vg_fe *all_sum_ = nullptr, *final_cw_ = nullptr;
-    size_t query_len_ = 0, fg_len_ = 0;
+    size_t query_len_ = 0, fg_len_ = 0, qoff_ = 0;
+    int init_next_ = 0, step_ = 0;                     // position in the query order
+
+    static hhash_digest digest(const uint8_t *p)
+    {
+        hhash_digest d;
+        std::memcpy(d.data(), p, 32);
+        return d;
+    }
+    // the next 65 value pairs and `path` digests of the query section
+    query_answer answer(int path, int &new_size)
+    {
+        size_t need = 65 * 32 + (size_t)path * 32;
+        if (!query_sec_ || qoff_ + need > query_len_) fail("no query answer left");
+        query_answer a;
+        const uint8_t *q = query_sec_ + qoff_;
+        for (int i = 0; i < 65; ++i) {
+            vg_fe x, y;
+            std::memcpy(&x, q + 32 * i, 16);
+            std::memcpy(&y, q + 32 * i + 16, 16);
+            a.first.push_back({x, y});
+        }
+        for (int k = 0; k < path; ++k) a.second.push_back(digest(q + 65 * 32 + 32 * k));
+        qoff_ += need;
+        new_size = (int)need;
+        return a;
+    }
 
     static void check(int rc)
     {

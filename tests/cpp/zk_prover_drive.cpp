@@ -1,6 +1,7 @@
 // Drives vg::zk_prover (include/virgo_prover.hpp) in the call order of the
-// reference verifier (zk_verifier.cpp:910-1253) and rebuilds the transcript
-// from the values the methods return.  Usage: zk_prover_drive circuit.bin out.bin
+// reference verifier (zk_verifier.cpp:910-1253; the commitment's
+// poly_commit_prover and fri:: calls in the order of vpd_verifier.cpp:46-335)
+// and rebuilds the transcript from the values the methods return.  Usage: zk_prover_drive circuit.bin out.bin
 // circuit.bin: int32 d, int64 n[d], then per gate int32 ty, int64 u, int64 v.
 #include <cstdio>
 #include <cstring>
@@ -72,13 +73,42 @@ int main(int argc, char **argv)
             put(out, &p.Iuv, 16); put(out, &p.preZu, 16); put(out, &p.preZv, 16);
         }
         put(out, &p.Zu, 16); put(out, &p.preu1, 16); put(out, &p.sumRc, 48);
-        auto roots = p.commitment_roots();
-        put(out, roots.first, 32); put(out, roots.second, 32);
-        put(out, p.all_sum(), 65 * 16);
-        put(out, p.fri_roots(), 32 * (nbits - 6));
-        put(out, p.final_codeword(), (2048 + 32) * 16);
-        auto qs = p.query_section();
-        put(out, qs.first, qs.second);
+        // the commitment, as verify_poly_commitment drives poly_commit_prover
+        // and fri:: (vpd_verifier.cpp:46-335)
+        std::vector<vg_fe> msk;
+        vg::hhash_digest rl = p.commit_private_array(nullptr, (int)nbits, msk);
+        std::vector<vg_fe> all_sum(65);
+        vg::hhash_digest rh = p.commit_public_array(msk, nullptr, 0, zero, all_sum.data());
+        put(out, rl.data(), 32); put(out, rh.data(), 32);
+        put(out, all_sum.data(), 65 * 16);
+        vg::ldt_commitment com = p.commit_phase((int)nbits);
+        if ((int)com.commitment_hhash.size() != (int)nbits - 6) { fprintf(stderr, "FRI root count\n"); return 1; }
+        for (auto &dg : com.commitment_hhash) put(out, dg.data(), 32);
+        put(out, com.final_rs_code, (2048 + 32) * 16);
+        auto emit = [&](const vg::query_answer &a, int new_size) {
+            size_t before = out.size();
+            for (auto &pr : a.first) { put(out, &pr.first, 16); put(out, &pr.second, 16); }
+            for (auto &dg : a.second) put(out, dg.data(), 32);
+            return (int)(out.size() - before) == new_size;
+        };
+        for (int rep = 0; rep < 33; ++rep)
+            for (int i = 0; i < (int)nbits - 6; ++i) {
+                int ns = 0;
+                bool ok = true;
+                if (i == 0) {
+                    vg::query_answer a = p.request_init_value_with_merkle(0, 0, ns, 0);
+                    ok &= emit(a, ns);
+                    vg::query_answer b = p.request_init_value_with_merkle(0, 0, ns, 1);
+                    ok &= emit(b, ns);
+                }
+                vg::query_answer c = p.request_step_commit(i, 0, ns);
+                ok &= emit(c, ns);
+                if (!ok) { fprintf(stderr, "query answer size\n"); return 1; }
+            }
+        // the verifier asks in its own order; any other order is refused
+        bool refused = false;
+        try { int ns; p.request_step_commit(0, 0, ns); } catch (const std::exception &) { refused = true; }
+        if (!refused) { fprintf(stderr, "out-of-order query accepted\n"); return 1; }
         auto fg = p.fft_gkr_section();
         put(out, fg.first, fg.second);
         FILE *o = fopen(argv[2], "wb");

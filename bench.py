@@ -79,6 +79,41 @@ def cpu_baseline(args):
                       % (lw, args.depth, dt, scale)}
 
 
+def _cpu_prove_spec(spec):
+    """One oracle proof of a batch job (runs in a worker process)."""
+    from oracle import oracle as O
+    from virgo_amd import workloads as W
+    circ = W.make_job(spec)
+    t0 = time.perf_counter()
+    O.prove(circ)
+    return time.perf_counter() - t0
+
+
+def _noop(_):
+    return 0
+
+
+def cpu_batch(args, procs=16, stride=8):
+    """CPU throughput baseline of the batched workload: the oracle
+    (single-threaded C restatement of the reference prover) proving every
+    `stride`-th job of workloads.batch_specs in `procs` worker processes at
+    once (the host cores of one GPU's share).  proofs/s = sample size over
+    (summed proof seconds / procs): the whole 256-job batch would keep the
+    processes evenly busy, which a sample's wall time, set by its longest
+    proof, does not show (reported beside it)."""
+    import multiprocessing as mp
+    from virgo_amd import workloads as W
+    sample = W.batch_specs(args.batch_jobs)[::stride]
+    with mp.get_context("spawn").Pool(procs) as pool:
+        pool.map(_noop, range(procs))                   # workers started and imported
+        t0 = time.perf_counter()
+        times = pool.map(_cpu_prove_spec, sample, chunksize=1)
+        wall = time.perf_counter() - t0
+    return {"value": round(len(sample) / (sum(times) / procs), 2), "unit": "proofs/s", "cores": procs, "kind": "port",
+            "sample": "oracle proofs of every %dth job (%d of %d), %d processes: %.1f s wall, %.1f s CPU"
+                      % (stride, len(sample), args.batch_jobs, procs, wall, sum(times))}
+
+
 def batched(args):
     """Batched throughput on the uneven workload (workloads.batch_jobs).
 
@@ -253,6 +288,10 @@ def main():
         line["serving"] = serving(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args)
+        if "batched" in line:
+            cb = cpu_batch(args)
+            line["batched"]["cpu_baseline"] = cb
+            line["batched"]["speedup_vs_cpu"] = round(line["batched"]["proofs_per_s"] / cb["value"], 1)
         if not args.no_verify:
             from oracle import oracle as O
             t0 = time.perf_counter()

@@ -124,22 +124,34 @@ def read_reference_files(circuit_path):
             "u": np.concatenate(u), "v": np.concatenate(v)}
 
 
+def batch_specs(count=256, seed=7):
+    """Sizes and seeds of the batched workload (see batch_jobs), without
+    building the circuits: a list of (name, kind, a, b, circuit seed)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(count):
+        if rng.integers(0, 2) == 0:
+            n = 1 << int(rng.integers(3, 9))
+            specs.append(("matmul%d_s%d" % (n, k), "matmul", n, 0, 1000 + k))
+        else:
+            lw, d = int(rng.integers(10, 19)), int(rng.integers(4, 21))
+            specs.append(("random_w%d_d%d_s%d" % (lw, d, k), "random", lw, d, 1000 + k))
+    return specs
+
+
+def make_job(spec):
+    """The circuit of one batch_specs entry."""
+    _, kind, a, b, seed = spec
+    return matmul_circuit(a, seed=seed) if kind == "matmul" else random_circuit(a, b, seed=seed)
+
+
 def batch_jobs(count=256, seed=7):
     """The batched workload: `count` independent proofs of uneven size, half
     n x n matmul circuits with n log-uniform in {8, ..., 256}, half random
     layered circuits with log2 width uniform in [10, 18] and depth uniform in
     [4, 20] (add/mult 50/50, uniform wiring, inputs in [0, 2^31)).  Returns
     a list of (name, circuit)."""
-    rng = np.random.default_rng(seed)
-    jobs = []
-    for k in range(count):
-        if rng.integers(0, 2) == 0:
-            n = 1 << int(rng.integers(3, 9))
-            jobs.append(("matmul%d_s%d" % (n, k), matmul_circuit(n, seed=1000 + k)))
-        else:
-            lw, d = int(rng.integers(10, 19)), int(rng.integers(4, 21))
-            jobs.append(("random_w%d_d%d_s%d" % (lw, d, k), random_circuit(lw, d, seed=1000 + k)))
-    return jobs
+    return [(sp[0], make_job(sp)) for sp in batch_specs(count, seed)]
 
 
 def serve_jobs(count=96, seed=11):

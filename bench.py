@@ -62,6 +62,38 @@ def leaf_traffic(n):
     return int(799016.2 * 2048 + 24575.5 * 1024)
 
 
+def c1(args):
+    """The reference's own CPU-runnable case: prove + verify of the 32 x 32
+    matrix-product circuit (inputs padded to 2^15), on the GPU (best of 5
+    after a warm-up; the device verifier) and, unless --no-cpu-baseline, with
+    the CPU oracle (1 core); transcripts compared byte for byte."""
+    import virgo_amd as V
+    from virgo_amd import workloads as W
+    circ = W.matmul_circuit(32, seed=1)
+    p = V.Prover(circ)
+    tr = p.prove()
+    tp, tv = [], []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        tr = p.prove()
+        tp.append(time.perf_counter() - t0)
+        ok, _, vms = p.verify(tr)
+        tv.append(vms)
+    p.close()
+    out = {"workload": "32x32 matmul, inputs uniform in [0, 2^31), seed 1", "gpu_prove_ms": round(min(tp) * 1e3, 3),
+           "gpu_verify_ms": round(min(tv), 3), "accepted": bool(ok)}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        ref, _ = O.prove(circ)
+        out["cpu_prove_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        t0 = time.perf_counter()
+        ok_o, _ = O.verify(circ, tr)
+        out["cpu_verify_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        out["bit_exact_vs_cpu"] = ref == tr and bool(ok_o)
+    return out
+
+
 def cpu_baseline(args):
     """Times the CPU oracle (single-threaded C restatement of the reference
     prover) on a bounded sample: the same circuit family at 2^sample gates per
@@ -285,6 +317,7 @@ def main():
             vt.append(vms)
         line["verify"] = {"accepted": bool(ok), "device_verifier_ms": round(min(vt), 2)}
     p.close()
+    line["c1"] = c1(args)
     if not args.no_batch:
         line["batched"] = batched(args)
     if not args.no_serve:

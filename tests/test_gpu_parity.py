@@ -166,3 +166,27 @@ def test_rejects_bad_circuit():
     circ["u"][int(circ["n"][0]) + 5] = 1 << 20          # out-of-range wire
     with pytest.raises(V.VirgoError):
         V.Prover(circ)
+
+
+def test_ntt_tiles_give_the_same_bits():
+    """Every NTT tile the autotuner may pick gives the same transcript: the
+    same proofs made in child processes with the tile forced to 2^10 and 2^12
+    (VIRGO_NTT_LM) equal this process's (tuned) ones."""
+    import hashlib
+    import os
+    import subprocess
+    import sys
+    circs = [("random_w17_d5", W.random_circuit(17, 5, seed=5)), ("w18_d4", W.random_circuit(18, 4, seed=9))]
+    want = [hashlib.sha256(V.prove(c)).hexdigest() for _, c in circs]
+    tile, _ = V.ntt_plan(12)
+    assert tile in (10, 11, 12)
+    code = ("import hashlib, sys; sys.path.insert(0, %r); import virgo_amd as V; from virgo_amd import workloads as W; "
+            "print(hashlib.sha256(V.prove(W.random_circuit(17, 5, seed=5))).hexdigest()); "
+            "print(hashlib.sha256(V.prove(W.random_circuit(18, 4, seed=9))).hexdigest())"
+            % os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    for lm in ("10", "12"):
+        print("tile 2^" + lm, flush=True)
+        env = dict(os.environ, VIRGO_NTT_LM=lm)
+        out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=240)
+        assert out.returncode == 0, out.stderr
+        assert out.stdout.split() == want, lm

@@ -368,6 +368,8 @@ static int circ_build(const vgo_circuit_desc *dsc, circuit *c)
             }
             if (ty == 6 || ty == 10) v = 0;                     /* :87-98 */
             if (ty == 13 && u != v) FAIL(-3, "bit-test gate needs u == v");
+            if (ty == 12 && v - u + 1 > 60) FAIL(-3, "power-sum gate spans more than 60 entries");   /* zk_prover.cpp:129 */
+            if (ty < 0 || ty > 13 || ty == 11) FAIL(-3, "unknown gate type %d", ty);                /* zk_prover.cpp:145 */
             if (i != 1) {
                 c->L[i].g[j].ty = ty; c->L[i].g[j].u = u; c->L[i].g[j].v = v;
             } else {

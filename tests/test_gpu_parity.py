@@ -69,6 +69,10 @@ CASES = {
     "random_w17_d5": lambda: W.random_circuit(17, 5, seed=5),
     # phase 1 spread evenly, phase 2's records crowded on 64 entries
     "random_w13_d3_skew": lambda: W.random_circuit(13, 3, seed=6, v_bits=6),
+    # every gate type a description may use (not, minus, xor, naab, relay,
+    # dummy, power sums, bit tests beside add/mult/sum)
+    "mixed_w10_d5": lambda: W.mixed_circuit(10, 5, seed=12),
+    "mixed_w16_d3_in18": lambda: W.mixed_circuit(16, 3, seed=13, log_inputs=18),
 }
 
 
@@ -126,7 +130,8 @@ def test_batch_matches_single():
     """vg_prove_batch over contexts on shared caller streams: every proof is
     byte-identical to the oracle's, twice in a row."""
     circs = [W.matmul_circuit(8, seed=3), W.random_circuit(10, 4, seed=11), W.random_circuit(12, 3, seed=12),
-             W.matmul_circuit(16, seed=4), W.random_circuit(11, 6, seed=13), W.random_circuit(10, 9, seed=14)]
+             W.matmul_circuit(16, seed=4), W.random_circuit(11, 6, seed=13), W.random_circuit(10, 9, seed=14),
+             W.mixed_circuit(11, 4, seed=15)]
     refs = [O.prove(c)[0] for c in circs]
     streams = [V.Stream() for _ in range(3)]
     provers = [V.Prover(c, stream=streams[k % 3]) for k, c in enumerate(circs)]
@@ -139,7 +144,7 @@ def test_batch_matches_single():
         st.close()
 
 
-@pytest.mark.parametrize("name", ["matmul8", "random_w12_d3"])
+@pytest.mark.parametrize("name", ["matmul8", "random_w12_d3", "mixed_w10_d5"])
 def test_cpp_zk_prover_mirror(name, tmp_path):
     """The C++ zk_prover mirror (include/virgo_prover.hpp), driven in the
     reference verifier's call order, returns exactly the oracle's messages."""
@@ -166,6 +171,18 @@ def test_rejects_bad_circuit():
     circ["u"][int(circ["n"][0]) + 5] = 1 << 20          # out-of-range wire
     with pytest.raises(V.VirgoError):
         V.Prover(circ)
+
+
+@pytest.mark.parametrize("ty,u,v,msg", [(12, 0, 60, "more than 60"), (11, 0, 0, "unknown gate type"),
+                                         (13, 1, 2, "u == v"), (5, 3, 3, "sum range")])
+def test_rejects_bad_gates_like_the_oracle(ty, u, v, msg):
+    c = W.mixed_circuit(7, 2, seed=5)
+    g = int(c["n"][0]) + 3
+    c["ty"][g], c["u"][g], c["v"][g] = ty, u, v
+    with pytest.raises(V.VirgoError, match=msg):
+        V.Prover(c)
+    with pytest.raises(RuntimeError, match=msg):
+        O.prove(c)
 
 
 def test_ntt_tiles_give_the_same_bits():

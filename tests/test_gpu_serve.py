@@ -21,6 +21,7 @@ MIX = [
     ("random_w11_d9", lambda: W.random_circuit(11, 9, seed=7)),
     ("random_w16_d4", lambda: W.random_circuit(16, 4, seed=8)),
     ("random_w13_d3_skew", lambda: W.random_circuit(13, 3, seed=9, v_bits=6)),
+    ("mixed_w12_d4", lambda: W.mixed_circuit(12, 4, seed=10)),
 ]
 
 

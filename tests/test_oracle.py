@@ -205,26 +205,53 @@ def test_eq_table_bigint():
 
 
 def brute_eval(circ):
-    """Independent Python evaluation of add/mult/sum circuits."""
+    """Independent Python evaluation over F_{p^2} of every gate type a
+    description may use, by each type's definition (zk_prover.cpp:78-142)."""
+    def add(a, b): return ((a[0] + b[0]) % P, (a[1] + b[1]) % P)
+    def sub(a, b): return ((a[0] - b[0]) % P, (a[1] - b[1]) % P)
+    def const(k): return (k % P, 0)
+    one = const(1)
     n = circ["n"]; off = 0
     vals = None
     for li, cnt in enumerate(n):
         ty = circ["ty"][off:off + cnt]; u = circ["u"][off:off + cnt]; v = circ["v"][off:off + cnt]
         off += cnt
         if li == 0:
-            vals = [(int(x) % P, 0) for x in u]
+            vals = [const(int(x)) for x in u]
             continue
         new = []
-        for t, a, b in zip(ty, u, v):
+        for t, a, b in zip(ty.tolist(), u.tolist(), v.tolist()):
+            x, y = vals[a], vals[b] if b < len(vals) else None
             if t == 0:
-                new.append(((vals[a][0] + vals[b][0]) % P, (vals[a][1] + vals[b][1]) % P))
+                new.append(add(x, y))
             elif t == 1:
-                new.append(py_mul(vals[a], vals[b]))
+                new.append(py_mul(x, y))
+            elif t == 2:
+                new.append(const(0))
             elif t == 5:
-                s = (0, 0)
+                s = const(0)
                 for k in range(a, b):
-                    s = ((s[0] + vals[k][0]) % P, (s[1] + vals[k][1]) % P)
+                    s = add(s, vals[k])
                 new.append(s)
+            elif t == 6:
+                new.append(sub(one, x))
+            elif t == 7:
+                new.append(sub(x, y))
+            elif t == 8:                            # x + y - 2xy
+                new.append(sub(add(x, y), py_mul(const(2), py_mul(x, y))))
+            elif t == 9:                            # (1 - x) y
+                new.append(py_mul(sub(one, x), y))
+            elif t == 10:
+                new.append(x)
+            elif t == 12:                           # sum of 2^(k-u) x_k over [u, v]
+                s = const(0)
+                for k in range(a, b + 1):
+                    s = add(s, py_mul(const(1 << (k - a)), vals[k]))
+                new.append(s)
+            elif t == 13:                           # x (1 - x): zero iff x is a bit
+                new.append(py_mul(x, sub(one, x)))
+            else:
+                raise AssertionError("type %d" % t)
         vals = new
     return vals
 
@@ -241,6 +268,51 @@ def test_evaluate_matmul_closed_form():
 def test_evaluate_random_circuit():
     c = W.random_circuit(6, 3, seed=4)
     assert to_int(O.evaluate(c)) == brute_eval(c)
+
+
+@pytest.mark.parametrize("log_width,depth,seed", [(5, 4, 1), (7, 3, 2), (4, 6, 3)])
+def test_evaluate_mixed_gate_types(log_width, depth, seed):
+    c = W.mixed_circuit(log_width, depth, seed=seed)
+    assert set(W.MIXED_TYPES) <= set(c["ty"][c["n"][0]:].tolist())
+    assert to_int(O.evaluate(c)) == brute_eval(c)
+
+
+def test_evaluate_bit_gates_on_bits():
+    """Closed forms on 0/1 inputs: not, xor, naab and the bit test are the
+    boolean functions they name; the power sum reads the bits as a number."""
+    bits = [0, 1, 1, 0, 1, 1, 1, 0]
+    inp = (np.full(8, 3, np.int32), np.array(bits, np.int64), np.zeros(8, np.int64))
+    gates = [(6, 1, 0), (8, 1, 2), (8, 0, 1), (9, 0, 1), (9, 1, 2), (13, 2, 2), (12, 1, 5), (7, 0, 4)]
+    lay = (np.array([g[0] for g in gates], np.int32), np.array([g[1] for g in gates], np.int64),
+           np.array([g[2] for g in gates], np.int64))
+    c = W._pack([inp, lay])
+    out = [x[0] for x in to_int(O.evaluate(c))]
+    assert out == [0, 0, 1, 1, 0, 0, 0b11011, P - 1]
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_prove_verify_mixed_gate_types(seed):
+    c = W.mixed_circuit(6, 4, seed=seed)
+    tr, _ = O.prove(c)
+    ok, msg = O.verify(c, tr)
+    assert ok, msg
+    # a sum range one entry longer changes the public predicate: rejected
+    c2 = {k: a.copy() for k, a in c.items()}
+    base = int(c["n"][0])
+    lay = slice(base, base + int(c["n"][1]))
+    g = base + int(np.flatnonzero((c["ty"][lay] == 12) & (c["v"][lay] > c["u"][lay]))[0])
+    c2["v"][g] -= 1
+    assert not O.verify(c2, tr)[0]
+
+
+@pytest.mark.parametrize("ty,u,v,msg", [(12, 0, 60, "more than 60"), (11, 0, 0, "unknown gate type"),
+                                         (13, 1, 2, "u == v"), (5, 3, 3, "sum range")])
+def test_loader_rejects_bad_gates(ty, u, v, msg):
+    c = W.mixed_circuit(7, 2, seed=5)
+    g = int(c["n"][0]) + 3
+    c["ty"][g], c["u"][g], c["v"][g] = ty, u, v
+    with pytest.raises(Exception, match=msg):
+        O.prove(c)
 
 
 @pytest.fixture(scope="module")

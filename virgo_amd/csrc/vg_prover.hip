@@ -119,6 +119,10 @@ int build_circuit(const vg_circuit_desc *d, host_circuit &C)
             }
             if (t == 6 || t == 10) vv = 0;
             if (t == 13 && uu != vv) { snprintf(b, sizeof b, "bit-test gate needs u == v"); return fail(-3); }
+            if (t == 12 && vv - uu + 1 > 60) {                                   // zk_prover.cpp:129
+                snprintf(b, sizeof b, "power-sum gate spans more than 60 entries");
+                return fail(-3);
+            }
             if (t < 0 || t > 13 || t == 11) { snprintf(b, sizeof b, "unknown gate type %d", t); return fail(-3); }
             if (i != 1) {
                 ty[j] = (uint8_t)t; u[j] = (int32_t)uu; v[j] = (int32_t)vv;

@@ -165,3 +165,44 @@ def serve_jobs(count=96, seed=11):
         lw, d = int(rng.integers(10, 21)), int(rng.integers(8, 25))
         jobs.append(("random_w%d_d%d_s%d" % (lw, d, k), random_circuit(lw, d, seed=5000 + k)))
     return jobs
+
+
+# gate types a circuit description may use (zk_prover.cpp:66-155): 0 add,
+# 1 mult, 2 dummy (zero), 5 sum of a range, 6 not, 7 minus, 8 xor, 9 naab,
+# 10 relay, 12 sum of a range weighted by powers of 2, 13 bit test (u == v).
+# Type 4 (direct relay) is the layer the loader inserts above the inputs
+# (verifier_fast_track.cpp:102-106), never written in a description.
+MIXED_TYPES = (0, 1, 2, 5, 6, 7, 8, 9, 10, 12, 13)
+
+
+def mixed_circuit(log_width, depth, seed=1, log_inputs=None):
+    """Random layered circuit using every gate type: each gate's type is
+    uniform over MIXED_TYPES, u and v uniform over the previous layer within
+    each type's rules (sums and power-weighted sums span 1..8 entries, bit
+    tests have u == v, not, relay and dummy gates carry v = 0); inputs
+    uniform in [0, 2^31)."""
+    if log_inputs is None:
+        log_inputs = log_width
+    W, NI = 1 << log_width, 1 << log_inputs
+    vals = (splitmix64(seed, NI, 1) >> np.uint64(33)).astype(np.int64)
+    layers = [(np.full(NI, 3, np.int32), vals, np.zeros(NI, np.int64))]
+    prev = NI
+    types = np.array(MIXED_TYPES, dtype=np.int32)
+    for layer in range(depth):
+        r = splitmix64(seed, 4 * W, 2 + layer)
+        ty = types[(r[:W] % np.uint64(len(types))).astype(np.int64)]
+        u = _mod(r[W:2 * W], prev)
+        v = _mod(r[2 * W:3 * W], prev)
+        span = (r[3 * W:] % np.uint64(8)).astype(np.int64) + 1
+        rng_sum = (ty == 5)
+        u[rng_sum] = np.minimum(u[rng_sum], prev - span[rng_sum])
+        v[rng_sum] = u[rng_sum] + span[rng_sum]                  # [u, v), v <= prev
+        rng_exp = (ty == 12)
+        u[rng_exp] = np.minimum(u[rng_exp], prev - span[rng_exp])
+        v[rng_exp] = u[rng_exp] + span[rng_exp] - 1              # [u, v], v < prev
+        v[ty == 13] = u[ty == 13]
+        v[(ty == 6) | (ty == 10) | (ty == 2)] = 0
+        u[ty == 2] = 0
+        layers.append((ty, u, v))
+        prev = W
+    return _pack(layers)

@@ -174,6 +174,35 @@ int vg_fri_fold(const vg_fe *cw, int log_n, const vg_fe *r, vg_fe *out);
 /* element-wise F_{p^2} product (prime_field.h:125-151) */
 int vg_fe_mul(const vg_fe *a, const vg_fe *b, vg_fe *out, size_t n);
 
+/* SHA-256 (FIPS 180-4) of n independent 64-byte messages, and the Merkle
+ * tree of vg_merkle_tree with SHA-256 node hashes (pad leaves SHA-256 of 64
+ * zero bytes).  No counterpart in the reference, whose Merkle hash is
+ * SHA3-256 (my_hhash.h:39-46): an alternative hash for the kernel micro-set,
+ * never used by proofs. */
+int vg_sha256_64(const uint8_t *in, uint8_t *out, size_t n);
+int vg_merkle_tree_sha256(const uint8_t *leaves, int n_leaf, uint8_t *tree);
+
+/* Kernel micro-set on one polynomial of N = 2^log_n coefficients, every
+ * stage device-resident and timed on the device: (0) forward transform of
+ * size N (fast_fourier_transform, RS_polynomial.cpp:27-167), (1) its
+ * inverse (RS_polynomial.cpp:169-240), (2) the Reed-Solomon extension to
+ * M = N 2^rate_log points (the coset extension of commit_private_array,
+ * poly_commit.h:91-104, at rate 2^-rate_log), (3) the Merkle tree over the
+ * M/2 32-byte leaves (E[j], E[j + M/2]) of the extension E in natural order
+ * (merkle_tree.cpp:4-57), (4) FRI commit: L - 5 folds of vg_fri_fold
+ * (L = log_n + rate_log) with challenges fri_r[0..L-6], each folded
+ * codeword of more than 32 values committed by the same tree.  Writes
+ * roots[32 k] = root of level k's tree (level 0: E), k < L - 5; final_cw =
+ * the 32 values left; *roundtrip_ok = 1 when the inverse returned coef
+ * exactly; stage_ms[5] = mean device ms of each stage over reps runs after
+ * one untimed run (either may be NULL).  hash: VG_HASH_SHA3_256 (the
+ * reference's) or VG_HASH_SHA256.  No counterpart in the reference as one
+ * call: it strings together the drop-ins above on device buffers. */
+#define VG_HASH_SHA3_256 0
+#define VG_HASH_SHA256 1
+int vg_microset(int log_n, int rate_log, int hash, const vg_fe *coef, const vg_fe *fri_r, int reps, uint8_t *roots,
+                vg_fe *final_cw, int *roundtrip_ok, double *stage_ms);
+
 /* fft_gkr (src/VPD/fft_circuit_GKR.cpp:827-845; the program that
  * vpd_verifier.cpp:90-99 runs as "./fft_gkr lg log_fftgkr.txt"): the GKR
  * proof of the FFT/evaluation circuit of a commitment with 2^lg

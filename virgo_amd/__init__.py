@@ -21,7 +21,7 @@ EXPORTS = ["vg_init", "vg_last_error", "vg_free", "vg_ctx_create", "vg_ctx_destr
            "vg_prove", "vg_prove_async", "vg_finish", "vg_output", "vg_fft", "vg_ifft", "vg_sha3_64",
            "vg_merkle_tree", "vg_fri_fold", "vg_fe_mul", "vg_fft_gkr", "vg_stream_create", "vg_stream_destroy", "vg_prove_batch",
            "vg_verify", "vg_server_create", "vg_server_submit", "vg_server_wait", "vg_server_stats",
-           "vg_server_destroy", "vg_ntt_plan"]
+           "vg_server_destroy", "vg_ntt_plan", "vg_sha256_64", "vg_merkle_tree_sha256", "vg_microset"]
 
 
 class VirgoError(RuntimeError):
@@ -61,6 +61,10 @@ def lib():
         L.vg_ifft.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.vg_sha3_64.argtypes = [vp, vp, ctypes.c_size_t]
         L.vg_merkle_tree.argtypes = [vp, ctypes.c_int, vp]
+        L.vg_sha256_64.argtypes = [vp, vp, ctypes.c_size_t]
+        L.vg_merkle_tree_sha256.argtypes = [vp, ctypes.c_int, vp]
+        L.vg_microset.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, vp, vp,
+                                  ctypes.POINTER(ctypes.c_int), vp]
         L.vg_fe_mul.argtypes = [vp, vp, vp, ctypes.c_size_t]
         L.vg_fri_fold.argtypes = [vp, ctypes.c_int, vp, vp]
         L.vg_ntt_plan.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), vp]
@@ -347,12 +351,48 @@ def fri_fold(cw, r):
     return o
 
 
-def merkle_tree(leaves):
+def sha256_64(blocks):
+    """SHA-256 of each 64-byte row (vg_sha256_64)."""
+    init()
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8).reshape(-1, 64)
+    o = np.zeros((len(blocks), 32), dtype=np.uint8)
+    _check(lib().vg_sha256_64(_p(blocks), _p(o), len(blocks)))
+    return o
+
+
+HASHES = {"sha3": 0, "sha256": 1}
+
+
+def merkle_tree(leaves, hash="sha3"):
+    """Heap-layout Merkle tree of 32-byte leaves (row 1 = root), node hash
+    SHA3-256 (the proofs') or SHA-256."""
     init()
     leaves = np.ascontiguousarray(leaves, dtype=np.uint8).reshape(-1, 32)
     pad = 1
     while pad < len(leaves):
         pad *= 2
     tree = np.zeros((2 * pad, 32), dtype=np.uint8)
-    _check(lib().vg_merkle_tree(_p(leaves), len(leaves), _p(tree)))
+    fn = lib().vg_merkle_tree if HASHES[hash] == 0 else lib().vg_merkle_tree_sha256
+    _check(fn(_p(leaves), len(leaves), _p(tree)))
     return tree
+
+
+def microset(coef, rate_log, fri_r, hash="sha3", reps=1):
+    """Kernel micro-set on one polynomial (vg_microset): transform, inverse,
+    rate 2^-rate_log extension, Merkle tree of the extension's pairs, FRI
+    commit down to 32 values.  Returns (roots [L-5, 32] uint8, final 32
+    values, inverse returned coef exactly, {stage: mean device ms})."""
+    init()
+    coef = np.ascontiguousarray(coef, dtype=FE)
+    n = len(coef)
+    assert n >= 32 and n & (n - 1) == 0
+    L = n.bit_length() - 1 + rate_log
+    fri_r = np.ascontiguousarray(fri_r, dtype=FE)
+    assert len(fri_r) >= L - 5
+    roots = np.zeros((L - 5, 32), dtype=np.uint8)
+    final = np.zeros(32, dtype=FE)
+    ok = ctypes.c_int()
+    ms = np.zeros(5, dtype=np.float64)
+    _check(lib().vg_microset(n.bit_length() - 1, rate_log, HASHES[hash], _p(coef), _p(fri_r), reps, _p(roots),
+                             _p(final), ctypes.byref(ok), _p(ms)))
+    return roots, final, bool(ok.value), dict(zip(("fft", "ifft", "extension", "merkle", "fri"), ms.tolist()))

@@ -4,6 +4,7 @@
 // every field operation returns the canonical representative.
 #include "vg_kernels.hpp"
 #include "vg_keccak.hpp"
+#include "vg_sha256.hpp"
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -958,6 +959,39 @@ __global__ void k_fri_fold_nat(const fe *in, int64_t n, const fe *r, const fe *i
     fe a = in[i], b = in[i + n];
     out[i] = mul_base(add(add(a, b), mul(mul(*r, iw[i]), sub(a, b))), 1ull << 60);    // 2^-1 = 2^60
 }
+// value k of a codeword held in natural order (log_rows = 0) or coset-major
+// (the extension's layout: value k in row k mod 2^log_rows, column
+// k >> log_rows of rows of length cols)
+__device__ __forceinline__ fe cw_at(const fe *cw, int64_t k, int log_rows, int64_t cols)
+{
+    return cw[(k & (((int64_t)1 << log_rows) - 1)) * cols + (k >> log_rows)];
+}
+// Merkle leaves of a codeword of 2 half values: leaf j = value j || value j + half
+__global__ void k_pair_leaves(const fe *cw, int64_t half, int log_rows, int64_t cols, uint8_t *tree)
+{
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= half) return;
+    fe *leaf = (fe *)(tree + 32 * (half + j));
+    leaf[0] = cw_at(cw, j, log_rows, cols);
+    leaf[1] = cw_at(cw, j + half, log_rows, cols);
+}
+// the FRI fold of k_fri_fold_nat on a natural or coset-major codeword of
+// 2 half values with w^-i read from the powers tw[e] = w_SS^e:
+// w_{2 half}^-i = tw[(SS - (i << sh)) mod SS], 2 half << sh = SS
+__global__ void k_fri_fold_any(const fe *in, int64_t half, int log_rows, int64_t cols, const fe *r, const fe *tw,
+                               int64_t SS, int sh, fe *out)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= half) return;
+    fe a = cw_at(in, i, log_rows, cols), b = cw_at(in, i + half, log_rows, cols);
+    fe iw = tw[(SS - (i << sh)) & (SS - 1)];
+    out[i] = mul_base(add(add(a, b), mul(mul(*r, iw), sub(a, b))), 1ull << 60);    // 2^-1 = 2^60
+}
+__global__ void k_count_diff(const fe *a, const fe *b, int64_t n, unsigned int *cnt)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n && (a[i].img != b[i].img || a[i].real != b[i].real)) atomicAdd(cnt, 1u);
+}
 __global__ void k_twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2)
 {
     int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1188,6 +1222,13 @@ struct tree_set {
 constexpr int MK_TAIL = launch::MERKLE_TAIL;
 // level h (h >= 1) of every tree whose level h has more than MK_TAIL nodes:
 // nodes [nleaf>>h, 2*(nleaf>>h))
+// the node hash of a tree: SHA3-256 (the proofs' trees) or SHA-256
+template <int HASH>
+__device__ __forceinline__ void node_hash(const uint64_t *m, uint64_t *h)
+{
+    if constexpr (HASH == launch::HASH_SHA256) sha256_64(m, h); else sha3_64(m, h);
+}
+template <int HASH>
 __global__ __launch_bounds__(TPB) void k_merkle_multi(tree_set T, int h, int64_t total)
 {
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1204,7 +1245,7 @@ __global__ __launch_bounds__(TPB) void k_merkle_multi(tree_set T, int h, int64_t
     const uint64_t *c = (const uint64_t *)(T.tree[sg] + 2 * k * 32);
     uint64_t m[8], hh[4];
     for (int q = 0; q < 8; ++q) m[q] = c[q];
-    sha3_64(m, hh);
+    node_hash<HASH>(m, hh);
     store_digest(T.tree[sg], k, hh);
 }
 // level h of many trees (a batch of proofs): item k covers the nodes
@@ -1229,6 +1270,7 @@ __global__ __launch_bounds__(TPB) void k_merkle_level_table(const launch::mk_ite
 // the remaining levels of one tree (every level with <= MK_TAIL nodes) in
 // one block of 256 threads: the first from the tree in memory, the rest
 // from LDS
+template <int HASH>
 __device__ __forceinline__ void merkle_tail(uint8_t *tree, int64_t nl)
 {
     __shared__ uint64_t dg[MK_TAIL][4];
@@ -1240,7 +1282,7 @@ __device__ __forceinline__ void merkle_tail(uint8_t *tree, int64_t nl)
         const uint64_t *c = (const uint64_t *)(tree + 2 * k * 32);
         uint64_t m[8], hh[4];
         for (int q = 0; q < 8; ++q) m[q] = c[q];
-        sha3_64(m, hh);
+        node_hash<HASH>(m, hh);
         store_digest(tree, k, hh);
         for (int q = 0; q < 4; ++q) dg[i][q] = hh[q];
     }
@@ -1252,7 +1294,7 @@ __device__ __forceinline__ void merkle_tail(uint8_t *tree, int64_t nl)
         for (int i = threadIdx.x; i < cnt; i += blockDim.x, ++c) {
             uint64_t m[8];
             for (int q = 0; q < 4; ++q) { m[q] = dg[2 * i][q]; m[4 + q] = dg[2 * i + 1][q]; }
-            sha3_64(m, hh[c]);
+            node_hash<HASH>(m, hh[c]);
             store_digest(tree, cnt + i, hh[c]);
         }
         __syncthreads();
@@ -1262,13 +1304,14 @@ __device__ __forceinline__ void merkle_tail(uint8_t *tree, int64_t nl)
         __syncthreads();
     }
 }
+template <int HASH>
 __global__ __launch_bounds__(256) void k_merkle_tails(tree_set T)
 {
-    merkle_tail(T.tree[blockIdx.x], T.nleaf[blockIdx.x]);
+    merkle_tail<HASH>(T.tree[blockIdx.x], T.nleaf[blockIdx.x]);
 }
 __global__ __launch_bounds__(256) void k_merkle_tails_table(const launch::leaf_seg *G)
 {
-    merkle_tail(G[blockIdx.x].tree, G[blockIdx.x].S / 2);
+    merkle_tail<launch::HASH_SHA3>(G[blockIdx.x].tree, G[blockIdx.x].S / 2);
 }
 __global__ void k_gather16(const uint64_t *src, int64_t n, fe *out)
 {
@@ -1284,13 +1327,23 @@ __global__ void k_sha3_batch(const uint64_t *in, uint64_t *out, int64_t n)
     sha3_64(m, h);
     for (int i = 0; i < 4; ++i) out[4 * k + i] = h[i];
 }
+__global__ void k_sha256_batch(const uint64_t *in, uint64_t *out, int64_t n)
+{
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint64_t m[8], h[4];
+    for (int i = 0; i < 8; ++i) m[i] = in[8 * k + i];
+    sha256_64(m, h);
+    for (int i = 0; i < 4; ++i) out[4 * k + i] = h[i];
+}
 // pad leaves of a Merkle tree: H(0^64) (merkle_tree.cpp:105-117)
+template <int HASH>
 __global__ void k_merkle_pad(uint8_t *tree, int64_t pad, int64_t n_leaf)
 {
     int64_t k = n_leaf + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= pad) return;
     uint64_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0}, h[4];
-    sha3_64(m, h);
+    node_hash<HASH>(m, h);
     store_digest(tree, pad + k, h);
 }
 __global__ void k_fe_mul(const fe *a, const fe *b, fe *o, int64_t n)
@@ -1802,6 +1855,20 @@ void fri_fold_nat(const fe *in, int64_t n, const fe *r, const fe *iw, fe *out, h
 {
     if (n > 0) hipLaunchKernelGGL(k_fri_fold_nat, dim3(nblk(n)), dim3(TPB), 0, s, in, n, r, iw, out);
 }
+void pair_leaves(const fe *cw, int64_t half, int log_rows, int64_t cols, uint8_t *tree, hipStream_t s)
+{
+    if (half > 0) hipLaunchKernelGGL(k_pair_leaves, dim3(nblk(half)), dim3(TPB), 0, s, cw, half, log_rows, cols, tree);
+}
+void fri_fold_any(const fe *in, int64_t half, int log_rows, int64_t cols, const fe *r, const fe *tw, int64_t SS, int sh,
+                  fe *out, hipStream_t s)
+{
+    if (half > 0)
+        hipLaunchKernelGGL(k_fri_fold_any, dim3(nblk(half)), dim3(TPB), 0, s, in, half, log_rows, cols, r, tw, SS, sh, out);
+}
+void count_diff(const fe *a, const fe *b, int64_t n, unsigned int *cnt, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_count_diff, dim3(nblk(n)), dim3(TPB), 0, s, a, b, n, cnt);
+}
 void twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2, hipStream_t s)
 {
     hipLaunchKernelGGL(k_twiddles, dim3(nblk(n)), dim3(TPB), 0, s, tw, n, T1, T2);
@@ -1839,9 +1906,17 @@ void sha3_batch(const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s)
 {
     if (n) hipLaunchKernelGGL(k_sha3_batch, dim3(nblk(n)), dim3(TPB), 0, s, (const uint64_t *)in, (uint64_t *)out, n);
 }
-void merkle_pad(uint8_t *tree, int64_t pad, int64_t n_leaf, hipStream_t s)
+void sha256_batch(const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s)
 {
-    if (pad > n_leaf) hipLaunchKernelGGL(k_merkle_pad, dim3(nblk(pad - n_leaf)), dim3(TPB), 0, s, tree, pad, n_leaf);
+    if (n) hipLaunchKernelGGL(k_sha256_batch, dim3(nblk(n)), dim3(TPB), 0, s, (const uint64_t *)in, (uint64_t *)out, n);
+}
+void merkle_pad(uint8_t *tree, int64_t pad, int64_t n_leaf, hipStream_t s, int hash)
+{
+    if (pad <= n_leaf) return;
+    if (hash == HASH_SHA256)
+        hipLaunchKernelGGL(k_merkle_pad<HASH_SHA256>, dim3(nblk(pad - n_leaf)), dim3(TPB), 0, s, tree, pad, n_leaf);
+    else
+        hipLaunchKernelGGL(k_merkle_pad<HASH_SHA3>, dim3(nblk(pad - n_leaf)), dim3(TPB), 0, s, tree, pad, n_leaf);
 }
 void fe_mul(const fe *a, const fe *b, fe *o, int64_t n, hipStream_t s)
 {
@@ -1902,7 +1977,7 @@ void merkle_tails_table(const leaf_seg *G, int ntree, hipStream_t s)
 {
     if (ntree) hipLaunchKernelGGL(k_merkle_tails_table, dim3(ntree), dim3(256), 0, s, G);
 }
-void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStream_t s)
+void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStream_t s, int hash)
 {
     tree_set T;
     memset(&T, 0, sizeof T);
@@ -1913,9 +1988,15 @@ void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStre
         for (int k = 0; k < ntree; ++k)
             if ((nleaf[k] >> h) > MK_TAIL) total += nleaf[k] >> h;
         if (!total) break;
-        hipLaunchKernelGGL(k_merkle_multi, dim3(nblk(total)), dim3(TPB), 0, s, T, h, total);
+        if (hash == HASH_SHA256)
+            hipLaunchKernelGGL(k_merkle_multi<HASH_SHA256>, dim3(nblk(total)), dim3(TPB), 0, s, T, h, total);
+        else
+            hipLaunchKernelGGL(k_merkle_multi<HASH_SHA3>, dim3(nblk(total)), dim3(TPB), 0, s, T, h, total);
     }
-    hipLaunchKernelGGL(k_merkle_tails, dim3(ntree), dim3(256), 0, s, T);
+    if (hash == HASH_SHA256)
+        hipLaunchKernelGGL(k_merkle_tails<HASH_SHA256>, dim3(ntree), dim3(256), 0, s, T);
+    else
+        hipLaunchKernelGGL(k_merkle_tails<HASH_SHA3>, dim3(ntree), dim3(256), 0, s, T);
 }
 void reduce_slots_table(const slot_job *J, const int64_t *start, int n, int64_t total, hipStream_t s)
 {

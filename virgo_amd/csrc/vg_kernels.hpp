@@ -151,6 +151,17 @@ void twiddles(fe *tw, int64_t n, const fe *T1, const fe *T2, hipStream_t s);
 // one FRI fold of a natural-order codeword of 2n values (fri.cpp:321-340):
 // out[i] = ((in[i] + in[i+n]) + r iw[i] (in[i] - in[i+n])) / 2, iw[i] = w^-i
 void fri_fold_nat(const fe *in, int64_t n, const fe *r, const fe *iw, fe *out, hipStream_t s);
+// kernel micro-set pieces.  A codeword of 2 half values is natural
+// (log_rows = 0) or coset-major (value k in row k mod 2^log_rows, column
+// k >> log_rows, rows of cols values).  pair_leaves: the leaf level of its
+// Merkle tree (leaf j = value j || value j + half, at tree + 32 (half + j)).
+// fri_fold_any: the fold of fri_fold_nat with w^-i = tw[(SS - (i << sh))
+// mod SS] from the powers tw[e] = w_SS^e (2 half << sh == SS).  count_diff:
+// *cnt += number of positions where a and b differ.
+void pair_leaves(const fe *cw, int64_t half, int log_rows, int64_t cols, uint8_t *tree, hipStream_t s);
+void fri_fold_any(const fe *in, int64_t half, int log_rows, int64_t cols, const fe *r, const fe *tw, int64_t SS, int sh,
+                  fe *out, hipStream_t s);
+void count_diff(const fe *a, const fe *b, int64_t n, unsigned int *cnt, hipStream_t s);
 // Twiddles of the NTTs: stage-major W[2^s + k] = w_{2^(s+1)}^k and Wi (the
 // inverse powers) for s < maxlog, so each stage reads a contiguous run; the
 // LDE's coset twist w^(s j) = TA[s][j mod 2^th] TB[s][j >> th] for rows
@@ -194,8 +205,12 @@ void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int invers
 int ntt_tune(int log_n, int64_t nb, const fe *in, fe *out, const ntt_tabs &T, hipStream_t s);
 int ntt_plan(int log_n, float *us);
 void sha3_batch(const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s);
+// node hashes of a Merkle tree: the proofs use SHA3-256; SHA-256 is the
+// alternative the kernel micro-set may build its trees with
+constexpr int HASH_SHA3 = 0, HASH_SHA256 = 1;
+void sha256_batch(const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s);
 // fills leaves [n_leaf, pad) of a tree with pad leaves with H(0^64)
-void merkle_pad(uint8_t *tree, int64_t pad, int64_t n_leaf, hipStream_t s);
+void merkle_pad(uint8_t *tree, int64_t pad, int64_t n_leaf, hipStream_t s, int hash = HASH_SHA3);
 void fe_mul(const fe *a, const fe *b, fe *o, int64_t n, hipStream_t s);
 // FRI fold only (no hashing): next (S/2 per slice) from prev (S per slice)
 void fri_fold(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, int64_t SS, fe *next, hipStream_t s);
@@ -249,7 +264,7 @@ struct mk_item {
 void merkle_level_table(const mk_item *items, int n, int64_t total, hipStream_t s);
 void merkle_tails_table(const leaf_seg *G, int ntree, hipStream_t s);
 // every Merkle tree of the set, level by level across trees
-void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStream_t s);
+void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStream_t s, int hash = HASH_SHA3);
 // out[k] = 16 bytes at address src[k]
 void gather16(const uint64_t *src, int64_t n, fe *out, hipStream_t s);
 

@@ -3,7 +3,9 @@
 // (RS_polynomial.cpp:27-240), my_hhash (my_hhash.h:39-46),
 // merkle_tree_prover::create_tree (merkle_tree.cpp:4-57), one FRI fold of a
 // codeword (fri.cpp:321-340) and the field product (prime_field.h:125-151).  Each call copies in, runs the HIP kernels
-// on the null stream and copies out.
+// on the null stream and copies out.  Also the kernel micro-set (transform,
+// inverse, extension, Merkle tree, FRI commit of one polynomial, timed per
+// stage on the device) and the SHA-256 variants of the hash and tree.
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -127,21 +129,152 @@ int vg_sha3_64(const uint8_t *in, uint8_t *out, size_t n)
     return 0;
 }
 
-int vg_merkle_tree(const uint8_t *leaves, int n_leaf, uint8_t *tree)
+int vg_sha256_64(const uint8_t *in, uint8_t *out, size_t n)
 {
-    if (n_leaf < 1) { op_err = "vg_merkle_tree: no leaves"; return -1; }
+    if (!n) return 0;
+    dbuf a, b;
+    OPCHK(hipMalloc(&a.p, 64 * n));
+    OPCHK(hipMalloc(&b.p, 32 * n));
+    OPCHK(hipMemcpy(a.p, in, 64 * n, hipMemcpyHostToDevice));
+    launch::sha256_batch((const uint8_t *)a.p, (uint8_t *)b.p, (int64_t)n, 0);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(out, b.p, 32 * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int merkle_tree_with(const uint8_t *leaves, int n_leaf, uint8_t *tree, int hash, const char *name)
+{
+    if (n_leaf < 1) { op_err = std::string(name) + ": no leaves"; return -1; }
     int64_t pad = 1;
     while (pad < n_leaf) pad *= 2;
     dbuf t;
     OPCHK(hipMalloc(&t.p, 64 * pad));
     OPCHK(hipMemset(t.p, 0, 64 * pad));
     OPCHK(hipMemcpy((uint8_t *)t.p + 32 * pad, leaves, 32 * (size_t)n_leaf, hipMemcpyHostToDevice));
-    launch::merkle_pad((uint8_t *)t.p, pad, n_leaf, 0);
+    launch::merkle_pad((uint8_t *)t.p, pad, n_leaf, 0, hash);
     uint8_t *tp = (uint8_t *)t.p;
     int64_t npad = pad;
-    launch::merkle_multi(1, &tp, &npad, 0);
+    launch::merkle_multi(1, &tp, &npad, 0, hash);
     OPCHK(hipGetLastError());
     OPCHK(hipMemcpy(tree, t.p, 64 * pad, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vg_merkle_tree(const uint8_t *leaves, int n_leaf, uint8_t *tree)
+{
+    return merkle_tree_with(leaves, n_leaf, tree, launch::HASH_SHA3, "vg_merkle_tree");
+}
+
+int vg_merkle_tree_sha256(const uint8_t *leaves, int n_leaf, uint8_t *tree)
+{
+    return merkle_tree_with(leaves, n_leaf, tree, launch::HASH_SHA256, "vg_merkle_tree_sha256");
+}
+
+int vg_microset(int log_n, int rate_log, int hash, const vg_fe *coef, const vg_fe *fri_r, int reps, uint8_t *roots,
+                vg_fe *final_cw, int *roundtrip_ok, double *stage_ms)
+{
+    if (log_n < 5 || log_n > 26 || rate_log < 1 || rate_log > 5 || log_n + rate_log > 30 ||
+        (hash != VG_HASH_SHA3_256 && hash != VG_HASH_SHA256) || reps < 1 || !coef || !fri_r || !roots || !final_cw) {
+        op_err = "vg_microset: bad arguments (5 <= log_n <= 26, 1 <= rate_log <= 5, log_n + rate_log <= 30, "
+                 "hash VG_HASH_SHA3_256 or VG_HASH_SHA256, reps >= 1, no null pointers)";
+        return -1;
+    }
+    const int L = log_n + rate_log, nfold = L - 5;
+    const int64_t N = (int64_t)1 << log_n, M = (int64_t)1 << L, R = (int64_t)1 << rate_log;
+    const int th = (log_n + 1) / 2, tb = log_n - th;
+    dbuf T12, tw, W, TA, TB, lens, cf, F, G, E, tree, cw, rr, rts, cnt;
+    // powers tw[e] = w_M^e, stage tables for transforms up to N, coset twists of R rows
+    std::vector<fe> t(1024 + (M >> 10) + 1);
+    fe w = root_of_unity(L), x = fone();
+    for (int j = 0; j < 1024; ++j) { t[j] = x; x = mul(x, w); }
+    fe y = fone();
+    for (int64_t j = 0; j < (M >> 10) + 1; ++j) { t[1024 + j] = y; y = mul(y, x); }
+    OPCHK(hipMalloc(&T12.p, sizeof(fe) * t.size()));
+    OPCHK(hipMemcpy(T12.p, t.data(), sizeof(fe) * t.size(), hipMemcpyHostToDevice));
+    OPCHK(hipMalloc(&tw.p, sizeof(fe) * M));
+    OPCHK(hipMalloc(&W.p, sizeof(fe) * 2 * N));
+    OPCHK(hipMalloc(&TA.p, sizeof(fe) * (R << th)));
+    OPCHK(hipMalloc(&TB.p, sizeof(fe) * (R << tb)));
+    fe *tw_ = (fe *)tw.p, *W0 = (fe *)W.p;
+    launch::twiddles(tw_, M, (const fe *)T12.p, (const fe *)T12.p + 1024, 0);
+    launch::ntt_tables(W0, W0 + N, log_n, (fe *)TA.p, (fe *)TB.p, rate_log, th, tb, tw_, M, 0);
+    launch::ntt_tabs T = {W0, W0 + N, (const fe *)TA.p, (const fe *)TB.p, th, tb};
+    int len = (int)N;
+    OPCHK(hipMalloc(&lens.p, sizeof(int)));
+    OPCHK(hipMemcpy(lens.p, &len, sizeof(int), hipMemcpyHostToDevice));
+    // data: coefficients, transform, inverse, extension (coset-major), one
+    // tree (reused level by level), the folded codewords of every level
+    OPCHK(hipMalloc(&cf.p, sizeof(fe) * N));
+    OPCHK(hipMalloc(&F.p, sizeof(fe) * N));
+    OPCHK(hipMalloc(&G.p, sizeof(fe) * N));
+    OPCHK(hipMalloc(&E.p, sizeof(fe) * M));
+    OPCHK(hipMalloc(&tree.p, (size_t)32 * M));
+    OPCHK(hipMalloc(&cw.p, sizeof(fe) * M));
+    OPCHK(hipMalloc(&rr.p, sizeof(fe) * nfold));
+    OPCHK(hipMalloc(&rts.p, (size_t)32 * nfold));
+    OPCHK(hipMalloc(&cnt.p, sizeof(unsigned int)));
+    OPCHK(hipMemcpy(cf.p, coef, sizeof(fe) * N, hipMemcpyHostToDevice));
+    OPCHK(hipMemcpy(rr.p, fri_r, sizeof(fe) * nfold, hipMemcpyHostToDevice));
+    const fe *cf_ = (const fe *)cf.p, *r_ = (const fe *)rr.p;
+    fe *F_ = (fe *)F.p, *G_ = (fe *)G.p, *E_ = (fe *)E.p;
+    uint8_t *tr_ = (uint8_t *)tree.p, *rt_ = (uint8_t *)rts.p;
+    // codeword of level k >= 1 (M >> k values) at lv[k]
+    std::vector<fe *> lv(nfold + 1, nullptr);
+    {
+        int64_t off = 0;
+        for (int k = 1; k <= nfold; ++k) { lv[k] = (fe *)cw.p + off; off += M >> k; }
+    }
+    if (launch::ntt_tune(log_n, R, E_, (fe *)tree.p, T, 0)) { op_err = "vg_microset: NTT tile tuning failed"; return -20; }
+    const fe inv_n = finv(mk((uint64_t)N, 0));
+    hipEvent_t ev[6] = {};
+    for (auto &e : ev) OPCHK(hipEventCreate(&e));
+    struct evs { hipEvent_t *e; ~evs() { for (int i = 0; i < 6; ++i) if (e[i]) (void)hipEventDestroy(e[i]); } } ev_guard{ev};
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int rep = 0; rep <= reps; ++rep) {          // rep 0 warms up, untimed
+        OPCHK(hipEventRecord(ev[0], 0));
+        launch::ntt({0, cf_, N, nullptr, nullptr, 0, 0, 0, 0}, F_, N, 1, log_n, 0, fone(), 0, T, 0);
+        OPCHK(hipEventRecord(ev[1], 0));
+        launch::ntt({0, F_, N, nullptr, nullptr, 0, 0, 0, 0}, G_, N, 1, log_n, 1, inv_n, 1, T, 0);
+        OPCHK(hipEventRecord(ev[2], 0));
+        launch::lde(cf_, N, (const int *)lens.p, 1, E_, log_n, rate_log, T, 0);
+        OPCHK(hipEventRecord(ev[3], 0));
+        // level 0: the tree of the extension's pairs (E[j], E[j + M/2])
+        int64_t half = M / 2;
+        launch::pair_leaves(E_, half, rate_log, N, tr_, 0);
+        launch::merkle_multi(1, &tr_, &half, 0, hash);
+        OPCHK(hipMemcpyAsync(rt_, tr_ + 32, 32, hipMemcpyDeviceToDevice, 0));
+        OPCHK(hipEventRecord(ev[4], 0));
+        // FRI: fold level k into k + 1; commit every level with more than 32 values
+        for (int k = 0; k < nfold; ++k) {
+            half = (M >> k) / 2;
+            if (k == 0) launch::fri_fold_any(E_, half, rate_log, N, r_, tw_, M, 0, lv[1], 0);
+            else launch::fri_fold_any(lv[k], half, 0, 0, r_ + k, tw_, M, k, lv[k + 1], 0);
+            if (k + 1 < nfold) {
+                int64_t h2 = half / 2;
+                launch::pair_leaves(lv[k + 1], h2, 0, 0, tr_, 0);
+                launch::merkle_multi(1, &tr_, &h2, 0, hash);
+                OPCHK(hipMemcpyAsync(rt_ + 32 * (k + 1), tr_ + 32, 32, hipMemcpyDeviceToDevice, 0));
+            }
+        }
+        OPCHK(hipEventRecord(ev[5], 0));
+        OPCHK(hipGetLastError());
+        OPCHK(hipEventSynchronize(ev[5]));
+        if (rep == 0) continue;
+        for (int sgi = 0; sgi < 5; ++sgi) {
+            float ms = 0;
+            OPCHK(hipEventElapsedTime(&ms, ev[sgi], ev[sgi + 1]));
+            acc[sgi] += ms;
+        }
+    }
+    OPCHK(hipMemset(cnt.p, 0, sizeof(unsigned int)));
+    launch::count_diff(G_, cf_, N, (unsigned int *)cnt.p, 0);
+    unsigned int nd = 0;
+    OPCHK(hipMemcpy(&nd, cnt.p, sizeof nd, hipMemcpyDeviceToHost));
+    OPCHK(hipMemcpy(roots, rts.p, (size_t)32 * nfold, hipMemcpyDeviceToHost));
+    OPCHK(hipMemcpy(final_cw, lv[nfold], sizeof(fe) * 32, hipMemcpyDeviceToHost));
+    if (roundtrip_ok) *roundtrip_ok = nd == 0;
+    if (stage_ms)
+        for (int sgi = 0; sgi < 5; ++sgi) stage_ms[sgi] = acc[sgi] / reps;
     return 0;
 }
 

@@ -47,7 +47,9 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-budget-gb", type=int, default=96)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--part-only", choices=["batch", "serve"],
+    ap.add_argument("--no-microset", action="store_true")
+    ap.add_argument("--microset-sizes", default="18,20,22,24")
+    ap.add_argument("--part-only", choices=["batch", "serve", "microset"],
                     help="run only the batched or the serving part and print its object (for profiling)")
     return ap.parse_args()
 
@@ -91,6 +93,90 @@ def c1(args):
         ok_o, _ = O.verify(circ, tr)
         out["cpu_verify_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
         out["bit_exact_vs_cpu"] = ref == tr and bool(ok_o)
+    return out
+
+
+MICRO_RATE_LOG = 3      # the kernel micro-set's extension rate 1/8
+KECCAK_OPS = 24 * 180   # 32-bit VALU ops per Keccak-f[1600] (as for the leaf chains)
+
+
+def _rand_fe(rng, n):
+    import numpy as np
+    import virgo_amd as V
+    a = np.zeros(n, dtype=V.FE)
+    a["real"] = rng.integers(0, (1 << 61) - 1, n, dtype=np.uint64)
+    a["img"] = rng.integers(0, (1 << 61) - 1, n, dtype=np.uint64)
+    return a
+
+
+def microset(args):
+    """The kernel micro-set: one polynomial of 2^k uniform F_{p^2}
+    coefficients (k in --microset-sizes), forward and inverse transform,
+    rate-1/8 extension, Merkle tree over the extension's 32-byte pairs, FRI
+    commit (fold + tree per level) down to 32 values; each stage timed on the
+    device (mean of 5 after one warm-up, vg_microset), with SHA3-256 (the
+    reference's hash) and SHA-256 trees.  Rates: transform butterflies/s,
+    tree hashes/s, and for SHA3 trees the VALU fraction at 4320 ops per
+    Keccak-f.  CPU: the oracle's stages (the reference's FFT, SHA3 tree and
+    fold, 1 core) at 2^18 and 2^20."""
+    import numpy as np
+    import virgo_amd as V
+    out = {"workload": "one polynomial of 2^k uniform F_{p^2} coefficients, rate 1/8 extension, Merkle tree of "
+                       "32-byte pairs, FRI commit to 32 values, seed 400+k", "sizes": {}}
+    for k in [int(x) for x in args.microset_sizes.split(",") if x]:
+        rng = np.random.default_rng(400 + k)
+        L = k + MICRO_RATE_LOG
+        c, rs = _rand_fe(rng, 1 << k), _rand_fe(rng, L - 5)
+        row = {}
+        for h in ("sha3", "sha256"):
+            roots, _, ok, ms = V.microset(c, MICRO_RATE_LOG, rs, hash=h, reps=5)
+            M = 1 << L
+            fly = (1 << k) // 2 * k
+            hashes = M // 2 - 1
+            e = {s + "_ms": round(v, 4) for s, v in ms.items()}
+            e["total_ms"] = round(sum(ms.values()), 3)
+            e["roundtrip_exact"] = ok
+            e["fft_gbfly_s"] = round(fly / (ms["fft"] * 1e-3) / 1e9, 2)
+            e["extension_gbfly_s"] = round((fly << MICRO_RATE_LOG) / (ms["extension"] * 1e-3) / 1e9, 2)
+            e["merkle_mhash_s"] = round(hashes / (ms["merkle"] * 1e-3) / 1e6, 1)
+            if h == "sha3":
+                gops = hashes * KECCAK_OPS / (ms["merkle"] * 1e-3) / 1e9
+                e["merkle_valu_frac"] = round(gops / PEAK_VALU_GOPS, 3)
+            e["root"] = bytes(roots[0]).hex()[:16]
+            row[h] = e
+        out["sizes"]["2^%d" % k] = row
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        cpu = {}
+        for k in (18, 20):
+            rng = np.random.default_rng(400 + k)
+            L = k + MICRO_RATE_LOG
+            N, M = 1 << k, 1 << L
+            c, rs = _rand_fe(rng, N), _rand_fe(rng, L - 5)
+            w = O.root_of_unity(k)
+            t = [time.perf_counter()]
+            f = O.fft(c, N, w)
+            t.append(time.perf_counter())
+            O.ifft(f, N, w)
+            t.append(time.perf_counter())
+            E = O.fft(c, M, O.root_of_unity(L))
+            t.append(time.perf_counter())
+            lv = np.ascontiguousarray(np.stack([E[:M // 2], E[M // 2:]], axis=1)).view(np.uint8).reshape(-1, 32)
+            root0 = bytes(O.merkle_tree(lv)[1]).hex()[:16]
+            t.append(time.perf_counter())
+            cw = E
+            for j in range(L - 5):
+                cw = O.fri_fold1(cw, rs[j:j + 1])
+                if j + 1 < L - 5:
+                    h2 = len(cw) // 2
+                    O.merkle_tree(np.ascontiguousarray(np.stack([cw[:h2], cw[h2:]], axis=1)).view(np.uint8).reshape(-1, 32))
+            t.append(time.perf_counter())
+            names = ("fft", "ifft", "extension", "merkle", "fri")
+            e = {n + "_ms": round((t[i + 1] - t[i]) * 1e3, 1) for i, n in enumerate(names)}
+            e["total_ms"] = round((t[-1] - t[0]) * 1e3, 1)
+            e["same_root_as_gpu"] = root0 == out["sizes"].get("2^%d" % k, {}).get("sha3", {}).get("root", root0)
+            cpu["2^%d" % k] = e
+        out["cpu_oracle_1core"] = cpu
     return out
 
 
@@ -252,7 +338,8 @@ def main():
     import virgo_amd as V
     from virgo_amd import workloads as W
     if args.part_only:
-        print(json.dumps({args.part_only: batched(args) if args.part_only == "batch" else serving(args)}))
+        part = {"batch": batched, "serve": serving, "microset": microset}[args.part_only]
+        print(json.dumps({args.part_only: part(args)}))
         return
     # the CPU throughput baseline of the batch first: its worker processes
     # start before this process initialises the GPU
@@ -322,6 +409,8 @@ def main():
         line["batched"] = batched(args)
     if not args.no_serve:
         line["serving"] = serving(args)
+    if not args.no_microset:
+        line["microset"] = microset(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args)
         if cpu_b:

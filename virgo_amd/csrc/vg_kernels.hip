@@ -781,10 +781,13 @@ struct ntt_in {
     const fe *src; int64_t stride;
     const fe *l, *q; int64_t cw_S, cw_stride, step; int log_R;
     const int *lens; int log_Rl; int hmode;
+    int brev_src;             // kinds 0 and 2 (hmode 0): src already in bit-reversed order
 };
-__device__ __forceinline__ fe ntt_fetch(const ntt_in &in, int64_t b, int64_t j, const launch::ntt_tabs &T)
+// input value j of row b; jm: where src holds it (j, or brev(j) for a
+// bit-reversed source)
+__device__ __forceinline__ fe ntt_fetch(const ntt_in &in, int64_t b, int64_t j, int64_t jm, const launch::ntt_tabs &T)
 {
-    if (in.kind == 0) return in.src[b * in.stride + j];
+    if (in.kind == 0) return in.src[b * in.stride + jm];
     if (in.kind == 1) {
         int64_t k = cw_idx(j * in.step, in.cw_S, in.log_R);
         return mul(in.l[b * in.cw_stride + k], in.q[b * in.cw_stride + k]);
@@ -792,7 +795,7 @@ __device__ __forceinline__ fe ntt_fetch(const ntt_in &in, int64_t b, int64_t j, 
     int64_t slice = b >> in.log_Rl, s = b & ((1 << in.log_Rl) - 1);
     int len = in.lens[slice];
     if (j >= len) return fzero();
-    fe c = in.src[slice * in.stride + (in.hmode ? len : 0) + j];
+    fe c = in.src[slice * in.stride + (in.hmode ? len : 0) + jm];
     // coset twist w_SS^(s j) from two small tables
     fe x = mul(T.TA[(s << T.th) + (j & ((1 << T.th) - 1))], T.TB[(s << T.tb) + (j >> T.th)]);
     return mul(c, x);
@@ -885,7 +888,7 @@ __device__ __forceinline__ void ntt_pass(fe *sh, int M, int s0, int first, int l
             if (first) {
                 int64_t pos = (q << log_m) + i;
                 int64_t j = log_n ? (int64_t)(__brevll((unsigned long long)pos) >> (64 - log_n)) : 0;
-                v[m] = ntt_fetch(in, b, j, T);
+                v[m] = ntt_fetch(in, b, j, in.brev_src ? pos : j, T);
             } else {
                 v[m] = sh[lpad(i)];
             }
@@ -911,7 +914,7 @@ __global__ __launch_bounds__(TPB) void k_ntt_lds(ntt_in in, fe *out, int64_t out
     int M = 1 << log_m;
     if (log_m == 0) {                                  // a transform of size 1
         if (threadIdx.x == 0) {
-            fe x = ntt_fetch(in, b, 0, T);
+            fe x = ntt_fetch(in, b, 0, 0, T);
             out[b * out_stride + q] = do_scale ? mul(x, scale) : x;
         }
         return;
@@ -977,15 +980,54 @@ __global__ void k_pair_leaves(const fe *cw, int64_t half, int log_rows, int64_t 
 }
 // the FRI fold of k_fri_fold_nat on a natural or coset-major codeword of
 // 2 half values with w^-i read from the powers tw[e] = w_SS^e:
-// w_{2 half}^-i = tw[(SS - (i << sh)) mod SS], 2 half << sh = SS
+// w_{2 half}^-i = tw[(SS - (i << sh)) mod SS], 2 half << sh = SS.  With
+// leaves set, value i of the result is also stored in the leaf level of
+// the result's own tree (half / 2 leaves: leaf i mod half/2, second value
+// when i >= half/2).
 __global__ void k_fri_fold_any(const fe *in, int64_t half, int log_rows, int64_t cols, const fe *r, const fe *tw,
-                               int64_t SS, int sh, fe *out)
+                               int64_t SS, int sh, fe *out, uint8_t *leaves)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= half) return;
     fe a = cw_at(in, i, log_rows, cols), b = cw_at(in, i + half, log_rows, cols);
     fe iw = tw[(SS - (i << sh)) & (SS - 1)];
-    out[i] = mul_base(add(add(a, b), mul(mul(*r, iw), sub(a, b))), 1ull << 60);    // 2^-1 = 2^60
+    fe o = mul_base(add(add(a, b), mul(mul(*r, iw), sub(a, b))), 1ull << 60);    // 2^-1 = 2^60
+    out[i] = o;
+    if (leaves) {
+        int64_t h2 = half >> 1;
+        *(fe *)(leaves + 32 * (h2 + (i & (h2 - 1))) + (i >= h2 ? 16 : 0)) = o;
+    }
+}
+// dst[brev(j)] = j < len ? src[j] : 0 over log_n >= 12 bits, in 64 x 64
+// tiles: the tile of middle bits mid holds j = a 2^(log_n-6) + mid 2^6 + c
+// (a, c < 64), read along c and written along brev6(a), both contiguous
+__global__ __launch_bounds__(256) void k_brev_tile(const fe *src, int64_t len, fe *dst, int log_n)
+{
+    __shared__ fe t[64][65];
+    int lm = log_n - 12, sh = log_n - 6;
+    int64_t mid = blockIdx.x;
+    int64_t rmid = lm ? (int64_t)(__brevll((unsigned long long)mid) >> (64 - lm)) : 0;
+    for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+        int a = e >> 6, c = e & 63;
+        int64_t j = ((int64_t)a << sh) | (mid << 6) | c;
+        t[a][c] = j < len ? src[j] : fzero();
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+        int r = e >> 6, u = e & 63;                  // r = brev6(c), u = brev6(a)
+        int c = (int)(__brev((unsigned)r) >> 26), a = (int)(__brev((unsigned)u) >> 26);
+        dst[((int64_t)r << sh) | (rmid << 6) | u] = t[a][c];
+    }
+}
+struct root_src {
+    const uint8_t *p[32];
+};
+// out[32 k ..] = the root (node 1) of tree k, k < n
+__global__ void k_copy_roots(root_src S, int n, uint8_t *out)
+{
+    int t = threadIdx.x;
+    if (t >= 4 * n) return;
+    ((uint64_t *)out)[t] = ((const uint64_t *)(S.p[t >> 2] + 32))[t & 3];
 }
 __global__ void k_count_diff(const fe *a, const fe *b, int64_t n, unsigned int *cnt)
 {
@@ -1860,10 +1902,26 @@ void pair_leaves(const fe *cw, int64_t half, int log_rows, int64_t cols, uint8_t
     if (half > 0) hipLaunchKernelGGL(k_pair_leaves, dim3(nblk(half)), dim3(TPB), 0, s, cw, half, log_rows, cols, tree);
 }
 void fri_fold_any(const fe *in, int64_t half, int log_rows, int64_t cols, const fe *r, const fe *tw, int64_t SS, int sh,
-                  fe *out, hipStream_t s)
+                  fe *out, uint8_t *leaves, hipStream_t s)
 {
     if (half > 0)
-        hipLaunchKernelGGL(k_fri_fold_any, dim3(nblk(half)), dim3(TPB), 0, s, in, half, log_rows, cols, r, tw, SS, sh, out);
+        hipLaunchKernelGGL(k_fri_fold_any, dim3(nblk(half)), dim3(TPB), 0, s, in, half, log_rows, cols, r, tw, SS, sh, out,
+                           leaves);
+}
+int brev_permute(const fe *src, int64_t len, fe *dst, int log_n, hipStream_t s)
+{
+    if (log_n < 12 || log_n > 40 || len > ((int64_t)1 << log_n)) return -1;
+    hipLaunchKernelGGL(k_brev_tile, dim3((unsigned)((int64_t)1 << (log_n - 12))), dim3(256), 0, s, src, len, dst, log_n);
+    return 0;
+}
+int copy_roots(const uint8_t *const *trees, int n, uint8_t *out, hipStream_t s)
+{
+    if (n < 1 || n > 32) return -1;
+    root_src S;
+    memset(&S, 0, sizeof S);
+    for (int k = 0; k < n; ++k) S.p[k] = trees[k];
+    hipLaunchKernelGGL(k_copy_roots, dim3(1), dim3(128), 0, s, S, n, out);
+    return 0;
 }
 void count_diff(const fe *a, const fe *b, int64_t n, unsigned int *cnt, hipStream_t s)
 {
@@ -1878,15 +1936,16 @@ void intt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, fe inv_n,
     ntt_in ni;
     ni.kind = in.kind; ni.src = in.src; ni.stride = in.src_stride;
     ni.l = in.l; ni.q = in.q; ni.cw_S = in.cw_S; ni.cw_stride = in.cw_stride; ni.step = in.step; ni.log_R = in.log_R;
-    ni.lens = nullptr; ni.log_Rl = 0; ni.hmode = 0;
+    ni.lens = nullptr; ni.log_Rl = 0; ni.hmode = 0; ni.brev_src = in.brev_src;
     ntt_run(ni, out, out_stride, nb, log_n, 1, inv_n, 1, T, s);
 }
 void lde(const fe *coef, int64_t coef_stride, const int *lens, int nb, fe *out, int log_C, int log_R, const ntt_tabs &T,
-         hipStream_t s, int hmode)
+         hipStream_t s, int hmode, int brev_src)
 {
     ntt_in ni;
     memset(&ni, 0, sizeof ni);
     ni.kind = 2; ni.src = coef; ni.stride = coef_stride; ni.lens = lens; ni.log_Rl = log_R; ni.hmode = hmode;
+    ni.brev_src = brev_src;
     ntt_run(ni, out, (int64_t)1 << log_C, (int64_t)nb << log_R, log_C, 0, fone(), 0, T, s);
 }
 void slice_sums(const fe *lq_coef, int64_t stride, int nslice, int64_t real, int64_t m, fe *c0, fe *all_sum, hipStream_t s)
@@ -1900,6 +1959,7 @@ void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int invers
     memset(&ni, 0, sizeof ni);
     ni.kind = in.kind; ni.src = in.src; ni.stride = in.src_stride;
     ni.l = in.l; ni.q = in.q; ni.cw_S = in.cw_S; ni.cw_stride = in.cw_stride; ni.step = in.step; ni.log_R = in.log_R;
+    ni.brev_src = in.brev_src;
     ntt_run(ni, out, out_stride, nb, log_n, inverse, scale, do_scale, T, s);
 }
 void sha3_batch(const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s)

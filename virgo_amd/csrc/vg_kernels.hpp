@@ -156,11 +156,14 @@ void fri_fold_nat(const fe *in, int64_t n, const fe *r, const fe *iw, fe *out, h
 // k >> log_rows, rows of cols values).  pair_leaves: the leaf level of its
 // Merkle tree (leaf j = value j || value j + half, at tree + 32 (half + j)).
 // fri_fold_any: the fold of fri_fold_nat with w^-i = tw[(SS - (i << sh))
-// mod SS] from the powers tw[e] = w_SS^e (2 half << sh == SS).  count_diff:
-// *cnt += number of positions where a and b differ.
+// mod SS] from the powers tw[e] = w_SS^e (2 half << sh == SS); with leaves
+// set it also writes the leaf level of the result's tree (half/2 leaves,
+// half >= 2).  copy_roots: out[32 k ..] = root of tree k (k < n <= 32).
+// count_diff: *cnt += number of positions where a and b differ.
 void pair_leaves(const fe *cw, int64_t half, int log_rows, int64_t cols, uint8_t *tree, hipStream_t s);
 void fri_fold_any(const fe *in, int64_t half, int log_rows, int64_t cols, const fe *r, const fe *tw, int64_t SS, int sh,
-                  fe *out, hipStream_t s);
+                  fe *out, uint8_t *leaves, hipStream_t s);
+int copy_roots(const uint8_t *const *trees, int n, uint8_t *out, hipStream_t s);
 void count_diff(const fe *a, const fe *b, int64_t n, unsigned int *cnt, hipStream_t s);
 // Twiddles of the NTTs: stage-major W[2^s + k] = w_{2^(s+1)}^k and Wi (the
 // inverse powers) for s < maxlog, so each stage reads a contiguous run; the
@@ -182,13 +185,18 @@ struct intt_src {
     int kind;
     const fe *src; int64_t src_stride;
     const fe *l, *q; int64_t cw_S, cw_stride; int64_t step; int log_R;
+    int brev_src = 0;      // kind 0: src holds the row in bit-reversed order (brev_permute)
 };
 void intt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, fe inv_n, const ntt_tabs &T, hipStream_t s);
 // Coset LDE: for each slice b and row s < R, out[b][s][t] = sum_{j<len_b} c_b[j] w_SS^{(tR+s) j}.
 // coef[b] has lens[b] (<= C) entries at coef + b*coef_stride.
 // hmode: coefficients of slice b start at coef + b*coef_stride + lens[b] (upper half of lq)
 void lde(const fe *coef, int64_t coef_stride, const int *lens, int nb, fe *out, int log_C, int log_R,
-         const ntt_tabs &T, hipStream_t s, int hmode = 0);
+         const ntt_tabs &T, hipStream_t s, int hmode = 0, int brev_src = 0);
+// dst[brev(j)] = j < len ? src[j] : 0 for j < 2^log_n (log_n >= 12): the
+// bit-reversed copy a transform reads with brev_src set, so that its first
+// pass reads contiguous tiles instead of one value per 2^(log_n-12) stride
+int brev_permute(const fe *src, int64_t len, fe *dst, int log_n, hipStream_t s);
 // all_sum[i] = (lq_coef[0] + lq_coef[len_i]) * len_i; c0[i] = lq_coef[0] + lq_coef[len_i]
 void slice_sums(const fe *lq_coef, int64_t stride, int nslice, int64_t real, int64_t m, fe *c0, fe *all_sum, hipStream_t s);
 

@@ -182,7 +182,7 @@ int vg_microset(int log_n, int rate_log, int hash, const vg_fe *coef, const vg_f
     const int L = log_n + rate_log, nfold = L - 5;
     const int64_t N = (int64_t)1 << log_n, M = (int64_t)1 << L, R = (int64_t)1 << rate_log;
     const int th = (log_n + 1) / 2, tb = log_n - th;
-    dbuf T12, tw, W, TA, TB, lens, cf, F, G, E, tree, cw, rr, rts, cnt;
+    dbuf T12, tw, W, TA, TB, lens, cf, F, G, E, tree, ftree, cw, rr, rts, cnt, pbuf;
     // powers tw[e] = w_M^e, stage tables for transforms up to N, coset twists of R rows
     std::vector<fe> t(1024 + (M >> 10) + 1);
     fe w = root_of_unity(L), x = fone();
@@ -202,13 +202,15 @@ int vg_microset(int log_n, int rate_log, int hash, const vg_fe *coef, const vg_f
     int len = (int)N;
     OPCHK(hipMalloc(&lens.p, sizeof(int)));
     OPCHK(hipMemcpy(lens.p, &len, sizeof(int), hipMemcpyHostToDevice));
-    // data: coefficients, transform, inverse, extension (coset-major), one
-    // tree (reused level by level), the folded codewords of every level
+    // data: coefficients, transform, inverse, extension (coset-major), its
+    // tree, the folded codewords of every level and their trees
     OPCHK(hipMalloc(&cf.p, sizeof(fe) * N));
+    OPCHK(hipMalloc(&pbuf.p, sizeof(fe) * N));
     OPCHK(hipMalloc(&F.p, sizeof(fe) * N));
     OPCHK(hipMalloc(&G.p, sizeof(fe) * N));
     OPCHK(hipMalloc(&E.p, sizeof(fe) * M));
     OPCHK(hipMalloc(&tree.p, (size_t)32 * M));
+    OPCHK(hipMalloc(&ftree.p, (size_t)32 * M));
     OPCHK(hipMalloc(&cw.p, sizeof(fe) * M));
     OPCHK(hipMalloc(&rr.p, sizeof(fe) * nfold));
     OPCHK(hipMalloc(&rts.p, (size_t)32 * nfold));
@@ -218,44 +220,73 @@ int vg_microset(int log_n, int rate_log, int hash, const vg_fe *coef, const vg_f
     const fe *cf_ = (const fe *)cf.p, *r_ = (const fe *)rr.p;
     fe *F_ = (fe *)F.p, *G_ = (fe *)G.p, *E_ = (fe *)E.p;
     uint8_t *tr_ = (uint8_t *)tree.p, *rt_ = (uint8_t *)rts.p;
-    // codeword of level k >= 1 (M >> k values) at lv[k]
+    // codeword of level k >= 1 (M >> k values) at lv[k]; its tree (levels
+    // with more than 32 values, (M >> k) / 2 leaves) at lt[k]
     std::vector<fe *> lv(nfold + 1, nullptr);
+    std::vector<uint8_t *> lt(nfold + 1, nullptr);
+    std::vector<int64_t> ln(nfold + 1, 0);
     {
-        int64_t off = 0;
-        for (int k = 1; k <= nfold; ++k) { lv[k] = (fe *)cw.p + off; off += M >> k; }
+        int64_t off = 0, toff = 0;
+        for (int k = 1; k <= nfold; ++k) {
+            lv[k] = (fe *)cw.p + off;
+            off += M >> k;
+            if (k < nfold) {
+                lt[k] = (uint8_t *)ftree.p + toff;
+                ln[k] = (M >> k) / 2;
+                toff += 64 * ln[k];
+            }
+        }
     }
+    lt[0] = tr_;
     if (launch::ntt_tune(log_n, R, E_, (fe *)tree.p, T, 0)) { op_err = "vg_microset: NTT tile tuning failed"; return -20; }
     const fe inv_n = finv(mk((uint64_t)N, 0));
+    const bool big = log_n >= 22;
+    fe *P_ = (fe *)pbuf.p;
     hipEvent_t ev[6] = {};
-    for (auto &e : ev) OPCHK(hipEventCreate(&e));
     struct evs { hipEvent_t *e; ~evs() { for (int i = 0; i < 6; ++i) if (e[i]) (void)hipEventDestroy(e[i]); } } ev_guard{ev};
+    for (auto &e : ev) OPCHK(hipEventCreate(&e));
     double acc[5] = {0, 0, 0, 0, 0};
     for (int rep = 0; rep <= reps; ++rep) {          // rep 0 warms up, untimed
+        // transforms of 2^22 values and more read a bit-reversed copy (one
+        // tiled permutation) instead of gathering at stride 2^(log_n-12):
+        // the extension of 2^24 values takes 7.8 instead of 8.6 ms; below
+        // 2^22 the permutation costs more than it saves
         OPCHK(hipEventRecord(ev[0], 0));
-        launch::ntt({0, cf_, N, nullptr, nullptr, 0, 0, 0, 0}, F_, N, 1, log_n, 0, fone(), 0, T, 0);
+        if (big) {
+            launch::brev_permute(cf_, N, P_, log_n, 0);
+            launch::ntt({0, P_, N, nullptr, nullptr, 0, 0, 0, 0, 1}, F_, N, 1, log_n, 0, fone(), 0, T, 0);
+        } else {
+            launch::ntt({0, cf_, N, nullptr, nullptr, 0, 0, 0, 0}, F_, N, 1, log_n, 0, fone(), 0, T, 0);
+        }
         OPCHK(hipEventRecord(ev[1], 0));
-        launch::ntt({0, F_, N, nullptr, nullptr, 0, 0, 0, 0}, G_, N, 1, log_n, 1, inv_n, 1, T, 0);
+        if (big) {
+            launch::brev_permute(F_, N, P_, log_n, 0);
+            launch::ntt({0, P_, N, nullptr, nullptr, 0, 0, 0, 0, 1}, G_, N, 1, log_n, 1, inv_n, 1, T, 0);
+        } else {
+            launch::ntt({0, F_, N, nullptr, nullptr, 0, 0, 0, 0}, G_, N, 1, log_n, 1, inv_n, 1, T, 0);
+        }
         OPCHK(hipEventRecord(ev[2], 0));
-        launch::lde(cf_, N, (const int *)lens.p, 1, E_, log_n, rate_log, T, 0);
+        if (big) {
+            launch::brev_permute(cf_, N, P_, log_n, 0);
+            launch::lde(P_, N, (const int *)lens.p, 1, E_, log_n, rate_log, T, 0, 0, 1);
+        } else {
+            launch::lde(cf_, N, (const int *)lens.p, 1, E_, log_n, rate_log, T, 0);
+        }
         OPCHK(hipEventRecord(ev[3], 0));
         // level 0: the tree of the extension's pairs (E[j], E[j + M/2])
         int64_t half = M / 2;
         launch::pair_leaves(E_, half, rate_log, N, tr_, 0);
         launch::merkle_multi(1, &tr_, &half, 0, hash);
-        OPCHK(hipMemcpyAsync(rt_, tr_ + 32, 32, hipMemcpyDeviceToDevice, 0));
         OPCHK(hipEventRecord(ev[4], 0));
-        // FRI: fold level k into k + 1; commit every level with more than 32 values
+        // FRI: every fold first (each writes the next codeword and its tree's
+        // leaves), then the trees of all levels together
         for (int k = 0; k < nfold; ++k) {
             half = (M >> k) / 2;
-            if (k == 0) launch::fri_fold_any(E_, half, rate_log, N, r_, tw_, M, 0, lv[1], 0);
-            else launch::fri_fold_any(lv[k], half, 0, 0, r_ + k, tw_, M, k, lv[k + 1], 0);
-            if (k + 1 < nfold) {
-                int64_t h2 = half / 2;
-                launch::pair_leaves(lv[k + 1], h2, 0, 0, tr_, 0);
-                launch::merkle_multi(1, &tr_, &h2, 0, hash);
-                OPCHK(hipMemcpyAsync(rt_ + 32 * (k + 1), tr_ + 32, 32, hipMemcpyDeviceToDevice, 0));
-            }
+            if (k == 0) launch::fri_fold_any(E_, half, rate_log, N, r_, tw_, M, 0, lv[1], lt[1], 0);
+            else launch::fri_fold_any(lv[k], half, 0, 0, r_ + k, tw_, M, k, lv[k + 1], lt[k + 1], 0);
         }
+        if (nfold > 1) launch::merkle_multi(nfold - 1, lt.data() + 1, ln.data() + 1, 0, hash);
+        if (launch::copy_roots(lt.data(), nfold, rt_, 0)) { op_err = "vg_microset: too many levels"; return -1; }
         OPCHK(hipEventRecord(ev[5], 0));
         OPCHK(hipGetLastError());
         OPCHK(hipEventSynchronize(ev[5]));

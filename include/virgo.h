@@ -63,12 +63,14 @@ int vg_stream_destroy(void *stream);
 int64_t vg_ctx_device_bytes(const vg_ctx *ctx);
 /* NTT schedule in use for transforms of 2^log_n values (no counterpart in
  * the reference, whose FFT has one fixed loop, RS_polynomial.cpp:27-240):
- * the LDS tile (log2 of its elements, 10..12) picked by the library's
- * autotuner -- the first context that needs a transform size times the
- * three tiles on its own buffers and keeps the fastest for the process --
- * and, in us[0..2] (may be NULL), the measured times of the 2^10, 2^11 and
- * 2^12 tiles (0: not measured; the default 2^11 until tuned, VIRGO_NTT_LM
- * forces one).  Every tile gives the same results. */
+ * the LDS tile (log2 of its elements, 10..12), or 1 for the one-block
+ * transform of sizes 2^13 and 2^14 (all stages in one launch), picked by
+ * the library's autotuner -- the first context that needs a transform size
+ * times the candidates on its own buffers and keeps the fastest for the
+ * process -- and, in us[0..3] (may be NULL), the measured times of the
+ * 2^10, 2^11 and 2^12 tiles and of the one-block transform (0: not
+ * measured; the default 2^11 until tuned, VIRGO_NTT_LM=10|11|12|1 forces
+ * one).  Every schedule gives the same results. */
 int vg_ntt_plan(int log_n, int *tile_log, float *us);
 
 /* Replaces zk_verifier::verify's prover calls (zk_verifier.cpp:922-1226) and

@@ -186,22 +186,25 @@ def test_rejects_bad_gates_like_the_oracle(ty, u, v, msg):
 
 
 def test_ntt_tiles_give_the_same_bits():
-    """Every NTT tile the autotuner may pick gives the same transcript: the
-    same proofs made in child processes with the tile forced to 2^10 and 2^12
-    (VIRGO_NTT_LM) equal this process's (tuned) ones."""
+    """Every NTT schedule the autotuner may pick gives the same transcript:
+    the same proofs made in child processes with the tile forced to 2^10 and
+    2^12, or to the one-block transform (VIRGO_NTT_LM=1; used for the 2^13
+    and 2^14 extensions of 2^19 and 2^20 inputs), equal this process's
+    (tuned) ones."""
     import hashlib
     import os
     import subprocess
     import sys
-    circs = [("random_w17_d5", W.random_circuit(17, 5, seed=5)), ("w18_d4", W.random_circuit(18, 4, seed=9))]
-    want = [hashlib.sha256(V.prove(c)).hexdigest() for _, c in circs]
+    specs = [(17, 5, 5), (18, 4, 9), (19, 2, 3), (20, 2, 4)]
+    want = [hashlib.sha256(V.prove(W.random_circuit(*sp[:2], seed=sp[2]))).hexdigest() for sp in specs]
     tile, _ = V.ntt_plan(12)
     assert tile in (10, 11, 12)
+    tile14, us14 = V.ntt_plan(14)
+    assert tile14 in (1, 10, 11, 12) and 1 in us14, us14
     code = ("import hashlib, sys; sys.path.insert(0, %r); import virgo_amd as V; from virgo_amd import workloads as W; "
-            "print(hashlib.sha256(V.prove(W.random_circuit(17, 5, seed=5))).hexdigest()); "
-            "print(hashlib.sha256(V.prove(W.random_circuit(18, 4, seed=9))).hexdigest())"
-            % os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-    for lm in ("10", "12"):
+            "[print(hashlib.sha256(V.prove(W.random_circuit(w, d, seed=s))).hexdigest()) for (w, d, s) in %r]"
+            % (os.path.dirname(os.path.dirname(os.path.abspath(__file__))), specs))
+    for lm in ("10", "12", "1"):
         print("tile 2^" + lm, flush=True)
         env = dict(os.environ, VIRGO_NTT_LM=lm)
         out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=240)

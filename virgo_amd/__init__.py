@@ -332,11 +332,13 @@ def sha3_64(blocks):
 
 def ntt_plan(log_n):
     """(tile_log, {tile_log: us}) of the NTT autotuner for 2^log_n
-    transforms (vg_ntt_plan)."""
+    transforms (vg_ntt_plan); tile_log 1 is the one-block transform of
+    2^13 and 2^14 values."""
     t = ctypes.c_int()
-    us = np.zeros(3, dtype=np.float32)
+    us = np.zeros(4, dtype=np.float32)
     _check(lib().vg_ntt_plan(log_n, ctypes.byref(t), _p(us)))
-    return t.value, {10 + k: round(float(us[k]), 1) for k in range(3) if us[k] > 0}
+    names = [10, 11, 12, 1]                    # tiles 2^10..2^12; 1: the one-block transform
+    return t.value, {names[k]: round(float(us[k]), 1) for k in range(4) if us[k] > 0}
 
 
 def fri_fold(cw, r):

@@ -870,37 +870,101 @@ __device__ __forceinline__ void radix_premul(fe *v, int s0, int64_t lo, const la
 // One block: positions [q 2^log_m, (q+1) 2^log_m) of batch row b, read in
 // bit-reversed order, stages [0, log_m) as radix-8/4/2 passes (the first
 // reading global memory, the last writing it, the rest in LDS).
-constexpr int NTT_PAD = 4;          // one pad element per 16 breaks power-of-two LDS strides
+constexpr int NTT_PAD = 3;          // one pad element per 8 breaks power-of-two LDS strides
 __device__ __forceinline__ int lpad(int i) { return i + (i >> NTT_PAD); }
+// group g (2^G values) of a radix-2^G pass over stages [s0, s0+G) of the
+// tile of positions [q 2^log_m, (q+1) 2^log_m) held in sh
+template <int G>
+__device__ __forceinline__ void ntt_group(fe *sh, int g, int s0, int first, int last, const ntt_in &in, int64_t b,
+                                          int64_t q, int log_m, int log_n, fe *out, int64_t out_stride, fe scale,
+                                          int do_scale, const launch::ntt_tabs &T, int inverse)
+{
+    int lo = g & ((1 << s0) - 1), hi = g >> s0;
+    int base = (hi << (s0 + G)) | lo;
+    fe v[1 << G];
+#pragma unroll
+    for (int m = 0; m < (1 << G); ++m) {
+        int i = base + (m << s0);
+        if (first) {
+            int64_t pos = (q << log_m) + i;
+            int64_t j = log_n ? (int64_t)(__brevll((unsigned long long)pos) >> (64 - log_n)) : 0;
+            v[m] = ntt_fetch(in, b, j, in.brev_src ? pos : j, T);
+        } else {
+            v[m] = sh[lpad(i)];
+        }
+    }
+    radix_premul<G>(v, s0, lo, T, inverse);
+    radix_core<G>(v, inverse);
+#pragma unroll
+    for (int m = 0; m < (1 << G); ++m) {
+        int i = base + (m << s0);
+        if (last) out[b * out_stride + (q << log_m) + i] = do_scale ? mul(v[m], scale) : v[m];
+        else sh[lpad(i)] = v[m];
+    }
+}
 template <int G>
 __device__ __forceinline__ void ntt_pass(fe *sh, int M, int s0, int first, int last, const ntt_in &in, int64_t b, int64_t q,
                                          int log_m, int log_n, fe *out, int64_t out_stride, fe scale, int do_scale,
                                          const launch::ntt_tabs &T, int inverse)
 {
     int ng = M >> G;
-    for (int g = threadIdx.x; g < ng; g += blockDim.x) {
-        int lo = g & ((1 << s0) - 1), hi = g >> s0;
-        int base = (hi << (s0 + G)) | lo;
-        fe v[1 << G];
-#pragma unroll
-        for (int m = 0; m < (1 << G); ++m) {
-            int i = base + (m << s0);
-            if (first) {
-                int64_t pos = (q << log_m) + i;
-                int64_t j = log_n ? (int64_t)(__brevll((unsigned long long)pos) >> (64 - log_n)) : 0;
-                v[m] = ntt_fetch(in, b, j, in.brev_src ? pos : j, T);
-            } else {
-                v[m] = sh[lpad(i)];
-            }
+    for (int g = threadIdx.x; g < ng; g += blockDim.x)
+        ntt_group<G>(sh, g, s0, first, last, in, b, q, log_m, log_n, out, out_stride, scale, do_scale, T, inverse);
+}
+// A whole transform of 2^(LM+3) values per block of NTT_FUSED_TPB threads
+// (row b = blockIdx.x), with one global write instead of an LDS pass and a
+// register pass: the 8 tiles of 2^LM bit-reversed positions go through
+// stages [0, LM) in LDS four at a time; each thread keeps the values of its
+// positions i < 2^LM in all 8 tiles in registers and finishes stages
+// [LM, LM+3) as one radix-8 step across the tiles (the k_ntt_regs<3> step
+// with s0 = LM, lo = i).
+constexpr int NTT_FUSED_TPB = 1024;
+// tiles 4 ROUND .. 4 ROUND + 3 of k_ntt_fused through stages [0, LM) in LDS,
+// then into v[p][4 ROUND + k] (constant indices, so v stays in registers)
+template <int LM, int ROUND, int PP>
+__device__ __forceinline__ void ntt_fused_round(fe (&v)[PP][8], fe *sh, int TL, const ntt_in &in, int64_t b, fe *out,
+                                                int64_t out_stride, fe scale, const launch::ntt_tabs &T, int inverse)
+{
+    constexpr int M = 1 << LM;
+    for (int s0 = 0; s0 < LM;) {
+        int G = LM - s0 >= 3 ? 3 : LM - s0;
+        int ng = M >> G;
+        for (int gg = threadIdx.x; gg < 4 * ng; gg += NTT_FUSED_TPB) {
+            int k = gg / ng, g = gg - k * ng;
+            fe *tile = sh + k * TL;
+            int64_t q = 4 * ROUND + k;
+            if (G == 3) ntt_group<3>(tile, g, s0, s0 == 0, 0, in, b, q, LM, LM + 3, out, out_stride, scale, 0, T, inverse);
+            else if (G == 2) ntt_group<2>(tile, g, s0, s0 == 0, 0, in, b, q, LM, LM + 3, out, out_stride, scale, 0, T, inverse);
+            else ntt_group<1>(tile, g, s0, s0 == 0, 0, in, b, q, LM, LM + 3, out, out_stride, scale, 0, T, inverse);
         }
-        radix_premul<G>(v, s0, lo, T, inverse);
-        radix_core<G>(v, inverse);
+        s0 += G;
+        __syncthreads();
+    }
 #pragma unroll
-        for (int m = 0; m < (1 << G); ++m) {
-            int i = base + (m << s0);
-            if (last) out[b * out_stride + (q << log_m) + i] = do_scale ? mul(v[m], scale) : v[m];
-            else sh[lpad(i)] = v[m];
-        }
+    for (int p = 0; p < PP; ++p)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[p][4 * ROUND + k] = sh[k * TL + lpad(threadIdx.x + p * NTT_FUSED_TPB)];
+    __syncthreads();
+}
+template <int LM>
+__global__ __launch_bounds__(NTT_FUSED_TPB) void k_ntt_fused(ntt_in in, fe *out, int64_t out_stride, int inverse, fe scale,
+                                                            int do_scale, launch::ntt_tabs T)
+{
+    constexpr int M = 1 << LM, TL = M + (M >> NTT_PAD), PP = M / NTT_FUSED_TPB;
+    static_assert(PP >= 1, "tile smaller than the block");
+    __shared__ fe sh[4 * TL];
+    const int64_t b = blockIdx.x;
+    fe v[PP][8];
+    ntt_fused_round<LM, 0, PP>(v, sh, TL, in, b, out, out_stride, scale, T, inverse);
+    ntt_fused_round<LM, 1, PP>(v, sh, TL, in, b, out, out_stride, scale, T, inverse);
+#pragma unroll
+    for (int p = 0; p < PP; ++p) {
+        int i = threadIdx.x + p * NTT_FUSED_TPB;
+        radix_premul<3>(v[p], LM, i, T, inverse);
+        radix_core<3>(v[p], inverse);
+        fe *o = out + b * out_stride + i;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) o[(int64_t)m << LM] = do_scale ? mul(v[p][m], scale) : v[p][m];
     }
 }
 // LM: log2 of the LDS tile (the block's transform size when log_n >= LM)
@@ -1400,6 +1464,10 @@ int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
 // each transform size, timed once per process on the first context that
 // needs that size; any tile gives the same bits (exact arithmetic) ----
 constexpr int NTT_LM_MIN = 10, NTT_LM_MAX = 12;
+// the one-block transform k_ntt_fused (sizes 2^13 and 2^14), a candidate
+// beside the tiles
+constexpr int NTT_FUSED = 1;
+bool fused_fits(int log_n) { return log_n == 13 || log_n == 14; }
 std::mutex g_tune_mu;
 int g_ntt_lm[64];                     // chosen tile per log_n (0: not tuned)
 float g_ntt_us[64][NTT_LM_MAX + 1];   // measured us per candidate (0: not run)
@@ -1408,13 +1476,13 @@ int forced_lm()
     static int v = [] {
         const char *e = getenv("VIRGO_NTT_LM");
         int x = e ? atoi(e) : 0;
-        return x >= NTT_LM_MIN && x <= NTT_LM_MAX ? x : 0;
+        return (x >= NTT_LM_MIN && x <= NTT_LM_MAX) || x == NTT_FUSED ? x : 0;
     }();
     return v;
 }
 int ntt_lm_for(int log_n)
 {
-    if (int f = forced_lm()) return f;
+    if (int f = forced_lm()) return f == NTT_FUSED && !fused_fits(log_n) ? NTT_LM : f;
     int t = (log_n >= 0 && log_n < 64) ? __atomic_load_n(&g_ntt_lm[log_n], __ATOMIC_ACQUIRE) : 0;
     return t ? t : NTT_LM;
 }
@@ -1422,6 +1490,19 @@ int ntt_lm_for(int log_n)
 void ntt_run_lm(const ntt_in &in, fe *out, int64_t out_stride, int64_t nb, int log_n, int inverse, fe scale, int do_scale,
                 const launch::ntt_tabs &T, hipStream_t st, int lm)
 {
+    if (lm == NTT_FUSED) {
+        if (log_n == 14) {
+            hipLaunchKernelGGL(k_ntt_fused<11>, dim3((unsigned)nb), dim3(NTT_FUSED_TPB), 0, st, in, out, out_stride, inverse,
+                               scale, do_scale, T);
+            return;
+        }
+        if (log_n == 13) {
+            hipLaunchKernelGGL(k_ntt_fused<10>, dim3((unsigned)nb), dim3(NTT_FUSED_TPB), 0, st, in, out, out_stride, inverse,
+                               scale, do_scale, T);
+            return;
+        }
+        lm = NTT_LM;
+    }
     int log_m = log_n < lm ? log_n : lm;
     int rest = log_n - log_m;
     dim3 grid((unsigned)((int64_t)1 << (log_n - log_m)), (unsigned)nb);
@@ -1867,7 +1948,8 @@ int ntt_tune(int log_n, int64_t nb, const fe *in, fe *out, const ntt_tabs &T, hi
     int best = NTT_LM;
     float best_us = 0;
     bool ok = true;
-    for (int lm = NTT_LM_MIN; lm <= NTT_LM_MAX && lm <= log_n && ok; ++lm) {
+    for (int lm = fused_fits(log_n) ? NTT_FUSED : NTT_LM_MIN; lm <= NTT_LM_MAX && lm <= log_n && ok;
+         lm = lm == NTT_FUSED ? NTT_LM_MIN : lm + 1) {
         float us = 0;
         for (int rep = 0; rep < 4 && ok; ++rep) {
             ok = hipEventRecord(e0, s) == hipSuccess;
@@ -1891,6 +1973,7 @@ int ntt_plan(int log_n, float *us)
     std::lock_guard<std::mutex> lk(g_tune_mu);
     if (log_n < 0 || log_n >= 64) return 0;
     for (int lm = NTT_LM_MIN; lm <= NTT_LM_MAX; ++lm) us[lm - NTT_LM_MIN] = g_ntt_us[log_n][lm];
+    us[NTT_LM_MAX - NTT_LM_MIN + 1] = g_ntt_us[log_n][NTT_FUSED];
     return ntt_lm_for(log_n);
 }
 void fri_fold_nat(const fe *in, int64_t n, const fe *r, const fe *iw, fe *out, hipStream_t s)

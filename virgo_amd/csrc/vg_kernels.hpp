@@ -204,12 +204,13 @@ void slice_sums(const fe *lq_coef, int64_t stride, int nslice, int64_t real, int
 void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int inverse, fe scale, int do_scale,
          const ntt_tabs &T, hipStream_t s);
 // NTT tile autotuner: times the LDS tile candidates (2^10..2^12 elements)
-// of a forward transform of nb rows of 2^log_n on stream s (in: any
-// nb * 2^log_n values, out: as many, overwritten) and keeps the fastest for
-// every later transform of that size in the process; once per size (later
-// calls return at once).  ntt_plan: the tile in use for log_n (the default
-// 2^11 until tuned; VIRGO_NTT_LM=k forces one) and the measured us of the
-// three candidates (0: not run).
+// and, for 2^13 and 2^14, the one-block transform (code 1) of a forward
+// transform of nb rows of 2^log_n on stream s (in: any nb * 2^log_n
+// values, out: as many, overwritten) and keeps the fastest for every later
+// transform of that size in the process; once per size (later calls return
+// at once).  ntt_plan: the schedule in use for log_n (the default 2^11
+// until tuned; VIRGO_NTT_LM=k forces one) and the measured us of the four
+// candidates in us[0..3] (tiles 2^10..2^12, one-block; 0: not run).
 int ntt_tune(int log_n, int64_t nb, const fe *in, fe *out, const ntt_tabs &T, hipStream_t s);
 int ntt_plan(int log_n, float *us);
 void sha3_batch(const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s);

@@ -1978,9 +1978,9 @@ int vg_ctx_destroy(vg_ctx *c)
 int vg_ntt_plan(int log_n, int *tile_log, float *us)
 {
     if (!tile_log) FAILV(-1, "vg_ntt_plan: null argument");
-    float t[3] = {0, 0, 0};
+    float t[4] = {0, 0, 0, 0};
     *tile_log = launch::ntt_plan(log_n, t);
-    if (us) for (int k = 0; k < 3; ++k) us[k] = t[k];
+    if (us) for (int k = 0; k < 4; ++k) us[k] = t[k];
     return 0;
 }
 

@@ -113,20 +113,22 @@ def _py_root(leaves, h):
     return level[0]
 
 
-@pytest.mark.parametrize("log_n,hash", [(18, "sha3"), (18, "sha256"), (20, "sha256"), (24, "sha3")])
-def test_microset_stages(log_n, hash):
+@pytest.mark.parametrize("log_n,hash,rate_log", [(18, "sha3", RATE_LOG), (18, "sha256", RATE_LOG), (20, "sha256", RATE_LOG),
+                                                  (24, "sha3", RATE_LOG), (16, "sha3", 5)])
+def test_microset_stages(log_n, hash, rate_log):
     """vg_microset (one call, device-resident, timed per stage) against an
     independent route: every FRI level's codeword is the transform of the
     coefficients folded in the coefficient domain (c'_k = c_2k + r c_2k+1),
     made by the single-operation FFT; each level's root is the
-    single-operation tree of its pairs (and, at 2^18, a hashlib tree)."""
-    print("microset", log_n, hash, flush=True)
-    N, L = 1 << log_n, log_n + RATE_LOG
+    single-operation tree of its pairs (and, at 2^18, a hashlib tree).  The
+    last case runs at the reference's rate 1/32."""
+    print("microset", log_n, hash, rate_log, flush=True)
+    N, L = 1 << log_n, log_n + rate_log
     M = 1 << L
     rng = np.random.default_rng(300 + log_n)
     c = rand_fe(rng, N)
     rs = rand_fe(rng, L - 5)
-    roots, final, ok, ms = V.microset(c, RATE_LOG, rs, hash=hash, reps=2)
+    roots, final, ok, ms = V.microset(c, rate_log, rs, hash=hash, reps=2)
     assert ok
     assert all(v > 0 for v in ms.values()), ms
     assert roots.shape == (L - 5, 32)

@@ -18,6 +18,9 @@ CASES = {
     "matmul32": lambda: W.matmul_circuit(32, seed=1),
     "random_w10_d4": lambda: W.random_circuit(10, 4, seed=3),
     "random_w13_d3_skew": lambda: W.random_circuit(13, 3, seed=6, v_bits=6),
+    # the device predicates of every gate type a description may use
+    "mixed_w10_d5": lambda: W.mixed_circuit(10, 5, seed=12),
+    "mixed_w14_d3": lambda: W.mixed_circuit(14, 3, seed=17),
 }
 
 
@@ -82,6 +85,38 @@ def test_verify_tampered_matches_oracle():
         assert p.verify(tr)[0]
     finally:
         p.close()
+
+
+@pytest.mark.gpu
+def test_verify_mixed_rejects_changed_wiring():
+    """A power-sum range one entry shorter, a not gate turned into a relay,
+    or a bit test on another wire: the device verifier rejects the proof of
+    the original circuit, like the oracle's."""
+    circ = W.mixed_circuit(11, 3, seed=18)
+    p = V.Prover(circ)
+    tr = p.prove()
+    p.close()
+    base = int(circ["n"][0])
+    lay = slice(base, base + int(circ["n"][1]))
+    ty, u, v = circ["ty"][lay], circ["u"][lay], circ["v"][lay]
+    g12 = base + int(np.flatnonzero((ty == 12) & (v > u))[0])
+    g6 = base + int(np.flatnonzero(ty == 6)[0])
+    g13 = base + int(np.flatnonzero(ty == 13)[0])
+    for what, g, field, val in (("power sum", g12, "v", circ["v"][g12] - 1), ("not", g6, "ty", 10),
+                                ("bit test", g13, "uv", (circ["u"][g13] + 1) % int(circ["n"][0]))):
+        c2 = {k: a.copy() for k, a in circ.items()}
+        if field == "uv":
+            c2["u"][g] = c2["v"][g] = val
+        else:
+            c2[field][g] = val
+        print("changed", what, flush=True)
+        q = V.Prover(c2)
+        try:
+            ok, why, _ = q.verify(tr)
+        finally:
+            q.close()
+        assert not ok, what
+        assert not O.verify(c2, tr)[0], what
 
 
 @pytest.mark.gpu

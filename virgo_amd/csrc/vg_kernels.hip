@@ -782,6 +782,7 @@ struct ntt_in {
     const fe *l, *q; int64_t cw_S, cw_stride, step; int log_R;
     const int *lens; int log_Rl; int hmode;
     int brev_src;             // kinds 0 and 2 (hmode 0): src already in bit-reversed order
+    const fe *qscale;         // kind 1: q of row b = qscale[b] times the one row at q
 };
 // input value j of row b; jm: where src holds it (j, or brev(j) for a
 // bit-reversed source)
@@ -790,7 +791,8 @@ __device__ __forceinline__ fe ntt_fetch(const ntt_in &in, int64_t b, int64_t j, 
     if (in.kind == 0) return in.src[b * in.stride + jm];
     if (in.kind == 1) {
         int64_t k = cw_idx(j * in.step, in.cw_S, in.log_R);
-        return mul(in.l[b * in.cw_stride + k], in.q[b * in.cw_stride + k]);
+        fe qv = in.qscale ? mul(in.qscale[b], in.q[k]) : in.q[b * in.cw_stride + k];
+        return mul(in.l[b * in.cw_stride + k], qv);
     }
     int64_t slice = b >> in.log_Rl, s = b & ((1 << in.log_Rl) - 1);
     int len = in.lens[slice];
@@ -1143,20 +1145,25 @@ __global__ void k_slice_sums(const fe *lq, int64_t stride, int nslice, int64_t r
     all_sum[i] = mul_base(c, (uint64_t)len);
 }
 // virtual oracle at natural position p of slice `slice` (poly_commit.h:246-263, 305-322):
-// ((l q - (x^e - 1) h - c0) / x) e at x = w_SS^p, e = slice length
-__device__ __forceinline__ fe vo_at(const fe *l, const fe *q, const fe *h, const fe *c0, int slice, int nslice, int64_t p,
-                                    int64_t SS, int log_R, int64_t real, int64_t m, const fe *tw)
+// ((l q - (x^e - 1) h - c0) / x) e at x = w_SS^p, e = slice length; q of
+// slice j < nslice - 1 is cq[j] times Q (q's first SS values), the last
+// slice's q its own (q's second SS values)
+__device__ __forceinline__ fe vo_at(const fe *l, const fe *q, const fe *cq, const fe *h, const fe *c0, int slice, int nslice,
+                                    int64_t p, int64_t SS, int log_R, int64_t real, int64_t m, const fe *tw)
 {
-    int64_t id = (int64_t)slice * SS + cw_idx(p, SS, log_R);
-    int64_t e = (slice == nslice - 1) ? m : real;
+    int64_t k = cw_idx(p, SS, log_R), id = (int64_t)slice * SS + k;
+    bool last = slice == nslice - 1;
+    int64_t e = last ? m : real;
+    fe qv = last ? q[SS + k] : mul(cq[slice], q[k]);
     fe xe = tw[(p * e) & (SS - 1)], xinv = tw[(SS - p) & (SS - 1)];
-    fe g = sub(mul(l[id], q[id]), mul(sub(xe, fone()), h[id]));
+    fe g = sub(mul(l[id], qv), mul(sub(xe, fone()), h[id]));
     return mul_base(mul(sub(g, c0[slice]), xinv), (uint64_t)e);
 }
 // the virtual oracle fused into the first FRI fold: next (SS/2 per slice,
 // codeword layout) from vo at p and p + SS/2, which is never stored
-__global__ __launch_bounds__(TPB) void k_vo_fold(const fe *l, const fe *q, const fe *h, const fe *c0, int nslice, int64_t SS,
-                                                int log_R, int64_t real, int64_t m, const fe *tw, const fe *rp, fe *next)
+__global__ __launch_bounds__(TPB) void k_vo_fold(const fe *l, const fe *q, const fe *cq, const fe *h, const fe *c0, int nslice,
+                                                int64_t SS, int log_R, int64_t real, int64_t m, const fe *tw, const fe *rp,
+                                                fe *next)
 {
     int64_t Sn = SS / 2;
     int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1166,8 +1173,8 @@ __global__ __launch_bounds__(TPB) void k_vo_fold(const fe *l, const fe *q, const
     int64_t cols = Sn >> log_R;
     int64_t srow = st / cols, t = st - srow * cols;
     int64_t p = (t << log_R) + srow;
-    fe a = vo_at(l, q, h, c0, j, nslice, p, SS, log_R, real, m, tw);
-    fe b = vo_at(l, q, h, c0, j, nslice, p + Sn, SS, log_R, real, m, tw);
+    fe a = vo_at(l, q, cq, h, c0, j, nslice, p, SS, log_R, real, m, tw);
+    fe b = vo_at(l, q, cq, h, c0, j, nslice, p + Sn, SS, log_R, real, m, tw);
     fe imr = mul(tw[(SS - p) & (SS - 1)], *rp);
     const fe inv2 = mk(1ull << 60, 0);
     next[tid] = mul(inv2, add(add(a, b), mul(imr, sub(a, b))));
@@ -2019,7 +2026,7 @@ void intt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, fe inv_n,
     ntt_in ni;
     ni.kind = in.kind; ni.src = in.src; ni.stride = in.src_stride;
     ni.l = in.l; ni.q = in.q; ni.cw_S = in.cw_S; ni.cw_stride = in.cw_stride; ni.step = in.step; ni.log_R = in.log_R;
-    ni.lens = nullptr; ni.log_Rl = 0; ni.hmode = 0; ni.brev_src = in.brev_src;
+    ni.lens = nullptr; ni.log_Rl = 0; ni.hmode = 0; ni.brev_src = in.brev_src; ni.qscale = in.qscale;
     ntt_run(ni, out, out_stride, nb, log_n, 1, inv_n, 1, T, s);
 }
 void lde(const fe *coef, int64_t coef_stride, const int *lens, int nb, fe *out, int log_C, int log_R, const ntt_tabs &T,
@@ -2042,7 +2049,7 @@ void ntt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, int invers
     memset(&ni, 0, sizeof ni);
     ni.kind = in.kind; ni.src = in.src; ni.stride = in.src_stride;
     ni.l = in.l; ni.q = in.q; ni.cw_S = in.cw_S; ni.cw_stride = in.cw_stride; ni.step = in.step; ni.log_R = in.log_R;
-    ni.brev_src = in.brev_src;
+    ni.brev_src = in.brev_src; ni.qscale = in.qscale;
     ntt_run(ni, out, out_stride, nb, log_n, inverse, scale, do_scale, T, s);
 }
 void sha3_batch(const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s)
@@ -2070,11 +2077,11 @@ void fri_fold(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, i
 {
     hipLaunchKernelGGL(k_fri_fold, dim3(nblk(65 * (S / 2))), dim3(TPB), 0, s, prev, S, log_R, r, tw, SS, next);
 }
-void fri_folds_vo(const fe *l, const fe *q, const fe *h, const fe *c0, int64_t real, int64_t m, int64_t SS, int nlev,
-                  int log_R, const fe *r, const fe *tw, fe *const *next, hipStream_t s)
+void fri_folds_vo(const fe *l, const fe *q, const fe *cq, const fe *h, const fe *c0, int64_t real, int64_t m, int64_t SS,
+                  int nlev, int log_R, const fe *r, const fe *tw, fe *const *next, hipStream_t s)
 {
     // level 0 straight from l, q, h; then as fri_folds
-    hipLaunchKernelGGL(k_vo_fold, dim3(nblk(65 * (SS / 2))), dim3(TPB), 0, s, l, q, h, c0, 65, SS, log_R, real, m, tw, r,
+    hipLaunchKernelGGL(k_vo_fold, dim3(nblk(65 * (SS / 2))), dim3(TPB), 0, s, l, q, cq, h, c0, 65, SS, log_R, real, m, tw, r,
                        next[0]);
     if (nlev > 1) fri_folds(next[0], SS / 2, SS, nlev - 1, log_R, r + 1, tw, next + 1, s);
 }

@@ -186,6 +186,7 @@ struct intt_src {
     const fe *src; int64_t src_stride;
     const fe *l, *q; int64_t cw_S, cw_stride; int64_t step; int log_R;
     int brev_src = 0;      // kind 0: src holds the row in bit-reversed order (brev_permute)
+    const fe *qscale = nullptr;   // kind 1: q of row b is qscale[b] times the one row at q
 };
 void intt(intt_src in, fe *out, int64_t out_stride, int nb, int log_n, fe inv_n, const ntt_tabs &T, hipStream_t s);
 // Coset LDE: for each slice b and row s < R, out[b][s][t] = sum_{j<len_b} c_b[j] w_SS^{(tR+s) j}.
@@ -228,9 +229,11 @@ void fri_fold(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, i
 void fri_folds(const fe *cw, int64_t S0, int64_t SS, int nlev, int log_R, const fe *r, const fe *tw, fe *const *next,
                hipStream_t s);
 // the same from the virtual oracle of l, q, h (poly_commit.h:246-263,
-// 305-322), computed inside the first fold and never stored
-void fri_folds_vo(const fe *l, const fe *q, const fe *h, const fe *c0, int64_t real, int64_t m, int64_t SS, int nlev,
-                  int log_R, const fe *r, const fe *tw, fe *const *next, hipStream_t s);
+// 305-322), computed inside the first fold and never stored; q holds two
+// extensions (codeword layout, SS each): Q's, which slice j < 64 scales by
+// cq[j], and the last slice's own
+void fri_folds_vo(const fe *l, const fe *q, const fe *cq, const fe *h, const fe *c0, int64_t real, int64_t m, int64_t SS,
+                  int nlev, int log_R, const fe *r, const fe *tw, fe *const *next, hipStream_t s);
 // leaf chains of nseg codewords in one launch (codeword k: S[k] positions per
 // slice, leaves S[k]/2 written to tree[k] + S[k]/2 nodes)
 void leaf_hash_multi(int nseg, const fe *const *cw, const int64_t *S, uint8_t *const *tree, int log_R, hipStream_t s);

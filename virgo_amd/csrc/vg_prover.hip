@@ -530,7 +530,7 @@ struct vg_ctx {
     fe *ntW = nullptr, *ntWi = nullptr, *ntTA = nullptr, *ntTB = nullptr;   // NTT stage and coset tables
     launch::ntt_tabs NT = {};
     fe *tw = nullptr, *T12 = nullptr, *coefP = nullptr, *coefQ = nullptr, *l = nullptr, *q = nullptr, *h = nullptr,
-       *qv = nullptr, *eqs = nullptr, *lqc = nullptr, *c0 = nullptr, *all_sum = nullptr, *pri = nullptr,
+       *qv = nullptr, *cq = nullptr, *eqs = nullptr, *lqc = nullptr, *c0 = nullptr, *all_sum = nullptr, *pri = nullptr,
        *pubm = nullptr, *cw = nullptr, *gout = nullptr;
     int *lens = nullptr;
     uint8_t *treeL = nullptr, *treeH = nullptr, *ctree = nullptr;
@@ -650,11 +650,15 @@ int64_t layout(vg_ctx *c, uint8_t *base)
         c->NT = {c->ntW, c->ntWi, c->ntTA, c->ntTB, th, tb};
     }
     c->coefP = carve<fe>(base, off, 65 * c->C_);
-    c->coefQ = carve<fe>(base, off, 65 * c->C_);
+    // the public array's slices 0..63 are cq[j] Q (eq factors over the top 6
+    // and the low n-6 input bits), so q holds two extensions: Q's and the
+    // public mask's
+    c->coefQ = carve<fe>(base, off, 2 * c->C_);
     c->l = carve<fe>(base, off, 65 * SS);
-    c->q = carve<fe>(base, off, 65 * SS);
+    c->q = carve<fe>(base, off, 2 * SS);
     c->h = carve<fe>(base, off, 65 * SS);
-    c->qv = carve<fe>(base, off, (int64_t)1 << c->n);
+    c->qv = carve<fe>(base, off, c->real);
+    c->cq = carve<fe>(base, off, 64);
     c->eqs = carve<fe>(base, off, 2 * ((int64_t)1 << (c->n / 2 + 1)));
     c->lqc = carve<fe>(base, off, 65 * c->LQ);
     c->c0 = carve<fe>(base, off, 65);
@@ -901,15 +905,24 @@ int enqueue_commit(vg_ctx *c)
     launch::lde(c->coefP, Cc, c->lens, 65, c->l, c->log_C, c->log_R, c->NT, st);
     HIPCHK(hipEventRecord(c->ev[3], st));
     // ---- commit_public_array (poly_commit.h:125-351) ----
+    // Slice j < 64 of the public array is cq[j] Q exactly (a product of the
+    // same eq factors), so its interpolation and extension are cq[j] times
+    // Q's: one extension of Q and one of the public mask replace the 65 of
+    // poly_commit.h:136-170; every consumer scales Q's values by cq[j].
     {
         fe *q = c->ht + (int64_t)D * 8 * c->ht_stride;
-        launch::eq_expand(c->qv, q, q + ((int64_t)1 << (n >> 1)), n >> 1, (int64_t)1 << n, st);
+        int fq = (n - 6) >> 1;
+        launch::eq_expand(c->qv, q, q + ((int64_t)1 << fq), fq, real, st);
     }
-    launch::intt({0, c->qv, real, nullptr, nullptr, 0, 0, 0, 0}, c->coefQ, Cc, 64, lr, inv_real, c->NT, st);
-    launch::intt({0, pub_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefQ + 64 * Cc, Cc, 1, lm, inv_m, c->NT, st);
-    launch::lde(c->coefQ, Cc, c->lens, 65, c->q, c->log_C, c->log_R, c->NT, st);
-    launch::intt({1, nullptr, 0, c->l, c->q, SS, SS, SS / (2 * real), c->log_R}, c->lqc, c->LQ, 64, lr + 1, inv_2real, c->NT, st);
-    launch::intt({1, nullptr, 0, c->l + 64 * SS, c->q + 64 * SS, SS, SS, SS / (2 * m), c->log_R}, c->lqc + 64 * c->LQ, c->LQ, 1,
+    launch::intt({0, c->qv, real, nullptr, nullptr, 0, 0, 0, 0}, c->coefQ, Cc, 1, lr, inv_real, c->NT, st);
+    launch::intt({0, pub_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefQ + Cc, Cc, 1, lm, inv_m, c->NT, st);
+    launch::lde(c->coefQ, Cc, c->lens + 63, 2, c->q, c->log_C, c->log_R, c->NT, st);     // lens[63] = real, lens[64] = m
+    {
+        launch::intt_src lq = {1, nullptr, 0, c->l, c->q, SS, SS, SS / (2 * real), c->log_R};
+        lq.qscale = c->cq;
+        launch::intt(lq, c->lqc, c->LQ, 64, lr + 1, inv_2real, c->NT, st);
+    }
+    launch::intt({1, nullptr, 0, c->l + 64 * SS, c->q + SS, SS, SS, SS / (2 * m), c->log_R}, c->lqc + 64 * c->LQ, c->LQ, 1,
                  lm + 1, inv_2m, c->NT, st);
     launch::slice_sums(c->lqc, c->LQ, 65, real, m, c->c0, c->msg + c->msg_allsum, st);
     launch::lde(c->lqc, c->LQ, c->lens, 65, c->h, c->log_C, c->log_R, c->NT, st, 1);
@@ -919,7 +932,8 @@ int enqueue_commit(vg_ctx *c)
     {
         fe *next[24];
         for (int k = 0; k < n - 6; ++k) next[k] = c->cw + c->cw_off[k];
-        launch::fri_folds_vo(c->l, c->q, c->h, c->c0, real, m, SS, n - 6, c->log_R, c->chal + c->chal_fri, c->tw, next, st);
+        launch::fri_folds_vo(c->l, c->q, c->cq, c->h, c->c0, real, m, SS, n - 6, c->log_R, c->chal + c->chal_fri, c->tw, next,
+                             st);
     }
     HIPCHK(hipEventRecord(c->ev[5], st));
     return 0;
@@ -1332,10 +1346,15 @@ int ctx_alloc(vg_ctx *c)
             r0 = ru; r1 = rv;
             al = c->chal + c->chal_ab[i]; be = al + 1;
         }
+        // the public array eq(rl, x) over the n input bits (zk_verifier.cpp:838-847)
+        // is cq[x >> (n-6)] Q[x mod 2^(n-6)]: half tables of Q over the low
+        // n-6 bits, and the 64 slice factors over the top 6
         fe *q = c->ht + (int64_t)C.depth * 8 * c->ht_stride;
         const fe *rl = c->chal + c->chal_ru[1];
-        addd(q, nullptr, rl, n >> 1);
-        addd(q + ((int64_t)1 << (n >> 1)), nullptr, rl + (n >> 1), n - (n >> 1));
+        int nq = n - 6;
+        addd(q, nullptr, rl, nq >> 1);
+        addd(q + ((int64_t)1 << (nq >> 1)), nullptr, rl + (nq >> 1), nq - (nq >> 1));
+        addd(c->cq, nullptr, rl + nq, 6);
         c->n_hdesc = (int)hd.size();
         c->ht_total = tot;
         HIPCHK(hipMemcpy(c->hdesc, hd.data(), sizeof(launch::half_desc) * hd.size(), hipMemcpyHostToDevice));
@@ -1401,9 +1420,10 @@ int ctx_alloc(vg_ctx *c)
     }
     // the LDS tile of this proof's transform sizes (the extensions: 65 x 2^R
     // rows of 2^log_C; the l q products: 65 rows of 2^(log_C+1)), tuned once
-    // per process on the context's codeword buffers before they hold anything
-    if (launch::ntt_tune(c->log_C, (int64_t)65 << c->log_R, c->l, c->q, c->NT, c->st) ||
-        launch::ntt_tune(c->log_C + 1, 65, c->l, c->q, c->NT, c->st))
+    // per process on the context's codeword buffers l and h before they hold
+    // anything
+    if (launch::ntt_tune(c->log_C, (int64_t)65 << c->log_R, c->l, c->h, c->NT, c->st) ||
+        launch::ntt_tune(c->log_C + 1, 65, c->l, c->h, c->NT, c->st))
         FAILV(-20, "NTT tile tuning failed");
     HIPCHK(hipStreamSynchronize(c->st));
     return 0;

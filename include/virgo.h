@@ -83,8 +83,9 @@ int vg_ntt_plan(int log_n, int *tile_log, float *us);
  * adds its prover time to the proof's).
  * Writes a malloc'd transcript (free with vg_free).  stage_ms (may be NULL)
  * receives 13 timings in ms.  Device: evaluate, gkr (incl. V_res), then the
- * commitment, which runs on a second stream beside gkr: lde_l, lde_qh (q and
- * h extensions), fri_fold (virtual oracle + folds), hash (all leaf chains
+ * commitment, which runs on a second stream beside gkr, or after it when the
+ * proof runs in order: lde_l, lde_qh (the public array's and h's
+ * extensions), fri_fold (virtual oracle + folds), hash (all leaf chains
  * and Merkle trees); then queries (gathers + copies to the host), the SHA3
  * leaf-chain kernel alone (part of hash), the whole device schedule, and
  * fft_gkr (on its own stream).  Host: prepare (challenge stream, mask

@@ -2037,11 +2037,13 @@ int vg_finish(vg_ctx *c, uint8_t **transcript, size_t *len, double *stage_ms)
     *len = T.b.size();
     if (stage_ms) {
         // evaluate, gkr | lde_l, lde_qh, fri_fold, hash (second stream) | queries, leaf kernel, total
+        // (in order on one stream, lde_l starts where gkr ends)
         static const int pairs[9][2] = {{0, 1}, {1, 2}, {1, 3}, {3, 4}, {4, 5}, {5, 6}, {8, 7}, {-1, -1}, {0, 7}};
         for (int k = 0; k < 9; ++k) {
             float ms = 0;
+            int a = k == 2 && c->serial ? 2 : pairs[k][0];
             if (pairs[k][0] < 0) (void)hipEventElapsedTime(&ms, c->ev_leaf[0], c->ev_leaf[1]);
-            else (void)hipEventElapsedTime(&ms, c->ev[pairs[k][0]], c->ev[pairs[k][1]]);
+            else (void)hipEventElapsedTime(&ms, c->ev[a], c->ev[pairs[k][1]]);
             stage_ms[k] = ms;
         }
         stage_ms[9] = fg_ms;

@@ -1144,23 +1144,24 @@ __global__ void k_slice_sums(const fe *lq, int64_t stride, int nslice, int64_t r
     c0[i] = c;
     all_sum[i] = mul_base(c, (uint64_t)len);
 }
-// virtual oracle at natural position p of slice `slice` (poly_commit.h:246-263, 305-322):
-// ((l q - (x^e - 1) h - c0) / x) e at x = w_SS^p, e = slice length; q of
-// slice j < nslice - 1 is cq[j] times Q (q's first SS values), the last
-// slice's q its own (q's second SS values)
-__device__ __forceinline__ fe vo_at(const fe *l, const fe *q, const fe *cq, const fe *h, const fe *c0, int slice, int nslice,
-                                    int64_t p, int64_t SS, int log_R, int64_t real, int64_t m, const fe *tw)
+// The virtual oracle of slice j (poly_commit.h:246-263, 305-322) at x = w_SS^p
+// is vo(p) = (G(p) - c0) x^-1 e with G = l q - (x^e - 1) h, e the slice
+// length; q of slice j < nslice - 1 is cq[j] times Q (q's first SS values),
+// the last slice's q its own (q's second SS values).  It is fused into the
+// first FRI fold (fri.cpp:337-338) and never stored: with a = vo(p),
+// b = vo(p + SS/2) and x(p + SS/2) = -x(p),
+//   next = ((a + b) + r x^-1 (a - b)) / 2
+//        = (e / 2) x^-1 ((G_a - G_b) + r x^-1 (G_a + G_b - 2 c0)),
+// the same field element with two general products fewer.
+__device__ __forceinline__ fe vo_numer(const fe *l, const fe *q, const fe *cq, const fe *h, int slice, bool last,
+                                       int64_t p, int64_t e, int64_t SS, int log_R, const fe *tw)
 {
     int64_t k = cw_idx(p, SS, log_R), id = (int64_t)slice * SS + k;
-    bool last = slice == nslice - 1;
-    int64_t e = last ? m : real;
     fe qv = last ? q[SS + k] : mul(cq[slice], q[k]);
-    fe xe = tw[(p * e) & (SS - 1)], xinv = tw[(SS - p) & (SS - 1)];
-    fe g = sub(mul(l[id], qv), mul(sub(xe, fone()), h[id]));
-    return mul_base(mul(sub(g, c0[slice]), xinv), (uint64_t)e);
+    fe xe = tw[(p * e) & (SS - 1)];
+    return sub(mul(l[id], qv), mul(sub(xe, fone()), h[id]));
 }
-// the virtual oracle fused into the first FRI fold: next (SS/2 per slice,
-// codeword layout) from vo at p and p + SS/2, which is never stored
+// next (SS/2 per slice, codeword layout) from the virtual oracle at p and p + SS/2
 __global__ __launch_bounds__(TPB) void k_vo_fold(const fe *l, const fe *q, const fe *cq, const fe *h, const fe *c0, int nslice,
                                                 int64_t SS, int log_R, int64_t real, int64_t m, const fe *tw, const fe *rp,
                                                 fe *next)
@@ -1173,11 +1174,14 @@ __global__ __launch_bounds__(TPB) void k_vo_fold(const fe *l, const fe *q, const
     int64_t cols = Sn >> log_R;
     int64_t srow = st / cols, t = st - srow * cols;
     int64_t p = (t << log_R) + srow;
-    fe a = vo_at(l, q, cq, h, c0, j, nslice, p, SS, log_R, real, m, tw);
-    fe b = vo_at(l, q, cq, h, c0, j, nslice, p + Sn, SS, log_R, real, m, tw);
-    fe imr = mul(tw[(SS - p) & (SS - 1)], *rp);
-    const fe inv2 = mk(1ull << 60, 0);
-    next[tid] = mul(inv2, add(add(a, b), mul(imr, sub(a, b))));
+    bool last = j == nslice - 1;
+    int64_t e = last ? m : real;
+    fe ga = vo_numer(l, q, cq, h, j, last, p, e, SS, log_R, tw);
+    fe gb = vo_numer(l, q, cq, h, j, last, p + Sn, e, SS, log_R, tw);
+    fe xinv = tw[(SS - p) & (SS - 1)];
+    fe sum = sub(add(ga, gb), add(c0[j], c0[j]));
+    fe v = mul(xinv, add(sub(ga, gb), mul(mul(*rp, xinv), sum)));
+    next[tid] = mul_base(v, mulp((uint64_t)e, 1ull << 60));                   // e / 2 = e 2^60
 }
 // ---------------------------------------------------------------------
 // hashing: leaf chains and Merkle levels

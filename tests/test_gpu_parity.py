@@ -100,20 +100,6 @@ def test_outputs_and_repeat_proofs_identical():
     p.close()
 
 
-def test_full_size_c3_accepted():
-    """2^20 gates x 16 layers: the restated reference verifier accepts the GPU
-    proof and the outputs match the oracle's evaluation (size-independent
-    checks; a full oracle proof of this size takes minutes on one core)."""
-    circ = W.random_circuit(20, 16, seed=1)
-    p = V.Prover(circ)
-    tr = p.prove()
-    print("stage ms", p.stage_ms.tolist(), flush=True)
-    ok, msg = O.verify(circ, tr)
-    assert ok, msg
-    assert np.array_equal(p.output(), O.evaluate(circ))
-    p.close()
-
-
 @pytest.mark.parametrize("lg", [1, 4, 9, 14])
 def test_fft_gkr_bit_exact(lg):
     """fft_gkr section (lg = 9: the 32x32 matmul commitment, 14: the 2^20

@@ -204,6 +204,15 @@ def test_eq_table_bigint():
         assert q[x] == acc
 
 
+def int_truncated(x):
+    """The input value as the reference stores it: read as long long, kept
+    as `int u` (the low 32 bits, two's complement) and widened to unsigned
+    long long for field_element(u) (zk_prover.cpp:57-66)."""
+    low = x & 0xFFFFFFFF
+    signed = low - (1 << 32) if low >= 1 << 31 else low
+    return signed % (1 << 64)
+
+
 def brute_eval(circ):
     """Independent Python evaluation over F_{p^2} of every gate type a
     description may use, by each type's definition (zk_prover.cpp:78-142)."""
@@ -217,7 +226,7 @@ def brute_eval(circ):
         ty = circ["ty"][off:off + cnt]; u = circ["u"][off:off + cnt]; v = circ["v"][off:off + cnt]
         off += cnt
         if li == 0:
-            vals = [const(int(x)) for x in u]
+            vals = [const(int_truncated(int(x))) for x in u]
             continue
         new = []
         for t, a, b in zip(ty.tolist(), u.tolist(), v.tolist()):
@@ -275,6 +284,23 @@ def test_evaluate_mixed_gate_types(log_width, depth, seed):
     c = W.mixed_circuit(log_width, depth, seed=seed)
     assert set(W.MIXED_TYPES) <= set(c["ty"][c["n"][0]:].tolist())
     assert to_int(O.evaluate(c)) == brute_eval(c)
+
+
+def test_evaluate_truncates_wide_inputs():
+    """Inputs in [2^40, 2^63) keep only their low 32 bits as a signed int;
+    about half turn negative and wrap to 2^64 + u before the reduction."""
+    c = W.random_circuit(6, 2, seed=8, wide=True)
+    ins = c["u"][:64].tolist()
+    assert min(ins) >= 1 << 40
+    neg = sum(1 for x in ins if x & 0x80000000)
+    assert 16 < neg < 48
+    assert int_truncated(0x1234_8000_0001) == (1 << 64) - (1 << 31) + 1
+    assert to_int(O.evaluate(c)) == brute_eval(c)
+    relay = W._pack([(np.full(4, 3, np.int32), np.array([5, (1 << 40) + 7, 0xFFFFFFFF, 1 << 31], np.int64),
+                      np.zeros(4, np.int64)),
+                     (np.full(4, 10, np.int32), np.arange(4, dtype=np.int64), np.zeros(4, np.int64))])
+    # 2^64 mod p = 8: int -1 -> 2^64 - 1 = 7 mod p; int -2^31 -> 8 - 2^31 mod p
+    assert [x[0] for x in to_int(O.evaluate(relay))] == [5, 7, 7, (8 - (1 << 31)) % P]
 
 
 def test_evaluate_bit_gates_on_bits():

@@ -61,3 +61,89 @@ def test_roots_match_reference():
         o = np.zeros(1, dtype=O.FE)
         L.ref_root(k, _p(o))
         assert np.array_equal(o, O.root_of_unity(k)), k
+
+
+REF_FG = os.path.join(ROOT, "oracle", "_ref", "libref_fftgkr.so")
+
+
+def _ref_fg():
+    if not os.path.exists(REF_FG):
+        pytest.skip("oracle/_ref/libref_fftgkr.so not built (needs the reference sources)")
+    L = ctypes.CDLL(REF_FG)
+    L.ref_fg_run.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    for name in ("ref_fg_quad", "ref_fg_lin"):
+        getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_long]
+        getattr(L, name).restype = ctypes.c_long
+    L.ref_fg_layer.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_long]
+    L.ref_fg_layer.restype = ctypes.c_long
+    return L
+
+
+def _records(L, name):
+    n = getattr(L, name)(None, 0)
+    out = np.zeros(4 * n, dtype=O.FE)
+    getattr(L, name)(_p(out), n)
+    return out.reshape(n, 4)
+
+
+def _section_fields(sec, lg):
+    """Splits an oracle "VGF1" section into (a_0, round polynomials, layer
+    values v_u / v_v) in the order the section stores them."""
+    assert sec[:4] == b"VGF1" and int.from_bytes(sec[4:8], "little") == lg
+    f = np.frombuffer(sec[8:], dtype=O.FE)
+    a0, k = f[0], 1
+    polys, vals = [], []
+
+    def rounds(m):
+        nonlocal k
+        for _ in range(m):
+            polys.append(f[k:k + 3])
+            k += 3
+
+    def val():
+        nonlocal k
+        vals.append(f[k])
+        k += 1
+    rounds(lg + 6); val()                   # addition layer, v_u
+    rounds(lg); val()                       # mult layer, v_u
+    for _ in range(lg):                     # inverse-DFT layers: v_u, v_v
+        rounds(lg); val(); rounds(lg); val()
+    assert k == len(f)
+    return a0, polys, vals
+
+
+@pytest.mark.parametrize("lg", [1, 4, 9, 14])
+def test_fft_gkr_matches_reference_program(lg):
+    """The oracle's fft_gkr against the reference's own program
+    (fft_circuit_GKR.cpp compiled from source, main body run on a fresh
+    rand() stream): every circuit layer, every round polynomial its prover
+    makes (recorded at each quadratic_poly::eval), every v_u / v_v (each
+    linear_poly::eval), the claim on the outputs, the verdict and the proof
+    size equal the oracle's section."""
+    L = _ref_fg()
+    res, psize = ctypes.c_int(), ctypes.c_int()
+    nl = L.ref_fg_run(lg, ctypes.byref(res), ctypes.byref(psize))
+    sec, ok, osize, _ = O.fft_gkr(lg)
+    assert res.value == 1 and ok
+    assert psize.value == osize
+    for idx in range(nl):
+        n = L.ref_fg_layer(idx, None, 0)
+        ref = np.zeros(n, dtype=O.FE)
+        L.ref_fg_layer(idx, _p(ref), n)
+        assert np.array_equal(ref, O.fft_gkr_layer(lg, idx)), "layer %d" % idx
+    a0, polys, vals = _section_fields(sec, lg)
+    quad = _records(L, "ref_fg_quad")
+    # each round evaluates its polynomial at 0 and 1 (the check) and at the challenge
+    assert len(quad) == 3 * len(polys)
+    for k, p in enumerate(polys):
+        grp = quad[3 * k:3 * k + 3]
+        assert all(np.array_equal(g[:3], grp[0][:3]) for g in grp), "round %d" % k
+        assert sorted(int(g[3]["real"]) for g in grp[:2]) == [0, 1] and not grp[0][3]["img"]
+        assert np.array_equal(grp[0][:3], p), "round polynomial %d" % k
+    # claim on the 64 outputs = p(0) + p(1) of the first round = a + b + 2c
+    a, b, c = (O.fe_add(polys[0][i:i + 1], np.zeros(1, O.FE)) for i in range(3))
+    assert np.array_equal(O.fe_add(O.fe_add(a, b), O.fe_add(c, c)), a0.reshape(1))
+    lin = _records(L, "ref_fg_lin")
+    assert len(lin) == len(vals)
+    for k, v in enumerate(vals):
+        assert lin[k][3] == v, "layer value %d" % k

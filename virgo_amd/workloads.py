@@ -65,15 +65,26 @@ def _mod(r, m):
     return (r % np.uint64(m)).astype(np.int64)
 
 
-def random_circuit(log_width, depth, seed=1, log_inputs=None, v_bits=None):
+def wide_inputs(seed, n):
+    """n input values uniform in [2^40, 2^63): the reference reads them as
+    long long and keeps only the low 32 bits through int u
+    (zk_prover.cpp:57-66), so about half become negative ints."""
+    return ((splitmix64(seed, n, 1) >> np.uint64(1)) | np.uint64(1 << 40)).astype(np.int64)
+
+
+def random_circuit(log_width, depth, seed=1, log_inputs=None, v_bits=None, wide=False):
     """Random layered circuit: `depth` compute layers of 2^log_width gates,
     each gate add or mult with probability 1/2, u and v uniform over the
-    previous layer; 2^log_inputs inputs uniform in [0, 2^31).  v_bits: v
-    uniform over the first 2^v_bits entries only (skewed fan-out)."""
+    previous layer; 2^log_inputs inputs uniform in [0, 2^31) (wide: in
+    [2^40, 2^63), see wide_inputs).  v_bits: v uniform over the first
+    2^v_bits entries only (skewed fan-out)."""
     if log_inputs is None:
         log_inputs = log_width
     W, NI = 1 << log_width, 1 << log_inputs
-    vals = (splitmix64(seed, NI, 1) >> np.uint64(33)).astype(np.int64)
+    if wide:
+        vals = wide_inputs(seed, NI)
+    else:
+        vals = (splitmix64(seed, NI, 1) >> np.uint64(33)).astype(np.int64)
     layers = [(np.full(NI, 3, np.int32), vals, np.zeros(NI, np.int64))]
     prev = NI
     for layer in range(depth):

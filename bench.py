@@ -37,7 +37,9 @@ def parse():
     ap.add_argument("--log-width", type=int, default=20)
     ap.add_argument("--depth", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-log-width", type=int, default=18)
+    ap.add_argument("--cpu-sample-log-width", type=int, default=0,
+                    help="oracle baseline at this width, scaled to --log-width (default: the full width)")
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--batch-workers", type=int, default=16)
     ap.add_argument("--no-batch", action="store_true")
@@ -180,23 +182,66 @@ def microset(args):
     return out
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def keccak_rates():
+    """Keccak-f[1600] permutations per second on this core: the oracle's
+    SHA3-256 of 64-byte blocks (one permutation each) against OpenSSL's
+    (hashlib.sha3_256 over one long message: one permutation per 136-byte
+    block, so no per-call Python overhead)."""
+    import hashlib
+    import numpy as np
+    from oracle import oracle as O
+    n = 1 << 18
+    blocks = np.random.default_rng(3).integers(0, 256, (n, 64), dtype=np.uint8)
+    O.sha3_64(blocks[:1024])
+    t0 = time.perf_counter()
+    O.sha3_64(blocks)
+    t_o = time.perf_counter() - t0
+    msg = bytes(136 * n)
+    t0 = time.perf_counter()
+    hashlib.sha3_256(msg).digest()
+    t_h = time.perf_counter() - t0
+    return {"oracle": round(n / t_o), "openssl_hashlib": round(n / t_h), "ratio": round(t_h / t_o, 3)}
+
+
 def cpu_baseline(args):
     """Times the CPU oracle (single-threaded C restatement of the reference
-    prover) on a bounded sample: the same circuit family at 2^sample gates per
-    layer, scaled linearly by width to the 2^20-gate workload."""
+    prover, -O3 with AVX2 and mulx as the reference builds) proving the same
+    circuit as the GPU (2^log_width gates x depth layers, seed 1), once, at
+    full size, pinned to one host core, before this process initialises the
+    GPU.  Also reports the host CPU model and the oracle's Keccak-f rate
+    against OpenSSL's on the same core."""
     from oracle import oracle as O
     from virgo_amd import workloads as W
-    lw = args.cpu_sample_log_width
+    lw = args.cpu_sample_log_width or args.log_width
     circ = W.random_circuit(lw, args.depth, seed=1)
-    t0 = time.perf_counter()
-    _, parts = O.prove(circ)
-    dt = time.perf_counter() - t0
+    cpus = sorted(os.sched_getaffinity(0))
+    os.sched_setaffinity(0, {cpus[0]})
+    try:
+        rates = keccak_rates()
+        t0 = time.perf_counter()
+        tr, parts = O.prove(circ)
+        dt = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, set(cpus))
     scale = 2 ** (args.log_width - lw)
-    return {"value": round(dt * 1e3 * scale, 1), "unit": "ms per proof (scaled)", "cores": 1, "kind": "port",
-            "sample": "oracle prove of random circuit 2^%d x %d layers: %.2f s, x%d width scaling"
-                      % (lw, args.depth, dt, scale),
-            "sample_stage_ms": {k: round(float(v) * 1e3, 1) for k, v in
-                                zip(("evaluate", "gkr", "commit", "queries", "fft_gkr"), parts)}}
+    out = {"value": round(dt * 1e3 * scale, 1), "unit": "ms per proof", "cores": 1, "kind": "port",
+           "sample": "one oracle proof of the random circuit 2^%d x %d layers, seed 1, on core %d, before GPU init"
+                     % (lw, args.depth, cpus[0]) + ("" if scale == 1 else "; x%d width scaling" % scale),
+           "cpu_model": _cpu_model(),
+           "stage_ms": {k: round(float(v) * 1e3, 1) for k, v in
+                        zip(("evaluate", "gkr", "commit", "queries", "fft_gkr"), parts)},
+           "keccak_f_per_s": rates}
+    return out, (tr if scale == 1 else None)
 
 
 def _cpu_prove_spec(spec):
@@ -331,6 +376,14 @@ def serving(args):
             "bit_exact_vs_alone": bool(same), "all_accepted": bool(accepted)}
 
 
+_T0 = time.perf_counter()
+
+
+def progress(what):
+    """One line on stderr as each part of the run finishes."""
+    print("bench: %s done at %.1f s" % (what, time.perf_counter() - _T0), file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -346,6 +399,11 @@ def main():
     # the CPU throughput baseline of the batch first: its worker processes
     # start before this process initialises the GPU
     cpu_b = cpu_batch(args) if not args.no_cpu_baseline and not args.no_batch else None
+    if cpu_b:
+        progress("cpu batch baseline")
+    cpu_1, cpu_tr = cpu_baseline(args) if not args.no_cpu_baseline else (None, None)
+    if cpu_1:
+        progress("cpu single-proof baseline")
     circ = W.random_circuit(args.log_width, args.depth, seed=1)
     p = V.Prover(circ)
     for _ in range(args.warmup):
@@ -357,6 +415,7 @@ def main():
         stage.append(p.stage_ms.copy())
     dt = time.perf_counter() - t0
     ms = dt * 1e3 / args.steps
+    progress("timed proofs")
     import numpy as np
     st = np.mean(np.array(stage), axis=0)
     names = V.STAGES
@@ -416,15 +475,22 @@ def main():
             vt.append(vms)
         line["verify"] = {"accepted": bool(ok), "device_verifier_ms": round(min(vt), 2)}
     p.close()
-    line["c1"] = c1(args)
+    if not args.no_c1:
+        line["c1"] = c1(args)
+        progress("c1")
     if not args.no_batch:
         line["batched"] = batched(args)
+        progress("batched")
     if not args.no_serve:
         line["serving"] = serving(args)
+        progress("serving")
     if not args.no_microset:
         line["microset"] = microset(args)
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args)
+        progress("microset")
+    if cpu_1:
+        line["cpu_baseline"] = cpu_1
+        if cpu_tr is not None:
+            line["cpu_baseline"]["bit_exact_vs_gpu"] = cpu_tr == tr
         if cpu_b:
             line["batched"]["cpu_baseline"] = cpu_b
             line["batched"]["speedup_vs_cpu"] = round(line["batched"]["proofs_per_s"] / cpu_b["value"], 1)

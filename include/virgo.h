@@ -161,6 +161,17 @@ int vg_verify(vg_ctx *ctx, const uint8_t *transcript, size_t len, int *accepted,
 /* fast_fourier_transform (RS_polynomial.cpp:27-167): result[k] =
  * sum_{j<coef_len} c_j w^{jk}, k < order, w = root of unity of order `order`. */
 int vg_fft(const vg_fe *coef, int coef_len, int order, const vg_fe *rou, vg_fe *result);
+/* Allocation-free variant of vg_fft.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  coef (coef_len values) and result (order
+ * values) are DEVICE pointers; rou points to one host value.  Same result
+ * as vg_fft.  Enqueues its kernels on `stream` (NULL: the null stream) and
+ * returns; result is ready once that stream's work is done.  All working
+ * memory is one static device array per GPU (64 MiB, held until the process
+ * ends), so 2 <= order <= 2^20 (a larger order fails with an error) and no
+ * hipMalloc or hipFree is called.  Calls that overlap on one GPU share that
+ * array and race: serialise them (one stream, or wait between calls). */
+int vg_fft_noalloc(const vg_fe *coef, int coef_len, int order, const vg_fe *rou, vg_fe *result, void *stream);
 /* inverse_fast_fourier_transform (RS_polynomial.cpp:169-240) */
 int vg_ifft(const vg_fe *evals, int coef_len, int order, const vg_fe *rou, vg_fe *result);
 /* my_hhash (my_hhash.h:39-46): n independent SHA3-256 digests of 64 bytes */

@@ -21,7 +21,7 @@ EXPORTS = ["vg_init", "vg_last_error", "vg_free", "vg_ctx_create", "vg_ctx_destr
            "vg_prove", "vg_prove_async", "vg_finish", "vg_output", "vg_fft", "vg_ifft", "vg_sha3_64",
            "vg_merkle_tree", "vg_fri_fold", "vg_fe_mul", "vg_fft_gkr", "vg_stream_create", "vg_stream_destroy", "vg_prove_batch",
            "vg_verify", "vg_server_create", "vg_server_submit", "vg_server_wait", "vg_server_stats",
-           "vg_server_destroy", "vg_ntt_plan", "vg_sha256_64", "vg_merkle_tree_sha256", "vg_microset"]
+           "vg_server_destroy", "vg_ntt_plan", "vg_sha256_64", "vg_merkle_tree_sha256", "vg_microset", "vg_fft_noalloc"]
 
 
 class VirgoError(RuntimeError):
@@ -67,6 +67,7 @@ def lib():
                                   ctypes.POINTER(ctypes.c_int), vp]
         L.vg_fe_mul.argtypes = [vp, vp, vp, ctypes.c_size_t]
         L.vg_fri_fold.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.vg_fft_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
         L.vg_ntt_plan.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), vp]
         L.vg_prove_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.vg_stream_create.argtypes = [ctypes.POINTER(vp)]

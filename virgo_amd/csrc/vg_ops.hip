@@ -8,6 +8,7 @@
 // stage on the device) and the SHA-256 variants of the hash and tree.
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,66 @@ int make_twiddles(const fe &w, int64_t n, dbuf &tw, dbuf &W, launch::ntt_tabs &T
     launch::ntt_tables(w0, w1, lg, nullptr, nullptr, 0, 0, 0, (const fe *)tw.p, n, 0);
     T = {w0, w1, nullptr, nullptr, 0, 0};
     OPCHK(hipDeviceSynchronize());
+    return 0;
+}
+}  // namespace
+
+// ---- working memory of the allocation-free transform (vg_fft_noalloc) ----
+// One static device array per GPU, held until the process ends: the powers
+// tw[e] = w^e (e < order), the forward and inverse stage-major tables
+// W[order], Wi[order] (k_ntt_tables), the zero-padded input pad[order], and
+// the two-level power table T12 (w^j for j < 1024, then w^(1024 k) for
+// k <= order / 1024) that k_twiddles multiplies.  Sized for order <= 2^20:
+// 4 * 2^20 + 2049 values of 16 bytes, 64.03 MiB.
+constexpr int NA_MAX_LOG = 20;
+constexpr int64_t NA_MAX = (int64_t)1 << NA_MAX_LOG;
+constexpr int64_t NA_T12 = 1024 + (NA_MAX >> 10) + 1;
+constexpr int64_t NA_VALUES = 4 * NA_MAX + NA_T12;
+static __device__ __align__(256) uint64_t scratch[2 * NA_VALUES];
+
+// T[j] = w^j for j < 1024 and T[1024 + k] = w^(1024 k) for k < m, each by
+// square-and-multiply (exact, so the same values as repeated products)
+__global__ void k_pow_tables(fe *T, fe w, int64_t m)
+{
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= 1024 + m) return;
+    uint64_t e = j < 1024 ? (uint64_t)j : (uint64_t)(j - 1024) << 10;
+    fe x = w, acc = fone();
+    while (e) {
+        if (e & 1) acc = mul(acc, x);
+        x = mul(x, x);
+        e >>= 1;
+    }
+    T[j] = acc;
+}
+// pad[i] = src[i] for i < len, 0 for len <= i < n (a kernel, not a memset, so
+// that the clear is an ordinary node when a caller captures the stream)
+__global__ void k_pad_copy(const fe *src, int64_t len, fe *pad, int64_t n)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) pad[i] = i < len ? src[i] : fzero();
+}
+
+namespace {
+std::mutex g_na_mu;
+constexpr int NA_DEVICES = 64;
+fe *g_na_ptr[NA_DEVICES];        // device address of scratch, per device
+// The device address of this GPU's copy of scratch, looked up once per
+// device with that device current.  The host-side name of the array is not a
+// device address and is never handed to a kernel or a copy.
+int na_scratch(fe **out)
+{
+    int dev = 0;
+    OPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_DEVICES) { op_err = "vg_fft_noalloc: device index out of range"; return -1; }
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        OPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) { op_err = "vg_fft_noalloc: no device address for the scratch array"; return -20; }
+        g_na_ptr[dev] = (fe *)p;
+    }
+    *out = g_na_ptr[dev];
     return 0;
 }
 }  // namespace
@@ -339,6 +400,38 @@ int vg_fe_mul(const vg_fe *a, const vg_fe *b, vg_fe *out, size_t n)
     launch::fe_mul((const fe *)x.p, (const fe *)y.p, (fe *)z.p, (int64_t)n, 0);
     OPCHK(hipGetLastError());
     OPCHK(hipMemcpy(out, z.p, 16 * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vg_fft_noalloc(const vg_fe *coef, int coef_len, int order, const vg_fe *rou, vg_fe *result, void *stream)
+{
+    if (!coef || !rou || !result) { op_err = "vg_fft_noalloc: null pointer"; return -1; }
+    if (!pow2(coef_len) || !pow2(order) || coef_len > order || order < 2) {
+        op_err = "vg_fft_noalloc: sizes must be powers of two, coef_len <= order, order >= 2";
+        return -1;
+    }
+    if (order > NA_MAX) {
+        op_err = "vg_fft_noalloc: order " + std::to_string(order) + " exceeds the scratch limit of 2^" +
+                 std::to_string(NA_MAX_LOG);
+        return -2;
+    }
+    fe *S = nullptr;
+    int rc = na_scratch(&S);
+    if (rc) return rc;
+    fe *tw = S, *W = S + NA_MAX, *Wi = S + 2 * NA_MAX, *pad = S + 3 * NA_MAX, *T12 = S + 4 * NA_MAX;
+    hipStream_t st = (hipStream_t)stream;
+    fe w;
+    memcpy(&w, rou, sizeof w);
+    int64_t n = order, m = (n >> 10) + 1;
+    int lg = ilog2(n);
+    hipLaunchKernelGGL(k_pow_tables, dim3((unsigned)((1024 + m + 255) / 256)), dim3(256), 0, st, T12, w, m);
+    launch::twiddles(tw, n, T12, T12 + 1024, st);
+    launch::ntt_tables(W, Wi, lg, nullptr, nullptr, 0, 0, 0, tw, n, st);
+    hipLaunchKernelGGL(k_pad_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const fe *)coef,
+                       (int64_t)coef_len, pad, n);
+    launch::ntt_tabs T = {W, Wi, nullptr, nullptr, 0, 0};
+    launch::ntt({0, pad, n, nullptr, nullptr, 0, 0, 0, 0}, (fe *)result, n, 1, lg, 0, fone(), 0, T, st);
+    OPCHK(hipGetLastError());
     return 0;
 }
 

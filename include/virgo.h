@@ -141,8 +141,49 @@ int vg_server_stats(vg_server *server, vg_server_stats_t *out);
  * collected are dropped). */
 int vg_server_destroy(vg_server *server);
 
-/* Output layer of the last proof (2^bits elements). */
+/* Output layer of the last proof (2^bits elements); out may be NULL to ask
+ * for *bits alone.  Fails when cap < 2^bits. */
 int vg_output(vg_ctx *ctx, vg_fe *out, size_t cap, int *bits);
+/* Values of layer `layer` (0: the padded input layer, circuit_value[0] of
+ * zk_prover.cpp:54-66; depth-1: the output layer) of the padded,
+ * relay-extended circuit, as last evaluated; out may be NULL to ask for
+ * *bits alone.  No proof may be in flight on ctx. */
+int vg_layer_values(vg_ctx *ctx, int layer, vg_fe *out, size_t cap, int *bits);
+
+/* Values a caller that drives the prover one message at a time checks its
+ * own arguments against (include/virgo_prover.hpp does, for each argument
+ * the reference verifier passes).  Available once a proof of ctx has
+ * finished.  Writes min(cap, *count) values to out (may be NULL) and the
+ * total to *count.  what:
+ *  VG_VALUES_DRAWS: every field element the proof takes from glibc rand()
+ *    (seed 1; each value = rand() % p as real, then rand() % p as img,
+ *    prime_field.cpp:35-41), in draw order: r_0[b], r_1[b] (b = bits of the
+ *    output layer, zk_verifier.cpp:929); then for each layer i = depth-1
+ *    .. 1 with bl = bits of layer i-1: rho (:955), maskR[6] when i > 1 and
+ *    maskr[2] (zk_prover.cpp:233-241, drawn in sumcheck_init), the mask
+ *    polynomial's (2 bl + 1) * 2 + 7 coefficients (zk_prover.cpp:162-172,
+ *    generate_maskpoly_pre_rho), r_u[bl], r_v[bl], r_c, random_combine,
+ *    linear_coeff, linear_combine (zk_verifier.cpp:968-1002), alpha, beta
+ *    (:1183); then the n-6 FRI challenges (vpd_verifier.cpp:60,
+ *    commit_phase).  Values as drawn: the verifier zeroes r_v and r_c of
+ *    layer 1 afterwards (zk_verifier.cpp:973-977).
+ *  VG_VALUES_GENERAL: per layer i = depth-1 .. 1, the value of the wiring
+ *    predicates at the layer's challenges that sumcheck_finalround adds
+ *    (zk_prover.cpp:1173-1178; the verifier computes it at
+ *    zk_verifier.cpp:1126-1147); 0 for layer 1.
+ *  VG_VALUES_PUB_MASK / VG_VALUES_PRI_MASK: all_pub_mask as the verifier
+ *    builds it (zk_verifier.cpp:983-1025, 1201-1202) and all_pri_mask as
+ *    the prover and verifier build it (zk_prover.cpp:195-201, 170,
+ *    zk_verifier.cpp:1203-1204), the lists the commitment used. */
+#define VG_VALUES_DRAWS 0
+#define VG_VALUES_GENERAL 1
+#define VG_VALUES_PUB_MASK 2
+#define VG_VALUES_PRI_MASK 3
+int vg_proof_values(vg_ctx *ctx, int what, vg_fe *out, size_t cap, size_t *count);
+/* The 33 query positions of ctx's last proof: each the first raw rand()
+ * draw r % 2^(n-1) that is even and at least 2^(n-6), after the draws
+ * above (vpd_verifier.cpp:127-130). */
+int vg_query_positions(vg_ctx *ctx, int64_t *pos);
 
 /* Verifies a transcript of this context's circuit, as vg_prove writes it,
  * with the checks of zk_verifier::verify (src/linear_gkr/zk_verifier.cpp:

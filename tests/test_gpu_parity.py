@@ -130,26 +130,61 @@ def test_batch_matches_single():
         st.close()
 
 
-@pytest.mark.parametrize("name", ["matmul8", "random_w12_d3", "mixed_w10_d5"])
-def test_cpp_zk_prover_mirror(name, tmp_path):
-    """The C++ zk_prover mirror (include/virgo_prover.hpp), driven in the
-    reference verifier's call order, returns exactly the oracle's messages."""
-    import os
+def _write_circuit(circ, path):
     import struct
-    import subprocess
-    circ = CASES[name]()
-    path = tmp_path / "c.bin"
     with open(path, "wb") as f:
         f.write(struct.pack("<i", len(circ["n"])))
         f.write(np.asarray(circ["n"], "<i8").tobytes())
         rec = np.zeros(len(circ["ty"]), dtype=[("ty", "<i4"), ("u", "<i8"), ("v", "<i8")])
         rec["ty"], rec["u"], rec["v"] = circ["ty"], circ["u"], circ["v"]
         f.write(rec.tobytes())
+
+
+def _drive(args):
+    import os
+    import subprocess
     exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "zk_prover_drive")
+    return subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
+
+
+@pytest.mark.parametrize("name", ["matmul8", "random_w12_d3", "mixed_w10_d5"])
+def test_cpp_zk_prover_mirror(name, tmp_path):
+    """The C++ mirror (include/virgo_prover.hpp), driven as the reference
+    verifier drives its prover -- challenges and query positions drawn with
+    glibc rand() in the verifier's order, all_pub_mask and the public array
+    built by the driver, the members the verifier reads read -- returns
+    exactly the oracle's messages."""
+    circ = CASES[name]()
+    _write_circuit(circ, tmp_path / "c.bin")
     print("case", name, flush=True)
-    subprocess.run([exe, str(path), str(tmp_path / "t.bin")], check=True, timeout=60)
+    r = _drive([tmp_path / "c.bin", tmp_path / "t.bin"])
+    assert r.returncode == 0, r.stderr
     ref, _ = O.prove(circ)
     assert (tmp_path / "t.bin").read_bytes() == ref
+
+
+MIRROR_REFUSALS = {
+    "challenge": "previous_random differs", "alpha": "sumcheck_init: alpha differs",
+    "stream": "sumcheck_init: maskR: the rand() stream is out of step",
+    "general": "sumcheck_finalround: general_value differs", "primask": "all_pri_mask[3] differs",
+    "pubmask": "all_pub_msk[3] differs", "public": "public_array[5] is not eq(r_0, x)",
+    "target": "target_sum differs", "fri": "commit_phase: the rand() stream is out of step",
+    "position": "query position differs", "order": "request_step_commit out of the verifier's order",
+}
+
+
+def test_cpp_mirror_refuses_wrong_arguments(tmp_path):
+    """Each planted mistake -- one wrong challenge, mask entry, public-array
+    entry, target sum, query position or call order, or a rand() stream out
+    of step -- makes the mirror throw an error naming it, instead of handing
+    out a message made for other values."""
+    circ = CASES["matmul8"]()
+    _write_circuit(circ, tmp_path / "c.bin")
+    for mode, msg in MIRROR_REFUSALS.items():
+        print("mode", mode, flush=True)
+        r = _drive([tmp_path / "c.bin", tmp_path / "t.bin", mode])
+        assert r.returncode == 0, (mode, r.stdout, r.stderr)
+        assert "refused:" in r.stdout and msg in r.stdout, (mode, r.stdout)
 
 
 def test_rejects_bad_circuit():

@@ -21,7 +21,8 @@ EXPORTS = ["vg_init", "vg_last_error", "vg_free", "vg_ctx_create", "vg_ctx_destr
            "vg_prove", "vg_prove_async", "vg_finish", "vg_output", "vg_fft", "vg_ifft", "vg_sha3_64",
            "vg_merkle_tree", "vg_fri_fold", "vg_fe_mul", "vg_fft_gkr", "vg_stream_create", "vg_stream_destroy", "vg_prove_batch",
            "vg_verify", "vg_server_create", "vg_server_submit", "vg_server_wait", "vg_server_stats",
-           "vg_server_destroy", "vg_ntt_plan", "vg_sha256_64", "vg_merkle_tree_sha256", "vg_microset", "vg_fft_noalloc"]
+           "vg_server_destroy", "vg_ntt_plan", "vg_sha256_64", "vg_merkle_tree_sha256", "vg_microset", "vg_fft_noalloc", "vg_layer_values",
+           "vg_proof_values", "vg_query_positions"]
 
 
 class VirgoError(RuntimeError):
@@ -55,6 +56,9 @@ def lib():
         L.vg_prove_async.argtypes = [vp]
         L.vg_finish.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t), vp]
         L.vg_output.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]
+        L.vg_layer_values.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]
+        L.vg_proof_values.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        L.vg_query_positions.argtypes = [vp, vp]
         L.vg_verify.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
                                 ctypes.POINTER(ctypes.c_double)]
         L.vg_fft.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
@@ -178,11 +182,38 @@ class Prover:
         return bool(ok.value), ("" if ok.value else lib().vg_last_error().decode()), ms.value
 
     def output(self):
-        cap = 1 << 26
-        buf = np.zeros(cap, dtype=FE)
-        bits = ctypes.c_int()
-        _check(lib().vg_output(self._ctx, _p(buf), cap, ctypes.byref(bits)))
-        return buf[: 1 << bits.value].copy()
+        return self.layer(-1)
+
+    def layer(self, idx):
+        """Values of layer idx of the padded circuit as last evaluated (0:
+        the input layer; -1: the output layer)."""
+        L, bits = lib(), ctypes.c_int()
+        if idx < 0:
+            def fn(out, cap):
+                return L.vg_output(self._ctx, out, cap, ctypes.byref(bits))
+        else:
+            def fn(out, cap):
+                return L.vg_layer_values(self._ctx, idx, out, cap, ctypes.byref(bits))
+        _check(fn(None, 0))
+        buf = np.zeros(1 << bits.value, dtype=FE)
+        _check(fn(_p(buf), len(buf)))
+        return buf
+
+    VALUES = {"draws": 0, "general": 1, "pub_mask": 2, "pri_mask": 3}
+
+    def proof_values(self, what):
+        """vg_proof_values of the last proof: "draws", "general", "pub_mask"
+        or "pri_mask"."""
+        n = ctypes.c_size_t()
+        _check(lib().vg_proof_values(self._ctx, self.VALUES[what], None, 0, ctypes.byref(n)))
+        buf = np.zeros(n.value, dtype=FE)
+        _check(lib().vg_proof_values(self._ctx, self.VALUES[what], _p(buf), n.value, ctypes.byref(n)))
+        return buf
+
+    def query_positions(self):
+        pos = np.zeros(33, dtype=np.int64)
+        _check(lib().vg_query_positions(self._ctx, _p(pos)))
+        return pos
 
     def close(self):
         if self._ctx:

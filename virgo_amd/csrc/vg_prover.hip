@@ -547,6 +547,8 @@ struct vg_ctx {
     // per-proof host state
     schedule S;
     std::vector<layer_host> LH;
+    std::vector<fe> pub_mask, pri_mask;       // all_pub_mask / all_pri_mask of the last proof (unpadded)
+    bool proved = false;                      // a finished proof's values are on the host
     std::vector<int64_t> chal_ru, chal_rv, chal_ab;   // offsets into chal per layer
     int64_t chal_r0 = 0, chal_r1 = 0, chal_fri = 0, chal_one = 0, chal_vc = 0;
     std::vector<int64_t> msg_layer;                   // offset of each layer's block in msg
@@ -746,6 +748,8 @@ int prepare(vg_ctx *c)
     pub.push_back(mul(M.Zu, M.preu1));
     M.pri.push_back(M.maskr[0]);
     M.pri.push_back(M.maskr[1]);
+    c->pub_mask = pub;
+    c->pri_mask = M.pri;
     c->post[0] = M.Zu; c->post[1] = M.preu1; c->post[2] = M.sumRc.a; c->post[3] = M.sumRc.b; c->post[4] = M.sumRc.c;
     // masks padded to m (poly_commit.h:56-69, 136-139)
     std::vector<fe> prim(c->m, fzero()), pubm(c->m, fzero());
@@ -1110,6 +1114,7 @@ void assemble(vg_ctx *c, tbuf &T)
     T.put(go + k, 64); k += 4;                                     // roots L, H
     for (int s = 0; s < 65; ++s) T.f(c->h_msg[c->msg_allsum + s]);
     T.put(go + k, (size_t)16 * (c->n_gather - k));
+    c->proved = true;
 }
 
 // Plan of the CSR transposes of one gate layer (the structure of
@@ -1439,6 +1444,7 @@ int prove_start_gkr(vg_ctx *c, bool gkr)
 {
     if (!c) FAILV(-1, "null context");
     if (c->pending) FAILV(-1, "a proof is already in flight on this context");
+    c->proved = false;
     c->t_start = std::chrono::steady_clock::now();
     // the evaluation runs while the host draws the challenges
     int rc = enqueue_eval(c);
@@ -2281,13 +2287,66 @@ int vg_verify(vg_ctx *c, const uint8_t *transcript, size_t len, int *accepted, d
     return rc;
 }
 
+int vg_layer_values(vg_ctx *c, int layer, vg_fe *out, size_t cap, int *bits)
+{
+    if (!c || !bits) FAILV(-1, "vg_layer_values: null argument");
+    if (layer < 0 || layer >= c->C.depth) FAILV(-1, "vg_layer_values: layer %d outside 0..%d", layer, c->C.depth - 1);
+    if (c->pending) FAILV(-1, "vg_layer_values: a proof is in flight on this context");
+    int bl = c->C.bl[layer];
+    *bits = bl;
+    if (!out) return 0;
+    if (((size_t)1 << bl) > cap) FAILV(-1, "vg_layer_values: buffer of %zu values holds less than 2^%d", cap, bl);
+    HIPCHK(hipMemcpy(out, c->cv[layer], sizeof(fe) << bl, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int vg_output(vg_ctx *c, vg_fe *out, size_t cap, int *bits)
 {
     if (!c) FAILV(-1, "null context");
-    int bl = c->C.bl[c->C.depth - 1];
-    *bits = bl;
-    if (((size_t)1 << bl) > cap) FAILV(-1, "output buffer too small");
-    HIPCHK(hipMemcpy(out, c->cv[c->C.depth - 1], sizeof(fe) << bl, hipMemcpyDeviceToHost));
+    return vg_layer_values(c, c->C.depth - 1, out, cap, bits);
+}
+
+int vg_proof_values(vg_ctx *c, int what, vg_fe *out, size_t cap, size_t *count)
+{
+    if (!c || !count) FAILV(-1, "vg_proof_values: null argument");
+    if (c->pending || !c->proved) FAILV(-1, "vg_proof_values: no finished proof on this context");
+    const schedule &S = c->S;
+    const host_circuit &C = c->C;
+    std::vector<fe> v;
+    switch (what) {
+    case VG_VALUES_DRAWS:
+        v.insert(v.end(), S.r0.begin(), S.r0.end());
+        v.insert(v.end(), S.r1.begin(), S.r1.end());
+        for (int i = C.depth - 1; i >= 1; --i) {
+            const layer_sched &l = S.L[i];
+            v.push_back(l.rho);
+            if (i > 1) v.insert(v.end(), l.maskR, l.maskR + 6);
+            v.insert(v.end(), l.maskr, l.maskr + 2);
+            v.insert(v.end(), l.maskpoly.begin(), l.maskpoly.end());
+            v.insert(v.end(), l.r_u.begin(), l.r_u.end());
+            v.insert(v.end(), l.r_v.begin(), l.r_v.end());
+            for (const fe &x : {l.r_c, l.random_combine, l.linear_coeff, l.linear_combine, l.alpha, l.beta}) v.push_back(x);
+        }
+        v.insert(v.end(), S.fri_r.begin(), S.fri_r.end());
+        break;
+    case VG_VALUES_GENERAL:
+        for (int i = C.depth - 1; i >= 1; --i)
+            v.push_back(i > 1 ? c->h_msg[c->msg_layer[i] + 6 * c->LH[i].bl + 10] : fzero());
+        break;
+    case VG_VALUES_PUB_MASK: v = c->pub_mask; break;
+    case VG_VALUES_PRI_MASK: v = c->pri_mask; break;
+    default: FAILV(-1, "vg_proof_values: unknown selector %d", what);
+    }
+    *count = v.size();
+    if (out) memcpy(out, v.data(), sizeof(fe) * (v.size() < cap ? v.size() : cap));
+    return 0;
+}
+
+int vg_query_positions(vg_ctx *c, int64_t *pos)
+{
+    if (!c || !pos) FAILV(-1, "vg_query_positions: null argument");
+    if (c->pending || !c->proved) FAILV(-1, "vg_query_positions: no finished proof on this context");
+    for (int r = 0; r < 33; ++r) pos[r] = c->S.pow[r];
     return 0;
 }
 

@@ -48,3 +48,33 @@ struct range_alloc {
 };
 
 }  // namespace vg
+
+namespace vg {
+
+// Admission order of the proof server's requests that wait for memory:
+// arrival order, except that a later request whose lease fits now may pass
+// the oldest waiter while that one does not fit, at most max_bypass times;
+// after that the oldest is admitted next.  Not thread-safe (the server holds
+// its lock).  Host code only.
+struct admit_queue {
+    int max_bypass = 8;
+    std::map<int64_t, int> waiting;        // id -> times passed while it was the oldest waiter
+    void wait(int64_t id) { waiting.emplace(id, 0); }
+    // may request id try to take its lease now?
+    bool may_try(int64_t id) const
+    {
+        if (waiting.empty()) return false;
+        auto head = waiting.begin();
+        return head->first == id || head->second < max_bypass;
+    }
+    // request id took its lease
+    void admitted(int64_t id)
+    {
+        auto head = waiting.begin();
+        if (head != waiting.end() && head->first != id) ++head->second;
+        waiting.erase(id);
+    }
+    void cancel(int64_t id) { waiting.erase(id); }
+};
+
+}  // namespace vg

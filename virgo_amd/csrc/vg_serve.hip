@@ -13,8 +13,12 @@
 // * `slots` worker threads, each with its own non-blocking stream, take the
 //   requests in arrival order.  A worker plans its request on the host
 //   (circuit build, CSR transposes, layout: the exact byte count), then waits
-//   for memory: requests waiting for memory are admitted strictly in arrival
-//   order, each as soon as its lease fits (first fit over the free ranges).
+//   for memory: requests waiting for memory are admitted in arrival order,
+//   each as soon as its lease fits (first fit over the free ranges), except
+//   that a later one whose lease fits may pass the oldest waiter while that
+//   one does not fit, at most VIRGO_SERVE_BYPASS (default 8) times per
+//   oldest waiter (vg::admit_queue), so small leases do not queue behind a
+//   large one and the large one is not starved.
 //   The proof then runs in order on the worker's stream, overlapping on the
 //   device with every other admitted proof.
 // * Each result is the transcript vg_prove makes for the same circuit alone:
@@ -93,7 +97,7 @@ struct vg_server {
     std::condition_variable cv_work, cv_mem, cv_done;
     std::deque<job *> queue;
     std::map<int64_t, job *> jobs;
-    std::set<int64_t> waiting;             // requests planned and waiting for memory
+    vg::admit_queue waiting;               // requests planned and waiting for memory
     int64_t next_id = 0;
     bool stop = false;
     std::vector<std::thread> workers;
@@ -137,16 +141,16 @@ void worker(vg_server *s, hipStream_t st)
                 int64_t doff = -1, hoff = -1;
                 {
                     std::unique_lock<std::mutex> lk(s->mu);
-                    s->waiting.insert(j->id);
+                    s->waiting.wait(j->id);
                     s->cv_mem.wait(lk, [&] {
-                        if (*s->waiting.begin() != j->id) return false;
+                        if (!s->waiting.may_try(j->id)) return false;
                         doff = s->dpool.take(dn);
                         if (doff < 0) return false;
                         hoff = s->hpool.take(hn);
                         if (hoff < 0) { s->dpool.give(doff, dn); doff = -1; return false; }
                         return true;
                     });
-                    s->waiting.erase(j->id);
+                    s->waiting.admitted(j->id);
                     s->in_use += up(dn);
                     s->peak_bytes = std::max(s->peak_bytes, s->in_use);
                     ++s->admitted;
@@ -201,6 +205,7 @@ int vg_server_create(vg_server **out, int64_t device_budget, int64_t pinned_budg
     if (device_budget <= 0 || slots < 1 || slots > 64) SFAIL(-1, "vg_server_create: bad budget or slot count");
     if (pinned_budget <= 0) pinned_budget = (int64_t)1 << 30;
     vg_server *s = new vg_server();
+    if (const char *e = getenv("VIRGO_SERVE_BYPASS")) s->waiting.max_bypass = std::max(0, atoi(e));
     s->dsize = up(device_budget);
     s->hsize = up(pinned_budget);
     hipError_t e = hipGetDevice(&s->device);

@@ -343,12 +343,13 @@ __device__ __forceinline__ void fold_terms(const lpoly &V, const lpoly &aV, cons
 //   a += d_am d_V,  c += am0 V0 + aV0,  p(1) += am1 V1 + aV1,  b = p(1) - a - c
 // (three products; the reference's (a, b, c) follow exactly).
 //
-// Streaming launches (k_fold_reg) run up to 3 rounds with no communication:
-// each lane folds 2^b consecutive values of every table down to one value,
-// keeping its round sums in registers; the block reduces them once at the
-// end into per-block partials.  When at most 64 values remain, one wave per
-// instance (k_fold_tail) runs the last rounds in LDS and writes the round
-// sums and the final pair directly.
+// Streaming launches (k_fold_reg) run up to 3 rounds: each wave folds 64 2^b
+// consecutive values of every table down to 64, loading them with wave-wide
+// contiguous 1 KB loads and pairing values across lanes (fold_wave), keeping
+// its round sums in registers; the block reduces them once at the end into
+// per-block partials.  When at most 64 values remain, one wave per instance
+// (k_fold_tail) runs the last rounds in LDS and writes the round sums and the
+// final pair directly.
 // ---------------------------------------------------------------------
 using launch::FB_B;
 using launch::FB_TAIL;
@@ -411,10 +412,67 @@ __device__ __forceinline__ void fold_partials(fsum *acc, const launch::fold_inst
         }
     }
 }
-// rounds j0 .. j0+B-1, lane g folding values [g 2^B, (g+1) 2^B) to out[g]
+// lane l's copy of lane (l ^ m)'s v, m a power of two below 64 (a constant
+// once the callers' loops are unrolled): DPP quad permutes for 1 and 2,
+// ds_swizzle's xor mode (within 32 lanes) up to 16, ds_bpermute for 32.
+// Every lane of the wave must be active.
+__device__ __forceinline__ uint32_t lane_xor32(uint32_t v, int m)
+{
+    switch (m) {
+    case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    case 4: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (4 << 10) | 0x1F);
+    case 8: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (8 << 10) | 0x1F);
+    case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (16 << 10) | 0x1F);
+    default: return (uint32_t)__shfl_xor((int)v, m);
+    }
+}
+__device__ __forceinline__ uint64_t lane_xor64(uint64_t v, int m)
+{
+    return (uint64_t)lane_xor32((uint32_t)v, m) | (uint64_t)lane_xor32((uint32_t)(v >> 32), m) << 32;
+}
+__device__ __forceinline__ fe lane_xor(const fe &x, int m)
+{
+    fe y;
+    y.real = lane_xor64(x.real, m);
+    y.img = lane_xor64(x.img, m);
+    return y;
+}
+// B rounds on the 64 2^B consecutive values E[0 .. 64 2^B) of each table one
+// wave holds, lane l holding E[64k + l] as value k (so each wave-wide load
+// reads 1 KB contiguous).  Before round q, lane l holds F_q[64k + rot_q(l)]
+// as value k < 2^(B-q), F_q being the tables after q rounds and rot_q
+// rotating the 6 lane bits right by q; so the pair (F_q[2i], F_q[2i+1]) lies
+// in lanes l and l ^ 2^q as the same value k.  The lane with bit q clear
+// folds the pairs of even k, its partner those of odd k, each receiving the
+// other's element; both keep the result as value k / 2.  Afterwards lane l
+// holds F_B[rot_B(l)] as value 0.  Round q's sums go into acc[q].
+template <int B>
+__device__ __forceinline__ void fold_wave(fe *V, fe *aV, fe *am, const fe *rq, fsum *acc, int lane)
+{
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+        const int m = 1 << q;
+        const bool hi = (lane >> q) & 1;
+#pragma unroll
+        for (int j = 0; j < (1 << (B - q - 1)); ++j) {
+            fe rV = lane_xor(hi ? V[2 * j] : V[2 * j + 1], m);
+            fe raV = lane_xor(hi ? aV[2 * j] : aV[2 * j + 1], m);
+            fe ram = lane_xor(hi ? am[2 * j] : am[2 * j + 1], m);
+            fe V0 = hi ? rV : V[2 * j], V1 = hi ? V[2 * j + 1] : rV;
+            fe aV0 = hi ? raV : aV[2 * j], aV1 = hi ? aV[2 * j + 1] : raV;
+            fe am0 = hi ? ram : am[2 * j], am1 = hi ? am[2 * j + 1] : ram;
+            fsum_add(acc[q], V0, V1, aV0, aV1, am0, am1);
+            V[j] = lerp(V0, V1, rq[q]); aV[j] = lerp(aV0, aV1, rq[q]); am[j] = lerp(am0, am1, rq[q]);
+        }
+    }
+}
+// rounds j0 .. j0+B-1: wave unit u (64 outputs; nout is a multiple of 64)
+// folds values [u 64 2^B, (u+1) 64 2^B) to out[u 64 .. u 64 + 64)
 template <int B>
 __global__ __launch_bounds__(256) void k_fold_reg(const launch::fold_inst *I, int cur, int j0, int64_t nout, int slot_round)
 {
+    constexpr int H = 1 << B;
     const launch::fold_inst &t = I[blockIdx.y];
     ftab X = fold_input(t, j0, cur);
     fe rq[B];
@@ -423,12 +481,17 @@ __global__ __launch_bounds__(256) void k_fold_reg(const launch::fold_inst *I, in
     fsum acc[B];
 #pragma unroll
     for (int q = 0; q < B; ++q) acc[q] = {fzero(), fzero(), fzero()};
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nout; g += (int64_t)gridDim.x * blockDim.x) {
-        int64_t o = g << B;
-        fe oV, oaV, oam;
-        fold_lane<B>([&](int k, fe &V, fe &aV, fe &am) { V = X.V[o + k]; aV = X.aV[o + k]; am = X.am[o + k]; }, rq, acc,
-                     oV, oaV, oam);
-        t.V[cur][g] = oV; t.aV[cur][g] = oaV; t.am[cur][g] = oam;
+    const int lane = threadIdx.x & 63;
+    const int64_t nu = nout >> 6, ustep = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    // u is uniform over the wave, so all its lanes take part in every exchange
+    for (int64_t u = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; u < nu; u += ustep) {
+        const int64_t o = (u << (B + 6)) + lane;
+        fe V[H], aV[H], am[H];
+#pragma unroll
+        for (int k = 0; k < H; ++k) { V[k] = X.V[o + 64 * k]; aV[k] = X.aV[o + 64 * k]; am[k] = X.am[o + 64 * k]; }
+        fold_wave<B>(V, aV, am, rq, acc, lane);
+        const int64_t g = (u << 6) + (((lane >> B) | (lane << (6 - B))) & 63);
+        t.V[cur][g] = V[0]; t.aV[cur][g] = aV[0]; t.am[cur][g] = am[0];
     }
     fold_partials<B>(acc, t, slot_round);
 }

@@ -110,13 +110,17 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
  * which proves one circuit per process, main_zk.cpp:11-25) ----
  * Requests of any size are submitted at any time and admitted while others
  * run.  The server reserves device_budget bytes of HBM and pinned_budget
- * bytes of page-locked host memory (<= 0: 1 GiB) once; every admitted proof
- * takes its whole working set from them as one lease, keeps it resident on
- * the device until its transcript is on the host, and gives it back, so no
- * device memory is allocated or freed while proofs run.  `slots` worker
- * threads (1..64), each on its own stream, take requests in arrival order;
- * requests waiting for memory are admitted in arrival order as soon as their
- * lease fits.  Every transcript is byte-identical to vg_prove of the same
+ * bytes of page-locked host memory (<= 0: 2 GiB) once; every admitted proof
+ * takes its whole working set from them as one lease (the pinned part holds
+ * its set-up uploads, about 9 bytes per gate, and its transcript staging),
+ * keeps it resident on the device until its transcript is on the host, and
+ * gives it back, so no device memory is allocated or freed while proofs run.
+ * `slots` worker threads (1..64), each on its own stream, take requests in
+ * arrival order; requests waiting for memory are admitted in arrival order
+ * as soon as their lease fits, except that a later request whose lease fits
+ * may pass the oldest waiter up to VIRGO_SERVE_BYPASS (default 8) times.  A
+ * proof's uploads and launches all go to its worker's stream, and its worker
+ * waits only for its own proof's end.  Every transcript is byte-identical to vg_prove of the same
  * circuit alone.  A request larger than the budgets fails (its wait returns
  * the error); the others are unaffected. */
 typedef struct vg_server vg_server;

@@ -77,6 +77,8 @@ struct fg_ctx {
     std::vector<launch::half_desc> h_hdesc;
     int64_t ht_total = 0;
     fe *msg = nullptr, *h_msg = nullptr;
+    uint8_t *h_up = nullptr;          // page-locked upload staging (when the source stages)
+    int64_t up_bytes = 0;
     int64_t msg_len = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool pending = false;
@@ -345,8 +347,17 @@ int fg_plan(fg_ctx **out, int lg, hipStream_t stream, int64_t *dev_bytes, int64_
         c->own_stream = true;
     }
     c->bytes = layout(c, nullptr);
+    {
+        // fg_alloc's uploads, put by put
+        auto sb = uploader::stage_bytes;
+        int lg2 = 2 + 2 * lg;
+        c->up_bytes = sb((int64_t)sizeof(launch::fold_inst) * lg2) + sb((int64_t)sizeof(launch::fg_layer_dev) * lg) +
+                      sb((int64_t)sizeof(launch::half_desc) * (int64_t)c->h_hdesc.size()) +
+                      sb((int64_t)sizeof(int) * (int64_t)c->h_slot_nb.size()) +
+                      sb((int64_t)sizeof(int64_t) * (int64_t)c->h_slot_dst.size());
+    }
     if (dev_bytes) *dev_bytes = c->bytes;
-    if (pin_bytes) *pin_bytes = (int64_t)sizeof(fe) * (c->n_chal + c->msg_len) + 512;
+    if (pin_bytes) *pin_bytes = (int64_t)sizeof(fe) * (c->n_chal + c->msg_len) + 512 + c->up_bytes;
     *out = c;
     return 0;
 }
@@ -398,12 +409,15 @@ int fg_alloc(fg_ctx *c, mem_source *mem)
         LD[dep].vu = c->msg + l.msg1 + 3 * lg + 4;
         LD[dep].dep = dep;
     }
-    bool ok = hipMemcpy(c->inst, I.data(), sizeof(launch::fold_inst) * I.size(), hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(c->ldev, LD.data(), sizeof(launch::fg_layer_dev) * LD.size(), hipMemcpyHostToDevice) == hipSuccess;
-    ok = ok && hipMemcpy(c->hdesc, d.data(), sizeof(launch::half_desc) * d.size(), hipMemcpyHostToDevice) == hipSuccess;
+    uploader U;
+    if (!U.init(mem, c->up_bytes, c->st)) FG_FAIL(-21, "fft_gkr: pinned upload staging failed");
+    c->h_up = U.stage;
+    bool ok = U.put(c->inst, I.data(), (int64_t)sizeof(launch::fold_inst) * (int64_t)I.size()) == hipSuccess &&
+              U.put(c->ldev, LD.data(), (int64_t)sizeof(launch::fg_layer_dev) * (int64_t)LD.size()) == hipSuccess;
+    ok = ok && U.put(c->hdesc, d.data(), (int64_t)sizeof(launch::half_desc) * (int64_t)d.size()) == hipSuccess;
     if (ok && !c->h_slot_nb.empty())
-        ok = hipMemcpy(c->slot_nb, c->h_slot_nb.data(), sizeof(int) * c->h_slot_nb.size(), hipMemcpyHostToDevice) == hipSuccess &&
-             hipMemcpy(c->slot_dst, c->h_slot_dst.data(), sizeof(int64_t) * c->h_slot_dst.size(), hipMemcpyHostToDevice) == hipSuccess;
+        ok = U.put(c->slot_nb, c->h_slot_nb.data(), (int64_t)sizeof(int) * (int64_t)c->h_slot_nb.size()) == hipSuccess &&
+             U.put(c->slot_dst, c->h_slot_dst.data(), (int64_t)sizeof(int64_t) * (int64_t)c->h_slot_dst.size()) == hipSuccess;
     if (!ok || hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess)
         FG_FAIL(-20, "fft_gkr: setup copies failed");
     return 0;
@@ -426,6 +440,7 @@ void fg_destroy(fg_ctx *c)
         if (c->arena) c->mem->device_free(c->arena);
         if (c->h_chal) c->mem->pinned_free(c->h_chal);
         if (c->h_msg) c->mem->pinned_free(c->h_msg);
+        if (c->h_up) c->mem->pinned_free(c->h_up);
     }
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);

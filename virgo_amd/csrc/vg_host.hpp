@@ -2,7 +2,9 @@
 // generator and a byte buffer for transcripts.
 #pragma once
 #include <stdint.h>
+#include <string.h>
 #include <vector>
+#include <hip/hip_runtime.h>
 #include "vg_field.hpp"
 
 namespace vg {
@@ -58,8 +60,49 @@ struct mem_source {
     virtual void device_free(void *p) = 0;
     virtual void *pinned(int64_t bytes) = 0;     // page-locked host memory
     virtual void pinned_free(void *p) = 0;
+    // true: a context stages its set-up uploads through page-locked memory
+    // of this source and copies them asynchronously on its stream (see
+    // uploader); the planned pinned byte count then includes the staging
+    virtual bool async_uploads() const { return false; }
 };
 mem_source *hip_mem();
+
+// ---------------------------------------------------------------------------
+// Host-to-device uploads of a context's static data (gates, descriptors,
+// tables) at creation.  A context of its own copies each array with a
+// synchronous hipMemcpy.  A context in a proof server's lease packs the
+// bytes into page-locked staging taken from the lease and enqueues
+// hipMemcpyAsync on its own stream: a synchronous copy runs on the null
+// stream, whose hardware queue other proofs share, so each one would wait
+// for whatever work of other proofs is queued ahead of it.  The staging
+// stays part of the lease until the context is destroyed (after its stream
+// is drained).  stage_bytes(n) is what put() of n bytes takes from it.
+// ---------------------------------------------------------------------------
+struct uploader {
+    uint8_t *stage = nullptr;
+    int64_t cap = 0, off = 0;
+    hipStream_t st = nullptr;
+    static int64_t stage_bytes(int64_t n) { return (n + 255) & ~(int64_t)255; }
+    // takes `bytes` of staging from m when it stages; false: no memory
+    bool init(mem_source *m, int64_t bytes, hipStream_t s)
+    {
+        st = s;
+        if (!m->async_uploads() || bytes <= 0) return true;
+        stage = (uint8_t *)m->pinned(bytes);
+        cap = bytes;
+        return stage != nullptr;
+    }
+    hipError_t put(void *dst, const void *src, int64_t n)
+    {
+        if (n <= 0) return hipSuccess;
+        if (!stage) return hipMemcpy(dst, src, (size_t)n, hipMemcpyHostToDevice);
+        if (off + n > cap) return hipErrorInvalidValue;      // more than the plan counted
+        memcpy(stage + off, src, (size_t)n);
+        hipError_t e = hipMemcpyAsync(dst, stage + off, (size_t)n, hipMemcpyHostToDevice, st);
+        off += stage_bytes(n);
+        return e;
+    }
+};
 
 // transcript bytes: F = 16 bytes {img, real}, little-endian u32
 struct tbuf {

@@ -470,6 +470,8 @@ struct vg_ctx {
     uint8_t *arena = nullptr;
     int64_t arena_bytes = 0;
     int64_t pin_bytes = 0, gout_len = 0;     // pinned staging: chal, gsrc, gout, msg
+    int64_t up_bytes = 0;                    // page-locked upload staging when the source stages (upload_bytes)
+    uint8_t *h_up = nullptr;
     int64_t fg_bytes = 0, fg_pin = 0;        // the fft_gkr context's
     mem_source *mem = nullptr;               // where arena and staging come from
     // device views
@@ -1182,6 +1184,27 @@ void csr_plan(vg_ctx *c, int i)
 
 // Host planning of a proof: sizes, challenge and message layout, layer
 // batches, CSR transposes and the arena layout (nothing allocated).
+// staging bytes of ctx_alloc's uploads, put by put in the same order
+int64_t upload_bytes(const vg_ctx *c)
+{
+    const host_circuit &C = c->C;
+    auto sb = uploader::stage_bytes;
+    int64_t t = 0;
+    for (int i = 0; i < C.depth; ++i) {
+        int64_t nn = (int64_t)1 << C.bl[i];
+        t += sb(4 * nn);
+        if (i > 0) t += sb(nn) + sb(4 * nn);
+    }
+    if (c->n_slots) t += sb((int64_t)sizeof(int) * c->n_slots) + sb((int64_t)sizeof(int64_t) * c->n_slots);
+    for (int i = 1; i < C.depth; ++i) t += sb(4 * (int64_t)c->h_z2[i].size());
+    t += sb((int64_t)sizeof(launch::half_desc) * (8 * (C.depth - 1) + 3));
+    int nd = C.depth - 1;
+    t += sb((int64_t)sizeof(launch::layer_dev) * (nd > 0 ? nd : 1));
+    t += sb((int64_t)sizeof(launch::fold_inst) * (nd > 0 ? 2 * nd : 1));
+    t += sb((int64_t)sizeof(int) * 65);
+    t += sb((int64_t)sizeof(fe) * (1024 + (c->SS >> 10) + 1));
+    return t;
+}
 int ctx_plan(vg_ctx *c)
 {
     host_circuit &C = c->C;
@@ -1285,6 +1308,7 @@ int ctx_plan(vg_ctx *c)
     c->arena_bytes = layout(c, nullptr);
     c->gout_len = c->gather_cap > (1024 + (c->SS >> 10) + 1) ? c->gather_cap : (1024 + (c->SS >> 10) + 1);
     c->pin_bytes = (int64_t)sizeof(fe) * (c->n_chal + c->gout_len + c->msg_len) + (int64_t)sizeof(uint64_t) * c->gather_cap + 1024;
+    c->up_bytes = upload_bytes(c);
     return 0;
 }
 
@@ -1306,21 +1330,25 @@ int ctx_alloc(vg_ctx *c)
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
     for (auto &e : c->gev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto &e : c->ev_leaf) HIPCHK(hipEventCreate(&e));
+    // every upload in the order upload_bytes counts them
+    uploader U;
+    if (!U.init(c->mem, c->up_bytes, c->st)) FAILV(-21, "pinned upload staging failed");
+    c->h_up = U.stage;
     // gates and inputs
     for (int i = 0; i < C.depth; ++i) {
         int64_t nn = (int64_t)1 << C.bl[i];
-        HIPCHK(hipMemcpy(c->d_u[i], C.u[i].data(), 4 * nn, hipMemcpyHostToDevice));
+        HIPCHK(U.put(c->d_u[i], C.u[i].data(), 4 * nn));
         if (i > 0) {
-            HIPCHK(hipMemcpy(c->d_ty[i], C.ty[i].data(), nn, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(c->d_v[i], C.v[i].data(), 4 * nn, hipMemcpyHostToDevice));
+            HIPCHK(U.put(c->d_ty[i], C.ty[i].data(), nn));
+            HIPCHK(U.put(c->d_v[i], C.v[i].data(), 4 * nn));
         }
     }
     if (c->n_slots) {
-        HIPCHK(hipMemcpy(c->slot_nb, c->h_slot_nb.data(), sizeof(int) * c->n_slots, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->slot_dst, c->h_slot_dst.data(), sizeof(int64_t) * c->n_slots, hipMemcpyHostToDevice));
+        HIPCHK(U.put(c->slot_nb, c->h_slot_nb.data(), (int64_t)sizeof(int) * c->n_slots));
+        HIPCHK(U.put(c->slot_dst, c->h_slot_dst.data(), (int64_t)sizeof(int64_t) * c->n_slots));
     }
     for (int i = 1; i < C.depth; ++i) {
-        if (!c->h_z2[i].empty()) HIPCHK(hipMemcpy(c->z2[i], c->h_z2[i].data(), 4 * c->h_z2[i].size(), hipMemcpyHostToDevice));
+        HIPCHK(U.put(c->z2[i], c->h_z2[i].data(), 4 * (int64_t)c->h_z2[i].size()));
         gates_dev g = {c->d_ty[i], c->d_u[i], c->d_v[i], C.bl[i]};
         launch::csr_build(g, (int64_t)C.ty[i].size(), (int64_t)1 << C.bl[i - 1], c->off1[i], c->off2[i], c->ent1[i],
                           c->ent2[i], c->csr_cur, c->csr_scan, c->st);
@@ -1362,7 +1390,7 @@ int ctx_alloc(vg_ctx *c)
         addd(c->cq, nullptr, rl + nq, 6);
         c->n_hdesc = (int)hd.size();
         c->ht_total = tot;
-        HIPCHK(hipMemcpy(c->hdesc, hd.data(), sizeof(launch::half_desc) * hd.size(), hipMemcpyHostToDevice));
+        HIPCHK(U.put(c->hdesc, hd.data(), (int64_t)sizeof(launch::half_desc) * (int64_t)hd.size()));
     }
     {
         // static per-layer descriptors of the batched sumcheck launches
@@ -1403,14 +1431,14 @@ int ctx_alloc(vg_ctx *c)
                     x.vconst = c->chal + c->chal_vc + 2 * i + ph;
                 }
             }
-        HIPCHK(hipMemcpy(c->d_layers, LD.data(), sizeof(launch::layer_dev) * LD.size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->d_inst, FI.data(), sizeof(launch::fold_inst) * FI.size(), hipMemcpyHostToDevice));
+        HIPCHK(U.put(c->d_layers, LD.data(), (int64_t)sizeof(launch::layer_dev) * (int64_t)LD.size()));
+        HIPCHK(U.put(c->d_inst, FI.data(), (int64_t)sizeof(launch::fold_inst) * (int64_t)FI.size()));
         c->h_layers = LD;
         c->h_inst = FI;
     }
     std::vector<int> lens(65, (int)c->real);
     lens[64] = (int)c->m;
-    HIPCHK(hipMemcpy(c->lens, lens.data(), sizeof(int) * 65, hipMemcpyHostToDevice));
+    HIPCHK(U.put(c->lens, lens.data(), (int64_t)sizeof(int) * 65));
     {
         // twiddle table w_SS^e, e < SS (depends only on the input size)
         int64_t SS = c->SS;
@@ -1419,7 +1447,7 @@ int ctx_alloc(vg_ctx *c)
         for (int j = 0; j < 1024; ++j) { T12[j] = x; x = mul(x, w); }
         fe w10 = x, y = fone();
         for (int64_t j = 0; j < (SS >> 10) + 1; ++j) { T12[1024 + j] = y; y = mul(y, w10); }
-        HIPCHK(hipMemcpy(c->T12, T12.data(), sizeof(fe) * T12.size(), hipMemcpyHostToDevice));
+        HIPCHK(U.put(c->T12, T12.data(), (int64_t)sizeof(fe) * (int64_t)T12.size()));
         launch::twiddles(c->tw, SS, c->T12, c->T12 + 1024, c->st);
         launch::ntt_tables(c->ntW, c->ntWi, c->log_C + 1, c->ntTA, c->ntTB, c->log_R, c->NT.th, c->NT.tb, c->tw, SS, c->st);
     }
@@ -1430,7 +1458,9 @@ int ctx_alloc(vg_ctx *c)
     if (launch::ntt_tune(c->log_C, (int64_t)65 << c->log_R, c->l, c->h, c->NT, c->st) ||
         launch::ntt_tune(c->log_C + 1, 65, c->l, c->h, c->NT, c->st))
         FAILV(-20, "NTT tile tuning failed");
-    HIPCHK(hipStreamSynchronize(c->st));
+    // a context of its own reports set-up errors here; a staged one at its
+    // proof's end (the host does not wait behind other proofs' work)
+    if (!U.stage) HIPCHK(hipStreamSynchronize(c->st));
     return 0;
 }
 
@@ -1948,7 +1978,7 @@ int ctx_create_planned(vg_ctx **out, const vg_circuit_desc *d, void *stream)
 void ctx_needs(const vg_ctx *c, int64_t *dev, int64_t *pin)
 {
     *dev = ((c->arena_bytes + 255) & ~(int64_t)255) + ((c->fg_bytes + 255) & ~(int64_t)255);
-    *pin = c->pin_bytes + c->fg_pin;
+    *pin = c->pin_bytes + uploader::stage_bytes(c->up_bytes) + c->fg_pin;
 }
 
 int ctx_materialize(vg_ctx *c, mem_source *mem, vg_ctx **out)
@@ -1989,6 +2019,7 @@ int vg_ctx_destroy(vg_ctx *c)
         if (c->h_gsrc) c->mem->pinned_free(c->h_gsrc);
         if (c->h_gout) c->mem->pinned_free(c->h_gout);
         if (c->h_msg) c->mem->pinned_free(c->h_msg);
+        if (c->h_up) c->mem->pinned_free(c->h_up);
     }
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto &e : c->ev_leaf) if (e) (void)hipEventDestroy(e);

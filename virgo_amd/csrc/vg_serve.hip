@@ -72,6 +72,7 @@ struct lease : vg::mem_source {
         return p;
     }
     void pinned_free(void *) override {}
+    bool async_uploads() const override { return true; }
 };
 
 struct job {
@@ -203,7 +204,7 @@ int vg_server_create(vg_server **out, int64_t device_budget, int64_t pinned_budg
     if (!out) SFAIL(-1, "vg_server_create: null argument");
     *out = nullptr;
     if (device_budget <= 0 || slots < 1 || slots > 64) SFAIL(-1, "vg_server_create: bad budget or slot count");
-    if (pinned_budget <= 0) pinned_budget = (int64_t)1 << 30;
+    if (pinned_budget <= 0) pinned_budget = (int64_t)2 << 30;   // staging of uploads and transcripts
     vg_server *s = new vg_server();
     if (const char *e = getenv("VIRGO_SERVE_BYPASS")) s->waiting.max_bypass = std::max(0, atoi(e));
     s->dsize = up(device_budget);
@@ -215,6 +216,20 @@ int vg_server_create(vg_server **out, int64_t device_budget, int64_t pinned_budg
         hipStream_t st = nullptr;
         e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
         if (e == hipSuccess) s->streams.push_back(st);
+    }
+    if (e == hipSuccess) {
+        // touch every page of the page-locked region once, on a few threads:
+        // its pages are mapped on first touch, and a proof packing its
+        // uploads into an untouched range wrote at about 2 GB/s instead of
+        // about 20
+        const int64_t pg = 4096;
+        const int nt = 8;
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([s, t, pg, nt] {
+                for (int64_t o = (int64_t)t * pg; o < s->hsize; o += (int64_t)nt * pg) s->hbase[o] = 0;
+            });
+        for (auto &x : th) x.join();
     }
     if (e != hipSuccess) {
         std::string m = std::string("vg_server_create: ") + hipGetErrorString(e);

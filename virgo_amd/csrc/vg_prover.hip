@@ -1133,6 +1133,7 @@ void csr_plan(vg_ctx *c, int i)
     int64_t nt = (int64_t)1 << C.bl[i - 1];
     int64_t ng = (int64_t)C.ty[i].size();
     int64_t tile = launch::gf_tile(C.bl[i - 1]);
+    const int tsh = ilog2(tile);                 // tiles are powers of two: x / tile = x >> tsh
     int64_t ntile = (nt + tile - 1) / tile;
     std::vector<int64_t> t1(ntile, 0), t2(ntile, 0);
     std::vector<int32_t> &z = c->h_z2[i];
@@ -1149,15 +1150,15 @@ void csr_plan(vg_ctx *c, int i)
         case 12: lo = u[g]; hi = (int64_t)v[g] + 1; break;
         default: break;
         }
-        if (hi == lo + 1) t1[lo / tile]++;
+        if (hi == lo + 1) t1[lo >> tsh]++;
         else
             for (int64_t x = lo; x < hi;) {
-                int64_t k = x / tile, end = (k + 1) * tile < hi ? (k + 1) * tile : hi;
+                int64_t k = x >> tsh, end = (k + 1) * tile < hi ? (k + 1) * tile : hi;
                 t1[k] += end - x;
                 x = end;
             }
         e1 += hi > lo ? hi - lo : 0;
-        if (t == 0 || t == 1 || t == 7 || t == 8 || t == 9 || t == 13) { t2[v[g] / tile]++; ++e2; }
+        if (t == 0 || t == 1 || t == 7 || t == 8 || t == 9 || t == 13) { t2[v[g] >> tsh]++; ++e2; }
         if (t == 5 || t == 12 || t == 6 || t == 10) z.push_back((int32_t)g);
     }
     c->n_ent1[i] = e1;

@@ -563,6 +563,13 @@ struct vg_ctx {
     bool pending = false;
     bool serial = false;        // the whole proof in order on st (caller's stream, large input or VIRGO_SERIAL=1)
     bool own_aux = false;       // st2 created by the context
+    // a proof in order on its own streams (large input) still runs the
+    // extension of its input layer (commit_private_array) on st2 beside the
+    // evaluation, whose gathers leave the VALU idle: ev_in (input layer
+    // evaluated, on st), ev_chal (challenges uploaded, on st2), ev_l0 (the
+    // extension's start, on st2); ev[3] marks its end
+    bool early_l = false;
+    hipEvent_t ev_in = nullptr, ev_chal = nullptr, ev_l0 = nullptr;
     int own_gst = 0;            // gst[0..own_gst) created by the context (else gst[] = st)
     fg_ctx *fg = nullptr;       // fft_gkr of the commitment (lg = n - 6), on its own stream
 };
@@ -824,12 +831,14 @@ int build_gather(vg_ctx *c)
 
 // ---- evaluate (zk_prover.cpp:54-157): needs no challenge, so it is
 // enqueued before the host draws them ----
+void enqueue_lde_l(vg_ctx *c, hipStream_t st);
 int enqueue_eval(vg_ctx *c)
 {
     hipStream_t st = c->st;
     host_circuit &C = c->C;
     HIPCHK(hipEventRecord(c->ev[0], st));
     launch::eval_input(c->d_u[0], c->cv[0], 1 << C.bl[0], st);
+    if (c->early_l) HIPCHK(hipEventRecord(c->ev_in, st));
     for (int i = 1; i < C.depth; ++i)
         launch::eval_layer({c->d_ty[i], c->d_u[i], c->d_v[i], C.bl[i]}, c->cv[i - 1], c->cv[i], st);
     HIPCHK(hipEventRecord(c->ev[1], st));
@@ -845,7 +854,19 @@ int enqueue_gkr(vg_ctx *c, bool gkr = true)
     hipStream_t st = c->st, st2 = c->st2;
     host_circuit &C = c->C;
     int D = C.depth;
-    HIPCHK(hipMemcpyAsync(c->chal, c->h_chal, sizeof(fe) * c->n_chal, hipMemcpyHostToDevice, st));
+    if (c->early_l) {
+        // the input layer's extension needs the input values and the private
+        // masks only: st2 takes it as soon as the input layer is evaluated
+        HIPCHK(hipStreamWaitEvent(st2, c->ev_in, 0));
+        HIPCHK(hipMemcpyAsync(c->chal, c->h_chal, sizeof(fe) * c->n_chal, hipMemcpyHostToDevice, st2));
+        HIPCHK(hipEventRecord(c->ev_chal, st2));
+        HIPCHK(hipEventRecord(c->ev_l0, st2));
+        enqueue_lde_l(c, st2);
+        HIPCHK(hipEventRecord(c->ev[3], st2));
+        HIPCHK(hipStreamWaitEvent(st, c->ev_chal, 0));
+    } else {
+        HIPCHK(hipMemcpyAsync(c->chal, c->h_chal, sizeof(fe) * c->n_chal, hipMemcpyHostToDevice, st));
+    }
     HIPCHK(hipMemcpyAsync(c->gsrc, c->h_gsrc, sizeof(uint64_t) * c->n_gather, hipMemcpyHostToDevice, st));
     // every eq half table of the proof, from the uploaded challenges
     launch::half_multi(c->hdesc, c->n_hdesc, c->ht_total, st);
@@ -894,22 +915,35 @@ int enqueue_gkr(vg_ctx *c, bool gkr = true)
 
 // the commitment up to its leaf chains: LDEs, public array, FRI folds (ends
 // on ev[5]); on the second stream when the context overlaps its stages
+// ---- commit_private_array (poly_commit.h:42-124): the 64 input slices and
+// the private mask interpolated and extended into l ----
+void enqueue_lde_l(vg_ctx *c, hipStream_t st)
+{
+    int64_t real = c->real, m = c->m, Cc = c->C_;
+    int lr = ilog2(real), lm = ilog2(m);
+    fe *pri_d = c->chal + (c->n_chal - 2 * m);
+    fe inv_real = finv(mk((uint64_t)real, 0)), inv_m = finv(mk((uint64_t)m, 0));
+    launch::intt({0, c->cv[0], real, nullptr, nullptr, 0, 0, 0, 0}, c->coefP, Cc, 64, lr, inv_real, c->NT, st);
+    launch::intt({0, pri_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefP + 64 * Cc, Cc, 1, lm, inv_m, c->NT, st);
+    launch::lde(c->coefP, Cc, c->lens, 65, c->l, c->log_C, c->log_R, c->NT, st);
+}
 int enqueue_commit(vg_ctx *c)
 {
     hipStream_t st = c->serial ? c->st : c->st2;
     host_circuit &C = c->C;
     int D = C.depth;
-    // ---- commit_private_array (poly_commit.h:42-124) ----
     int n = c->n;
     int64_t SS = c->SS, real = c->real, m = c->m, Cc = c->C_;
     int lr = ilog2(real), lm = ilog2(m);
-    fe *pri_d = c->chal + (c->n_chal - 2 * m), *pub_d = c->chal + (c->n_chal - m);
+    fe *pub_d = c->chal + (c->n_chal - m);
     fe inv_real = finv(mk((uint64_t)real, 0)), inv_m = finv(mk((uint64_t)m, 0));
     fe inv_2real = finv(mk((uint64_t)(2 * real), 0)), inv_2m = finv(mk((uint64_t)(2 * m), 0));
-    launch::intt({0, c->cv[0], real, nullptr, nullptr, 0, 0, 0, 0}, c->coefP, Cc, 64, lr, inv_real, c->NT, st);
-    launch::intt({0, pri_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefP + 64 * Cc, Cc, 1, lm, inv_m, c->NT, st);
-    launch::lde(c->coefP, Cc, c->lens, 65, c->l, c->log_C, c->log_R, c->NT, st);
-    HIPCHK(hipEventRecord(c->ev[3], st));
+    if (c->early_l) {
+        HIPCHK(hipStreamWaitEvent(st, c->ev[3], 0));       // l, extended on st2 beside the evaluation
+    } else {
+        enqueue_lde_l(c, st);
+        HIPCHK(hipEventRecord(c->ev[3], st));
+    }
     // ---- commit_public_array (poly_commit.h:125-351) ----
     // Slice j < 64 of the public array is cq[j] Q exactly (a product of the
     // same eq factors), so its interpolation and extension are cq[j] times
@@ -1331,6 +1365,11 @@ int ctx_alloc(vg_ctx *c)
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
     for (auto &e : c->gev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto &e : c->ev_leaf) HIPCHK(hipEventCreate(&e));
+    if (c->early_l) {
+        HIPCHK(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_chal, hipEventDisableTiming));
+        HIPCHK(hipEventCreate(&c->ev_l0));
+    }
     // every upload in the order upload_bytes counts them
     uploader U;
     if (!U.init(c->mem, c->up_bytes, c->st)) FAILV(-21, "pinned upload staging failed");
@@ -1944,6 +1983,17 @@ int ctx_create_planned(vg_ctx **out, const vg_circuit_desc *d, void *stream)
         if (c->serial) {
             c->st2 = c->st;
             for (int w = 0; w < vg_ctx::NGS; ++w) c->gst[w] = c->st;
+            // in order on its own streams because of its size: the input
+            // layer's extension still runs beside the evaluation (VIRGO_EARLY_L=0
+            // keeps it in order too)
+            const char *es = getenv("VIRGO_SERIAL"), *el = getenv("VIRGO_EARLY_L");
+            if (c->own_stream && !(es && es[0] == '1') && !(el && el[0] == '0')) {
+                e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
+                if (e == hipSuccess) {
+                    c->own_aux = true;
+                    c->early_l = true;
+                }
+            }
         } else {
             e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
             c->own_aux = true;
@@ -2024,6 +2074,7 @@ int vg_ctx_destroy(vg_ctx *c)
     }
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto &e : c->ev_leaf) if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {c->ev_in, c->ev_chal, c->ev_l0}) if (e) (void)hipEventDestroy(e);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
     if (c->own_aux && c->st2) (void)hipStreamDestroy(c->st2);
     for (int w = 0; w < c->own_gst; ++w) (void)hipStreamDestroy(c->gst[w]);
@@ -2080,7 +2131,9 @@ int vg_finish(vg_ctx *c, uint8_t **transcript, size_t *len, double *stage_ms)
         for (int k = 0; k < 9; ++k) {
             float ms = 0;
             int a = k == 2 && c->serial ? 2 : pairs[k][0];
+            if (c->early_l && k == 3) a = 2;                   // lde_qh starts once the sumcheck is done
             if (pairs[k][0] < 0) (void)hipEventElapsedTime(&ms, c->ev_leaf[0], c->ev_leaf[1]);
+            else if (c->early_l && k == 2) (void)hipEventElapsedTime(&ms, c->ev_l0, c->ev[3]);
             else (void)hipEventElapsedTime(&ms, c->ev[a], c->ev[pairs[k][1]]);
             stage_ms[k] = ms;
         }

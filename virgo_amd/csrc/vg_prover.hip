@@ -566,7 +566,7 @@ struct vg_ctx {
     // a proof in order on its own streams (large input) still runs the
     // extension of its input layer (commit_private_array) on st2 beside the
     // evaluation, whose gathers leave the VALU idle: ev_in (input layer
-    // evaluated, on st), ev_chal (challenges uploaded, on st2), ev_l0 (the
+    // evaluated, on st), ev_chal (challenges uploaded, on st), ev_l0 (the
     // extension's start, on st2); ev[3] marks its end
     bool early_l = false;
     hipEvent_t ev_in = nullptr, ev_chal = nullptr, ev_l0 = nullptr;
@@ -831,14 +831,21 @@ int build_gather(vg_ctx *c)
 
 // ---- evaluate (zk_prover.cpp:54-157): needs no challenge, so it is
 // enqueued before the host draws them ----
-void enqueue_lde_l(vg_ctx *c, hipStream_t st);
+void enqueue_lde_l(vg_ctx *c, hipStream_t st, int part);
 int enqueue_eval(vg_ctx *c)
 {
     hipStream_t st = c->st;
     host_circuit &C = c->C;
     HIPCHK(hipEventRecord(c->ev[0], st));
     launch::eval_input(c->d_u[0], c->cv[0], 1 << C.bl[0], st);
-    if (c->early_l) HIPCHK(hipEventRecord(c->ev_in, st));
+    if (c->early_l) {
+        // the extension of the 64 input slices needs the input layer only:
+        // st2 runs it beside the evaluation
+        HIPCHK(hipEventRecord(c->ev_in, st));
+        HIPCHK(hipStreamWaitEvent(c->st2, c->ev_in, 0));
+        HIPCHK(hipEventRecord(c->ev_l0, c->st2));
+        enqueue_lde_l(c, c->st2, 1);
+    }
     for (int i = 1; i < C.depth; ++i)
         launch::eval_layer({c->d_ty[i], c->d_u[i], c->d_v[i], C.bl[i]}, c->cv[i - 1], c->cv[i], st);
     HIPCHK(hipEventRecord(c->ev[1], st));
@@ -854,18 +861,13 @@ int enqueue_gkr(vg_ctx *c, bool gkr = true)
     hipStream_t st = c->st, st2 = c->st2;
     host_circuit &C = c->C;
     int D = C.depth;
+    HIPCHK(hipMemcpyAsync(c->chal, c->h_chal, sizeof(fe) * c->n_chal, hipMemcpyHostToDevice, st));
     if (c->early_l) {
-        // the input layer's extension needs the input values and the private
-        // masks only: st2 takes it as soon as the input layer is evaluated
-        HIPCHK(hipStreamWaitEvent(st2, c->ev_in, 0));
-        HIPCHK(hipMemcpyAsync(c->chal, c->h_chal, sizeof(fe) * c->n_chal, hipMemcpyHostToDevice, st2));
-        HIPCHK(hipEventRecord(c->ev_chal, st2));
-        HIPCHK(hipEventRecord(c->ev_l0, st2));
-        enqueue_lde_l(c, st2);
+        // the private mask's slice of l, once the challenges are uploaded
+        HIPCHK(hipEventRecord(c->ev_chal, st));
+        HIPCHK(hipStreamWaitEvent(st2, c->ev_chal, 0));
+        enqueue_lde_l(c, st2, 2);
         HIPCHK(hipEventRecord(c->ev[3], st2));
-        HIPCHK(hipStreamWaitEvent(st, c->ev_chal, 0));
-    } else {
-        HIPCHK(hipMemcpyAsync(c->chal, c->h_chal, sizeof(fe) * c->n_chal, hipMemcpyHostToDevice, st));
     }
     HIPCHK(hipMemcpyAsync(c->gsrc, c->h_gsrc, sizeof(uint64_t) * c->n_gather, hipMemcpyHostToDevice, st));
     // every eq half table of the proof, from the uploaded challenges
@@ -916,16 +918,22 @@ int enqueue_gkr(vg_ctx *c, bool gkr = true)
 // the commitment up to its leaf chains: LDEs, public array, FRI folds (ends
 // on ev[5]); on the second stream when the context overlaps its stages
 // ---- commit_private_array (poly_commit.h:42-124): the 64 input slices and
-// the private mask interpolated and extended into l ----
-void enqueue_lde_l(vg_ctx *c, hipStream_t st)
+// the private mask interpolated and extended into l (slice-major, SS values
+// per slice; every row is independent, so the slices may be extended in
+// parts): part 0 all 65 slices, part 1 the 64 input slices (they need the
+// input layer only), part 2 the mask slice (it needs the challenge upload)
+void enqueue_lde_l(vg_ctx *c, hipStream_t st, int part = 0)
 {
     int64_t real = c->real, m = c->m, Cc = c->C_;
     int lr = ilog2(real), lm = ilog2(m);
     fe *pri_d = c->chal + (c->n_chal - 2 * m);
     fe inv_real = finv(mk((uint64_t)real, 0)), inv_m = finv(mk((uint64_t)m, 0));
-    launch::intt({0, c->cv[0], real, nullptr, nullptr, 0, 0, 0, 0}, c->coefP, Cc, 64, lr, inv_real, c->NT, st);
-    launch::intt({0, pri_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefP + 64 * Cc, Cc, 1, lm, inv_m, c->NT, st);
-    launch::lde(c->coefP, Cc, c->lens, 65, c->l, c->log_C, c->log_R, c->NT, st);
+    if (part != 2) launch::intt({0, c->cv[0], real, nullptr, nullptr, 0, 0, 0, 0}, c->coefP, Cc, 64, lr, inv_real, c->NT, st);
+    if (part == 1) launch::lde(c->coefP, Cc, c->lens, 64, c->l, c->log_C, c->log_R, c->NT, st);
+    if (part != 1) launch::intt({0, pri_d, m, nullptr, nullptr, 0, 0, 0, 0}, c->coefP + 64 * Cc, Cc, 1, lm, inv_m, c->NT, st);
+    if (part == 0) launch::lde(c->coefP, Cc, c->lens, 65, c->l, c->log_C, c->log_R, c->NT, st);
+    if (part == 2)
+        launch::lde(c->coefP + 64 * Cc, Cc, c->lens + 64, 1, c->l + 64 * c->SS, c->log_C, c->log_R, c->NT, st);
 }
 int enqueue_commit(vg_ctx *c)
 {
@@ -941,7 +949,7 @@ int enqueue_commit(vg_ctx *c)
     if (c->early_l) {
         HIPCHK(hipStreamWaitEvent(st, c->ev[3], 0));       // l, extended on st2 beside the evaluation
     } else {
-        enqueue_lde_l(c, st);
+        enqueue_lde_l(c, st, 0);
         HIPCHK(hipEventRecord(c->ev[3], st));
     }
     // ---- commit_public_array (poly_commit.h:125-351) ----

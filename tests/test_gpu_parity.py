@@ -206,6 +206,31 @@ def test_rejects_bad_gates_like_the_oracle(ty, u, v, msg):
         O.prove(c)
 
 
+def _relay_circuit(later_bad_wire=False):
+    """Inputs and a first layer of 2^7 gates, then a layer of 4 add gates
+    whose gate 1 is a relay reading u = 10: in range of the previous layer,
+    beyond its own layer's 4 entries (zk_prover.cpp:496-497 would index its
+    beta table out of bounds)."""
+    ni = 1 << 7
+    idx = np.arange(ni, dtype=np.int64)
+    top = (np.zeros(4, np.int32), np.arange(4, dtype=np.int64), np.arange(4, dtype=np.int64))
+    top[0][1], top[1][1], top[2][1] = 4, 10, 0
+    if later_bad_wire:
+        top[1][3] = ni                                    # a later gate wired past the previous layer
+    return W._pack([(np.full(ni, 3, np.int32), idx, np.zeros(ni, np.int64)),
+                    (np.zeros(ni, np.int32), idx, idx), top])
+
+
+def test_rejects_relay_beyond_its_layer():
+    """A relay gate reading past its own layer is refused; the layer's wiring
+    checks run first, so a later out-of-range wire in the same layer is the
+    error reported."""
+    with pytest.raises(V.VirgoError, match="relay gate 1 reads u=10 beyond"):
+        V.Prover(_relay_circuit())
+    with pytest.raises(V.VirgoError, match="gate 3: input index out of bounds"):
+        V.Prover(_relay_circuit(later_bad_wire=True))
+
+
 def test_ntt_tiles_give_the_same_bits():
     """Every NTT schedule the autotuner may pick gives the same transcript:
     the same proofs made in child processes with the tile forced to 2^10 and

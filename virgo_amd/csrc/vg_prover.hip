@@ -37,11 +37,22 @@ namespace {
 // ---------------------------------------------------------------------------
 // circuit, as zk_verifier::read_circuit builds it (zk_verifier.cpp:15-217)
 // ---------------------------------------------------------------------------
+// std::allocator whose resize() leaves new elements uninitialised: the
+// loader writes every gate it keeps, so a layer's arrays are not zeroed
+// first (hundreds of MB for the largest circuits)
+template <class T> struct noinit_alloc : std::allocator<T> {
+    template <class U> struct rebind { using other = noinit_alloc<U>; };
+    noinit_alloc() = default;
+    template <class U> noinit_alloc(const noinit_alloc<U> &) {}
+    template <class U> void construct(U *p) noexcept { ::new ((void *)p) U; }
+    template <class U, class... A> void construct(U *p, A &&...a) { ::new ((void *)p) U(std::forward<A>(a)...); }
+};
+template <class T> using hvec = std::vector<T, noinit_alloc<T>>;
 struct host_circuit {
     int depth = 0;                         // total_depth (layer 0 = inputs)
     std::vector<int> bl;
-    std::vector<std::vector<uint8_t>> ty;
-    std::vector<std::vector<int32_t>> u, v;
+    std::vector<hvec<uint8_t>> ty;
+    std::vector<hvec<int32_t>> u, v;
 };
 
 int bitlen(int64_t x) { int c = 0; while (x) { ++c; x >>= 1; } return c; }
@@ -95,14 +106,24 @@ int build_circuit(const vg_circuit_desc *d, host_circuit &C)
         char b[256];
         auto fail = [&](int rc) { code[i] = rc; err[i] = b; };
         int64_t alloc = n[i] == 1 ? 2 : n_pad[i];
-        int li = i == 1 ? 0 : i;
-        C.ty[li].assign(alloc, 2); C.u[li].assign(alloc, 0); C.v[li].assign(alloc, 0);
-        if (i == 1) { C.ty[1].assign(alloc, 4); C.u[1].assign(alloc, 0); C.v[1].assign(alloc, 0); }
+        if (i == 1) {
+            C.ty[0].assign(alloc, 2); C.u[0].assign(alloc, 0); C.v[0].assign(alloc, 0);
+            C.ty[1].assign(alloc, 4); C.u[1].assign(alloc, 0); C.v[1].assign(alloc, 0);
+        } else {
+            // every element is written below: [0, n) by the gate loop, the
+            // second of a single-gate layer after it
+            C.ty[i].resize(alloc); C.u[i].resize(alloc); C.v[i].resize(alloc);
+        }
         int prev_bl = i == 1 ? 0 : C.bl[i - 1];      // the input layer's gates are checked against 2^0
         const int32_t *dty = d->ty + goff[i];
         const int64_t *du = d->u + goff[i], *dv = d->v + goff[i];
         uint8_t *ty = C.ty[i].data();
         int32_t *u = C.u[i].data(), *v = C.v[i].data();
+        // a direct-relay gate indexes the layer's own beta table with u
+        // (zk_prover.cpp:496-497); the reference reads past it otherwise.
+        // The first such gate is reported once the layer's other checks pass.
+        const int64_t own = (int64_t)1 << C.bl[i];
+        int64_t relay_bad = -1;
         for (int64_t j = 0; j < n[i]; ++j) {
             int t = dty[j];
             int64_t uu = du[j], vv = dv[j];
@@ -126,6 +147,7 @@ int build_circuit(const vg_circuit_desc *d, host_circuit &C)
             if (t < 0 || t > 13 || t == 11) { snprintf(b, sizeof b, "unknown gate type %d", t); return fail(-3); }
             if (i != 1) {
                 ty[j] = (uint8_t)t; u[j] = (int32_t)uu; v[j] = (int32_t)vv;
+                if (t == 4 && uu >= own && relay_bad < 0) relay_bad = j;
             } else {
                 if (t != 2 && t != 3) { snprintf(b, sizeof b, "input layer gates must be type 2 or 3"); return fail(-3); }
                 C.ty[1][j] = 4; C.u[1][j] = (int32_t)j; C.v[1][j] = 0;
@@ -135,13 +157,12 @@ int build_circuit(const vg_circuit_desc *d, host_circuit &C)
         if (i == 1)
             for (int64_t g = n[i]; g < n_pad[i]; ++g) { C.ty[1][g] = 4; C.u[1][g] = (int32_t)g; C.ty[0][g] = 3; C.u[0][g] = 0; }
         if (n[i] == 1) { C.ty[i][1] = 2; C.u[i][1] = 0; C.v[i][1] = 0; }
-        // a direct-relay gate indexes the layer's own beta table with u
-        // (zk_prover.cpp:496-497); the reference reads past it otherwise
-        for (size_t j = 0; j < C.ty[i].size(); ++j)
-            if (C.ty[i][j] == 4 && C.u[i][j] >= (1LL << C.bl[i])) {
-                snprintf(b, sizeof b, "layer %d: relay gate %zu reads u=%d beyond its own layer size", i, j, C.u[i][j]);
-                return fail(-3);
-            }
+        // (the input layer's relay gates read u = j < its size: never out of range)
+        if (relay_bad >= 0) {
+            snprintf(b, sizeof b, "layer %d: relay gate %zu reads u=%d beyond its own layer size", i, (size_t)relay_bad,
+                     u[relay_bad]);
+            return fail(-3);
+        }
     };
     for_layers(1, D + 1, off, layer);
     for (int i = 1; i <= D; ++i)

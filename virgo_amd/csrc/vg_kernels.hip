@@ -1736,14 +1736,19 @@ __global__ void k_fg_ifft_p2m(const launch::fg_layer_dev *D, const fe *plo, cons
 }
 
 // ---------------------------------------------------------------------
-// CSR transposes of one gate layer, built on the device when a context is
+// CSR transposes of every gate layer, built on the device when a context is
 // created (the structure of zk_prover.cpp:420-587 and :845-972): phase 1
 // lists, for every entry x of the previous layer, the gates that add into
 // it through u (a sum gate into every x of its range, its record carrying x
 // in place of v); phase 2 lists by v the gates that add into entry v.
-// Counts by atomics, offsets by a two-level scan, records placed by atomic
-// cursors: the order of records within one entry is arbitrary, which no
-// result depends on (the table builds only add field elements).
+// All layers in six launches (blockIdx.y = layer, or layer and list): the
+// offsets A (nt + 1 words) are zeroed, entry x's records counted into
+// A[x + 2] (the last entry's count is never needed), A scanned inclusively
+// so that A[x + 1] = start(x), and each record placed at atomicAdd(&A[x +
+// 1], 1), which leaves A[x + 1] = end(x) = start(x + 1): A ends as the
+// offsets, A[0] = 0 and A[nt] = the total, with no cursor array.  The order
+// of records within one entry is arbitrary, which no result depends on (the
+// table builds only add field elements).
 // ---------------------------------------------------------------------
 __device__ __forceinline__ void csr_p1_range(int t, int32_t u, int32_t v, int64_t &lo, int64_t &hi)
 {
@@ -1756,25 +1761,34 @@ __device__ __forceinline__ void csr_p1_range(int t, int32_t u, int32_t v, int64_
 }
 __device__ __forceinline__ bool csr_p2_by_v(int t) { return t == 0 || t == 1 || t == 7 || t == 8 || t == 9 || t == 13; }
 
-__global__ void k_csr_count(const uint8_t *ty, const int32_t *u, const int32_t *v, int64_t ng, uint32_t *c1, uint32_t *c2)
+__global__ void k_csr_zero(const launch::csr_layer *L)
 {
+    const launch::csr_layer &d = L[blockIdx.y];
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x <= d.nt; x += (int64_t)gridDim.x * blockDim.x) {
+        d.off1[x] = 0u;
+        d.off2[x] = 0u;
+    }
+}
+__global__ void k_csr_count(const launch::csr_layer *L)
+{
+    const launch::csr_layer &d = L[blockIdx.y];
     int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (g >= ng) return;
-    int t = ty[g];
-    int32_t uu = u[g], vv = v[g];
+    if (g >= d.ng) return;
+    int t = d.ty[g];
+    int32_t uu = d.u[g], vv = d.v[g];
     int64_t lo, hi;
     csr_p1_range(t, uu, vv, lo, hi);
-    for (int64_t x = lo; x < hi; ++x) atomicAdd(&c1[x + 1], 1u);
-    if (csr_p2_by_v(t)) atomicAdd(&c2[vv + 1], 1u);
+    for (int64_t x = lo; x < hi; ++x)
+        if (x + 2 <= d.nt) atomicAdd(&d.off1[x + 2], 1u);
+    if (csr_p2_by_v(t) && (int64_t)vv + 2 <= d.nt) atomicAdd(&d.off2[vv + 2], 1u);
 }
-
-__global__ void k_csr_fill(const uint8_t *ty, const int32_t *u, const int32_t *v, int64_t ng, const uint32_t *o1,
-                           const uint32_t *o2, uint32_t *cur1, uint32_t *cur2, launch::gate_rec *e1, launch::gate_rec *e2)
+__global__ void k_csr_fill(const launch::csr_layer *L)
 {
+    const launch::csr_layer &d = L[blockIdx.y];
     int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (g >= ng) return;
-    int t = ty[g];
-    int32_t uu = u[g], vv = v[g];
+    if (g >= d.ng) return;
+    int t = d.ty[g];
+    int32_t uu = d.u[g], vv = d.v[g];
     launch::gate_rec rec = {(int32_t)g, uu, vv, t};
     int64_t lo, hi;
     csr_p1_range(t, uu, vv, lo, hi);
@@ -1782,24 +1796,25 @@ __global__ void k_csr_fill(const uint8_t *ty, const int32_t *u, const int32_t *v
     for (int64_t x = lo; x < hi; ++x) {
         launch::gate_rec r1 = rec;
         if (range) r1.v = (int32_t)x;
-        e1[o1[x] + atomicAdd(&cur1[x], 1u)] = r1;
+        d.ent1[atomicAdd(&d.off1[x + 1], 1u)] = r1;
     }
-    if (csr_p2_by_v(t)) e2[o2[vv] + atomicAdd(&cur2[vv], 1u)] = rec;
+    if (csr_p2_by_v(t)) d.ent2[atomicAdd(&d.off2[vv + 1], 1u)] = rec;
 }
 
-// inclusive scan of 2048 consecutive values per block; tops[b] = block total
-constexpr int SCAN_T = 256, SCAN_E = 8;
-__global__ void __launch_bounds__(SCAN_T) k_scan_block(uint32_t *a, int64_t n, uint32_t *tops)
+// inclusive scan of the 2048 consecutive words of block blockIdx.x of list
+// blockIdx.y (layer y / 2, off1 or off2); tops[b] = block b's total
+constexpr int SCAN_T = 256, SCAN_E = 8, SCAN_B = SCAN_T * SCAN_E;
+__device__ __forceinline__ void scan_list(const launch::csr_layer &d, int which, uint32_t *&a, int64_t &n, uint32_t *&tops)
 {
-    __shared__ uint32_t sh[SCAN_T];
-    int64_t base = blockIdx.x * (int64_t)(SCAN_T * SCAN_E) + threadIdx.x * SCAN_E;
-    uint32_t x[SCAN_E], sum = 0;
-#pragma unroll
-    for (int k = 0; k < SCAN_E; ++k) {
-        sum += base + k < n ? a[base + k] : 0u;
-        x[k] = sum;
-    }
-    sh[threadIdx.x] = sum;
+    a = which ? d.off2 : d.off1;
+    tops = which ? d.tops2 : d.tops1;
+    n = d.nt + 1;
+}
+// block-wide inclusive scan of one value per thread; returns the thread's
+// inclusive prefix, *total the block's sum
+__device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t *sh, uint32_t *total)
+{
+    sh[threadIdx.x] = v;
     __syncthreads();
     for (int off = 1; off < SCAN_T; off <<= 1) {
         uint32_t y = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0u;
@@ -1807,19 +1822,69 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_block(uint32_t *a, int64_t n, u
         sh[threadIdx.x] += y;
         __syncthreads();
     }
-    uint32_t pre = threadIdx.x ? sh[threadIdx.x - 1] : 0u;
+    uint32_t r = sh[threadIdx.x];
+    *total = sh[SCAN_T - 1];
+    __syncthreads();
+    return r;
+}
+__global__ void __launch_bounds__(SCAN_T) k_scan_block(const launch::csr_layer *L)
+{
+    __shared__ uint32_t sh[SCAN_T];
+    uint32_t *a, *tops;
+    int64_t n;
+    scan_list(L[blockIdx.y >> 1], blockIdx.y & 1, a, n, tops);
+    int64_t b0 = blockIdx.x * (int64_t)SCAN_B;
+    if (b0 >= n) return;                       // the same for the whole block
+    int64_t base = b0 + threadIdx.x * SCAN_E;
+    uint32_t x[SCAN_E], sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_E; ++k) {
+        sum += base + k < n ? a[base + k] : 0u;
+        x[k] = sum;
+    }
+    uint32_t tot;
+    uint32_t pre = block_scan_u32(sum, sh, &tot) - sum;
 #pragma unroll
     for (int k = 0; k < SCAN_E; ++k)
         if (base + k < n) a[base + k] = x[k] + pre;
-    if (tops && threadIdx.x == SCAN_T - 1) tops[blockIdx.x] = sh[SCAN_T - 1];
+    if (threadIdx.x == 0) tops[blockIdx.x] = tot;
+}
+// inclusive scan of one list's block totals, one block per list, 2048 at a
+// time with the running total carried
+__global__ void __launch_bounds__(SCAN_T) k_scan_tops(const launch::csr_layer *L)
+{
+    __shared__ uint32_t sh[SCAN_T];
+    uint32_t *a, *tops;
+    int64_t n;
+    scan_list(L[blockIdx.x >> 1], blockIdx.x & 1, a, n, tops);
+    int64_t nb = (n + SCAN_B - 1) / SCAN_B;
+    uint32_t carry = 0;
+    for (int64_t c0 = 0; c0 < nb; c0 += SCAN_B) {
+        int64_t base = c0 + threadIdx.x * SCAN_E;
+        uint32_t x[SCAN_E], sum = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_E; ++k) {
+            sum += base + k < nb ? tops[base + k] : 0u;
+            x[k] = sum;
+        }
+        uint32_t tot;
+        uint32_t pre = block_scan_u32(sum, sh, &tot) - sum + carry;
+#pragma unroll
+        for (int k = 0; k < SCAN_E; ++k)
+            if (base + k < nb) tops[base + k] = x[k] + pre;
+        carry += tot;
+    }
 }
 // adds the inclusive prefix of the previous blocks' totals
-__global__ void k_scan_add(uint32_t *a, int64_t n, const uint32_t *tops)
+__global__ void k_scan_add(const launch::csr_layer *L)
 {
-    if (blockIdx.x == 0) return;
+    uint32_t *a, *tops;
+    int64_t n;
+    scan_list(L[blockIdx.y >> 1], blockIdx.y & 1, a, n, tops);
+    int64_t base = blockIdx.x * (int64_t)SCAN_B;
+    if (blockIdx.x == 0 || base >= n) return;
     uint32_t add = tops[blockIdx.x - 1];
-    int64_t base = blockIdx.x * (int64_t)(SCAN_T * SCAN_E);
-    for (int k = threadIdx.x; k < SCAN_T * SCAN_E; k += SCAN_T)
+    for (int k = threadIdx.x; k < SCAN_B; k += blockDim.x)
         if (base + k < n) a[base + k] += add;
 }
 
@@ -1836,39 +1901,20 @@ void zero(void *p, size_t bytes, hipStream_t s)
     hipLaunchKernelGGL(k_zero, dim3(g), dim3(TPB), 0, s, (uint32_t *)p, n);
 }
 // ---- CSR transposes ----
-int64_t scan_scratch_len(int64_t n)
+int64_t csr_tops_len(int64_t nt) { return (nt + 1 + SCAN_B - 1) / SCAN_B; }
+void csr_build_multi(const csr_layer *L, int n, int64_t max_ng, int64_t max_nt, hipStream_t s)
 {
-    const int64_t B = SCAN_T * SCAN_E;
-    int64_t tot = 0;
-    while (n > B) { n = (n + B - 1) / B; tot += n; }
-    return tot + 1;
-}
-// inclusive prefix sum of a[0..n) in place; scratch: scan_scratch_len(n)
-void scan_u32(uint32_t *a, int64_t n, uint32_t *scratch, hipStream_t s)
-{
-    const int64_t B = SCAN_T * SCAN_E;
-    int64_t nb = (n + B - 1) / B;
-    if (nb <= 1) {
-        hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(SCAN_T), 0, s, a, n, (uint32_t *)nullptr);
-        return;
-    }
-    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(SCAN_T), 0, s, a, n, scratch);
-    scan_u32(scratch, nb, scratch + nb, s);
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SCAN_T), 0, s, a, n, (const uint32_t *)scratch);
-}
-void csr_build(gates_dev g, int64_t ng, int64_t nt, uint32_t *off1, uint32_t *off2, gate_rec *ent1, gate_rec *ent2,
-               uint32_t *cur, uint32_t *scratch, hipStream_t s)
-{
-    zero(off1, 4 * (size_t)(nt + 1), s);
-    zero(off2, 4 * (size_t)(nt + 1), s);
-    zero(cur, 8 * (size_t)nt, s);
-    if (ng <= 0) return;
-    unsigned nb = nblk(ng);
-    hipLaunchKernelGGL(k_csr_count, dim3(nb), dim3(TPB), 0, s, g.ty, g.u, g.v, ng, off1, off2);
-    scan_u32(off1, nt + 1, scratch, s);
-    scan_u32(off2, nt + 1, scratch, s);
-    hipLaunchKernelGGL(k_csr_fill, dim3(nb), dim3(TPB), 0, s, g.ty, g.u, g.v, ng, (const uint32_t *)off1,
-                       (const uint32_t *)off2, cur, cur + nt, ent1, ent2);
+    if (n <= 0) return;
+    unsigned zb = nblk(max_nt + 1);
+    if (zb > 4096) zb = 4096;
+    hipLaunchKernelGGL(k_csr_zero, dim3(zb, n), dim3(TPB), 0, s, L);
+    unsigned gb = nblk(max_ng > 0 ? max_ng : 1);
+    hipLaunchKernelGGL(k_csr_count, dim3(gb, n), dim3(TPB), 0, s, L);
+    unsigned sb = (unsigned)csr_tops_len(max_nt);
+    hipLaunchKernelGGL(k_scan_block, dim3(sb, 2 * n), dim3(SCAN_T), 0, s, L);
+    hipLaunchKernelGGL(k_scan_tops, dim3(2 * n), dim3(SCAN_T), 0, s, L);
+    hipLaunchKernelGGL(k_scan_add, dim3(sb, 2 * n), dim3(SCAN_T), 0, s, L);
+    hipLaunchKernelGGL(k_csr_fill, dim3(gb, n), dim3(TPB), 0, s, L);
 }
 void eval_input(const int32_t *u, fe *out, int n, hipStream_t s)
 {

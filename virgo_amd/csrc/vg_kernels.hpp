@@ -49,14 +49,22 @@ struct beta_u { const fe *f, *s; int fh; };
 struct __align__(16) gate_rec {
     int32_t g, u, v, ty;
 };
-// CSR transposes of one gate layer (ng gates over nt = 2^bl(i-1) table
-// entries): off1 / off2 (nt + 1 offsets each) and ent1 / ent2 by u and by v
-// (record counts as the host's plan counted them).  cur: 2 nt words,
-// scratch: scan_scratch_len(nt + 1) words.  Enqueued on s.
-int64_t scan_scratch_len(int64_t n);
-void scan_u32(uint32_t *a, int64_t n, uint32_t *scratch, hipStream_t s);
-void csr_build(gates_dev g, int64_t ng, int64_t nt, uint32_t *off1, uint32_t *off2, gate_rec *ent1, gate_rec *ent2,
-               uint32_t *cur, uint32_t *scratch, hipStream_t s);
+// CSR transposes of gate layers (ng gates over nt = 2^bl(i-1) table
+// entries each): off1 / off2 (nt + 1 offsets each) and ent1 / ent2 by u and
+// by v (record counts as the host's plan counted them); tops1 / tops2:
+// csr_tops_len(nt) words of scan scratch each.  csr_build_multi builds the
+// n layers of the device array L in six launches on s (max_ng, max_nt: the
+// largest layer's).
+struct csr_layer {
+    const uint8_t *ty;
+    const int32_t *u, *v;
+    int64_t ng, nt;
+    uint32_t *off1, *off2;
+    gate_rec *ent1, *ent2;
+    uint32_t *tops1, *tops2;
+};
+int64_t csr_tops_len(int64_t nt);
+void csr_build_multi(const csr_layer *L, int n, int64_t max_ng, int64_t max_nt, hipStream_t s);
 // One GKR layer of the main prover on the device (all pointers fixed per
 // context).  Batches of layers with equal table size run in one launch.
 struct layer_dev {

@@ -527,7 +527,8 @@ struct vg_ctx {
     std::vector<char> crowd1, crowd2;         // a gather_fold tile holds more than GF_MAX_FANIN records per entry
     std::vector<int> kl1, kl2;                // log2 lanes per entry for a flat build (densest tile's records per entry)
     std::vector<std::vector<int32_t>> h_z2;
-    uint32_t *csr_cur = nullptr, *csr_scan = nullptr;   // scratch of the device CSR build
+    launch::csr_layer *d_csr = nullptr;                 // descriptors of the device CSR build, one per gate layer
+    std::vector<uint32_t *> tops1, tops2;               // its scan scratch per layer
     std::vector<uint32_t *> off1, off2;
     std::vector<launch::gate_rec *> ent1, ent2;
     std::vector<int32_t *> z2;
@@ -644,18 +645,18 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     c->d_inst = carve<launch::fold_inst>(base, off, 2 * D);
     c->off1.assign(D, nullptr); c->off2.assign(D, nullptr);
     c->ent1.assign(D, nullptr); c->ent2.assign(D, nullptr); c->z2.assign(D, nullptr);
-    int64_t max_nt = 1;
+    c->tops1.assign(D, nullptr); c->tops2.assign(D, nullptr);
     for (int i = 1; i < D; ++i) {
         int64_t nt = (int64_t)1 << C.bl[i - 1];
-        if (nt > max_nt) max_nt = nt;
         c->off1[i] = carve<uint32_t>(base, off, nt + 1);
         c->off2[i] = carve<uint32_t>(base, off, nt + 1);
         c->ent1[i] = carve<launch::gate_rec>(base, off, c->n_ent1[i] + 1);
         c->ent2[i] = carve<launch::gate_rec>(base, off, c->n_ent2[i] + 1);
         c->z2[i] = carve<int32_t>(base, off, (int64_t)c->h_z2[i].size() + 1);
+        c->tops1[i] = carve<uint32_t>(base, off, launch::csr_tops_len(nt));
+        c->tops2[i] = carve<uint32_t>(base, off, launch::csr_tops_len(nt));
     }
-    c->csr_cur = carve<uint32_t>(base, off, 2 * max_nt);
-    c->csr_scan = carve<uint32_t>(base, off, launch::scan_scratch_len(max_nt + 1));
+    c->d_csr = carve<launch::csr_layer>(base, off, D);
     c->ht_stride = (int64_t)1 << ((maxbl >> 1) + 1);
     c->ht = carve<fe>(base, off, (int64_t)D * 8 * c->ht_stride + 2 * ((int64_t)1 << (C.bl[0] / 2 + 1)));
     c->hdesc = carve<launch::half_desc>(base, off, (int64_t)D * 8 + 2);
@@ -1184,7 +1185,7 @@ void assemble(vg_ctx *c, tbuf &T)
 
 // Plan of the CSR transposes of one gate layer (the structure of
 // zk_prover.cpp:420-587 and :845-972; the transposes themselves are built on
-// the device, launch::csr_build): phase 1 lists, for every entry u of the
+// the device, launch::csr_build_multi): phase 1 lists, for every entry u of the
 // previous layer, the gates that add into it through u (a sum gate into
 // every u of its range); phase 2 lists by v the gates that add into entry v,
 // and separately the gates that always add into entry 0 (z2, kept here).
@@ -1261,6 +1262,7 @@ int64_t upload_bytes(const vg_ctx *c)
     }
     if (c->n_slots) t += sb((int64_t)sizeof(int) * c->n_slots) + sb((int64_t)sizeof(int64_t) * c->n_slots);
     for (int i = 1; i < C.depth; ++i) t += sb(4 * (int64_t)c->h_z2[i].size());
+    t += sb((int64_t)sizeof(launch::csr_layer) * (C.depth - 1));
     t += sb((int64_t)sizeof(launch::half_desc) * (8 * (C.depth - 1) + 3));
     int nd = C.depth - 1;
     t += sb((int64_t)sizeof(launch::layer_dev) * (nd > 0 ? nd : 1));
@@ -1416,11 +1418,23 @@ int ctx_alloc(vg_ctx *c)
         HIPCHK(U.put(c->slot_nb, c->h_slot_nb.data(), (int64_t)sizeof(int) * c->n_slots));
         HIPCHK(U.put(c->slot_dst, c->h_slot_dst.data(), (int64_t)sizeof(int64_t) * c->n_slots));
     }
-    for (int i = 1; i < C.depth; ++i) {
-        HIPCHK(U.put(c->z2[i], c->h_z2[i].data(), 4 * (int64_t)c->h_z2[i].size()));
-        gates_dev g = {c->d_ty[i], c->d_u[i], c->d_v[i], C.bl[i]};
-        launch::csr_build(g, (int64_t)C.ty[i].size(), (int64_t)1 << C.bl[i - 1], c->off1[i], c->off2[i], c->ent1[i],
-                          c->ent2[i], c->csr_cur, c->csr_scan, c->st);
+    {
+        // the CSR transposes of every gate layer, in six launches
+        std::vector<launch::csr_layer> CL;
+        int64_t max_ng = 0, max_nt = 0;
+        for (int i = 1; i < C.depth; ++i) {
+            HIPCHK(U.put(c->z2[i], c->h_z2[i].data(), 4 * (int64_t)c->h_z2[i].size()));
+            launch::csr_layer d;
+            d.ty = c->d_ty[i]; d.u = c->d_u[i]; d.v = c->d_v[i];
+            d.ng = (int64_t)C.ty[i].size(); d.nt = (int64_t)1 << C.bl[i - 1];
+            d.off1 = c->off1[i]; d.off2 = c->off2[i]; d.ent1 = c->ent1[i]; d.ent2 = c->ent2[i];
+            d.tops1 = c->tops1[i]; d.tops2 = c->tops2[i];
+            max_ng = std::max(max_ng, d.ng);
+            max_nt = std::max(max_nt, d.nt);
+            CL.push_back(d);
+        }
+        HIPCHK(U.put(c->d_csr, CL.data(), (int64_t)sizeof(launch::csr_layer) * (int64_t)CL.size()));
+        launch::csr_build_multi(c->d_csr, (int)CL.size(), max_ng, max_nt, c->st);
     }
     {
         // eq half tables of every layer, computed in one launch per proof

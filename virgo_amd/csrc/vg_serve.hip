@@ -222,11 +222,11 @@ int vg_server_create(vg_server **out, int64_t device_budget, int64_t pinned_budg
         // its pages are mapped on first touch, and a proof packing its
         // uploads into an untouched range wrote at about 2 GB/s instead of
         // about 20
-        const int64_t pg = 4096;
-        const int nt = 8;
+        constexpr int64_t pg = 4096;
+        constexpr int nt = 8;
         std::vector<std::thread> th;
         for (int t = 0; t < nt; ++t)
-            th.emplace_back([s, t, pg, nt] {
+            th.emplace_back([s, t] {
                 for (int64_t o = (int64_t)t * pg; o < s->hsize; o += (int64_t)nt * pg) s->hbase[o] = 0;
             });
         for (auto &x : th) x.join();

@@ -418,6 +418,7 @@ int fg_alloc(fg_ctx *c, mem_source *mem)
     if (ok && !c->h_slot_nb.empty())
         ok = U.put(c->slot_nb, c->h_slot_nb.data(), (int64_t)sizeof(int) * (int64_t)c->h_slot_nb.size()) == hipSuccess &&
              U.put(c->slot_dst, c->h_slot_dst.data(), (int64_t)sizeof(int64_t) * (int64_t)c->h_slot_dst.size()) == hipSuccess;
+    ok = ok && U.flush() == hipSuccess;
     if (!ok || hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess)
         FG_FAIL(-20, "fft_gkr: setup copies failed");
     return 0;

@@ -77,11 +77,17 @@ mem_source *hip_mem();
 // for whatever work of other proofs is queued ahead of it.  The staging
 // stays part of the lease until the context is destroyed (after its stream
 // is drained).  stage_bytes(n) is what put() of n bytes takes from it.
+// Staged puts whose destinations lie at the same distances apart as their
+// staging (256-byte aligned blocks carved back to back, like the staging)
+// become one copy; flush() enqueues the pending copy and must come before
+// any launch that reads what was put.
 // ---------------------------------------------------------------------------
 struct uploader {
     uint8_t *stage = nullptr;
     int64_t cap = 0, off = 0;
     hipStream_t st = nullptr;
+    uint8_t *run_dst = nullptr;                  // pending copy: staging [run_off, run_end) to run_dst
+    int64_t run_off = 0, run_end = 0;
     static int64_t stage_bytes(int64_t n) { return (n + 255) & ~(int64_t)255; }
     // takes `bytes` of staging from m when it stages; false: no memory
     bool init(mem_source *m, int64_t bytes, hipStream_t s)
@@ -97,9 +103,19 @@ struct uploader {
         if (n <= 0) return hipSuccess;
         if (!stage) return hipMemcpy(dst, src, (size_t)n, hipMemcpyHostToDevice);
         if (off + n > cap) return hipErrorInvalidValue;      // more than the plan counted
+        hipError_t e = hipSuccess;
+        if (run_dst && (uint8_t *)dst != run_dst + (off - run_off)) e = flush();
+        if (!run_dst) { run_dst = (uint8_t *)dst; run_off = off; }
         memcpy(stage + off, src, (size_t)n);
-        hipError_t e = hipMemcpyAsync(dst, stage + off, (size_t)n, hipMemcpyHostToDevice, st);
+        run_end = off + n;                       // the padding after the last block is not copied
         off += stage_bytes(n);
+        return e;
+    }
+    hipError_t flush()
+    {
+        if (!run_dst) return hipSuccess;
+        hipError_t e = hipMemcpyAsync(run_dst, stage + run_off, (size_t)(run_end - run_off), hipMemcpyHostToDevice, st);
+        run_dst = nullptr;
         return e;
     }
 };

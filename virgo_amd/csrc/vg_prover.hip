@@ -615,6 +615,8 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     int D = C.depth;
     c->d_ty.assign(D, nullptr); c->d_u.assign(D, nullptr); c->d_v.assign(D, nullptr); c->cv.assign(D, nullptr);
     int maxbl = 0;
+    // the gate arrays back to back in the order ctx_alloc uploads them, so
+    // that a staged upload copies them all at once
     for (int i = 0; i < D; ++i) {
         int64_t n = (int64_t)1 << C.bl[i];
         if (C.bl[i] > maxbl) maxbl = C.bl[i];
@@ -623,8 +625,8 @@ int64_t layout(vg_ctx *c, uint8_t *base)
             c->d_ty[i] = carve<uint8_t>(base, off, n);
             c->d_v[i] = carve<int32_t>(base, off, n);
         }
-        c->cv[i] = carve<fe>(base, off, n);
     }
+    for (int i = 0; i < D; ++i) c->cv[i] = carve<fe>(base, off, (int64_t)1 << C.bl[i]);
     c->chal = carve<fe>(base, off, c->n_chal);
     c->amb_l.assign(D, nullptr); c->aVb_l.assign(D, nullptr); c->part_l.assign(D, nullptr);
     c->ws_l.assign(6 * D, nullptr);
@@ -1434,6 +1436,7 @@ int ctx_alloc(vg_ctx *c)
             CL.push_back(d);
         }
         HIPCHK(U.put(c->d_csr, CL.data(), (int64_t)sizeof(launch::csr_layer) * (int64_t)CL.size()));
+        HIPCHK(U.flush());
         launch::csr_build_multi(c->d_csr, (int)CL.size(), max_ng, max_nt, c->st);
     }
     {
@@ -1531,6 +1534,7 @@ int ctx_alloc(vg_ctx *c)
         fe w10 = x, y = fone();
         for (int64_t j = 0; j < (SS >> 10) + 1; ++j) { T12[1024 + j] = y; y = mul(y, w10); }
         HIPCHK(U.put(c->T12, T12.data(), (int64_t)sizeof(fe) * (int64_t)T12.size()));
+        HIPCHK(U.flush());                       // the last upload
         launch::twiddles(c->tw, SS, c->T12, c->T12 + 1024, c->st);
         launch::ntt_tables(c->ntW, c->ntWi, c->log_C + 1, c->ntTA, c->ntTB, c->log_R, c->NT.th, c->NT.tb, c->tw, SS, c->st);
     }

@@ -114,7 +114,7 @@ def _py_root(leaves, h):
 
 
 @pytest.mark.parametrize("log_n,hash,rate_log", [(18, "sha3", RATE_LOG), (18, "sha256", RATE_LOG), (20, "sha256", RATE_LOG),
-                                                  (24, "sha3", RATE_LOG), (16, "sha3", 5)])
+                                                  (22, "sha256", RATE_LOG), (24, "sha3", RATE_LOG), (16, "sha3", 5)])
 def test_microset_stages(log_n, hash, rate_log):
     """vg_microset (one call, device-resident, timed per stage) against an
     independent route: every FRI level's codeword is the transform of the

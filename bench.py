@@ -340,9 +340,12 @@ def serving(args):
     offered rate, through the library's proof server: each request is
     admitted as a slot and its lease of the HBM budget free up, with every
     admitted proof's state resident on the device until its transcript is on
-    the host.  Latency = submit to transcript, measured in the library.  Each
-    transcript is compared with the same circuit proved alone and checked by
-    the device verifier."""
+    the host.  Latency = submit to transcript, measured in the library.  Then
+    every request again through the same server, one at a time: its alone
+    latency (the floor of its latency under this server) and, per request,
+    the slowdown latency / alone latency.  Each transcript is compared with
+    the same circuit alone (through the server and on a context of its own)
+    and checked by the device verifier."""
     import numpy as np
     import virgo_amd as V
     from virgo_amd import workloads as W
@@ -362,10 +365,18 @@ def serving(args):
         ids.append(srv.submit(c))
     res = [srv.wait(i) for i in ids]
     st = srv.stats()
+    # the same requests through the same server one at a time: each one's
+    # latency with nothing else on the device (its floor under this server)
+    alone = []
+    for _, c in jobs:
+        tr, t = srv.wait(srv.submit(c))
+        alone.append((tr, t["latency"]))
     srv.close()
     lat = np.array([t["latency"] for _, t in res])
+    lat_alone = np.array([a for _, a in alone])
+    slow = lat / lat_alone
     t_end = max(ts + t["latency"] * 1e-3 for ts, (_, t) in zip(t_sub, res))
-    same, accepted = True, True
+    same, accepted = all(a == tr for (a, _), (tr, _) in zip(alone, res)), True
     for (_, c), (tr, _) in zip(jobs, res):
         p = V.Prover(c)
         same &= p.prove() == tr
@@ -376,7 +387,12 @@ def serving(args):
                         "arrivals, seed 13" % len(jobs),
             "offered_per_s": args.serve_rate, "proofs_per_s": round(len(jobs) / (t_end - t0), 1),
             "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
-            "mean_ms": round(float(lat.mean()), 2), "mean_queued_ms": mean("queued"), "mean_plan_ms": mean("plan"),
+            "mean_ms": round(float(lat.mean()), 2),
+            "alone_p50_ms": round(float(np.percentile(lat_alone, 50)), 2),
+            "alone_p99_ms": round(float(np.percentile(lat_alone, 99)), 2),
+            "slowdown_p50": round(float(np.percentile(slow, 50)), 2),
+            "slowdown_p99": round(float(np.percentile(slow, 99)), 2),
+            "mean_queued_ms": mean("queued"), "mean_plan_ms": mean("plan"),
             "mean_prove_ms": mean("prove"), "slots": args.serve_slots, "budget_gb": args.serve_budget_gb,
             "peak_admitted": st["peak_admitted"], "peak_device_gb": round(st["device_peak"] / 1e9, 1),
             "bit_exact_vs_alone": bool(same), "all_accepted": bool(accepted)}

@@ -6,6 +6,7 @@
 #include <vector>
 #include <hip/hip_runtime.h>
 #include "vg_field.hpp"
+#include "vg_pool.hpp"
 
 namespace vg {
 
@@ -106,7 +107,7 @@ struct uploader {
         hipError_t e = hipSuccess;
         if (run_dst && (uint8_t *)dst != run_dst + (off - run_off)) e = flush();
         if (!run_dst) { run_dst = (uint8_t *)dst; run_off = off; }
-        memcpy(stage + off, src, (size_t)n);
+        parallel_copy(stage + off, src, (size_t)n);   // large gate arrays on the host pool
         run_end = off + n;                       // the padding after the last block is not copied
         off += stage_bytes(n);
         return e;

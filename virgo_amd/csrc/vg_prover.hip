@@ -23,6 +23,7 @@
 #include "vg_host.hpp"
 #include "vg_fftgkr.hpp"
 #include "vg_internal.hpp"
+#include "vg_pool.hpp"
 #include "../../include/virgo.h"
 
 using namespace vg;
@@ -37,44 +38,49 @@ namespace {
 // ---------------------------------------------------------------------------
 // circuit, as zk_verifier::read_circuit builds it (zk_verifier.cpp:15-217)
 // ---------------------------------------------------------------------------
-// std::allocator whose resize() leaves new elements uninitialised: the
+// std::allocator whose resize() leaves new elements uninitialised (the
 // loader writes every gate it keeps, so a layer's arrays are not zeroed
-// first (hundreds of MB for the largest circuits)
+// first: hundreds of MB for the largest circuits) and whose large blocks
+// come from the process's recycled host blocks (no page faults for a
+// circuit loaded after another of similar size)
 template <class T> struct noinit_alloc : std::allocator<T> {
+    using value_type = T;
     template <class U> struct rebind { using other = noinit_alloc<U>; };
     noinit_alloc() = default;
     template <class U> noinit_alloc(const noinit_alloc<U> &) {}
+    T *allocate(size_t n) { return (T *)host_take(n * sizeof(T)); }
+    void deallocate(T *p, size_t n) { host_give(p, n * sizeof(T)); }
     template <class U> void construct(U *p) noexcept { ::new ((void *)p) U; }
     template <class U, class... A> void construct(U *p, A &&...a) { ::new ((void *)p) U(std::forward<A>(a)...); }
 };
+template <class T> bool operator==(const noinit_alloc<T> &, const noinit_alloc<T> &) { return true; }
+template <class T> bool operator!=(const noinit_alloc<T> &, const noinit_alloc<T> &) { return false; }
 template <class T> using hvec = std::vector<T, noinit_alloc<T>>;
+// The CSR transposes of a gate layer, as counted while the layer is loaded:
+// records by u (phase 1) and by v (phase 2), the densest gather tile's
+// records, and the gates phase 2 adds into entry 0 (types 5, 12, 6, 10), in
+// gate order.
+struct wiring_plan {
+    int64_t e1 = 0, e2 = 0, m1 = 0, m2 = 0;
+    std::vector<int32_t> z2;
+};
 struct host_circuit {
     int depth = 0;                         // total_depth (layer 0 = inputs)
     std::vector<int> bl;
     std::vector<hvec<uint8_t>> ty;
     std::vector<hvec<int32_t>> u, v;
+    std::vector<wiring_plan> W;            // gate layers 1 .. depth-1
 };
 
 int bitlen(int64_t x) { int c = 0; while (x) { ++c; x >>= 1; } return c; }
+int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
 
-// Runs fn(i) for i in [lo, hi) on up to 8 host threads when the work is
-// large (a proof's per-layer planning), else in order on this thread.
-void for_layers(int lo, int hi, int64_t work, const std::function<void(int)> &fn)
-{
-    int nth = (int)std::min<int64_t>({8, (int64_t)std::thread::hardware_concurrency(), (int64_t)(hi - lo)});
-    if (work < (1 << 20) || nth < 2) {
-        for (int i = lo; i < hi; ++i) fn(i);
-        return;
-    }
-    std::atomic<int> next(lo);
-    std::vector<std::thread> th;
-    for (int k = 0; k < nth; ++k)
-        th.emplace_back([&] {
-            for (int i = next++; i < hi; i = next++) fn(i);
-        });
-    for (auto &t : th) t.join();
-}
-
+// Loads a circuit as zk_verifier::read_circuit builds it (padding of the
+// input layer, the relay layer above it, bit lengths), checks every gate,
+// and counts each gate layer's wiring records for the CSR plan, in one pass
+// over the gates.  The pass is split into chunks of 2^16 gates run on the
+// host pool; the first error in file order (layer, then gate) is reported,
+// a layer's relay check only when its other checks pass, as before.
 int build_circuit(const vg_circuit_desc *d, host_circuit &C)
 {
     int D = d->d;
@@ -84,8 +90,7 @@ int build_circuit(const vg_circuit_desc *d, host_circuit &C)
     C.ty.assign(D + 1, {});
     C.u.assign(D + 1, {});
     C.v.assign(D + 1, {});
-    // sizes of every file layer first, then the gates of all layers in
-    // parallel; the first error in file order is reported
+    C.W.assign(D + 1, {});
     std::vector<int64_t> n(D + 1), n_pad(D + 1), goff(D + 1);
     int64_t off = 0;
     for (int i = 1; i <= D; ++i) {
@@ -100,73 +105,140 @@ int build_circuit(const vg_circuit_desc *d, host_circuit &C)
         C.bl[i] = n_pad[i] == 1 ? 1 : bitlen(n_pad[i] - 1);
     }
     C.bl[0] = C.bl[1];
-    std::vector<std::string> err(D + 1);
-    std::vector<int> code(D + 1, 0);
-    auto layer = [&](int i) {
-        char b[256];
-        auto fail = [&](int rc) { code[i] = rc; err[i] = b; };
+    for (int i = 1; i <= D; ++i) {
         int64_t alloc = n[i] == 1 ? 2 : n_pad[i];
-        if (i == 1) {
-            C.ty[0].assign(alloc, 2); C.u[0].assign(alloc, 0); C.v[0].assign(alloc, 0);
-            C.ty[1].assign(alloc, 4); C.u[1].assign(alloc, 0); C.v[1].assign(alloc, 0);
-        } else {
-            // every element is written below: [0, n) by the gate loop, the
-            // second of a single-gate layer after it
-            C.ty[i].resize(alloc); C.u[i].resize(alloc); C.v[i].resize(alloc);
+        C.ty[i].resize(alloc); C.u[i].resize(alloc); C.v[i].resize(alloc);
+        if (i == 1) { C.ty[0].resize(alloc); C.u[0].resize(alloc); C.v[0].resize(alloc); }
+    }
+    // work items: (layer, chunk) over each layer's loaded range (the input
+    // layer's padding included)
+    constexpr int64_t CH = (int64_t)1 << 16;
+    struct item {
+        int layer;
+        int64_t lo, hi;
+        int code = 0;
+        int64_t bad = -1;             // gate of the first failed check
+        std::string err;
+        int64_t relay_bad = -1;
+        int64_t e1 = 0, e2 = 0;
+        std::vector<int64_t> t1, t2;  // records per gather tile
+        std::vector<int32_t> z2;
+    };
+    std::vector<item> items;
+    for (int i = 1; i <= D; ++i) {
+        int64_t len = i == 1 ? n_pad[1] : n[i];
+        for (int64_t lo = 0; lo < len; lo += CH) {
+            item it;
+            it.layer = i; it.lo = lo; it.hi = std::min(len, lo + CH);
+            items.push_back(std::move(it));
         }
-        int prev_bl = i == 1 ? 0 : C.bl[i - 1];      // the input layer's gates are checked against 2^0
+    }
+    auto chunk = [&](int64_t k) {
+        item &it = items[k];
+        const int i = it.layer;
+        const int tsh = ilog2(launch::gf_tile(C.bl[i - 1]));
+        const int64_t ntile = std::max<int64_t>(1, ((int64_t)1 << C.bl[i - 1]) >> tsh);
+        it.t1.assign(ntile, 0); it.t2.assign(ntile, 0);
+        int64_t *t1 = it.t1.data(), *t2 = it.t2.data();
+        int64_t e1 = 0, e2 = 0;
+        // the CSR records of one stored gate (zk_prover.cpp:375-592, 789-983)
+        auto count = [&](int64_t g, int t, int64_t uu, int64_t vv) {
+            int64_t lo = 0, hi = 0;
+            switch (t) {
+            case 0: case 1: case 4: case 6: case 7: case 8: case 9: case 10: case 13: lo = uu; hi = uu + 1; break;
+            case 5: lo = uu; hi = vv; break;
+            case 12: lo = uu; hi = vv + 1; break;
+            default: break;
+            }
+            if (hi == lo + 1) t1[lo >> tsh]++;
+            else
+                for (int64_t x = lo; x < hi;) {
+                    int64_t q = x >> tsh, end = std::min((q + 1) << tsh, hi);
+                    t1[q] += end - x;
+                    x = end;
+                }
+            e1 += hi > lo ? hi - lo : 0;
+            if (t == 0 || t == 1 || t == 7 || t == 8 || t == 9 || t == 13) { t2[vv >> tsh]++; ++e2; }
+            if (t == 5 || t == 12 || t == 6 || t == 10) it.z2.push_back((int32_t)g);
+        };
+        char b[256];
+        auto fail = [&](int64_t j) { it.code = -3; it.bad = j; it.err = b; };
+        const int prev_bl = i == 1 ? 0 : C.bl[i - 1];   // the input layer's gates are checked against 2^0
+        const int64_t lim = (int64_t)1 << prev_bl;
+        const int64_t own = (int64_t)1 << C.bl[i];
         const int32_t *dty = d->ty + goff[i];
         const int64_t *du = d->u + goff[i], *dv = d->v + goff[i];
         uint8_t *ty = C.ty[i].data();
         int32_t *u = C.u[i].data(), *v = C.v[i].data();
-        // a direct-relay gate indexes the layer's own beta table with u
-        // (zk_prover.cpp:496-497); the reference reads past it otherwise.
-        // The first such gate is reported once the layer's other checks pass.
-        const int64_t own = (int64_t)1 << C.bl[i];
-        int64_t relay_bad = -1;
-        for (int64_t j = 0; j < n[i]; ++j) {
+        const int64_t end = std::min(it.hi, n[i]);
+        for (int64_t j = it.lo; j < end; ++j) {
             int t = dty[j];
             int64_t uu = du[j], vv = dv[j];
             if (t != 3) {
                 if (t == 5) {
-                    if (!(uu >= 0 && uu < (1LL << prev_bl)) || !(vv > uu && vv <= (1LL << prev_bl))) {
+                    if (!(uu >= 0 && uu < lim) || !(vv > uu && vv <= lim)) {
                         snprintf(b, sizeof b, "layer %d gate %lld: sum range out of bounds", i, (long long)j);
-                        return fail(-3);
+                        return fail(j);
                     }
-                } else if (!(uu >= 0 && uu < (1LL << prev_bl)) || !(vv >= 0 && vv < (1LL << prev_bl))) {
+                } else if (!(uu >= 0 && uu < lim) || !(vv >= 0 && vv < lim)) {
                     snprintf(b, sizeof b, "layer %d gate %lld: input index out of bounds", i, (long long)j);
-                    return fail(-3);
+                    return fail(j);
                 }
             }
             if (t == 6 || t == 10) vv = 0;
-            if (t == 13 && uu != vv) { snprintf(b, sizeof b, "bit-test gate needs u == v"); return fail(-3); }
+            if (t == 13 && uu != vv) { snprintf(b, sizeof b, "bit-test gate needs u == v"); return fail(j); }
             if (t == 12 && vv - uu + 1 > 60) {                                   // zk_prover.cpp:129
                 snprintf(b, sizeof b, "power-sum gate spans more than 60 entries");
-                return fail(-3);
+                return fail(j);
             }
-            if (t < 0 || t > 13 || t == 11) { snprintf(b, sizeof b, "unknown gate type %d", t); return fail(-3); }
+            if (t < 0 || t > 13 || t == 11) { snprintf(b, sizeof b, "unknown gate type %d", t); return fail(j); }
             if (i != 1) {
                 ty[j] = (uint8_t)t; u[j] = (int32_t)uu; v[j] = (int32_t)vv;
-                if (t == 4 && uu >= own && relay_bad < 0) relay_bad = j;
+                if (t == 4 && uu >= own && it.relay_bad < 0) it.relay_bad = j;
+                count(j, t, uu, vv);
             } else {
-                if (t != 2 && t != 3) { snprintf(b, sizeof b, "input layer gates must be type 2 or 3"); return fail(-3); }
-                C.ty[1][j] = 4; C.u[1][j] = (int32_t)j; C.v[1][j] = 0;
+                if (t != 2 && t != 3) { snprintf(b, sizeof b, "input layer gates must be type 2 or 3"); return fail(j); }
+                ty[j] = 4; u[j] = (int32_t)j; v[j] = 0;
                 C.ty[0][j] = (uint8_t)t; C.u[0][j] = (int32_t)uu; C.v[0][j] = (int32_t)vv;   // (int) truncation, zk_prover.cpp:60
+                count(j, 4, j, 0);
             }
         }
-        if (i == 1)
-            for (int64_t g = n[i]; g < n_pad[i]; ++g) { C.ty[1][g] = 4; C.u[1][g] = (int32_t)g; C.ty[0][g] = 3; C.u[0][g] = 0; }
-        if (n[i] == 1) { C.ty[i][1] = 2; C.u[i][1] = 0; C.v[i][1] = 0; }
-        // (the input layer's relay gates read u = j < its size: never out of range)
-        if (relay_bad >= 0) {
-            snprintf(b, sizeof b, "layer %d: relay gate %zu reads u=%d beyond its own layer size", i, (size_t)relay_bad,
-                     u[relay_bad]);
-            return fail(-3);
-        }
+        if (i == 1)                                      // padding: zero inputs relayed
+            for (int64_t g = std::max(it.lo, n[1]); g < it.hi; ++g) {
+                ty[g] = 4; u[g] = (int32_t)g; v[g] = 0;
+                C.ty[0][g] = 3; C.u[0][g] = 0; C.v[0][g] = 0;
+                count(g, 4, g, 0);
+            }
+        if (n[i] == 1 && it.lo == 0) { ty[1] = 2; u[1] = 0; v[1] = 0; }   // single-gate layer: a dummy second gate
+        it.e1 = e1; it.e2 = e2;
     };
-    for_layers(1, D + 1, off, layer);
-    for (int i = 1; i <= D; ++i)
-        if (code[i]) FAILV(code[i], "%s", err[i].c_str());
+    parallel_for((int64_t)items.size(), chunk);
+    // errors: the first failing layer in file order; in it, the first failed
+    // check by gate, else its first relay gate reading beyond its own layer
+    size_t k = 0;
+    for (int i = 1; i <= D; ++i) {
+        size_t k0 = k;
+        int64_t relay = -1;
+        for (; k < items.size() && items[k].layer == i; ++k) {
+            if (items[k].code) FAILV(items[k].code, "%s", items[k].err.c_str());
+            if (relay < 0) relay = items[k].relay_bad;
+        }
+        if (relay >= 0)
+            FAILV(-3, "layer %d: relay gate %zu reads u=%d beyond its own layer size", i, (size_t)relay, C.u[i][relay]);
+        // the layer's wiring counts
+        wiring_plan &W = C.W[i];
+        std::vector<int64_t> t1, t2;
+        for (size_t q = k0; q < k; ++q) {
+            item &it = items[q];
+            if (t1.empty()) { t1.swap(it.t1); t2.swap(it.t2); }
+            else
+                for (size_t x = 0; x < t1.size(); ++x) { t1[x] += it.t1[x]; t2[x] += it.t2[x]; }
+            W.e1 += it.e1;
+            W.e2 += it.e2;
+            W.z2.insert(W.z2.end(), it.z2.begin(), it.z2.end());
+        }
+        for (size_t x = 0; x < t1.size(); ++x) { W.m1 = std::max(W.m1, t1[x]); W.m2 = std::max(W.m2, t2[x]); }
+    }
     if (C.bl[0] < 6 + 1) FAILV(-2, "input layer too small for the commitment");
     return 0;
 }
@@ -476,7 +548,6 @@ void lastbit_terms(const fe *lb, fe d, fe e, fe &A, fe &B, fe &Dd, fe &E, fe &Fv
     B = add(B, mul(mul(c, e), Z));
 }
 
-int ilog2(int64_t x) { int l = 0; while (((int64_t)1 << l) < x) ++l; return l; }
 
 }  // namespace
 
@@ -1191,52 +1262,20 @@ void assemble(vg_ctx *c, tbuf &T)
 // previous layer, the gates that add into it through u (a sum gate into
 // every u of its range); phase 2 lists by v the gates that add into entry v,
 // and separately the gates that always add into entry 0 (z2, kept here).
-// Counts the records of each phase and whether any gather_fold tile of the
-// layer's table holds more than GF_MAX_FANIN records per entry.
+// Takes the layer's record counts (counted while loading, build_circuit)
+// and decides whether any gather_fold tile of the layer's table holds more
+// than GF_MAX_FANIN records per entry, and the lanes per entry of a flat
+// build.
 void csr_plan(vg_ctx *c, int i)
 {
-    const host_circuit &C = c->C;
-    int64_t nt = (int64_t)1 << C.bl[i - 1];
-    int64_t ng = (int64_t)C.ty[i].size();
-    int64_t tile = launch::gf_tile(C.bl[i - 1]);
-    const int tsh = ilog2(tile);                 // tiles are powers of two: x / tile = x >> tsh
-    int64_t ntile = (nt + tile - 1) / tile;
-    std::vector<int64_t> t1(ntile, 0), t2(ntile, 0);
-    std::vector<int32_t> &z = c->h_z2[i];
-    z.clear();
-    const uint8_t *ty = C.ty[i].data();
-    const int32_t *u = C.u[i].data(), *v = C.v[i].data();
-    int64_t e1 = 0, e2 = 0;
-    for (int64_t g = 0; g < ng; ++g) {
-        int t = ty[g];
-        int64_t lo = 0, hi = 0;
-        switch (t) {
-        case 0: case 1: case 4: case 6: case 7: case 8: case 9: case 10: case 13: lo = u[g]; hi = (int64_t)u[g] + 1; break;
-        case 5: lo = u[g]; hi = v[g]; break;
-        case 12: lo = u[g]; hi = (int64_t)v[g] + 1; break;
-        default: break;
-        }
-        if (hi == lo + 1) t1[lo >> tsh]++;
-        else
-            for (int64_t x = lo; x < hi;) {
-                int64_t k = x >> tsh, end = (k + 1) * tile < hi ? (k + 1) * tile : hi;
-                t1[k] += end - x;
-                x = end;
-            }
-        e1 += hi > lo ? hi - lo : 0;
-        if (t == 0 || t == 1 || t == 7 || t == 8 || t == 9 || t == 13) { t2[v[g] >> tsh]++; ++e2; }
-        if (t == 5 || t == 12 || t == 6 || t == 10) z.push_back((int32_t)g);
-    }
-    c->n_ent1[i] = e1;
-    c->n_ent2[i] = e2;
-    c->crowd1[i] = c->crowd2[i] = 0;
-    int64_t m1 = 0, m2 = 0;
-    for (int64_t k = 0; k < ntile; ++k) {
-        m1 = std::max(m1, t1[k]);
-        m2 = std::max(m2, t2[k]);
-    }
-    c->crowd1[i] = m1 > launch::GF_MAX_FANIN * tile;
-    c->crowd2[i] = m2 > launch::GF_MAX_FANIN * tile;
+    wiring_plan &W = c->C.W[i];
+    int64_t nt = (int64_t)1 << c->C.bl[i - 1];
+    int64_t tile = launch::gf_tile(c->C.bl[i - 1]);
+    c->n_ent1[i] = W.e1;
+    c->n_ent2[i] = W.e2;
+    c->h_z2[i].swap(W.z2);
+    c->crowd1[i] = W.m1 > launch::GF_MAX_FANIN * tile;
+    c->crowd2[i] = W.m2 > launch::GF_MAX_FANIN * tile;
     // lanes per entry of a flat build: the densest tile's records per entry,
     // rounded up to a power of two, at most a wave
     auto lanes = [&](int64_t m) {
@@ -1245,8 +1284,8 @@ void csr_plan(vg_ctx *c, int i)
         while (k < 6 && ((int64_t)1 << k) < per) ++k;
         return k;
     };
-    c->kl1[i] = lanes(m1);
-    c->kl2[i] = lanes(m2);
+    c->kl1[i] = lanes(W.m1);
+    c->kl2[i] = lanes(W.m2);
 }
 
 // Host planning of a proof: sizes, challenge and message layout, layer
@@ -1356,11 +1395,7 @@ int ctx_plan(vg_ctx *c)
     c->n_ent1.assign(C.depth, 0); c->n_ent2.assign(C.depth, 0);
     c->crowd1.assign(C.depth, 0); c->crowd2.assign(C.depth, 0); c->h_z2.assign(C.depth, {});
     c->kl1.assign(C.depth, 0); c->kl2.assign(C.depth, 0);
-    {
-        int64_t gates = 0;
-        for (int i = 1; i < C.depth; ++i) gates += (int64_t)C.ty[i].size();
-        for_layers(1, C.depth, gates, [&](int i) { csr_plan(c, i); });
-    }
+    for (int i = 1; i < C.depth; ++i) csr_plan(c, i);
     for (int i = 1; i < C.depth; ++i)
         if (c->n_ent1[i] >= (1LL << 31) || c->n_ent2[i] >= (1LL << 31)) FAILV(-2, "layer %d: too many wiring records", i);
     // gather_fold walks a block's records in sequence: phases whose records

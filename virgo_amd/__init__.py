@@ -7,6 +7,7 @@ HIP library or the GPU is missing, every entry point raises.
 """
 import ctypes
 import os
+import time
 
 import numpy as np
 
@@ -153,10 +154,18 @@ class Prover:
     def device_bytes(self):
         return int(lib().vg_ctx_device_bytes(self._ctx))
 
-    def prove(self):
+    def prove(self, clock=None):
+        """Proves the context's circuit; returns the transcript as bytes.
+        clock (a list): the seconds vg_prove took (inputs resident on the
+        device to the transcript in the library's host buffer) are appended;
+        the copy into a Python object after it is not counted."""
         out = ctypes.c_void_p()
         ln = ctypes.c_size_t()
-        _check(lib().vg_prove(self._ctx, ctypes.byref(out), ctypes.byref(ln), _p(self.stage_ms)))
+        t0 = time.perf_counter()
+        rc = lib().vg_prove(self._ctx, ctypes.byref(out), ctypes.byref(ln), _p(self.stage_ms))
+        if clock is not None:
+            clock.append(time.perf_counter() - t0)
+        _check(rc)
         data = ctypes.string_at(out, ln.value)
         lib().vg_free(out)
         return data
@@ -296,14 +305,19 @@ def prove(circ):
         p.close()
 
 
-def prove_batch(provers, workers):
+def prove_batch(provers, workers, clock=None):
     """Proves every Prover's circuit with `workers` host threads in the
-    library (vg_prove_batch); returns the transcripts in order."""
+    library (vg_prove_batch); returns the transcripts in order.  clock (a
+    list): the seconds vg_prove_batch took are appended (the copies into
+    Python objects after it are not counted)."""
     n = len(provers)
     ctxs = (ctypes.c_void_p * n)(*[p._ctx.value for p in provers])
     outs = (ctypes.c_void_p * n)()
     lens = (ctypes.c_size_t * n)()
+    t0 = time.perf_counter()
     rc = lib().vg_prove_batch(ctxs, n, workers, outs, lens)
+    if clock is not None:
+        clock.append(time.perf_counter() - t0)
     res = []
     for k in range(n):
         if outs[k]:

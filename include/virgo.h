@@ -61,6 +61,14 @@ int vg_stream_create(void **stream);
 int vg_stream_destroy(void *stream);
 /* Bytes of device memory held by the context. */
 int64_t vg_ctx_device_bytes(const vg_ctx *ctx);
+/* Loads and checks circuit c as vg_ctx_create does (the checks and padding
+ * of zk_verifier::read_circuit, zk_verifier.cpp:15-217; the same error codes
+ * and messages) and plans one proof of it on the host alone -- no HIP call,
+ * no device memory: *device_bytes and *pinned_bytes are what a context of it
+ * on a caller's stream takes (a proof server's lease).  No counterpart in
+ * the reference, which has no separate planning step; for admission control
+ * and capacity planning by callers. */
+int vg_plan(const vg_circuit_desc *c, int64_t *device_bytes, int64_t *pinned_bytes);
 /* NTT schedule in use for transforms of 2^log_n values (no counterpart in
  * the reference, whose FFT has one fixed loop, RS_polynomial.cpp:27-240):
  * the LDS tile (log2 of its elements, 10..12), or 1 for the one-block

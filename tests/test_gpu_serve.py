@@ -121,3 +121,21 @@ def test_server_bad_request_fails_alone(alone):
     st = srv.stats()
     srv.close()
     assert st["failed"] == 1 and st["completed"] == 2
+
+
+def test_plan_sizes_equal_the_contexts(alone):
+    """vg_plan (host only) reports what a context of the circuit on a
+    caller's stream allocates: the device bytes up to the 256-byte alignment
+    of its two blocks."""
+    circs, _ = alone
+    st = V.Stream()
+    try:
+        for name, c in circs.items():
+            dev, pin = V.plan(c)
+            p = V.Prover(c, stream=st)
+            held = p.device_bytes()
+            p.close()
+            assert 0 <= dev - held < 512, name
+            assert pin > 0
+    finally:
+        st.close()

@@ -23,7 +23,7 @@ EXPORTS = ["vg_init", "vg_last_error", "vg_free", "vg_ctx_create", "vg_ctx_destr
            "vg_merkle_tree", "vg_fri_fold", "vg_fe_mul", "vg_fft_gkr", "vg_stream_create", "vg_stream_destroy", "vg_prove_batch",
            "vg_verify", "vg_server_create", "vg_server_submit", "vg_server_wait", "vg_server_stats",
            "vg_server_destroy", "vg_ntt_plan", "vg_sha256_64", "vg_merkle_tree_sha256", "vg_microset", "vg_fft_noalloc", "vg_layer_values",
-           "vg_proof_values", "vg_query_positions"]
+           "vg_proof_values", "vg_query_positions", "vg_plan"]
 
 
 class VirgoError(RuntimeError):
@@ -53,6 +53,7 @@ def lib():
         L.vg_ctx_destroy.argtypes = [vp]
         L.vg_ctx_device_bytes.argtypes = [vp]
         L.vg_ctx_device_bytes.restype = ctypes.c_int64
+        L.vg_plan.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
         L.vg_prove.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t), vp]
         L.vg_prove_async.argtypes = [vp]
         L.vg_finish.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t), vp]
@@ -295,6 +296,25 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def _desc(circ):
+    keep = tuple(np.ascontiguousarray(circ[k], dtype=t) for k, t in
+                 (("n", np.int64), ("ty", np.int32), ("u", np.int64), ("v", np.int64)))
+    n, ty, u, v = keep
+    return _Desc(len(n), _p(n), _p(ty), _p(u), _p(v)), keep
+
+
+def plan(circ):
+    """Checks the circuit as a context would and plans one proof on the host
+    (vg_plan; no GPU needed): returns (device bytes, page-locked host bytes)
+    of a context of it on a caller's stream.  Raises VirgoError with the
+    loader's message for a bad circuit."""
+    d, keep = _desc(circ)
+    dev, pin = ctypes.c_int64(), ctypes.c_int64()
+    _check(lib().vg_plan(ctypes.byref(d), ctypes.byref(dev), ctypes.byref(pin)))
+    del keep
+    return dev.value, pin.value
 
 
 def prove(circ):

@@ -330,7 +330,7 @@ bool check(const fg_ctx *c, const fe *M)
 
 }  // namespace
 
-int fg_plan(fg_ctx **out, int lg, hipStream_t stream, int64_t *dev_bytes, int64_t *pin_bytes)
+int fg_plan(fg_ctx **out, int lg, hipStream_t stream, int64_t *dev_bytes, int64_t *pin_bytes, bool host_only)
 {
     *out = nullptr;
     if (lg < 1 || lg > 20) FG_FAIL(-1, "fft_gkr: lg = %d out of range [1, 20]", lg);
@@ -341,7 +341,7 @@ int fg_plan(fg_ctx **out, int lg, hipStream_t stream, int64_t *dev_bytes, int64_
     c->hi = lg - c->lo + 1;            // exponents up to 2^lg - 1 (and 2^(lg-1) for twiddles)
     c->inv_n = finv(from_u64((uint64_t)c->N));
     plan(c);
-    if (stream) c->st = stream;
+    if (stream || host_only) c->st = stream;
     else {
         if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; FG_FAIL(-20, "stream"); }
         c->own_stream = true;

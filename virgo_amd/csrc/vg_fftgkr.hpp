@@ -18,7 +18,8 @@ int fg_create(fg_ctx **out, int lg, hipStream_t stream);
 // and pinned host bytes it needs (nothing allocated); fg_alloc takes them
 // from `mem` and uploads the static tables.  fg_destroy after either.
 struct mem_source;
-int fg_plan(fg_ctx **out, int lg, hipStream_t stream, int64_t *dev_bytes, int64_t *pin_bytes);
+// host_only: no stream is made (the plan's sizes only; fg_destroy releases it)
+int fg_plan(fg_ctx **out, int lg, hipStream_t stream, int64_t *dev_bytes, int64_t *pin_bytes, bool host_only = false);
 int fg_alloc(fg_ctx *c, mem_source *mem);
 void fg_destroy(fg_ctx *c);
 int64_t fg_device_bytes(const fg_ctx *c);

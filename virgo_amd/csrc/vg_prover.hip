@@ -2034,11 +2034,20 @@ void vg_free(void *p) { free(p); }
 // and plans the proof (host work only); ctx_materialize takes the device
 // arena and pinned staging from `mem` (nullptr: hipMalloc / hipHostMalloc)
 // and uploads.  Both destroy the context on failure.
-int ctx_create_planned(vg_ctx **out, const vg_circuit_desc *d, void *stream)
+int ctx_create_planned(vg_ctx **out, const vg_circuit_desc *d, void *stream, bool host_only)
 {
     vg_ctx *c = new vg_ctx();
     int rc = build_circuit(d, c->C);
     if (rc) { delete c; return rc; }
+    if (host_only) {
+        // sizes only: no stream, the proof's layout as on a caller's stream
+        c->serial = true;
+        rc = ctx_plan(c);
+        if (!rc) rc = fg_plan(&c->fg, c->n - 6, nullptr, &c->fg_bytes, &c->fg_pin, true);
+        if (rc) { vg_ctx_destroy(c); return rc; }
+        *out = c;
+        return 0;
+    }
     if (stream) c->st = (hipStream_t)stream;
     else {
         hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
@@ -2134,6 +2143,17 @@ int vg_ctx_create(vg_ctx **out, const vg_circuit_desc *d, void *stream)
     int rc = ctx_create_planned(&c, d, stream);
     if (rc) return rc;
     return ctx_materialize(c, nullptr, out);
+}
+
+int vg_plan(const vg_circuit_desc *d, int64_t *device_bytes, int64_t *pinned_bytes)
+{
+    if (!d || !device_bytes || !pinned_bytes) FAILV(-1, "null argument");
+    *device_bytes = *pinned_bytes = 0;
+    vg_ctx *c = nullptr;
+    int rc = ctx_create_planned(&c, d, nullptr, true);
+    if (rc) return rc;
+    ctx_needs(c, device_bytes, pinned_bytes);
+    return vg_ctx_destroy(c);     // host state only: no stream, event or device memory was made
 }
 
 int vg_ctx_destroy(vg_ctx *c)

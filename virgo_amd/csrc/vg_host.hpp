@@ -3,6 +3,7 @@
 #pragma once
 #include <stdint.h>
 #include <string.h>
+#include <algorithm>
 #include <vector>
 #include <hip/hip_runtime.h>
 #include "vg_field.hpp"
@@ -81,7 +82,9 @@ mem_source *hip_mem();
 // Staged puts whose destinations lie at the same distances apart as their
 // staging (256-byte aligned blocks carved back to back, like the staging)
 // become one copy; flush() enqueues the pending copy and must come before
-// any launch that reads what was put.
+// any launch that reads what was put.  A staged put of 1 MB or more (a gate
+// layer's arrays) is packed at the next flush, all such puts together on the
+// host pool: its source must stay valid until then.
 // ---------------------------------------------------------------------------
 struct uploader {
     uint8_t *stage = nullptr;
@@ -89,6 +92,9 @@ struct uploader {
     hipStream_t st = nullptr;
     uint8_t *run_dst = nullptr;                  // pending copy: staging [run_off, run_end) to run_dst
     int64_t run_off = 0, run_end = 0;
+    struct piece { int64_t off; const void *src; int64_t n; };
+    std::vector<piece> deferred;                 // large puts not yet packed
+    static constexpr int64_t kDefer = (int64_t)1 << 20;
     static int64_t stage_bytes(int64_t n) { return (n + 255) & ~(int64_t)255; }
     // takes `bytes` of staging from m when it stages; false: no memory
     bool init(mem_source *m, int64_t bytes, hipStream_t s)
@@ -107,13 +113,26 @@ struct uploader {
         hipError_t e = hipSuccess;
         if (run_dst && (uint8_t *)dst != run_dst + (off - run_off)) e = flush();
         if (!run_dst) { run_dst = (uint8_t *)dst; run_off = off; }
-        parallel_copy(stage + off, src, (size_t)n);   // large gate arrays on the host pool
+        if (n >= kDefer) deferred.push_back({off, src, n});
+        else memcpy(stage + off, src, (size_t)n);
         run_end = off + n;                       // the padding after the last block is not copied
         off += stage_bytes(n);
         return e;
     }
+    // packs every deferred put, in 1 MB pieces over the host pool
+    void pack()
+    {
+        if (deferred.empty()) return;
+        std::vector<piece> pc;
+        for (const piece &d : deferred)
+            for (int64_t o = 0; o < d.n; o += kDefer)
+                pc.push_back({d.off + o, (const uint8_t *)d.src + o, std::min(kDefer, d.n - o)});
+        deferred.clear();
+        parallel_for((int64_t)pc.size(), [&](int64_t k) { memcpy(stage + pc[k].off, pc[k].src, (size_t)pc[k].n); });
+    }
     hipError_t flush()
     {
+        pack();
         if (!run_dst) return hipSuccess;
         hipError_t e = hipMemcpyAsync(run_dst, stage + run_off, (size_t)(run_end - run_off), hipMemcpyHostToDevice, st);
         run_dst = nullptr;

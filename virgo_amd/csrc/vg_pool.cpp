@@ -103,20 +103,6 @@ void parallel_for(int64_t n, const std::function<void(int64_t)> &fn)
     j->cv.wait(lk, [&] { return j->done.load() == n; });
 }
 
-void parallel_copy(void *dst, const void *src, size_t n)
-{
-    constexpr size_t piece = (size_t)2 << 20;
-    if (n < 4 * piece || pool_width() < 2) {
-        memcpy(dst, src, n);
-        return;
-    }
-    int64_t np = (int64_t)((n + piece - 1) / piece);
-    parallel_for(np, [&](int64_t k) {
-        size_t o = (size_t)k * piece;
-        memcpy((uint8_t *)dst + o, (const uint8_t *)src + o, std::min(piece, n - o));
-    });
-}
-
 namespace {
 struct block_cache {
     std::mutex mu;

@@ -16,8 +16,6 @@ void parallel_for(int64_t n, const std::function<void(int64_t)> &fn);
 // threads parallel_for may use at once (the calling thread included):
 // min(16, hardware threads), or VIRGO_HOST_THREADS
 int pool_width();
-// memcpy of large blocks split over the pool
-void parallel_copy(void *dst, const void *src, size_t n);
 
 // Large host blocks are kept for reuse when freed (up to a fixed total), so
 // a context built after another of similar size writes into pages that are

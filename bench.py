@@ -25,8 +25,17 @@ sys.path.insert(0, ROOT)
 os.environ["GPU_MAX_HW_QUEUES"] = "4"
 
 PEAK_HBM_GBS = 8000.0
-# int32 VALU lane-ops/s of MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz
-PEAK_VALU_GOPS = 256 * 64 * 2.4
+# MI355X VALU issue at the published 2.4 GHz: 256 CUs x 4 SIMDs, each
+# issuing one wave64 three-input bitwise operation (v_bitop3_b32, v_xor_b32)
+# every 2 cycles -- 32 lanes per cycle, the issue width behind the published
+# 157.3 TFLOP/s of packed FP32 -- and one funnel shift (v_alignbit_b32) every
+# 4 cycles (16 lanes per cycle); vg_valu_rates measures both on the device
+# (about 0.40 and 0.23 wave instructions per SIMD per cycle at 8 waves per
+# SIMD).
+VALU_SIMDS = 256 * 4
+CLOCK_GHZ = 2.4
+# one op per lane per cycle at 16 lanes: the single-rate figure
+PEAK_VALU_GOPS = VALU_SIMDS * 16 * CLOCK_GHZ
 
 
 def parse():
@@ -106,6 +115,17 @@ MICRO_RATE_LOG = 3      # the kernel micro-set's extension rate 1/8
 # reaches the four digest lanes dropped; derived symbolically in
 # tests/test_keccak_opcount.py (a general state costs 24 x 180 - 11 = 4309).
 KECCAK_F_OPS = 4174
+# the same split by issue class (tests/test_keccak_opcount.py derives both)
+KECCAK_F_BIT_OPS = 2822
+KECCAK_F_SHIFT_OPS = 1352
+
+
+def keccak_peak_gops(bit_cpi=2.0, shift_cpi=4.0, ghz=CLOCK_GHZ):
+    """Algorithmic ops/s of SHA3-256 (64-byte block) permutations at full
+    VALU issue: a SIMD spends bit_cpi cycles per wave64 bitwise operation and
+    shift_cpi per funnel shift, 64 lanes each."""
+    cycles = KECCAK_F_BIT_OPS * bit_cpi + KECCAK_F_SHIFT_OPS * shift_cpi   # per 64 permutations on one SIMD
+    return KECCAK_F_OPS * 64 * VALU_SIMDS * ghz / cycles
 
 
 def _rand_fe(rng, n):
@@ -149,7 +169,7 @@ def microset(args):
             e["merkle_mhash_s"] = round(hashes / (ms["merkle"] * 1e-3) / 1e6, 1)
             if h == "sha3":
                 gops = hashes * KECCAK_F_OPS / (ms["merkle"] * 1e-3) / 1e9
-                e["merkle_valu_frac"] = round(gops / PEAK_VALU_GOPS, 3)
+                e["merkle_valu_frac"] = round(gops / keccak_peak_gops(), 3)
             e["root"] = bytes(roots[0]).hex()[:16]
             row[h] = e
         out["sizes"]["2^%d" % k] = row
@@ -448,6 +468,7 @@ def main():
     ops = chains * 65 * KECCAK_F_OPS
     leaf_ms = float(st[7])
     achieved = ops / (leaf_ms * 1e-3) / 1e9
+    peak = keccak_peak_gops()
     line = {
         "metric": "prover ms @2^20-gate circuit; batched proofs/s vs plain per-proof GPU loop",
         "value": round(ms, 3),
@@ -465,11 +486,22 @@ def main():
                                % (args.log_width, args.depth, args.log_width),
                    "transcript_bytes": len(tr),
                    "stage_ms": {k: round(float(v), 3) for k, v in zip(names, st)}},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 1), "peak": PEAK_VALU_GOPS, "unit": "Gop/s",
-                     "frac": round(achieved / PEAK_VALU_GOPS, 4), "traffic": leaf_traffic(n),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 1), "peak": round(peak, 1), "unit": "Gop/s",
+                     "frac": round(achieved / peak, 4), "traffic": leaf_traffic(n),
                      "kernel": "k_leaf_multi (SHA3-256 leaf chains, %d chains x 65 Keccak-f)" % chains,
-                     "kernel_ms": round(leaf_ms, 3)},
+                     "kernel_ms": round(leaf_ms, 3),
+                     "peak_model": "2.4 GHz x 1024 SIMDs; per Keccak-f %d bitwise ops at 2 cycles and %d funnel "
+                                   "shifts at 4 cycles per wave64" % (KECCAK_F_BIT_OPS, KECCAK_F_SHIFT_OPS),
+                     "single_rate_peak": PEAK_VALU_GOPS},
     }
+    # the issue rates the peak assumes, measured on this device (vg_valu_rates)
+    try:
+        r = V.valu_rates()
+        mp = keccak_peak_gops(1.0 / r["bitwise_per_cycle"], 1.0 / r["shift_per_cycle"], CLOCK_GHZ)
+        line["roofline"]["measured_issue"] = {k: round(v, 3) for k, v in r.items()}
+        line["roofline"]["frac_at_measured_issue"] = round(achieved / mp, 4)
+    except V.VirgoError:
+        pass
     # the reference's own accounting (zk_verifier.cpp:1232 adds the prover's
     # and the commitment prover's total_time, vpd_verifier.cpp:74 and
     # fft_circuit_GKR.cpp:812-823 the fft_gkr prover time; evaluate is not

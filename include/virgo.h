@@ -225,6 +225,14 @@ int vg_fft(const vg_fe *coef, int coef_len, int order, const vg_fe *rou, vg_fe *
  * hipMalloc or hipFree is called.  Calls that overlap on one GPU share that
  * array and race: serialise them (one stream, or wait between calls). */
 int vg_fft_noalloc(const vg_fe *coef, int coef_len, int order, const vg_fe *rou, vg_fe *result, void *stream);
+/* Measures this device's VALU issue rate for the two instruction classes of
+ * the SHA3-256 leaf chains, 8 waves per SIMD, 16 independent chains per
+ * lane: rates[0] = three-input bitwise operations (v_bitop3_b32) and
+ * rates[1] = funnel shifts (v_alignbit_b32), in wave instructions per SIMD
+ * per shader cycle; rates[2] = the shader clock during the runs, MHz (shader
+ * cycle counter over the 100 MHz constant counter).  No counterpart in the
+ * reference: the measured side of the leaf kernel's roofline. */
+int vg_valu_rates(double *rates);
 /* inverse_fast_fourier_transform (RS_polynomial.cpp:169-240) */
 int vg_ifft(const vg_fe *evals, int coef_len, int order, const vg_fe *rou, vg_fe *result);
 /* my_hhash (my_hhash.h:39-46): n independent SHA3-256 digests of 64 bytes */

@@ -31,8 +31,8 @@ class Graph:
     def __init__(self):
         self.nodes = []
 
-    def node(self, cost, ins):
-        self.nodes.append((cost, [x for x in ins if x[0] == "v"]))
+    def node(self, cost, ins, cls="bit"):
+        self.nodes.append((cost, [x for x in ins if x[0] == "v"], cls))
         return ("v", len(self.nodes) - 1)
 
     def xor(self, *ops):            # three-input XORs: ceil((m - 1) / 2) ops for m live inputs
@@ -52,7 +52,7 @@ class Graph:
     def align(self, hi, lo, s):     # low 32 bits of (hi:lo) >> s: one funnel shift
         if hi[0] == "c" and lo[0] == "c":
             return ("c", (((hi[1] << 32) | lo[1]) >> s) & M32)
-        return self.node(1, [hi, lo])
+        return self.node(1, [hi, lo], "shift")
 
     def rot(self, lane, r):         # 64-bit rotate left of (lo, hi)
         lo, hi = lane
@@ -75,17 +75,19 @@ class Graph:
             return self.xor(a, c)
         return self.node(1, [a, b, c])
 
-    def live_ops(self, outs):
+    def live_ops(self, outs, by_class=False):
         live, stack = set(), [o[1] for o in outs if o[0] == "v"]
         while stack:
             k = stack.pop()
             if k not in live:
                 live.add(k)
                 stack.extend(v[1] for v in self.nodes[k][1])
+        if by_class:
+            return {c: sum(self.nodes[k][0] for k in live if self.nodes[k][2] == c) for c in ("bit", "shift")}
         return sum(self.nodes[k][0] for k in live)
 
 
-def keccak_ops(symbolic_lanes, consts, out_lanes):
+def keccak_ops(symbolic_lanes, consts, out_lanes, by_class=False):
     """Ops of one Keccak-f[1600] whose lanes < symbolic_lanes are unknown,
     lanes in consts are (lo, hi) constants, the rest zero; counted over the
     operations that reach lanes < out_lanes."""
@@ -111,12 +113,12 @@ def keccak_ops(symbolic_lanes, consts, out_lanes):
         st = [tuple(g.chi(B[x + 5 * y][h], B[(x + 1) % 5 + 5 * y][h], B[(x + 2) % 5 + 5 * y][h]) for h in (0, 1))
               for y in range(5) for x in range(5)]
         st[0] = (g.xor(st[0][0], ("c", RC[rd] & M32)), g.xor(st[0][1], ("c", RC[rd] >> 32)))
-    return g.live_ops([h for lane in st[:out_lanes] for h in lane])
+    return g.live_ops([h for lane in st[:out_lanes] for h in lane], by_class)
 
 
-def sha3_256_64_ops():
+def sha3_256_64_ops(by_class=False):
     # 64-byte message in lanes 0..7; 0x06 at byte 64 (lane 8), 0x80 at byte 135 (lane 16 high byte)
-    return keccak_ops(8, {8: (0x06, 0), 16: (0, 0x80000000)}, 4)
+    return keccak_ops(8, {8: (0x06, 0), 16: (0, 0x80000000)}, 4, by_class)
 
 
 def test_general_state_costs_the_full_round_count():
@@ -131,3 +133,13 @@ def test_bench_constant_is_the_derived_count():
     n = sha3_256_64_ops()
     assert n == bench.KECCAK_F_OPS
     assert 4100 < n < 24 * 180
+
+
+def test_bench_constants_split_by_issue_class():
+    """The roofline's peak prices the two classes apart: three-input bitwise
+    operations (bitop3 / xor) and funnel shifts (alignbit)."""
+    c = sha3_256_64_ops(by_class=True)
+    assert c["bit"] == bench.KECCAK_F_BIT_OPS and c["shift"] == bench.KECCAK_F_SHIFT_OPS
+    assert c["bit"] + c["shift"] == bench.KECCAK_F_OPS
+    g = keccak_ops(25, {}, 25, by_class=True)
+    assert g["shift"] == 24 * 58                 # theta's five rotations by one and rho's 24 lanes, two halves each

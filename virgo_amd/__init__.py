@@ -23,7 +23,7 @@ EXPORTS = ["vg_init", "vg_last_error", "vg_free", "vg_ctx_create", "vg_ctx_destr
            "vg_merkle_tree", "vg_fri_fold", "vg_fe_mul", "vg_fft_gkr", "vg_stream_create", "vg_stream_destroy", "vg_prove_batch",
            "vg_verify", "vg_server_create", "vg_server_submit", "vg_server_wait", "vg_server_stats",
            "vg_server_destroy", "vg_ntt_plan", "vg_sha256_64", "vg_merkle_tree_sha256", "vg_microset", "vg_fft_noalloc", "vg_layer_values",
-           "vg_proof_values", "vg_query_positions", "vg_plan"]
+           "vg_proof_values", "vg_query_positions", "vg_plan", "vg_valu_rates"]
 
 
 class VirgoError(RuntimeError):
@@ -54,6 +54,7 @@ def lib():
         L.vg_ctx_device_bytes.argtypes = [vp]
         L.vg_ctx_device_bytes.restype = ctypes.c_int64
         L.vg_plan.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        L.vg_valu_rates.argtypes = [vp]
         L.vg_prove.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t), vp]
         L.vg_prove_async.argtypes = [vp]
         L.vg_finish.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t), vp]
@@ -315,6 +316,16 @@ def plan(circ):
     _check(lib().vg_plan(ctypes.byref(d), ctypes.byref(dev), ctypes.byref(pin)))
     del keep
     return dev.value, pin.value
+
+
+def valu_rates():
+    """Measured VALU issue rates (vg_valu_rates): wave instructions per SIMD
+    per cycle for three-input bitwise operations and for funnel shifts, and
+    the shader clock in MHz during the measurement."""
+    init()
+    r = np.zeros(3, dtype=np.float64)
+    _check(lib().vg_valu_rates(_p(r)))
+    return {"bitwise_per_cycle": float(r[0]), "shift_per_cycle": float(r[1]), "clock_mhz": float(r[2])}
 
 
 def prove(circ):

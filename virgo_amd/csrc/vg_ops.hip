@@ -106,6 +106,30 @@ __global__ void k_pad_copy(const fe *src, int64_t len, fe *pad, int64_t n)
     if (i < n) pad[i] = i < len ? src[i] : fzero();
 }
 
+// VALU issue-rate kernels for vg_valu_rates: 16 independent 32-bit chains
+// per lane, iters steps of one instruction class (CLS 0: three-input
+// bitwise op, v_bitop3_b32; CLS 1: funnel shift, v_alignbit_b32); thread 0
+// of block 0 records its shader clock and 100 MHz constant clock spans.
+template <int CLS>
+__global__ __launch_bounds__(256) void k_issue(uint32_t *out, int iters, unsigned long long *clk)
+{
+    uint32_t a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = threadIdx.x * (k + 3) + blockIdx.x;
+    unsigned long long t0 = __builtin_amdgcn_s_memtime(), r0 = __builtin_amdgcn_s_memrealtime();
+    for (int i = 0; i < iters; ++i)
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            a[k] = CLS == 0 ? a[k] ^ (a[(k + 5) & 15] & ~a[(k + 9) & 15])
+                            : __builtin_amdgcn_alignbit(a[k], a[(k + 5) & 15], 13);
+    unsigned long long t1 = __builtin_amdgcn_s_memtime(), r1 = __builtin_amdgcn_s_memrealtime();
+    uint32_t x = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x ^= a[k];
+    out[blockIdx.x * (int64_t)blockDim.x + threadIdx.x] = x;
+    if (threadIdx.x == 0 && blockIdx.x == 0) { clk[0] = t1 - t0; clk[1] = r1 - r0; }
+}
+
 namespace {
 std::mutex g_na_mu;
 constexpr int NA_DEVICES = 64;
@@ -385,6 +409,46 @@ int vg_fri_fold(const vg_fe *cw, int log_n, const vg_fe *r, vg_fe *out)
     launch::fri_fold_nat((const fe *)in.p, half, (const fe *)rr.p, (const fe *)iw.p, (fe *)o.p, 0);
     OPCHK(hipGetLastError());
     OPCHK(hipMemcpy(out, o.p, sizeof(fe) * half, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vg_valu_rates(double *rates)
+{
+    if (!rates) { op_err = "vg_valu_rates: null argument"; return -1; }
+    int dev = 0, ncu = 0;
+    OPCHK(hipGetDevice(&dev));
+    OPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    // 8 blocks of 256 threads per CU: 8 waves on each SIMD
+    const int blocks = 8 * ncu, iters = 20000;
+    dbuf out, clk;
+    OPCHK(hipMalloc(&out.p, (size_t)blocks * 256 * 4));
+    OPCHK(hipMalloc(&clk.p, 16));
+    double mhz = 0;
+    for (int cls = 0; cls < 2; ++cls) {
+        for (int rep = 0; rep < 2; ++rep) {              // the first run warms up and is discarded
+            auto kfn = cls == 0 ? k_issue<0> : k_issue<1>;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            OPCHK(hipEventCreate(&e0));
+            OPCHK(hipEventCreate(&e1));
+            OPCHK(hipEventRecord(e0, 0));
+            hipLaunchKernelGGL(kfn, dim3(blocks), dim3(256), 0, 0, (uint32_t *)out.p, iters, (unsigned long long *)clk.p);
+            OPCHK(hipGetLastError());
+            OPCHK(hipEventRecord(e1, 0));
+            OPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            OPCHK(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            unsigned long long h[2] = {0, 0};
+            OPCHK(hipMemcpy(h, clk.p, 16, hipMemcpyDeviceToHost));
+            if (rep == 0) continue;
+            double f = h[1] ? h[0] * 100.0 / h[1] : 0.0;   // MHz: shader clocks per 100 MHz tick
+            double per_simd = (double)blocks * 4 * iters * 16 / (ncu * 4.0);
+            rates[cls] = f > 0 ? per_simd / (ms * 1e-3 * f * 1e6) : 0.0;
+            mhz += f / 2;
+        }
+    }
+    rates[2] = mhz;
     return 0;
 }
 

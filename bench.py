@@ -68,11 +68,11 @@ def parse():
 def leaf_traffic(n):
     """HBM bytes per k_leaf_multi launch, from rocprofv3 FETCH_SIZE and
     WRITE_SIZE of the 2^20-gate proof scaled by the same counters for a 1 GiB
-    copy (FETCH_SIZE x 2048 B/KiB, WRITE_SIZE x 1024 B/KiB): 799016.2 KiB
+    copy (FETCH_SIZE x 2048 B/KiB, WRITE_SIZE x 1024 B/KiB): 798999.8 KiB
     read, 24575.5 KiB written.  Other widths: null (not measured)."""
     if n != 20:
         return None
-    return int(799016.2 * 2048 + 24575.5 * 1024)
+    return int(798999.8 * 2048 + 24575.5 * 1024)
 
 
 def c1(args):

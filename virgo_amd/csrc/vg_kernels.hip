@@ -71,24 +71,33 @@ __global__ void k_eval_input(const int32_t *u, fe *out, int n)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = mk(red64((uint64_t)(int64_t)u[i]), 0);   // field_element((int)u), :60-65
 }
+// Both operands are gathered before the gate type is looked at: a wave holds
+// add and mult gates at random, and loads inside the type's branch would be
+// issued once per branch the wave takes, each batch waiting for its own
+// round trip.  Every index read here is in range (the loader checks u and v
+// of every gate but the input type 3, whose u is a value, and a sum's v may
+// equal the layer size, so neither of those is read as an operand).
 __global__ void k_eval_layer(const uint8_t *ty, const int32_t *gu, const int32_t *gv, const fe *p, fe *o, int n)
 {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
-    int u = gu[g], v = gv[g];
+    const int t = ty[g], u = gu[g], v = gv[g];
+    fe x = fzero(), y = fzero();
+    if (t != 3) x = p[u];
+    if (t != 3 && t != 5) y = p[v];
     fe r;
-    switch (ty[g]) {
-    case 0: r = add(p[u], p[v]); break;
-    case 1: r = mul(p[u], p[v]); break;
+    switch (t) {
+    case 0: r = add(x, y); break;
+    case 1: r = mul(x, y); break;
     case 3: r = mk(red64((uint64_t)(int64_t)u), 0); break;
-    case 4: case 10: r = p[u]; break;
-    case 5: { r = fzero(); for (int k = u; k < v; ++k) r = add(r, p[k]); break; }
-    case 6: r = sub(fone(), p[u]); break;
-    case 7: r = sub(p[u], p[v]); break;
-    case 8: { fe x = p[u], y = p[v]; r = sub(add(x, y), mul(mul(mk(2, 0), x), y)); break; }
-    case 9: { fe x = p[u], y = p[v]; r = sub(y, mul(x, y)); break; }
+    case 4: case 10: r = x; break;
+    case 5: { r = x; for (int k = u + 1; k < v; ++k) r = add(r, p[k]); break; }
+    case 6: r = sub(fone(), x); break;
+    case 7: r = sub(x, y); break;
+    case 8: r = sub(add(x, y), mul(mul(mk(2, 0), x), y)); break;
+    case 9: r = sub(y, mul(x, y)); break;
     case 12: { r = fzero(); for (int k = u; k <= v; ++k) r = add(r, mul_base(p[k], 1ull << ((k - u) % 61))); break; }
-    case 13: r = mul(p[u], sub(fone(), p[v])); break;
+    case 13: r = mul(x, sub(fone(), y)); break;
     default: r = fzero(); break;
     }
     o[g] = r;

@@ -145,3 +145,13 @@ def test_microset_stages(log_n, hash, rate_log):
         del cw, leaves
         cf = O.fe_add(cf[0::2], O.fe_mul(np.repeat(rs[k:k + 1], n_k // 2), cf[1::2]))
     assert np.array_equal(final, O.fft(cf, 32, O.root_of_unity(5)))
+
+
+def test_valu_issue_rates_match_the_roofline_model():
+    """The leaf kernel's roofline prices three-input bitwise operations at 2
+    cycles and funnel shifts at 4 per wave64: the device's measured issue
+    rates (vg_valu_rates) must show that split."""
+    r = V.valu_rates()
+    assert 1000 < r["clock_mhz"] < 3000
+    assert 0.3 < r["bitwise_per_cycle"] <= 0.5 and 0.15 < r["shift_per_cycle"] <= 0.25
+    assert r["bitwise_per_cycle"] > 1.5 * r["shift_per_cycle"]

@@ -127,3 +127,28 @@ def test_plan_independent_of_host_threads():
         r = subprocess.run([sys.executable, "-c", PROBE % ROOT], env=env, capture_output=True, text=True, check=True)
         outs.append(r.stdout.strip())
     assert outs[0] == outs[1]
+
+
+def test_plan_from_many_threads_at_once():
+    """Several callers plan at the same time (a proof server's workers): they
+    share the library's host pool and recycled blocks; every plan equals the
+    one made alone."""
+    import threading
+    cs = [W.random_circuit(17, 3, seed=s) for s in range(3)] + [W.mixed_circuit(11, 4, seed=1), W.matmul_circuit(16, seed=3)]
+    want = [V.plan(c) for c in cs]
+    got, errs = {}, []
+
+    def work(t):
+        try:
+            for r in range(4):
+                k = (t + r) % len(cs)
+                got[(t, r)] = (k, V.plan(cs[k]))
+        except Exception as e:          # surfaced below
+            errs.append(e)
+    th = [threading.Thread(target=work, args=(t,)) for t in range(8)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not errs
+    assert len(got) == 32 and all(v == want[k] for k, v in got.values())

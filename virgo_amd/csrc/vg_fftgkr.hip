@@ -70,8 +70,9 @@ struct fg_ctx {
     fe *slots = nullptr;
     int *slot_nb = nullptr;
     int64_t *slot_dst = nullptr;
+    int64_t *slot_rof = nullptr;               // offset of r[j-1] in chal; -1 for round 0
     std::vector<int> h_slot_nb;
-    std::vector<int64_t> h_slot_dst;
+    std::vector<int64_t> h_slot_dst, h_slot_rof;
     fe *ht = nullptr;
     launch::half_desc *hdesc = nullptr;
     std::vector<launch::half_desc> h_hdesc;
@@ -131,7 +132,7 @@ void plan(fg_ctx *c)
     c->n_chal = k;
     c->msg_len = m;
     // slot rounds of the multi-block fold launches, instance by instance
-    c->h_slot_nb.clear(); c->h_slot_dst.clear(); c->slot0.clear();
+    c->h_slot_nb.clear(); c->h_slot_dst.clear(); c->h_slot_rof.clear(); c->slot0.clear();
     int ninst = 2 + 2 * lg;
     for (int t = 0; t < ninst; ++t) {
         const fg_layer &l = c->L[t < 2 + lg ? t : t - lg];
@@ -143,7 +144,9 @@ void plan(fg_ctx *c)
         for (int q = 0; q < nl; ++q)
             for (int r = 0; r < ks[q]; ++r) {
                 c->h_slot_nb.push_back(launch::fold_blocks(l.bits, js[q]));
-                c->h_slot_dst.push_back(sums + 3 * (js[q] + r));
+                int j = js[q] + r;
+                c->h_slot_dst.push_back(sums + 3 * j);
+                c->h_slot_rof.push_back(j == 0 ? -1 : (ph == 0 ? l.ru : l.rv) + j - 1);
             }
     }
     // half tables: beta_g halves (weight alpha / beta) and, for the
@@ -197,6 +200,7 @@ int64_t layout(fg_ctx *c, uint8_t *base)
     c->slots = carve<fe>(base, off, (int64_t)(ns > 0 ? ns : 1) * 3 * launch::FW_BLOCKS);
     c->slot_nb = carve<int>(base, off, ns + 1);
     c->slot_dst = carve<int64_t>(base, off, ns + 1);
+    c->slot_rof = carve<int64_t>(base, off, ns + 1);
     c->ht = carve<fe>(base, off, c->ht_total);
     c->hdesc = carve<launch::half_desc>(base, off, (int64_t)c->h_hdesc.size());
     c->msg = carve<fe>(base, off, c->msg_len);
@@ -354,7 +358,7 @@ int fg_plan(fg_ctx **out, int lg, hipStream_t stream, int64_t *dev_bytes, int64_
         c->up_bytes = sb((int64_t)sizeof(launch::fold_inst) * lg2) + sb((int64_t)sizeof(launch::fg_layer_dev) * lg) +
                       sb((int64_t)sizeof(launch::half_desc) * (int64_t)c->h_hdesc.size()) +
                       sb((int64_t)sizeof(int) * (int64_t)c->h_slot_nb.size()) +
-                      sb((int64_t)sizeof(int64_t) * (int64_t)c->h_slot_dst.size());
+                      2 * sb((int64_t)sizeof(int64_t) * (int64_t)c->h_slot_dst.size());
     }
     if (dev_bytes) *dev_bytes = c->bytes;
     if (pin_bytes) *pin_bytes = (int64_t)sizeof(fe) * (c->n_chal + c->msg_len) + 512 + c->up_bytes;
@@ -417,7 +421,8 @@ int fg_alloc(fg_ctx *c, mem_source *mem)
     ok = ok && U.put(c->hdesc, d.data(), (int64_t)sizeof(launch::half_desc) * (int64_t)d.size()) == hipSuccess;
     if (ok && !c->h_slot_nb.empty())
         ok = U.put(c->slot_nb, c->h_slot_nb.data(), (int64_t)sizeof(int) * (int64_t)c->h_slot_nb.size()) == hipSuccess &&
-             U.put(c->slot_dst, c->h_slot_dst.data(), (int64_t)sizeof(int64_t) * (int64_t)c->h_slot_dst.size()) == hipSuccess;
+             U.put(c->slot_dst, c->h_slot_dst.data(), (int64_t)sizeof(int64_t) * (int64_t)c->h_slot_dst.size()) == hipSuccess &&
+             U.put(c->slot_rof, c->h_slot_rof.data(), (int64_t)sizeof(int64_t) * (int64_t)c->h_slot_rof.size()) == hipSuccess;
     ok = ok && U.flush() == hipSuccess;
     if (!ok || hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess)
         FG_FAIL(-20, "fft_gkr: setup copies failed");
@@ -480,7 +485,7 @@ int fg_enqueue(fg_ctx *c)
     launch::fg_ifft_tables(1, c->ldev, lg, pw, lg, s);
     slot += lg * launch::sumcheck_multi(c->inst + 2 + lg, lg, lg, s);
     if (slot != (int)c->h_slot_nb.size()) FG_FAIL(-30, "fft_gkr: fold slot plan mismatch");
-    launch::reduce_slots(c->slots, slot, c->slot_nb, c->slot_dst, c->msg, s);
+    launch::reduce_slots(c->slots, slot, c->slot_nb, c->slot_dst, c->slot_rof, c->chal, c->msg, s);
     FG_CHK(hipMemcpyAsync(c->h_msg, c->msg, sizeof(fe) * c->msg_len, hipMemcpyDeviceToHost, s));
     FG_CHK(hipEventRecord(c->e1, s));
     FG_CHK(hipGetLastError());

@@ -372,6 +372,16 @@ __device__ __forceinline__ void fsum_add(fsum &s, const fe &V0, const fe &V1, co
     s.c = add(s.c, add(mul(am0, V0), aV0));
     s.p1 = add(s.p1, add(mul(am1, V1), aV1));
 }
+// a and p(0) only.  For every round j >= 1 of a phase the tables are those of
+// round j-1 folded at r[j-1], so p_j(0) + p_j(1) = p_{j-1}(r[j-1]) exactly;
+// k_derive_b sets b = p_{j-1}(r[j-1]) - a - 2 c once the sums are reduced,
+// and the fold skips the product of p(1).
+__device__ __forceinline__ void fsum_add_ac(fsum &s, const fe &V0, const fe &V1, const fe &aV0, const fe &am0,
+                                            const fe &am1)
+{
+    s.a = add(s.a, mul(sub(am1, am0), sub(V1, V0)));
+    s.c = add(s.c, add(mul(am0, V0), aV0));
+}
 __device__ __forceinline__ fe lerp(const fe &x0, const fe &x1, const fe &r) { return add(x0, mul(sub(x1, x0), r)); }
 __device__ __forceinline__ ftab fold_input(const launch::fold_inst &t, int j0, int cur)
 {
@@ -407,16 +417,17 @@ __device__ __forceinline__ void fold_lane(Load ld, const fe *rq, fsum *acc, fe &
     }
     oV = V[0]; oaV = aV[0]; oam = am[0];
 }
-// per-block partials (a, b, c) of B rounds, b = p(1) - a - c
-template <int B>
+// per-block partials (a, b, c) of B rounds, b = p(1) - a - c; without P1
+// (a and p(0) summed only) b = 0, set after the reduction by k_derive_b
+template <int B, bool P1 = true>
 __device__ __forceinline__ void fold_partials(fsum *acc, const launch::fold_inst &t, int slot_round)
 {
 #pragma unroll
     for (int q = 0; q < B; ++q) {
-        fe v3[3] = {acc[q].a, acc[q].c, acc[q].p1};
-        block_sum<3>(v3);
+        fe v3[3] = {acc[q].a, acc[q].c, P1 ? acc[q].p1 : fzero()};
+        block_sum<P1 ? 3 : 2>(v3);
         if (threadIdx.x == 0) {
-            fe o3[3] = {v3[0], sub(sub(v3[2], v3[0]), v3[1]), v3[1]};
+            fe o3[3] = {v3[0], P1 ? sub(sub(v3[2], v3[0]), v3[1]) : fzero(), v3[1]};
             for (int c = 0; c < 3; ++c) t.slots[((int64_t)(slot_round + q) * 3 + c) * FB_MAXB + blockIdx.x] = o3[c];
         }
     }
@@ -456,32 +467,39 @@ __device__ __forceinline__ fe lane_xor(const fe &x, int m)
 // folds the pairs of even k, its partner those of odd k, each receiving the
 // other's element; both keep the result as value k / 2.  Afterwards lane l
 // holds F_B[rot_B(l)] as value 0.  Round q's sums go into acc[q].
-template <int B>
-__device__ __forceinline__ void fold_wave(fe *V, fe *aV, fe *am, const fe *rq, fsum *acc, int lane)
+//
+// The values are folded as a post-order walk: values [k0, k0 + 2^Q) become
+// value k0 / 2^Q of round Q once both halves are folded, so they are loaded
+// two at a time and at most Q + 2 of each table are live (4 waves per SIMD
+// instead of 2 with all 2^B loaded first; the round sums are field sums, so
+// their order is free).  P1: round sums with p(1) (else a and p(0) only).
+template <int Q, bool P1>
+__device__ __forceinline__ void fold_tree(const ftab &X, int64_t o, int k0, const fe *rq, fsum *acc, int lane, fe &V,
+                                          fe &aV, fe &am)
 {
-#pragma unroll
-    for (int q = 0; q < B; ++q) {
-        const int m = 1 << q;
+    if constexpr (Q == 0) {
+        V = X.V[o + 64 * k0]; aV = X.aV[o + 64 * k0]; am = X.am[o + 64 * k0];
+    } else {
+        constexpr int q = Q - 1, m = 1 << q;
+        fe Va, aVa, ama, Vb, aVb, amb;
+        fold_tree<Q - 1, P1>(X, o, k0, rq, acc, lane, Va, aVa, ama);
+        fold_tree<Q - 1, P1>(X, o, k0 + m, rq, acc, lane, Vb, aVb, amb);
         const bool hi = (lane >> q) & 1;
-#pragma unroll
-        for (int j = 0; j < (1 << (B - q - 1)); ++j) {
-            fe rV = lane_xor(hi ? V[2 * j] : V[2 * j + 1], m);
-            fe raV = lane_xor(hi ? aV[2 * j] : aV[2 * j + 1], m);
-            fe ram = lane_xor(hi ? am[2 * j] : am[2 * j + 1], m);
-            fe V0 = hi ? rV : V[2 * j], V1 = hi ? V[2 * j + 1] : rV;
-            fe aV0 = hi ? raV : aV[2 * j], aV1 = hi ? aV[2 * j + 1] : raV;
-            fe am0 = hi ? ram : am[2 * j], am1 = hi ? am[2 * j + 1] : ram;
-            fsum_add(acc[q], V0, V1, aV0, aV1, am0, am1);
-            V[j] = lerp(V0, V1, rq[q]); aV[j] = lerp(aV0, aV1, rq[q]); am[j] = lerp(am0, am1, rq[q]);
-        }
+        fe rV = lane_xor(hi ? Va : Vb, m), raV = lane_xor(hi ? aVa : aVb, m), ram = lane_xor(hi ? ama : amb, m);
+        fe V0 = hi ? rV : Va, V1 = hi ? Vb : rV;
+        fe aV0 = hi ? raV : aVa, aV1 = hi ? aVb : raV;
+        fe am0 = hi ? ram : ama, am1 = hi ? amb : ram;
+        if constexpr (P1) fsum_add(acc[q], V0, V1, aV0, aV1, am0, am1);
+        else fsum_add_ac(acc[q], V0, V1, aV0, am0, am1);
+        V = lerp(V0, V1, rq[q]); aV = lerp(aV0, aV1, rq[q]); am = lerp(am0, am1, rq[q]);
     }
 }
 // rounds j0 .. j0+B-1: wave unit u (64 outputs; nout is a multiple of 64)
-// folds values [u 64 2^B, (u+1) 64 2^B) to out[u 64 .. u 64 + 64)
-template <int B>
-__global__ __launch_bounds__(256) void k_fold_reg(const launch::fold_inst *I, int cur, int j0, int64_t nout, int slot_round)
+// folds values [u 64 2^B, (u+1) 64 2^B) to out[u 64 .. u 64 + 64).  P1 for
+// the launch that starts a phase (j0 = 0), whose round 0 has no predecessor.
+template <int B, bool P1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P1 ? 3 : 4))) void k_fold_reg(const launch::fold_inst *I, int cur, int j0, int64_t nout, int slot_round)
 {
-    constexpr int H = 1 << B;
     const launch::fold_inst &t = I[blockIdx.y];
     ftab X = fold_input(t, j0, cur);
     fe rq[B];
@@ -495,14 +513,12 @@ __global__ __launch_bounds__(256) void k_fold_reg(const launch::fold_inst *I, in
     // u is uniform over the wave, so all its lanes take part in every exchange
     for (int64_t u = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; u < nu; u += ustep) {
         const int64_t o = (u << (B + 6)) + lane;
-        fe V[H], aV[H], am[H];
-#pragma unroll
-        for (int k = 0; k < H; ++k) { V[k] = X.V[o + 64 * k]; aV[k] = X.aV[o + 64 * k]; am[k] = X.am[o + 64 * k]; }
-        fold_wave<B>(V, aV, am, rq, acc, lane);
+        fe V, aV, am;
+        fold_tree<B, P1>(X, o, 0, rq, acc, lane, V, aV, am);
         const int64_t g = (u << 6) + (((lane >> B) | (lane << (6 - B))) & 63);
-        t.V[cur][g] = V[0]; t.aV[cur][g] = aV[0]; t.am[cur][g] = am[0];
+        t.V[cur][g] = V; t.aV[cur][g] = aV; t.am[cur][g] = am;
     }
-    fold_partials<B>(acc, t, slot_round);
+    fold_partials<B, P1>(acc, t, slot_round);
 }
 // Table build of one phase fused with its first B fold rounds: the full
 // tables are never stored.  A block takes tiles of 256 2^B consecutive
@@ -679,6 +695,35 @@ __global__ void k_reduce_slots_table(const launch::slot_job *J, const int64_t *s
         for (int c = 0; c < 3; ++c) acc[c] = add(acc[c], q.part[((int64_t)R * 3 + c) * FB_MAXB + b]);
     block_sum<3>(acc);
     if (threadIdx.x == 0) { q.msg[q.dst[R]] = acc[0]; q.msg[q.dst[R] + 1] = acc[1]; q.msg[q.dst[R] + 2] = acc[2]; }
+}
+// b of the slot rounds j >= 1 of the phase whose round 0 is slot R (rof[R] <
+// 0), in round order: p_j(0) + p_j(1) = p_{j-1}(r[j-1]) exactly, because
+// round j's tables are round j-1's folded at r[j-1], so
+// b_j = p_j(1) - a_j - c_j = (c' + r (b' + r a')) - a_j - 2 c_j with
+// (a', b', c') round j-1's message (msg order a, b, c: a x^2 + b x + c).
+__device__ __forceinline__ void derive_chain(const int64_t *dst, const int64_t *rof, int n, int R, const fe *chal, fe *msg)
+{
+    if (rof[R] >= 0) return;
+    for (int k = R + 1; k < n && rof[k] >= 0; ++k) {
+        const fe *pv = msg + dst[k - 1];
+        fe *cv = msg + dst[k];
+        fe r = chal[rof[k]];
+        fe s = add(pv[2], mul(r, add(pv[1], mul(r, pv[0]))));
+        cv[1] = sub(sub(s, cv[0]), dbl(cv[2]));
+    }
+}
+__global__ void k_derive_b(const int64_t *dst, const int64_t *rof, int n, const fe *chal, fe *msg)
+{
+    int R = blockIdx.x * blockDim.x + threadIdx.x;
+    if (R < n) derive_chain(dst, rof, n, R, chal, msg);
+}
+__global__ void k_derive_b_table(const launch::slot_job *J, const int64_t *start, int n, int64_t total)
+{
+    int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (x >= total) return;
+    int j = job_of(start, n, x);
+    const launch::slot_job &q = J[j];
+    derive_chain(q.dst, q.rof, (int)(start[j + 1] - start[j]), (int)(x - start[j]), q.chal, q.msg);
 }
 __global__ void k_copy_table(const launch::copy_job *J, const int64_t *start, int n, int64_t total)
 {
@@ -2044,9 +2089,15 @@ int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s, int firs
         if (k > 0 || !skip_first) {
             int64_t nout = (int64_t)1 << (L - j - b);
             dim3 g((unsigned)fold_blocks(L, j, first), ninst);
-            if (b == 3) hipLaunchKernelGGL(k_fold_reg<3>, g, dim3(256), 0, s, I, cur, j, nout, rounds);
-            else if (b == 2) hipLaunchKernelGGL(k_fold_reg<2>, g, dim3(256), 0, s, I, cur, j, nout, rounds);
-            else hipLaunchKernelGGL(k_fold_reg<1>, g, dim3(256), 0, s, I, cur, j, nout, rounds);
+            if (j == 0) {
+                if (b == 3) hipLaunchKernelGGL((k_fold_reg<3, true>), g, dim3(256), 0, s, I, cur, j, nout, rounds);
+                else if (b == 2) hipLaunchKernelGGL((k_fold_reg<2, true>), g, dim3(256), 0, s, I, cur, j, nout, rounds);
+                else hipLaunchKernelGGL((k_fold_reg<1, true>), g, dim3(256), 0, s, I, cur, j, nout, rounds);
+            } else {
+                if (b == 3) hipLaunchKernelGGL((k_fold_reg<3, false>), g, dim3(256), 0, s, I, cur, j, nout, rounds);
+                else if (b == 2) hipLaunchKernelGGL((k_fold_reg<2, false>), g, dim3(256), 0, s, I, cur, j, nout, rounds);
+                else hipLaunchKernelGGL((k_fold_reg<1, false>), g, dim3(256), 0, s, I, cur, j, nout, rounds);
+            }
         }
         cur ^= 1;
         rounds += b;
@@ -2055,9 +2106,12 @@ int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s, int firs
     hipLaunchKernelGGL(k_fold_tail, dim3(1, ninst), dim3(64), 0, s, I, cur, j, L - j);
     return rounds;
 }
-void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s)
+void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, const int64_t *rof, const fe *chal,
+                  fe *msg, hipStream_t s)
 {
-    if (nslots) hipLaunchKernelGGL(k_reduce_slots, dim3(nslots), dim3(256), 0, s, slots, nb, dst, msg);
+    if (!nslots) return;
+    hipLaunchKernelGGL(k_reduce_slots, dim3(nslots), dim3(256), 0, s, slots, nb, dst, msg);
+    hipLaunchKernelGGL(k_derive_b, dim3((nslots + 63) / 64), dim3(64), 0, s, dst, rof, nslots, chal, msg);
 }
 void half_multi(const half_desc *d, int nd, int64_t total, hipStream_t s)
 {
@@ -2286,7 +2340,9 @@ void merkle_multi(int ntree, uint8_t *const *tree, const int64_t *nleaf, hipStre
 }
 void reduce_slots_table(const slot_job *J, const int64_t *start, int n, int64_t total, hipStream_t s)
 {
-    if (total) hipLaunchKernelGGL(k_reduce_slots_table, dim3((unsigned)total), dim3(256), 0, s, J, start, n);
+    if (!total) return;
+    hipLaunchKernelGGL(k_reduce_slots_table, dim3((unsigned)total), dim3(256), 0, s, J, start, n);
+    hipLaunchKernelGGL(k_derive_b_table, dim3((unsigned)((total + 63) / 64)), dim3(64), 0, s, J, start, n, total);
 }
 void copy_table(const copy_job *J, const int64_t *start, int n, int64_t total, hipStream_t s)
 {

@@ -139,8 +139,13 @@ struct fold_inst {
 // plan with first-launch cap `first`; skip_first: the first launch already
 // ran (gather_fold)
 int sumcheck_multi(const fold_inst *I, int ninst, int L, hipStream_t s, int first = FB_B, int skip_first = 0);
-// msg[dst[R]..+3] = sum over the nb[R] block partials of slot round R
-void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, fe *msg, hipStream_t s);
+// msg[dst[R]..+3] = sum over the nb[R] block partials of slot round R; then
+// for every slot round j >= 1 of a phase (rof[R] >= 0, the offset of r[j-1]
+// in chal) b = p_{j-1}(r[j-1]) - a - 2c from the previous round's message
+// (the folds of rounds j >= 1 sum a and p(0) only).  A phase's slot rounds
+// are consecutive slots from its round 0 (rof = -1).
+void reduce_slots(const fe *slots, int nslots, const int *nb, const int64_t *dst, const int64_t *rof, const fe *chal,
+                  fe *msg, hipStream_t s);
 struct half_desc {            // one eq half table: out[j] = f0 * prod_{i<nbits} eq(bit_i(j), r[i])
     fe *out;
     const fe *f0;
@@ -268,6 +273,8 @@ struct slot_job {
     const int *nb;
     const int64_t *dst;
     fe *msg;
+    const int64_t *rof;     // as reduce_slots
+    const fe *chal;
 };
 struct copy_job {
     const uint64_t *addr;     // gather: out[k] = *(fe *)addr[k]; null: out[k] = base[k]

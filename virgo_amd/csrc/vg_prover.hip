@@ -614,8 +614,9 @@ struct vg_ctx {
     int n_slots = 0;
     int *slot_nb = nullptr;
     int64_t *slot_dst = nullptr;
+    int64_t *slot_rof = nullptr;               // offset of r[j-1] in chal; -1 for round 0
     std::vector<int> h_slot_nb;
-    std::vector<int64_t> h_slot_dst;
+    std::vector<int64_t> h_slot_dst, h_slot_rof;
     fe *bt[8] = {};             // beta half tables: g0f g0s g1f g1s uf us vf vs
     fe *msg = nullptr;          // device results of the GKR part
     fe *mle_t[2] = {};
@@ -736,6 +737,7 @@ int64_t layout(vg_ctx *c, uint8_t *base)
     c->slots = carve<fe>(base, off, (int64_t)(c->n_slots > 0 ? c->n_slots : 1) * 3 * launch::FW_BLOCKS);
     c->slot_nb = carve<int>(base, off, c->n_slots + 1);
     c->slot_dst = carve<int64_t>(base, off, c->n_slots + 1);
+    c->slot_rof = carve<int64_t>(base, off, c->n_slots + 1);
     int half = (maxbl >> 1) + 1;
     for (int k = 0; k < 8; ++k) c->bt[k] = carve<fe>(base, off, (int64_t)1 << half);
     c->msg = carve<fe>(base, off, c->msg_len);
@@ -1004,7 +1006,7 @@ int enqueue_gkr(vg_ctx *c, bool gkr = true)
                 HIPCHK(hipStreamWaitEvent(st, c->gev[w], 0));
             }
         if (slot != c->n_slots) FAILV(-30, "fold slot plan mismatch");
-        launch::reduce_slots(c->slots, c->n_slots, c->slot_nb, c->slot_dst, c->msg, st);
+        launch::reduce_slots(c->slots, c->n_slots, c->slot_nb, c->slot_dst, c->slot_rof, c->chal, c->msg, st);
         HIPCHK(hipEventRecord(c->ev[2], st));
     }
     return 0;
@@ -1301,7 +1303,7 @@ int64_t upload_bytes(const vg_ctx *c)
         t += sb(4 * nn);
         if (i > 0) t += sb(nn) + sb(4 * nn);
     }
-    if (c->n_slots) t += sb((int64_t)sizeof(int) * c->n_slots) + sb((int64_t)sizeof(int64_t) * c->n_slots);
+    if (c->n_slots) t += sb((int64_t)sizeof(int) * c->n_slots) + 2 * sb((int64_t)sizeof(int64_t) * c->n_slots);
     for (int i = 1; i < C.depth; ++i) t += sb(4 * (int64_t)c->h_z2[i].size());
     t += sb((int64_t)sizeof(launch::csr_layer) * (C.depth - 1));
     t += sb((int64_t)sizeof(launch::half_desc) * (8 * (C.depth - 1) + 3));
@@ -1372,7 +1374,7 @@ int ctx_plan(vg_ctx *c)
     }
     // slot rounds of the block folds: batch by batch, phase 1 of its layers,
     // then phase 2; each instance's rounds contiguous
-    c->h_slot_nb.clear(); c->h_slot_dst.clear();
+    c->h_slot_nb.clear(); c->h_slot_dst.clear(); c->h_slot_rof.clear();
     c->slot0.assign(2 * C.depth, 0);
     for (auto &b : c->batches) {
         int js[32], ks[32];
@@ -1383,10 +1385,13 @@ int ctx_plan(vg_ctx *c)
                 int i = b.layers[q];
                 c->slot0[2 * (b.d0 + q) + ph] = (int)c->h_slot_nb.size();
                 int64_t sums = c->msg_layer[i] + (ph == 0 ? 0 : 3 * b.bl + 5);
+                int64_t r0 = ph == 0 ? c->chal_ru[i] : c->chal_rv[i];
                 for (int k = 0; k < nl; ++k)
                     for (int t = 0; t < ks[k]; ++t) {
+                        int j = js[k] + t;
                         c->h_slot_nb.push_back(launch::fold_blocks(b.bl, js[k], launch::GF_B));
-                        c->h_slot_dst.push_back(sums + 3 * (js[k] + t));
+                        c->h_slot_dst.push_back(sums + 3 * j);
+                        c->h_slot_rof.push_back(j == 0 ? -1 : r0 + j - 1);
                     }
             }
     }
@@ -1454,6 +1459,7 @@ int ctx_alloc(vg_ctx *c)
     if (c->n_slots) {
         HIPCHK(U.put(c->slot_nb, c->h_slot_nb.data(), (int64_t)sizeof(int) * c->n_slots));
         HIPCHK(U.put(c->slot_dst, c->h_slot_dst.data(), (int64_t)sizeof(int64_t) * c->n_slots));
+        HIPCHK(U.put(c->slot_rof, c->h_slot_rof.data(), (int64_t)sizeof(int64_t) * c->n_slots));
     }
     {
         // the CSR transposes of every gate layer, in six launches
@@ -2387,7 +2393,10 @@ int vg_prove_batch(vg_ctx *const *ctxs, int n, int workers, uint8_t **transcript
         std::vector<launch::copy_job> cj;
         std::vector<int64_t> ss(1, 0), cs(1, 0);
         for (vg_ctx *c : G.S) {
-            if (c->n_slots) { sj.push_back({c->slots, c->slot_nb, c->slot_dst, c->msg}); ss.push_back(ss.back() + c->n_slots); }
+            if (c->n_slots) {
+                sj.push_back({c->slots, c->slot_nb, c->slot_dst, c->msg, c->slot_rof, c->chal});
+                ss.push_back(ss.back() + c->n_slots);
+            }
             if (c->n_gather) { cj.push_back({c->gsrc, nullptr, c->h_gout}); cs.push_back(cs.back() + c->n_gather); }
             cj.push_back({nullptr, c->msg, c->h_msg});
             cs.push_back(cs.back() + c->msg_len);

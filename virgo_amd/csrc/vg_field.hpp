@@ -81,6 +81,12 @@ VG_HD fe mul(const fe &x, const fe &y)
 }
 // multiply by an element of the base field F_p (img == 0)
 VG_HD fe mul_base(const fe &x, uint64_t c) { return mk(mulp(x.real, c), mulp(x.img, c)); }
+// a 2^k mod p for a < p, 0 <= k < 61: since 2^61 == 1, a rotation of the 61
+// bits (the result is canonical: only p's own pattern is all ones)
+VG_HD uint64_t rotp(uint64_t a, int k) { return ((a << k) & P) | (a >> (61 - k)); }
+VG_HD fe mul_pow2(const fe &x, int k) { return mk(rotp(x.real, k), rotp(x.img, k)); }
+// x / 2 = x 2^60
+VG_HD fe halve(const fe &x) { return mul_pow2(x, 60); }
 
 inline fe fpow(fe x, unsigned __int128 e)
 {

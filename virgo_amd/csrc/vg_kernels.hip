@@ -106,11 +106,26 @@ __global__ void k_eval_layer(const uint8_t *ty, const int32_t *gu, const int32_t
 // ---------------------------------------------------------------------
 // multilinear evaluation by halving (V_res, zk_prover.cpp:30-52)
 // ---------------------------------------------------------------------
-__global__ void k_mle_fold(const fe *in, fe *out, int64_t n_out, const fe *rp)
+// h rounds of an MLE evaluation in one block of 2^(h-1) threads: block b
+// folds in[b 2^h, (b+1) 2^h) to out[b], round k taking each pair (x0, x1)
+// to x0 (1 - r[k]) + x1 r[k] = x0 + r[k] (x1 - x0); the first round in
+// registers, the rest in LDS
+__global__ __launch_bounds__(1024) void k_mle_block(const fe *in, fe *out, const fe *r, int h)
 {
-    fe r = *rp, om = sub(fone(), r);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_out; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = add(mul(in[2 * i], om), mul(in[2 * i + 1], r));
+    __shared__ fe buf[1024];
+    const int t = threadIdx.x;
+    const fe *src = in + ((int64_t)blockIdx.x << h);
+    int no = 1 << (h - 1);
+    fe v = fzero();
+    if (t < no) { fe a = src[2 * t], b = src[2 * t + 1]; v = add(a, mul(sub(b, a), r[0])); }
+    for (int k = 1; k < h; ++k) {
+        if (t < no) buf[t] = v;
+        __syncthreads();
+        no >>= 1;
+        if (t < no) { fe a = buf[2 * t], b = buf[2 * t + 1]; v = add(a, mul(sub(b, a), r[k])); }
+        __syncthreads();
+    }
+    if (t == 0) out[blockIdx.x] = v;
 }
 // the last rounds of an MLE evaluation (n <= 2 FOLD_TAIL entries) in LDS
 __global__ __launch_bounds__(FOLD_TAIL) void k_mle_tail(const fe *in, int n, const fe *r, int nr, fe *dst)
@@ -763,7 +778,9 @@ __global__ __launch_bounds__(TPB) void k_general(const launch::layer_dev *D)
     bu_t bu = {d.bu.f, d.bu.s, d.bu.fh}, bv = {d.bv.f, d.bv.s, d.bv.fh};
     fe vu = *d.vu, vv = *d.vv, one = fone();
     fe uv = mul(vu, vv);
-    fe acc = fzero();
+    // add and mult gates (nearly all of a layer) sum their beta products by
+    // type, times v_u + v_v and v_u v_v once at the end
+    fe acc = fzero(), sa = fzero(), sm = fzero();
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += gridDim.x * blockDim.x) {
         int ty = d.ty[i], u = d.u[i], v = d.v[i];
         if (ty == 2 || ty == 3 || ty == 4) continue;
@@ -789,9 +806,13 @@ __global__ __launch_bounds__(TPB) void k_general(const launch::layer_dev *D)
             acc = add(acc, mul(s, f));
         } else {
             int vv_idx = (ty == 10) ? 0 : v;
-            acc = add(acc, mul(mul(mul(tg, beta_u(bu, u)), beta_u(bv, vv_idx)), f));
+            fe t = mul(mul(tg, beta_u(bu, u)), beta_u(bv, vv_idx));
+            if (ty == 0) sa = add(sa, t);
+            else if (ty == 1) sm = add(sm, t);
+            else acc = add(acc, mul(t, f));
         }
     }
+    acc = add(acc, add(mul(sa, add(vu, vv)), mul(sm, uv)));
     block_sum<1>(&acc);
     if (threadIdx.x == 0) d.part[blockIdx.x] = acc;
 }
@@ -1143,7 +1164,7 @@ __global__ void k_fri_fold_nat(const fe *in, int64_t n, const fe *r, const fe *i
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     fe a = in[i], b = in[i + n];
-    out[i] = mul_base(add(add(a, b), mul(mul(*r, iw[i]), sub(a, b))), 1ull << 60);    // 2^-1 = 2^60
+    out[i] = add(halve(add(a, b)), mul(mul(halve(*r), iw[i]), sub(a, b)));
 }
 // value k of a codeword held in natural order (log_rows = 0) or coset-major
 // (the extension's layout: value k in row k mod 2^log_rows, column
@@ -1174,7 +1195,7 @@ __global__ void k_fri_fold_any(const fe *in, int64_t half, int log_rows, int64_t
     if (i >= half) return;
     fe a = cw_at(in, i, log_rows, cols), b = cw_at(in, i + half, log_rows, cols);
     fe iw = tw[(SS - (i << sh)) & (SS - 1)];
-    fe o = mul_base(add(add(a, b), mul(mul(*r, iw), sub(a, b))), 1ull << 60);    // 2^-1 = 2^60
+    fe o = add(halve(add(a, b)), mul(mul(halve(*r), iw), sub(a, b)));
     out[i] = o;
     if (leaves) {
         int64_t h2 = half >> 1;
@@ -1298,7 +1319,7 @@ __global__ __launch_bounds__(TPB) void k_vo_fold(const fe *l, const fe *q, const
     fe xinv = tw[(SS - p) & (SS - 1)];
     fe sum = sub(add(ga, gb), add(c0[j], c0[j]));
     fe v = mul(xinv, add(sub(ga, gb), mul(mul(*rp, xinv), sum)));
-    next[tid] = mul_base(v, mulp((uint64_t)e, 1ull << 60));                   // e / 2 = e 2^60
+    next[tid] = mul_pow2(v, (63 - __builtin_clzll((unsigned long long)e) + 60) % 61);     // e / 2 = 2^(log2 e - 1)
 }
 // ---------------------------------------------------------------------
 // hashing: leaf chains and Merkle levels
@@ -1323,7 +1344,9 @@ __device__ __forceinline__ int64_t leaf_of(int64_t tid, int64_t nleaf, int log_R
     return (t << log_R) + s;
 }
 // FRI fold of 65 slices (fri.cpp:321-340, 367-379): next[p] = ((a+b) + w_S^-p r (a-b)) / 2,
-// a = prev[p], b = prev[p + S/2]; one thread per (slice, position), row-contiguous.
+// a = prev[p], b = prev[p + S/2], as (a+b)/2 + w_S^-p (r/2) (a-b) (halving is a
+// rotation: two general products per position); one thread per (slice,
+// position), row-contiguous.
 __global__ __launch_bounds__(TPB) void k_fri_fold(const fe *prev, int64_t S, int log_R, const fe *rp, const fe *tw, int64_t SS,
                                                  fe *next)
 {
@@ -1335,9 +1358,8 @@ __global__ __launch_bounds__(TPB) void k_fri_fold(const fe *prev, int64_t S, int
     int64_t srow = st / cols, t = st - srow * cols;
     int64_t p = (t << log_R) + srow;
     fe a = prev[j * S + cw_idx(p, S, log_R)], b = prev[j * S + cw_idx(p + Sn, S, log_R)];
-    fe imr = mul(tw[(SS - p * (SS / S)) & (SS - 1)], *rp);
-    const fe inv2 = mk(1ull << 60, 0);
-    next[tid] = mul(inv2, add(add(a, b), mul(imr, sub(a, b))));
+    fe imr = mul(tw[(SS - p * (SS / S)) & (SS - 1)], halve(*rp));
+    next[tid] = add(halve(add(a, b)), mul(imr, sub(a, b)));
 }
 // FRI levels with at most FRI_TAIL positions per slice run in k_fri_tail
 constexpr int FRI_TAIL = 4096;
@@ -1354,16 +1376,15 @@ __global__ __launch_bounds__(256) void k_fri_tail(const fe *prev, int64_t S, int
     int64_t j = blockIdx.x;
     for (int64_t p = threadIdx.x; p < S; p += blockDim.x) v[p] = prev[j * S + cw_idx(p, S, log_R)];
     __syncthreads();
-    const fe inv2 = mk(1ull << 60, 0);
     for (int k = 0; k < F.n; ++k) {
         int64_t Sn = S / 2;
-        fe rr = r[k];
+        fe rr = halve(r[k]);
         fe nv[FRI_TAIL / 2 / 256];
         int c = 0;
         for (int64_t p = threadIdx.x; p < Sn; p += blockDim.x, ++c) {
             fe a = v[p], b = v[p + Sn];
             fe imr = mul(tw[(SS - p * (SS / S)) & (SS - 1)], rr);
-            nv[c] = mul(inv2, add(add(a, b), mul(imr, sub(a, b))));
+            nv[c] = add(halve(add(a, b)), mul(imr, sub(a, b)));
             F.next[k][j * Sn + cw_idx(p, Sn, log_R)] = nv[c];
         }
         __syncthreads();
@@ -2000,15 +2021,15 @@ void mle_eval(const fe *in, int log_n, const fe *r, fe *t0, fe *t1, fe *dst, hip
     const fe *src = in;
     fe *bufs[2] = {t0, t1};
     int cur = 0;
+    // up to 11 rounds per launch (a 2^20-entry layer: one block launch, then the tail)
     while (n > FOLD_TAIL) {
-        int64_t no = n / 2;
-        unsigned g = nblk(no);
-        if (g > 4096) g = 4096;
-        hipLaunchKernelGGL(k_mle_fold, dim3(g), dim3(TPB), 0, s, src, bufs[cur], no, r + k);
+        int h = log_n - k < 11 ? log_n - k : 11;
+        int64_t no = n >> h;
+        hipLaunchKernelGGL(k_mle_block, dim3((unsigned)no), dim3(1 << (h - 1)), 0, s, src, bufs[cur], r + k, h);
         src = bufs[cur];
         cur ^= 1;
         n = no;
-        ++k;
+        k += h;
     }
     hipLaunchKernelGGL(k_mle_tail, dim3(1), dim3(FOLD_TAIL), 0, s, src, (int)n, r + k, log_n - k, dst);
 }

@@ -403,10 +403,12 @@ __device__ __forceinline__ ftab fold_input(const launch::fold_inst &t, int j0, i
     if (j0 == 0) return {t.Vb, t.aVb, t.amb};
     return {t.V[cur ^ 1], t.aV[cur ^ 1], t.am[cur ^ 1]};
 }
-// B rounds on the 2^B consecutive values one lane holds of each table;
-// ld(k, V, aV, am) supplies value k of the three tables (called in order,
-// one pair at a time, so at most half of them are live at once).  Round q's
-// sums go into acc[q]; the folded values into oV, oaV, oam.
+// B rounds (the first of a phase and its successors) on the 2^B consecutive
+// values one lane holds of each table; ld(k, V, aV, am) supplies value k of
+// the three tables (called in order, one pair at a time, so at most half of
+// them are live at once).  Round q's sums go into acc[q] (p(1) for round 0
+// only: later rounds' b comes from k_derive_b); the folded values into oV,
+// oaV, oam.
 template <int B, class Load>
 __device__ __forceinline__ void fold_lane(Load ld, const fe *rq, fsum *acc, fe &oV, fe &oaV, fe &oam)
 {
@@ -424,7 +426,7 @@ __device__ __forceinline__ void fold_lane(Load ld, const fe *rq, fsum *acc, fe &
     for (int q = 1; q < B; ++q) {
 #pragma unroll
         for (int p2 = 0; p2 < (H >> q); ++p2) {
-            fsum_add(acc[q], V[2 * p2], V[2 * p2 + 1], aV[2 * p2], aV[2 * p2 + 1], am[2 * p2], am[2 * p2 + 1]);
+            fsum_add_ac(acc[q], V[2 * p2], V[2 * p2 + 1], aV[2 * p2], am[2 * p2], am[2 * p2 + 1]);
             V[p2] = lerp(V[2 * p2], V[2 * p2 + 1], rq[q]);
             aV[p2] = lerp(aV[2 * p2], aV[2 * p2 + 1], rq[q]);
             am[p2] = lerp(am[2 * p2], am[2 * p2 + 1], rq[q]);
@@ -550,7 +552,6 @@ __global__ __launch_bounds__(256) void k_gather_fold(const launch::layer_dev *D,
     constexpr int E = 1 << B;
     const launch::layer_dev &d = D[blockIdx.y];
     const launch::fold_inst &t = I[blockIdx.y];
-    static_assert(256 << B <= 2 * GF_CH, "a tile's V values must fit the chunk buffers");
     __shared__ fe cam[2 * GF_CH];
     __shared__ fe tot[3 * B];             // the block's round sums (a, c, p(1)) over its tiles
     fe *caV = cam + GF_CH;
@@ -606,18 +607,15 @@ __global__ __launch_bounds__(256) void k_gather_fold(const launch::layer_dev *D,
                 }
             }
         }
-        // the tile's V values, staged through LDS by coalesced loads
-        __syncthreads();
-        fe *sv = cam;
-        for (int k = threadIdx.x; k < tile && T0 + k < nt; k += blockDim.x) sv[k] = d.pv[T0 + k];
-        __syncthreads();
         fsum acc[B];
 #pragma unroll
         for (int q = 0; q < B; ++q) acc[q] = {fzero(), fzero(), fzero()};
         if (live) {
             if (PH == 1 && tl == 0 && d.nz > 0) aV[0] = add(aV[0], d.part[launch::MAX_PARTS]);
             fe oV, oaV, oam;
-            const fe *myv = sv + threadIdx.x * E;
+            // the entries' own V values straight from memory (64 contiguous
+            // bytes per thread; staging them through LDS cost two barriers)
+            const fe *myv = d.pv + tl;
             fold_lane<B>([&](int k, fe &V, fe &a, fe &m) { V = myv[k]; a = aV[k]; m = am[k]; }, t.r, acc, oV, oaV,
                          oam);
             int64_t g = tl >> B;

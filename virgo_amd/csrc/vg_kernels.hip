@@ -776,9 +776,7 @@ __global__ __launch_bounds__(TPB) void k_general(const launch::layer_dev *D)
     bu_t bu = {d.bu.f, d.bu.s, d.bu.fh}, bv = {d.bv.f, d.bv.s, d.bv.fh};
     fe vu = *d.vu, vv = *d.vv, one = fone();
     fe uv = mul(vu, vv);
-    // add and mult gates (nearly all of a layer) sum their beta products by
-    // type, times v_u + v_v and v_u v_v once at the end
-    fe acc = fzero(), sa = fzero(), sm = fzero();
+    fe acc = fzero();
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += gridDim.x * blockDim.x) {
         int ty = d.ty[i], u = d.u[i], v = d.v[i];
         if (ty == 2 || ty == 3 || ty == 4) continue;
@@ -804,13 +802,9 @@ __global__ __launch_bounds__(TPB) void k_general(const launch::layer_dev *D)
             acc = add(acc, mul(s, f));
         } else {
             int vv_idx = (ty == 10) ? 0 : v;
-            fe t = mul(mul(tg, beta_u(bu, u)), beta_u(bv, vv_idx));
-            if (ty == 0) sa = add(sa, t);
-            else if (ty == 1) sm = add(sm, t);
-            else acc = add(acc, mul(t, f));
+            acc = add(acc, mul(mul(mul(tg, beta_u(bu, u)), beta_u(bv, vv_idx)), f));
         }
     }
-    acc = add(acc, add(mul(sa, add(vu, vv)), mul(sm, uv)));
     block_sum<1>(&acc);
     if (threadIdx.x == 0) d.part[blockIdx.x] = acc;
 }

@@ -1549,8 +1549,8 @@ __device__ __forceinline__ void merkle_tail(uint8_t *tree, int64_t nl)
     }
     __syncthreads();
     while (cnt > 1) {
-        cnt >>= 1;                                    // <= MK_TAIL / 2 nodes: at most 2 per thread
-        uint64_t hh[MK_TAIL / 512][4];
+        cnt >>= 1;                                    // <= MK_TAIL / 2 nodes: at most one per thread
+        uint64_t hh[(MK_TAIL + 511) / 512][4];
         int c = 0;
         for (int i = threadIdx.x; i < cnt; i += blockDim.x, ++c) {
             uint64_t m[8];

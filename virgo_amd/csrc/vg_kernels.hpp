@@ -264,7 +264,7 @@ void leaf_hash_table(const leaf_seg *G, const int64_t *start, int nseg, int64_t 
 // by level from item tables (item: a tree's nodes [first, 2 first), start =
 // prefix sum of the node counts of the level), the rest one block per tree
 // G[k] (nleaf = S / 2)
-constexpr int MERKLE_TAIL = 1024;
+constexpr int MERKLE_TAIL = 256;
 // phase 3 of many proofs in two launches (batched proving): every proof's
 // slot reduction (as reduce_slots), then every proof's query answers and
 // message block copied straight into its page-locked host staging

@@ -495,13 +495,14 @@ int fg_enqueue(fg_ctx *c)
 
 hipEvent_t fg_done_event(const fg_ctx *c) { return c->e1; }
 
-int fg_finish(fg_ctx *c, std::vector<uint8_t> &out, int *accepted, int *proof_size, double *p_ms)
+int fg_finish(fg_ctx *c, tbuf &out, int *accepted, int *proof_size, double *p_ms)
 {
     if (!c->pending) FG_FAIL(-3, "fft_gkr: no proof in flight");
     c->pending = false;
     FG_CHK(hipEventSynchronize(c->e1));
     int lg = c->lg;
     tbuf t;
+    t.reserve((size_t)(16 * c->msg_len) + 4096);
     t.put("VGF1", 4);
     t.u32((uint32_t)lg);
     t.f(c->h_msg[0]);
@@ -512,7 +513,7 @@ int fg_finish(fg_ctx *c, std::vector<uint8_t> &out, int *accepted, int *proof_si
             t.f(c->h_msg[m + 3 * l.bits + 4]);           // v_u or v_v
         }
     }
-    out.insert(out.end(), t.b.begin(), t.b.end());
+    out.append(t);
     if (accepted) *accepted = check(c, c->h_msg) ? 1 : 0;
     if (proof_size) {                                    // sizeof(quadratic_poly) per round (:342 and like)
         int q = (lg + 6) + lg + 2 * lg * lg;
@@ -575,14 +576,13 @@ extern "C" int vg_fft_gkr(int lg, uint8_t **section, size_t *len, int *accepted,
     vg::fg_ctx *c = nullptr;
     int rc = vg::fg_create(&c, lg, nullptr);
     if (rc) return rc;
-    std::vector<uint8_t> out;
+    vg::tbuf out;
     rc = vg::fg_enqueue(c);
     if (!rc) rc = vg::fg_finish(c, out, accepted, proof_size, p_ms);
     vg::fg_destroy(c);
     if (rc) return rc;
-    *section = (uint8_t *)malloc(out.size());
-    if (!*section) { vg_set_error("vg_fft_gkr: out of host memory"); return -21; }
-    memcpy(*section, out.data(), out.size());
     *len = out.size();
+    *section = out.release();
+    if (!*section) { vg_set_error("vg_fft_gkr: out of host memory"); return -21; }
     return 0;
 }

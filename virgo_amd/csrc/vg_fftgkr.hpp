@@ -10,6 +10,7 @@
 namespace vg {
 
 struct fg_ctx;
+struct tbuf;
 
 // Allocates every device buffer of one fft_gkr proof with 2^lg coefficients;
 // work runs on `stream` (nullptr: a stream owned by the context).
@@ -28,7 +29,7 @@ int64_t fg_device_bytes(const fg_ctx *c);
 int fg_enqueue(fg_ctx *c);
 // Waits for the proof, appends the "VGF1" transcript section to `out` and
 // replays the program's inline checks (its verdict).  p_ms: device time.
-int fg_finish(fg_ctx *c, std::vector<uint8_t> &out, int *accepted, int *proof_size, double *p_ms);
+int fg_finish(fg_ctx *c, tbuf &out, int *accepted, int *proof_size, double *p_ms);
 // Recorded on the proof's stream after its last device work (fg_enqueue).
 hipEvent_t fg_done_event(const fg_ctx *c);
 // Verifies a "VGF1" section (as fg_finish writes it) with the program's

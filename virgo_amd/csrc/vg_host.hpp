@@ -3,7 +3,9 @@
 #pragma once
 #include <stdint.h>
 #include <string.h>
+#include <stdlib.h>
 #include <algorithm>
+#include <new>
 #include <vector>
 #include <hip/hip_runtime.h>
 #include "vg_field.hpp"
@@ -140,12 +142,41 @@ struct uploader {
     }
 };
 
-// transcript bytes: F = 16 bytes {img, real}, little-endian u32
+// transcript bytes: F = 16 bytes {img, real}, little-endian u32.  The bytes
+// live in malloc'd memory that release() hands to the caller as is (freed
+// with free(), i.e. vg_free), so a finished transcript is never copied.
 struct tbuf {
-    std::vector<uint8_t> b;
-    void put(const void *p, size_t n) { const uint8_t *c = (const uint8_t *)p; b.insert(b.end(), c, c + n); }
+    uint8_t *p = nullptr;
+    size_t n = 0, cap = 0;
+    tbuf() = default;
+    tbuf(const tbuf &) = delete;
+    tbuf &operator=(const tbuf &) = delete;
+    ~tbuf() { free(p); }
+    void reserve(size_t c)
+    {
+        if (c <= cap) return;
+        uint8_t *q = (uint8_t *)realloc(p, c);
+        if (!q) throw std::bad_alloc();
+        p = q;
+        cap = c;
+    }
+    void put(const void *src, size_t k)
+    {
+        if (n + k > cap) reserve(std::max(2 * cap, n + k + 4096));
+        memcpy(p + n, src, k);
+        n += k;
+    }
     void f(const fe &x) { put(&x, 16); }
     void u32(uint32_t x) { put(&x, 4); }
+    void append(const tbuf &o) { if (o.n) put(o.p, o.n); }
+    size_t size() const { return n; }
+    uint8_t *release()
+    {
+        uint8_t *q = p;
+        p = nullptr;
+        n = cap = 0;
+        return q;
+    }
 };
 
 }  // namespace vg

@@ -1291,6 +1291,18 @@ __device__ __forceinline__ fe vo_numer(const fe *l, const fe *q, const fe *cq, c
     fe xe = tw[(p * e) & (SS - 1)];
     return sub(mul(l[id], qv), mul(sub(xe, fone()), h[id]));
 }
+// the first FRI fold's value at p < SS/2 of slice j from the virtual oracle
+// at p and p + SS/2
+__device__ __forceinline__ fe vo_value(const fe *l, const fe *q, const fe *cq, const fe *h, const fe *c0, int j, bool last,
+                                       int64_t p, int64_t e, int64_t SS, int log_R, const fe *tw, const fe &r)
+{
+    fe ga = vo_numer(l, q, cq, h, j, last, p, e, SS, log_R, tw);
+    fe gb = vo_numer(l, q, cq, h, j, last, p + SS / 2, e, SS, log_R, tw);
+    fe xinv = tw[(SS - p) & (SS - 1)];
+    fe sum = sub(add(ga, gb), add(c0[j], c0[j]));
+    fe v = mul(xinv, add(sub(ga, gb), mul(mul(r, xinv), sum)));
+    return mul_pow2(v, (63 - __builtin_clzll((unsigned long long)e) + 60) % 61);     // e / 2 = 2^(log2 e - 1)
+}
 // next (SS/2 per slice, codeword layout) from the virtual oracle at p and p + SS/2
 __global__ __launch_bounds__(TPB) void k_vo_fold(const fe *l, const fe *q, const fe *cq, const fe *h, const fe *c0, int nslice,
                                                 int64_t SS, int log_R, int64_t real, int64_t m, const fe *tw, const fe *rp,
@@ -1305,13 +1317,34 @@ __global__ __launch_bounds__(TPB) void k_vo_fold(const fe *l, const fe *q, const
     int64_t srow = st / cols, t = st - srow * cols;
     int64_t p = (t << log_R) + srow;
     bool last = j == nslice - 1;
+    next[tid] = vo_value(l, q, cq, h, c0, j, last, p, last ? m : real, SS, log_R, tw, *rp);
+}
+// The first two FRI folds in one pass: thread = a position p < SS/4 of
+// slice j of the second fold's output (row-contiguous, as k_fri_fold); it
+// forms the first fold's values a at p and b at p + SS/4 from the virtual
+// oracle, writes both (they are committed) and folds them at r[1] as
+// k_fri_fold does, instead of reading them back.
+__global__ __launch_bounds__(TPB) void k_vo_fold2(const fe *l, const fe *q, const fe *cq, const fe *h, const fe *c0,
+                                                 int nslice, int64_t SS, int log_R, int64_t real, int64_t m, const fe *tw,
+                                                 const fe *rp, fe *next0, fe *next1)
+{
+    int64_t S0 = SS / 2, S1 = SS / 4;
+    int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)nslice * S1) return;
+    int j = (int)(tid / S1);
+    int64_t st = tid - (int64_t)j * S1;
+    int64_t cols = S1 >> log_R;
+    int64_t srow = st / cols, t = st - srow * cols;
+    int64_t p = (t << log_R) + srow;
+    bool last = j == nslice - 1;
     int64_t e = last ? m : real;
-    fe ga = vo_numer(l, q, cq, h, j, last, p, e, SS, log_R, tw);
-    fe gb = vo_numer(l, q, cq, h, j, last, p + Sn, e, SS, log_R, tw);
-    fe xinv = tw[(SS - p) & (SS - 1)];
-    fe sum = sub(add(ga, gb), add(c0[j], c0[j]));
-    fe v = mul(xinv, add(sub(ga, gb), mul(mul(*rp, xinv), sum)));
-    next[tid] = mul_pow2(v, (63 - __builtin_clzll((unsigned long long)e) + 60) % 61);     // e / 2 = 2^(log2 e - 1)
+    fe r0 = rp[0];
+    fe a = vo_value(l, q, cq, h, c0, j, last, p, e, SS, log_R, tw, r0);
+    fe b = vo_value(l, q, cq, h, c0, j, last, p + S1, e, SS, log_R, tw, r0);
+    next0[(int64_t)j * S0 + cw_idx(p, S0, log_R)] = a;
+    next0[(int64_t)j * S0 + cw_idx(p + S1, S0, log_R)] = b;
+    fe imr = mul(tw[(SS - p * (SS / S0)) & (SS - 1)], halve(rp[1]));
+    next1[tid] = add(halve(add(a, b)), mul(imr, sub(a, b)));
 }
 // ---------------------------------------------------------------------
 // hashing: leaf chains and Merkle levels
@@ -2283,10 +2316,16 @@ void fri_fold(const fe *prev, int64_t S, int log_R, const fe *r, const fe *tw, i
 void fri_folds_vo(const fe *l, const fe *q, const fe *cq, const fe *h, const fe *c0, int64_t real, int64_t m, int64_t SS,
                   int nlev, int log_R, const fe *r, const fe *tw, fe *const *next, hipStream_t s)
 {
-    // level 0 straight from l, q, h; then as fri_folds
+    // levels 0 and 1 straight from l, q, h (level 0 alone for a single
+    // level); then as fri_folds
+    if (nlev > 1) {
+        hipLaunchKernelGGL(k_vo_fold2, dim3(nblk(65 * (SS / 4))), dim3(TPB), 0, s, l, q, cq, h, c0, 65, SS, log_R, real, m, tw,
+                           r, next[0], next[1]);
+        if (nlev > 2) fri_folds(next[1], SS / 4, SS, nlev - 2, log_R, r + 2, tw, next + 2, s);
+        return;
+    }
     hipLaunchKernelGGL(k_vo_fold, dim3(nblk(65 * (SS / 2))), dim3(TPB), 0, s, l, q, cq, h, c0, 65, SS, log_R, real, m, tw, r,
                        next[0]);
-    if (nlev > 1) fri_folds(next[0], SS / 2, SS, nlev - 1, log_R, r + 1, tw, next + 1, s);
 }
 void fri_folds(const fe *vo, int64_t S0, int64_t SS, int nlev, int log_R, const fe *r, const fe *tw, fe *const *next,
                hipStream_t s)

@@ -369,9 +369,10 @@ __device__ __forceinline__ void fold_terms(const lpoly &V, const lpoly &aV, cons
 //
 // Streaming launches (k_fold_reg) run up to 3 rounds: each wave folds 64 2^b
 // consecutive values of every table down to 64, loading them with wave-wide
-// contiguous 1 KB loads and pairing values across lanes (fold_wave), keeping
+// contiguous 1 KB loads and pairing values across lanes (fold_tree), keeping
 // its round sums in registers; the block reduces them once at the end into
-// per-block partials.  When at most 64 values remain, one wave per instance
+// per-block partials.  Rounds j >= 1 sum a and p(0) only; k_derive_b sets
+// their b after the reduction.  When at most 64 values remain, one wave per instance
 // (k_fold_tail) runs the last rounds in LDS and writes the round sums and the
 // final pair directly.
 // ---------------------------------------------------------------------

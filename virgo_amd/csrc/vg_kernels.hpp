@@ -102,8 +102,9 @@ int gather_fold(int phase, const layer_dev *D, const fold_inst *I, int n, int L,
 // lane, no communication), then the last <= FB_TAIL rounds in one wave per
 // instance.  Round j's pair polynomials are evaluated at r[j].  Each
 // streaming launch writes its block partials of round q to slots[(q*3 + c) *
-// FW_BLOCKS + block] (q counted over the instance's launches); the caller
-// reduces every slot of the proof once with reduce_slots.  Tail round sums go
+// FW_BLOCKS + block] (q counted over the instance's launches; rounds j >= 1
+// with b = 0); the caller reduces every slot of the proof once with
+// reduce_slots, which also sets those rounds' b.  Tail round sums go
 // straight to sums[3j..].  After the last round: end[0..3] = the final pair's
 // V.a, V.b, am.a, am.b and end[4] = V.a * r[L-1] + V.b + *vconst (= v_u or
 // v_v).  Returns the slot rounds per instance.

@@ -490,7 +490,8 @@ __device__ __forceinline__ fe lane_xor(const fe &x, int m)
 // value k0 / 2^Q of round Q once both halves are folded, so they are loaded
 // two at a time and at most Q + 2 of each table are live (4 waves per SIMD
 // instead of 2 with all 2^B loaded first; the round sums are field sums, so
-// their order is free).  P1: round sums with p(1) (else a and p(0) only).
+// their order is free).  P1: round 0 (q = 0 of a launch at j0 = 0) sums
+// p(1) too; every other round a and p(0) only.
 template <int Q, bool P1>
 __device__ __forceinline__ void fold_tree(const ftab &X, int64_t o, int k0, const fe *rq, fsum *acc, int lane, fe &V,
                                           fe &aV, fe &am)
@@ -507,14 +508,15 @@ __device__ __forceinline__ void fold_tree(const ftab &X, int64_t o, int k0, cons
         fe V0 = hi ? rV : Va, V1 = hi ? Vb : rV;
         fe aV0 = hi ? raV : aVa, aV1 = hi ? aVb : raV;
         fe am0 = hi ? ram : ama, am1 = hi ? amb : ram;
-        if constexpr (P1) fsum_add(acc[q], V0, V1, aV0, aV1, am0, am1);
+        if constexpr (P1 && q == 0) fsum_add(acc[q], V0, V1, aV0, aV1, am0, am1);
         else fsum_add_ac(acc[q], V0, V1, aV0, am0, am1);
         V = lerp(V0, V1, rq[q]); aV = lerp(aV0, aV1, rq[q]); am = lerp(am0, am1, rq[q]);
     }
 }
 // rounds j0 .. j0+B-1: wave unit u (64 outputs; nout is a multiple of 64)
 // folds values [u 64 2^B, (u+1) 64 2^B) to out[u 64 .. u 64 + 64).  P1 for
-// the launch that starts a phase (j0 = 0), whose round 0 has no predecessor.
+// the launch that starts a phase (j0 = 0): its round 0, which has no
+// predecessor, sums p(1) too.
 template <int B, bool P1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P1 ? 3 : 4))) void k_fold_reg(const launch::fold_inst *I, int cur, int j0, int64_t nout, int slot_round)
 {

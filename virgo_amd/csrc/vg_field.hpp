@@ -68,6 +68,25 @@ VG_HD uint64_t dot2p(uint64_t a, uint64_t b, uint64_t c, uint64_t d)
     return fold61(t0 + (t2 << 1) + ((t1 & 0x3FFFFFFFull) << 31) + (t1 >> 30));
 }
 
+// a*b + c*d + e*f + g*h mod p for operands < 2^61, one reduction: t0 sums
+// four 31x31-bit products (< 2^64), t1 eight 30x31-bit ones (< 2^64), t2
+// four 30x30-bit ones (< 2^62); t0 is folded once first so that the final
+// sum stays below 2^64
+VG_HD uint64_t dot4p(uint64_t a, uint64_t b, uint64_t c, uint64_t d, uint64_t e, uint64_t f, uint64_t g, uint64_t h)
+{
+    const uint64_t x[4] = {a, c, e, g}, y[4] = {b, d, f, h};
+    uint64_t t0 = 0, t1 = 0, t2 = 0;
+    for (int k = 0; k < 4; ++k) {
+        uint32_t x0 = (uint32_t)x[k] & 0x7FFFFFFFu, x1 = (uint32_t)(x[k] >> 31);
+        uint32_t y0 = (uint32_t)y[k] & 0x7FFFFFFFu, y1 = (uint32_t)(y[k] >> 31);
+        t0 += (uint64_t)x0 * y0;
+        t1 += (uint64_t)x1 * y0 + (uint64_t)x0 * y1;
+        t2 += (uint64_t)x1 * y1;
+    }
+    uint64_t s0 = (t0 & P) + (t0 >> 61);
+    return fold61(s0 + (t2 << 1) + ((t1 & 0x3FFFFFFFull) << 31) + (t1 >> 30));
+}
+
 VG_HD fe add(const fe &a, const fe &b) { return mk(addp(a.real, b.real), addp(a.img, b.img)); }
 VG_HD fe sub(const fe &a, const fe &b) { return mk(subp(a.real, b.real), subp(a.img, b.img)); }
 VG_HD fe neg(const fe &a) { return mk(negp(a.real), negp(a.img)); }
@@ -78,6 +97,12 @@ VG_HD fe dbl(const fe &a) { return add(a, a); }
 VG_HD fe mul(const fe &x, const fe &y)
 {
     return mk(dot2p(x.real, y.real, P - x.img, y.img), dot2p(x.real, y.img, x.img, y.real));
+}
+// x y + z w with one reduction per component
+VG_HD fe dot2(const fe &x, const fe &y, const fe &z, const fe &w)
+{
+    return mk(dot4p(x.real, y.real, P - x.img, y.img, z.real, w.real, P - z.img, w.img),
+              dot4p(x.real, y.img, x.img, y.real, z.real, w.img, z.img, w.real));
 }
 // multiply by an element of the base field F_p (img == 0)
 VG_HD fe mul_base(const fe &x, uint64_t c) { return mk(mulp(x.real, c), mulp(x.img, c)); }

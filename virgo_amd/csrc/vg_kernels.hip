@@ -158,7 +158,7 @@ struct bu_t { const fe *f, *s; int fh; };
 __device__ __forceinline__ fe beta_g(const bg_t &b, int i)
 {
     int lo = i & ((1 << b.fh) - 1), hi = i >> b.fh;
-    return add(mul(b.g0f[lo], b.g0s[hi]), mul(b.g1f[lo], b.g1s[hi]));
+    return dot2(b.g0f[lo], b.g0s[hi], b.g1f[lo], b.g1s[hi]);
 }
 __device__ __forceinline__ fe beta_u(const bu_t &b, int u)
 {
@@ -550,7 +550,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P1 ? 3 : 4)
 constexpr int GF_CH = 1152;                  // records per chunk (a tile holds ~1024 at one per entry)
 constexpr int GF_NR = (GF_CH + 255) / 256;   // records per thread and chunk
 template <int PH, int B>
-__global__ __launch_bounds__(256) void k_gather_fold(const launch::layer_dev *D, const launch::fold_inst *I)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gather_fold(const launch::layer_dev *D, const launch::fold_inst *I)
 {
     constexpr int E = 1 << B;
     const launch::layer_dev &d = D[blockIdx.y];
@@ -771,7 +771,7 @@ __global__ void k_half_multi(const launch::half_desc *d, int nd, int64_t total)
 // ---------------------------------------------------------------------
 // wiring predicates combined with v_u, v_v (zk_verifier.cpp:219-606, :1149)
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(TPB) void k_general(const launch::layer_dev *D)
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(6))) void k_general(const launch::layer_dev *D)
 {
     const launch::layer_dev &d = D[blockIdx.y];
     if (!d.gv) return;                                   // layer 1: no general value
@@ -1292,7 +1292,7 @@ __device__ __forceinline__ fe vo_numer(const fe *l, const fe *q, const fe *cq, c
     int64_t k = cw_idx(p, SS, log_R), id = (int64_t)slice * SS + k;
     fe qv = last ? q[SS + k] : mul(cq[slice], q[k]);
     fe xe = tw[(p * e) & (SS - 1)];
-    return sub(mul(l[id], qv), mul(sub(xe, fone()), h[id]));
+    return dot2(l[id], qv, sub(fone(), xe), h[id]);               // l q - (x^e - 1) h
 }
 // the first FRI fold's value at p < SS/2 of slice j from the virtual oracle
 // at p and p + SS/2

@@ -550,7 +550,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P1 ? 3 : 4)
 constexpr int GF_CH = 1152;                  // records per chunk (a tile holds ~1024 at one per entry)
 constexpr int GF_NR = (GF_CH + 255) / 256;   // records per thread and chunk
 template <int PH, int B>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gather_fold(const launch::layer_dev *D, const launch::fold_inst *I)
+__global__ __launch_bounds__(256) void k_gather_fold(const launch::layer_dev *D, const launch::fold_inst *I)
 {
     constexpr int E = 1 << B;
     const launch::layer_dev &d = D[blockIdx.y];
